@@ -1,0 +1,198 @@
+"""Benchmark: 28-qubit ComplexF64 gate sweeps on one MI355X (plus the energy,
+gradient, batched and serving figures of the other workloads).
+
+One step = one pass of the 480-gate, 28-qubit Haar circuit (20 layers of
+14 one-, 7 two- and 3 three-qubit gates) over the state resident in HBM, one
+kernel per gate (the unfused drop-in path). `value` is the algorithmic HBM
+bandwidth of those sweeps: every gate reads and writes all 2^28 amplitudes once,
+2 * 16 * 2^28 bytes per gate.
+
+Prints one JSON line. --gpus N > 1 is refused: this build runs on one GPU.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "vqc.jl_amd"))
+sys.path.insert(0, os.path.join(ROOT, "oracle"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "28q gate HBM GB/s; energy+grad evals/s and p99 latency, batched vs plain"
+
+
+class HipTimer:
+    """hipEvent pairs recorded on the library's stream."""
+
+    def __init__(self, stream):
+        self.hip = ctypes.CDLL("libamdhip64.so")
+        self.stream = stream
+        self.e0, self.e1 = ctypes.c_void_p(), ctypes.c_void_p()
+        for e in (self.e0, self.e1):
+            assert self.hip.hipEventCreate(ctypes.byref(e)) == 0
+
+    def start(self):
+        assert self.hip.hipEventRecord(self.e0, self.stream) == 0
+
+    def stop_ms(self):
+        assert self.hip.hipEventRecord(self.e1, self.stream) == 0
+        assert self.hip.hipEventSynchronize(self.e1) == 0
+        ms = ctypes.c_float()
+        assert self.hip.hipEventElapsedTime(ctypes.byref(ms), self.e0, self.e1) == 0
+        return ms.value
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def unfused_sweep(vq, _lib, state, mats):
+    """One kernel per gate (vqc_sv_apply_dense)."""
+    for k, pos, U, _keep in mats:  # _keep holds the array U points into
+        _lib.call("vqc_sv_apply_dense", state.handle, k, pos, U)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=28)
+    ap.add_argument("--layers", type=int, default=20)
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--skip-extra", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding of a state is a design note, not implemented)", file=sys.stderr)
+        sys.exit(2)
+
+    import vqchip as vq
+    from vqchip import _lib
+    from vqchip.workloads import config1_circuit, random_pauli_operator
+
+    n = args.qubits
+    circuit = config1_circuit(n=n, layers=args.layers)
+    G = len(circuit)
+    mats = []
+    for g in circuit:
+        pos = g.ordered_positions()
+        U = np.ascontiguousarray(g.ordered_mat().reshape(-1, order="F"))
+        mats.append((len(pos), _lib.i32ptr([q - 1 for q in pos]), _lib.dptr(U), U))
+    log(f"config1: n={n} gates={G}")
+    state = vq.StateVector(n)
+    stream = ctypes.c_void_p()
+    _lib.call("vqc_get_stream", ctypes.byref(stream))
+    timer = HipTimer(stream)
+    bytes_per_gate = 2 * 16 * (1 << n)
+
+    # warmup
+    for _ in range(args.warmup):
+        unfused_sweep(vq, _lib, state, mats)
+    _lib.call("vqc_sync")
+    # timed: K steps bracketed by synchronisation
+    t0 = time.perf_counter()
+    timer.start()
+    for _ in range(args.steps):
+        unfused_sweep(vq, _lib, state, mats)
+    ms = timer.stop_ms()
+    _lib.call("vqc_sync")
+    wall = time.perf_counter() - t0
+    gbs = bytes_per_gate * G * args.steps / (ms * 1e-3) / 1e9
+    log(f"unfused: {ms / args.steps:.1f} ms/step, {gbs:.0f} GB/s (wall {wall:.2f}s)")
+    norm = state.norm()
+
+    # dominant kernel: the one-qubit dense sweep, timed alone
+    one_q = [m for m, g in zip(mats, circuit) if len(g.positions) == 1]
+    timer.start()
+    for k, p, U, _keep in one_q:
+        _lib.call("vqc_sv_apply_dense", state.handle, k, p, U)
+    ms1 = timer.stop_ms()
+    avg1_ms = ms1 / len(one_q)
+    ach1 = bytes_per_gate / (avg1_ms * 1e-3) / 1e9
+    per_class = {}
+    for kq in (1, 2, 3):
+        sub = [m for m, g in zip(mats, circuit) if len(g.positions) == kq]
+        timer.start()
+        for k, p, U, _keep in sub:
+            _lib.call("vqc_sv_apply_dense", state.handle, k, p, U)
+        per_class[f"{kq}q"] = round(bytes_per_gate / (timer.stop_ms() / len(sub) * 1e-3) / 1e9, 1)
+    log(f"per-class GB/s {per_class}")
+
+    extra = {"norm_error": abs(norm - 1.0), "per_gate_class_GBps": per_class}
+    # fused (LDS tile phases)
+    prog = vq.Program(circuit, n=n)
+    ops, pool = prog.op_array(), prog.pool()
+    _lib.call("vqc_apply_circuit", state.handle, len(prog.ops), ops, _lib.dptr(pool))
+    _lib.call("vqc_sync")
+    timer.start()
+    for _ in range(args.steps):
+        _lib.call("vqc_apply_circuit", state.handle, len(prog.ops), ops, _lib.dptr(pool))
+    msf = timer.stop_ms() / args.steps
+    extra["fused_ms_per_step"] = round(msf, 3)
+    extra["fused_effective_gate_GBps"] = round(bytes_per_gate * G / (msf * 1e-3) / 1e9, 1)
+    log(f"fused: {msf:.1f} ms/step, effective {extra['fused_effective_gate_GBps']} GB/s")
+
+    if not args.skip_extra:
+        Hp = random_pauli_operator(n, 200)
+        arr, nt = vq.pack_pauli(Hp)
+        s2 = np.zeros(2)
+        timer.start()
+        _lib.call("vqc_sv_expec_pauli", state.handle, None, nt, arr, None, _lib.dptr(s2))
+        mse = timer.stop_ms()
+        extra["pauli200_energy"] = s2[0]
+        extra["pauli200_ms"] = round(mse, 2)
+        extra["pauli200_read_GBps"] = round(16 * (1 << n) * nt / (mse * 1e-3) / 1e9, 1)
+        log(f"pauli energy {s2[0]:.6f} in {mse:.1f} ms")
+
+    # CPU baseline: the oracle (C++/OpenMP restatement of the reference's
+    # threaded kernels) on the host cores, on the first gates of the same circuit
+    import oracle
+    cores = int(os.environ.get("VQC_CPU_THREADS", min(16, os.cpu_count() or 1)))
+    oracle.set_threads(cores)
+    v = np.zeros(1 << n, dtype=np.complex128)
+    v[0] = 1
+    done, t0 = 0, time.perf_counter()
+    while done < G and time.perf_counter() - t0 < args.cpu_seconds:
+        g = circuit[done]
+        pos = [q - 1 for q in g.ordered_positions()]
+        oracle.apply_dense(v, len(pos), pos, g.ordered_mat())
+        done += 1
+    cpu_t = time.perf_counter() - t0
+    cpu_gbs = bytes_per_gate * done / cpu_t / 1e9
+    log(f"cpu oracle: {done} gates in {cpu_t:.1f}s = {cpu_gbs:.1f} GB/s on {cores} threads")
+
+    out = {
+        "metric": METRIC,
+        "value": round(gbs, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "complex128 (f64)",
+        "data": "synthetic (seeded Haar gates)",
+        "config": {"workload": f"{n}-qubit state vector, {args.layers} layers x (14 1q + 7 2q + 3 3q) Haar gates = {G} "
+                               "gates, one kernel per gate",
+                   "qubits": n, "gates": G, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(ach1, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(ach1 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_dense_small<1, 4> (1-qubit dense sweep)",
+                     "avg_launch_ms": round(avg1_ms, 4), "bytes_per_launch": bytes_per_gate},
+        "cpu_baseline": {"value": round(cpu_gbs, 2), "unit": "GB/s", "cores": cores, "kind": "port",
+                         "sample": f"first {done} gates of the same {n}-qubit circuit, oracle restatement"},
+        "extra": extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

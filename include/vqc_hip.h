@@ -1,0 +1,213 @@
+/*
+ * vqc_hip.h -- C ABI of libvqc_hip.so, the MI355X (gfx950) engine for the
+ * gate-application -> Hamiltonian-expectation -> adjoint-gradient path of
+ * VQC.jl.
+ *
+ * Conventions (all taken from the reference, cited per entry point):
+ *   - Amplitudes are ComplexF64, passed as interleaved (re, im) doubles; this is
+ *     byte-identical to Julia's Vector{ComplexF64} and C's double _Complex[].
+ *   - Qubit q (0-based here, 1-based in Julia) is bit q of the amplitude index
+ *     (little-endian, src/statevector.jl:167-176).
+ *   - A k-qubit matrix is 2^k x 2^k, column-major, over ASCENDING positions:
+ *     row/column index bit j corresponds to the j-th smallest position
+ *     (ordered_positions / ordered_mat, src/applygates/generic/generic.jl:43-50).
+ *   - A density matrix of n qubits is vec(rho), column-major, 4^n entries;
+ *     rho[i,j] lives at i + j*2^n, row bits are 0..n-1 and column bits n..2n-1
+ *     (src/densitymatrix.jl:27-30).
+ *   - Every call returns 0 on success and a negative VQC_E* code on failure;
+ *     vqc_last_error() gives a message (thread-local). This replaces the
+ *     reference's Julia exceptions (@assert / throw / error).
+ *   - Calls are ordered on one HIP stream per device. Calls that return data
+ *     to the host (downloads, expectations, gradients) synchronise; the rest
+ *     return once the work is enqueued. A state may not be used by two host
+ *     threads at once (the reference's apply! is not re-entrant either).
+ */
+#ifndef VQC_HIP_H
+#define VQC_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------- */
+#define VQC_OK            0
+#define VQC_EINVAL       -1   /* bad argument: qubit index, k, size, null pointer */
+#define VQC_ENOMEM       -2   /* device allocation failed or budget exceeded */
+#define VQC_EHIP         -3   /* a HIP runtime call failed */
+#define VQC_EUNSUPPORTED -4   /* valid request this build does not handle */
+
+/* ---- named gates (src/applygates/specific/) ------------------------------- */
+#define VQC_GATE_X        1   /* phasegate.jl:27-43   pos[0] */
+#define VQC_GATE_PHASE    2   /* phasegate.jl:6-22    |1> -> e^{i phi}|1> */
+#define VQC_GATE_CZ       3   /* czgate.jl:6-25       |11> -> -|11> */
+#define VQC_GATE_CNOT     4   /* czgate.jl:30-68      pos[0] = control, pos[1] = target */
+#define VQC_GATE_CPHASE   5   /* czgate.jl:73-95      |11> -> e^{i phi}|11> */
+#define VQC_GATE_SWAP     6   /* swapgates.jl:3-23 */
+#define VQC_GATE_ISWAP    7   /* swapgates.jl:25-49   |01>,|10> swapped times  i */
+#define VQC_GATE_ISWAPDAG 8   /* swapgates.jl:47-49   |01>,|10> swapped times -i */
+#define VQC_GATE_FREDKIN  9   /* fredkingate.jl:4-77  pos[0] = control */
+#define VQC_GATE_CCPHASE 10   /* fredkingate.jl:80-104 |111> -> e^{i phi}|111> */
+
+/* ---- circuit program ---------------------------------------------------- */
+#define VQC_OP_UNITARY 0      /* QuantumCircuits Gate: U (SV) or U.rho.U^dagger (DM) */
+#define VQC_OP_CHANNEL 1      /* QuantumMap: superoperator on (pos, pos+n) of a DM */
+
+/* One circuit element (a QCircuit entry, src/applygates/generic/generic.jl:26-31).
+ * Matrices live in a caller-owned pool of interleaved complex doubles; the
+ * offsets below count complex elements. */
+typedef struct vqc_op {
+    int32_t kind;        /* VQC_OP_UNITARY or VQC_OP_CHANNEL */
+    int32_t nq;          /* qubits acted on: 1..5 for unitaries, 1..2 for channels */
+    int32_t pos[5];      /* ascending 0-based qubit indices */
+    int32_t nparams;     /* differentiable parameters of this op: 0..3 */
+    int32_t slot[3];     /* gradient index of each parameter (-1: inactive) */
+    int32_t reserved;    /* must be 0 */
+    int64_t mat;         /* U (2^nq square) or superoperator S (4^nq square) */
+    int64_t aux;         /* channels: S^-1 for the reverse sweep; unitaries: -1 */
+    int64_t gen[3];      /* G_j = U^dagger dU/dtheta_j (2^nq square), one per parameter */
+} vqc_op_t;
+
+/* One merged Hamiltonian key: sum of the terms sharing a position tuple as a
+ * dense 2^nq matrix (src/hamiltonian/util.jl:6-15, expecs.jl:22-28). */
+typedef struct vqc_term {
+    int32_t nq;          /* 1..5 */
+    int32_t pos[5];      /* ascending 0-based qubit indices */
+    int64_t mat;         /* pool offset of M */
+} vqc_term_t;
+
+/* One Pauli string c * prod_q sigma_q: sigma_q = X (x bit only), Y (x and z),
+ * Z (z bit only). Acts as P|b> = c * i^{popcount(x&z)} * (-1)^{popcount(b&z)} |b^x>. */
+typedef struct vqc_pauli {
+    uint64_t x;
+    uint64_t z;
+    double   re;
+    double   im;
+} vqc_pauli_t;
+
+/* ---- library ----------------------------------------------------------- */
+const char* vqc_last_error(void);
+int vqc_version(void);
+/* Device facts used by planners and benchmarks. */
+int vqc_device_info(int* cu_count, int* lds_bytes_per_cu, uint64_t* hbm_bytes);
+/* Wait for all enqueued work. */
+int vqc_sync(void);
+/* The HIP stream every call is ordered on (a hipStream_t), for callers that
+ * time or order their own work against the library's (no reference counterpart). */
+int vqc_get_stream(void** stream);
+
+/* ---- state handles (StateVector / DensityMatrix storage) ----------------
+ * src/statevector.jl:1-21 (StateVector{T}(n) = |0..0>), src/densitymatrix.jl:38-42. */
+typedef struct vqc_state vqc_state;
+
+int vqc_sv_create(int nqubits, vqc_state** out);             /* |0...0> */
+int vqc_dm_create(int nqubits, vqc_state** out);             /* |0...0><0...0| */
+/* Borrow caller-owned device memory of 2^nbits complex doubles (not freed). */
+int vqc_state_wrap(int nbits, int is_dm, void* device_ptr, vqc_state** out);
+int vqc_state_destroy(vqc_state* s);
+int vqc_state_nbits(const vqc_state* s, int* nbits);
+int vqc_state_device_ptr(vqc_state* s, void** out);
+int vqc_state_upload(vqc_state* s, const double* host, uint64_t ncomplex);
+int vqc_state_download(const vqc_state* s, double* host, uint64_t ncomplex);
+int vqc_state_reset(vqc_state* s);                            /* reset!, statevector.jl:111-115 */
+int vqc_state_copy(vqc_state* dst, const vqc_state* src);     /* Base.copy, statevector.jl:35 */
+int vqc_state_scale(vqc_state* s, double re, double im);      /* x * y, statevector.jl:50 */
+/* <a|b> = sum conj(a_i) b_i (LinearAlgebra.dot, statevector.jl:58). */
+int vqc_state_dot(const vqc_state* a, const vqc_state* b, double* out2);
+
+/* ---- state-vector gates --------------------------------------------------
+ * Replaces _apply_gate_threaded2!(key, U, v) for k = 1..5:
+ * threaded_short_range.jl:100,364,753 and threaded_long_range.jl:51,131,
+ * and the serial fallback _apply_gate_2! (serial_short_range.jl:68-71). */
+int vqc_sv_apply_dense(vqc_state* s, int k, const int32_t* pos, const double* U);
+/* Replaces apply_threaded!(::NamedGate, v) (the kernels of src/applygates/specific/).
+ * pos is in the gate's own order (control first for CNOT and FREDKIN). */
+int vqc_sv_apply_named(vqc_state* s, int kind, const int32_t* pos, double phi);
+
+/* ---- Hamiltonians on state vectors -------------------------------------- */
+/* out += M_pos * in. Replaces _apply_gate_threaded2!(key, U, v, vout):
+ * short_range_threaded.jl:155,756,1504 and long_range_threaded.jl:68,176. */
+int vqc_sv_accum_dense(const vqc_state* in, vqc_state* out, int k, const int32_t* pos, const double* M);
+/* <bra|M|ket> (bra = ket when bra is NULL). Replaces expectation_value(key, M, v[, v_c]):
+ * src/hamiltonian/expecs/expecs.jl:85-131 and the kernels it dispatches to. */
+int vqc_sv_expec_dense(const vqc_state* ket, const vqc_state* bra, int k, const int32_t* pos,
+                       const double* M, double* out2);
+/* sum_t <bra|P_t|ket>, one complex per term written to out (2*nterms doubles) and
+ * their sum to sum2. Replaces the >5-site fallback dot(psi, H*psi) of
+ * expecs.jl:40 / apply_qterms.jl:20-25 for Pauli strings. */
+int vqc_sv_expec_pauli(const vqc_state* ket, const vqc_state* bra, int nterms,
+                       const vqc_pauli_t* terms, double* out, double* sum2);
+/* out = sum_t c_t P_t in  ((m::QubitsOperator)(v), apply_qterms.jl:13-28). */
+int vqc_sv_apply_pauli_sum(const vqc_state* in, vqc_state* out, int nterms, const vqc_pauli_t* terms);
+/* out = sum_keys M_key in; energy2 = <in|out> (fused H*psi and expectation). */
+int vqc_sv_apply_terms(const vqc_state* in, vqc_state* out, int nterms, const vqc_term_t* terms,
+                       const double* pool, double* energy2);
+
+/* ---- density matrices ----------------------------------------------------
+ * rho <- U rho U^dagger: U on pos, conj(U) on pos+n (generic.jl:54-64). */
+int vqc_dm_apply_gate(vqc_state* rho, int k, const int32_t* pos, const double* U);
+/* Superoperator S (4^N square) on (pos..., pos.+n...) (apply_qmaps.jl:52-86);
+ * mode 0 applies S (apply!), mode 1 applies S^dagger (apply_dagger!). For
+ * apply_inverse! the caller passes inv(S) with mode 0. */
+int vqc_dm_apply_superop(vqc_state* rho, int N, const int32_t* pos, const double* S, int mode);
+/* tr(M rho) for a k-site M (expecs_dm.jl:7-27, expec_dm_serial.jl:2-129). */
+int vqc_dm_expec_dense(const vqc_state* rho, int k, const int32_t* pos, const double* M, double* out2);
+
+/* ---- circuits -----------------------------------------------------------
+ * apply!(circuit, state) for SV or DM (generic.jl:26-31, apply_qmaps.jl:8-16),
+ * fused into LDS-tiled passes. */
+int vqc_apply_circuit(vqc_state* s, int nops, const vqc_op_t* ops, const double* pool);
+
+#define VQC_FLAG_UNFUSED 1    /* one kernel per gate and per reduction (the plain path) */
+
+/* Energy E = sum_keys <psi|M|psi> with psi = circuit * psi0 and the adjoint
+ * gradient dRe(E)/dtheta for every active parameter. Replaces Zygote.gradient(
+ * c -> real(expectation(H, c * psi0)), circuit) through @adjoint *(circuit, x)
+ * (circuitdiff.jl:2-8, 23-32, 61-78) and @adjoint expectation(H, psi)
+ * (expecdiff.jl:30-68). grad has nparams entries, zeroed first. psi_out (may be
+ * NULL) receives circuit * psi0. */
+int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_t* ops, const double* pool,
+                       int nterms, const vqc_term_t* terms, const double* hpool,
+                       int nparams, double* energy2, double* grad, vqc_state* psi_out, int flags);
+/* Same for a density matrix: E = sum_keys tr(M rho), the adjoint seeded with
+ * H^dagger (expecdiff.jl:64-68), gates and channels reversed as in
+ * circuitdiff.jl:36-59 (channels undone with S^-1). */
+int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_t* ops, const double* pool,
+                       int nterms, const vqc_term_t* terms, const double* hpool,
+                       int nparams, double* energy2, double* grad, vqc_state* rho_out, int flags);
+
+/* ---- batched evaluation (no counterpart in the reference) ----------------
+ * Many independent state-vector instances of uneven size, each |0..0> pushed
+ * through its own circuit, its energy and its full adjoint gradient, run as
+ * one persistent kernel over a packed state arena and replayed as a HIP graph. */
+typedef struct vqc_instance {
+    int32_t nqubits;     /* 1..30 */
+    int32_t nops;
+    int64_t op_begin;    /* first op in the shared op array */
+    int32_t nterms;
+    int32_t nparams;
+    int64_t term_begin;  /* first term in the shared term array */
+    int64_t grad_begin;  /* first gradient slot in the shared gradient output */
+} vqc_instance_t;
+
+typedef struct vqc_batch vqc_batch;
+/* Plan (fusion, tiles, work queue), allocate the arena and capture the graph.
+ * ops[].mat/gen and terms[].mat index the pools given to vqc_batch_run. */
+int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc_op_t* ops, int64_t nops_total,
+                     const vqc_term_t* terms, int64_t nterms_total, int64_t pool_len,
+                     int64_t hpool_len, int flags, vqc_batch** out);
+/* Upload the matrices (new angles) and run; energies2 gets 2 doubles per
+ * instance, grads the concatenated gradients. */
+int vqc_batch_run(vqc_batch* b, const double* pool, const double* hpool, double* energies2, double* grads);
+/* Enqueue only (no wait); vqc_batch_wait copies results back. */
+int vqc_batch_launch(vqc_batch* b, const double* pool, const double* hpool);
+int vqc_batch_wait(vqc_batch* b, double* energies2, double* grads);
+int vqc_batch_arena_bytes(const vqc_batch* b, uint64_t* bytes);
+int vqc_batch_destroy(vqc_batch* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VQC_HIP_H */

@@ -1,0 +1,148 @@
+"""ctypes wrapper of liboracle.so, the CPU restatement of VQC.jl's kernels.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker. The product path never calls it.
+Every function here forwards to vqc_oracle.cpp, whose routines cite the
+reference file:line they restate. Pinning (see tests/test_oracle.py): dense
+gates, expectations, Pauli strings and DM maps against numpy brute force on the
+full 2^n operator; the adjoint gradient against exact parameter shift and
+central finite differences; layout against the reference's known-answer tests.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        _lib = ctypes.CDLL(LIB)
+        _lib.orc_get_threads.restype = ctypes.c_int
+    return _lib
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
+
+
+def _c(a):
+    assert a.dtype == np.complex128 and a.flags.c_contiguous
+    return a.ctypes.data_as(_D)
+
+
+def _ipos(pos):
+    return (ctypes.c_int * len(pos))(*[int(p) for p in pos])
+
+
+def _mat(M):
+    return np.ascontiguousarray(np.asarray(M, dtype=np.complex128).reshape(-1, order="F"))
+
+
+def _rc(rc):
+    if rc != 0:
+        raise ValueError("oracle rejected the arguments")
+
+
+def apply_dense(v, k, pos0, U):
+    """v (complex128, 2^nbits) updated in place; pos0 ascending 0-based."""
+    nb = int(np.log2(v.size))
+    m = _mat(U)
+    _rc(lib().orc_apply_dense(nb, _c(v), k, _ipos(pos0), _c(m)))
+    return v
+
+
+def apply_named(v, kind, pos0_given, phi=0.0):
+    nb = int(np.log2(v.size))
+    _rc(lib().orc_apply_named(nb, _c(v), int(kind), _ipos(pos0_given), ctypes.c_double(phi)))
+    return v
+
+
+def accum_dense(v, vout, k, pos0, M):
+    nb = int(np.log2(v.size))
+    m = _mat(M)
+    _rc(lib().orc_accum_dense(nb, _c(v), _c(vout), k, _ipos(pos0), _c(m)))
+    return vout
+
+
+def expec_dense(v, k, pos0, M, vc=None):
+    nb = int(np.log2(v.size))
+    m = _mat(M)
+    out = np.zeros(2)
+    _rc(lib().orc_expec_dense(nb, _c(v), _c(vc) if vc is not None else None, k, _ipos(pos0), _c(m),
+                              out.ctypes.data_as(_D)))
+    return complex(out[0], out[1])
+
+
+def dot(a, b):
+    out = np.zeros(2)
+    lib().orc_dot(int(np.log2(a.size)), _c(a), _c(b), out.ctypes.data_as(_D))
+    return complex(out[0], out[1])
+
+
+def expec_pauli(v, paulis, nterms, vc=None):
+    """Per-term <vc|P_t|v> and their sum, the reference's per-term route."""
+    nb = int(np.log2(v.size))
+    out = np.zeros(2 * max(nterms, 1))
+    s = np.zeros(2)
+    lib().orc_expec_pauli(nb, _c(v), _c(vc) if vc is not None else None, nterms, paulis,
+                          out.ctypes.data_as(_D), s.ctypes.data_as(_D))
+    return out[0:2 * nterms:2] + 1j * out[1:2 * nterms:2], complex(s[0], s[1])
+
+
+def apply_pauli_sum(v, paulis, nterms):
+    out = np.zeros_like(v)
+    lib().orc_apply_pauli_sum(int(np.log2(v.size)), _c(v), _c(out), nterms, paulis)
+    return out
+
+
+def dm_apply_gate(rho, n, k, pos0, U):
+    m = _mat(U)
+    _rc(lib().orc_dm_apply_gate(n, _c(rho), k, _ipos(pos0), _c(m)))
+    return rho
+
+
+def dm_apply_superop(rho, n, N, pos0, S, mode=0):
+    m = _mat(S)
+    _rc(lib().orc_dm_apply_superop(n, _c(rho), N, _ipos(pos0), _c(m), mode))
+    return rho
+
+
+def dm_expec(rho, n, k, pos0, M):
+    m = _mat(M)
+    out = np.zeros(2)
+    _rc(lib().orc_dm_expec(n, _c(rho), k, _ipos(pos0), _c(m), out.ctypes.data_as(_D)))
+    return complex(out[0], out[1])
+
+
+def apply_circuit(v, n, is_dm, ops, nops, pool):
+    _rc(lib().orc_apply_circuit(n, int(is_dm), _c(v), nops, ops, _c(pool)))
+    return v
+
+
+def energy_grad(state0, n, is_dm, ops, nops, pool, terms, nterms, hpool, nparams):
+    """(E, grad, final state) of the restated adjoint (circuitdiff.jl / expecdiff.jl)."""
+    e2 = np.zeros(2)
+    g = np.zeros(max(nparams, 1))
+    out = np.zeros_like(state0)
+    fn = lib().orc_dm_energy_grad if is_dm else lib().orc_sv_energy_grad
+    _rc(fn(n, _c(np.ascontiguousarray(state0)), nops, ops, _c(pool), nterms, terms, _c(hpool), nparams,
+           e2.ctypes.data_as(_D), g.ctypes.data_as(_D), _c(out)))
+    return complex(e2[0], e2[1]), g[:nparams], out

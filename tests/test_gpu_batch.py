@@ -1,0 +1,91 @@
+"""Batched persistent-kernel evaluation against the oracle, per instance.
+
+Tolerance: energies and gradients within 1e-10 (relative to max(1, |value|)).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from brute import rand_unitary
+import vqchip as vq
+from vqchip.batch import Batch
+from vqchip.circuit import Program
+from vqchip.operators import pack_terms
+from vqchip.workloads import hea_circuit, heisenberg, tfim
+
+pytestmark = pytest.mark.gpu
+
+
+def oracle_eval(c, H, n):
+    prog = Program(c, n=n)
+    terms, nt, hpool = pack_terms(H)
+    psi0 = np.zeros(1 << n, dtype=complex)
+    psi0[0] = 1
+    E, g, _ = oracle.energy_grad(psi0, n, 0, prog.op_array(), len(prog.ops), prog.pool(), terms, nt, hpool, prog.nparams)
+    return E, g
+
+
+def make_instances(seed, ns):
+    rng = np.random.default_rng(seed)
+    out = []
+    for n in ns:
+        depth = int(rng.integers(1, 4))
+        c = hea_circuit(n, depth, rng.uniform(0, 2 * np.pi, 3 * n * depth))
+        if n >= 5 and rng.random() < 0.5:
+            c.append(vq.QuantumGate((1, 3, n - 1, n), rand_unitary(4, rng)))
+        if n >= 6 and rng.random() < 0.3:
+            c.append(vq.QuantumGate((1, 2, 4, n - 1, n), rand_unitary(5, rng)))
+        if rng.random() < 0.3:
+            c.insert(1, vq.FSIMGate(1, n, 0.4, 0.9))
+        H = heisenberg(n, rng.standard_normal(n - 1)) if rng.random() < 0.5 else tfim(n, float(rng.uniform(0.5, 1.5)))
+        out.append((c, H, n))
+    return out
+
+
+def check(instances, E, grads):
+    for i, (c, H, n) in enumerate(instances):
+        Er, gr = oracle_eval(c, H, n)
+        assert abs(E[i] - Er) < 1e-10 * max(1, abs(Er)), (i, n)
+        assert grads[i].shape == gr.shape
+        if gr.size:
+            assert np.max(np.abs(grads[i] - gr)) < 1e-10 * max(1, np.max(np.abs(gr))), (i, n)
+
+
+def test_mixed_batch_matches_oracle():
+    ns = [3, 8, 12, 13, 5, 16, 9, 14, 11, 17, 4, 15, 10, 12, 6]
+    inst = make_instances(1, ns)
+    b = Batch(inst)
+    E, grads = b.run()
+    check(inst, E, grads)
+    assert b.arena_bytes() == sum(32 << n for n in ns if n > 12)
+
+
+def test_graph_replay_with_new_angles():
+    ns = [8, 14, 12, 15, 7]
+    rng = np.random.default_rng(2)
+    Hs = [heisenberg(n, rng.standard_normal(n - 1)) for n in ns]
+    inst = [(hea_circuit(n, 2, rng.uniform(0, 6.3, 6 * n)), H, n) for n, H in zip(ns, Hs)]
+    b = Batch(inst)
+    E1, g1 = b.run()
+    check(inst, E1, g1)
+    inst2 = [(hea_circuit(n, 2, rng.uniform(0, 6.3, 6 * n)), H, n) for n, H in zip(ns, Hs)]
+    b.update([c for c, _, _ in inst2])
+    for _ in range(3):  # every replay agrees with the oracle
+        E2, g2 = b.run()
+        check(inst2, E2, g2)
+
+
+def test_alone_equals_in_mix():
+    ns = [9, 13, 11, 15]
+    inst = make_instances(4, ns)
+    E_mix, g_mix = Batch(inst).run()
+    for i, one in enumerate(inst):
+        E1, g1 = Batch([one]).run()
+        assert abs(E1[0] - E_mix[i]) < 1e-12
+        assert np.max(np.abs(g1[0] - g_mix[i])) < 1e-12
+
+
+def test_instance_without_ops_or_terms():
+    inst = [(vq.QCircuit(), tfim(5), 5), (vq.QCircuit(), tfim(14), 14), (hea_circuit(6, 1, np.ones(18)), vq.QubitsOperator(), 6)]
+    E, grads = Batch(inst).run()
+    check(inst, E, grads)

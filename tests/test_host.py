@@ -1,0 +1,116 @@
+"""Host-side logic: gate matrices and generators, packing, the library's exported ABI."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from brute import full_op, rand_matrix
+from vqchip import (CNOTGate, CRxGate, FSIMGate, PHASEGate, QCircuit, QuantumGate, QubitsOperator, QubitsTerm, RxGate,
+                    RyGate, RzGate, CPHASEGate, CCPHASEGate, Depolarizing, AmplitudeDamping)
+from vqchip import _lib
+from vqchip.circuit import Program
+from vqchip.gates import reorder
+from vqchip.operators import pack_pauli, pack_terms
+from vqchip.workloads import config0, config1_circuit, config2_instances, config3, hea_circuit
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_reorder_matches_bit_permutation():
+    rng = np.random.default_rng(0)
+    M = rand_matrix(3, rng)
+    given = [5, 1, 3]  # matrix bit b <-> qubit given[b]
+    # apply on a 6-qubit space through the given order, compare with the sorted form
+    v = rng.standard_normal(64) + 1j * rng.standard_normal(64)
+    idx = np.arange(64)
+    sub = sum(((idx >> q) & 1) << b for b, q in enumerate(given))
+    rest = idx & ~sum(1 << q for q in given)
+    F = M[sub[:, None], sub[None, :]] * (rest[:, None] == rest[None, :])
+    assert np.allclose(full_op(6, sorted(given), reorder(M, given)) @ v, F @ v)
+
+
+def test_generators_are_udagger_du():
+    h = 1e-6
+    for make, nparam in [(lambda p: RxGate(1, *p), 1), (lambda p: RyGate(2, *p), 1), (lambda p: RzGate(1, *p), 1),
+                         (lambda p: PHASEGate(1, *p), 1), (lambda p: CRxGate(3, 1, *p), 1),
+                         (lambda p: CPHASEGate(1, 2, *p), 1), (lambda p: FSIMGate(2, 1, *p), 2),
+                         (lambda p: CCPHASEGate(3, 1, 2, *p), 1)]:
+        p0 = list(np.linspace(0.3, 1.1, nparam))
+        g = make(p0)
+        U = g.ordered_mat()
+        assert np.allclose(U.conj().T @ U, np.eye(U.shape[0]), atol=1e-14)
+        for j, G in enumerate(g.differentiate()):
+            pp, pm = list(p0), list(p0)
+            pp[j] += h
+            pm[j] -= h
+            dU = (make(pp).ordered_mat() - make(pm).ordered_mat()) / (2 * h)
+            assert np.allclose(U.conj().T @ dU, G, atol=1e-8), (g, j)
+
+
+def test_inactive_parameters_and_slots():
+    # test/parameter.jl: isparas=false parameters are not counted and get no slot
+    c = QCircuit([RxGate(1, 0.1), RxGate(1, 1.1, isparas=False), CNOTGate(1, 2), RyGate(2, 0.2)])
+    assert c.nparameters() == 2 and c.active_parameters() == [0.1, 0.2]
+    prog = Program(c, n=2)
+    assert prog.nparams == 2
+    assert [o.slot[0] for o in prog.ops] == [0, -1, -1, 1]
+    c.reset_parameters([0.5, 0.6])
+    assert c.active_parameters() == [0.5, 0.6] and c[1].params == (1.1,)
+
+
+def test_operator_merge_and_packing():
+    H = QubitsOperator([QubitsTerm({1: "Z", 2: "Z"}), QubitsTerm({1: "X"}, coeff=0.5),
+                        QubitsTerm({1: "Y", 2: "Y"}, coeff=2.0)])
+    data = H.data()
+    assert list(data) == [(1, 2), (1,)]
+    Z, Y = np.diag([1, -1]), np.array([[0, -1j], [1j, 0]])
+    assert np.allclose(data[(1, 2)], np.kron(Z, Z) + 2 * np.kron(Y, Y))
+    terms, nt, pool = pack_terms(H)
+    assert nt == 2 and terms[0].nq == 2 and list(terms[0].pos)[:2] == [0, 1]
+    # first site is the least significant bit: QubitsTerm(1=>"X", 2=>"Z") = kron(Z, X)
+    t = QubitsTerm({1: "X", 2: "Z"})
+    assert np.allclose(t.matrix(), np.kron(Z, np.array([[0, 1], [1, 0]])))
+    arr, n = pack_pauli(QubitsOperator([QubitsTerm({1: "X", 3: "Y", 4: "Z"}, coeff=0.5)]))
+    assert (arr[0].x, arr[0].z, arr[0].re) == (0b0101, 0b1100, 0.5)
+
+
+def test_superoperator_layout():
+    ch = Depolarizing(1, 0.2)
+    S = ch.ordered_supermat()
+    rho = np.array([[0.7, 0.1 - 0.2j], [0.1 + 0.2j, 0.3]])
+    out = (S @ rho.reshape(-1, order="F")).reshape(2, 2, order="F")
+    ref = sum(K @ rho @ K.conj().T for K in ch.kraus)
+    assert np.allclose(out, ref)
+    ad = AmplitudeDamping(1, 0.3)
+    assert np.allclose(sum(K.conj().T @ K for K in ad.kraus), np.eye(2))
+
+
+def test_workload_shapes():
+    c0, H0, th = config0()
+    assert len(c0) == 188 and len(th) == 144 and len(H0.data()) == 23
+    c1 = config1_circuit(layers=2)
+    assert len(c1) == 48 and sorted({len(g.positions) for g in c1}) == [1, 2, 3]
+    inst = config2_instances(count=4)
+    assert all(8 <= n <= 20 for _, _, n in inst)
+    c3, H3, th3 = config3(n=4, depth=2)
+    assert len(th3) == 24 and sum(1 for g in c3 if g.nparameters() == 0 and not hasattr(g, "kraus")) == 6
+
+
+def _header_functions():
+    text = open(os.path.join(ROOT, "include", "vqc_hip.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(?:int|const char\*)\s+(vqc_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib_path = _lib.LIB_PATH
+    if not os.path.exists(lib_path):
+        pytest.skip("libvqc_hip.so not built")
+    lib = ctypes.CDLL(lib_path)  # loads without a GPU; no compute call is made
+    names = _header_functions()
+    assert len(names) > 30
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(_lib.SIGNATURES) | {"vqc_last_error"} >= set(names) - set()

@@ -1,0 +1,549 @@
+// vqc_batch.hip -- many independent energy + adjoint-gradient evaluations in
+// one persistent kernel, replayed as a HIP graph.
+//
+// Planning (once per batch shape):
+//   * an instance whose state fits one tile (n <= t_small) becomes ONE work
+//     item: the whole forward / Hamiltonian / reverse evaluation in LDS;
+//   * a larger instance becomes a chain of phases (vqc_tile.h); each phase is
+//     split into tile chunks, one work item per chunk; the items of phase p of
+//     an instance may start only when every item of its phase p-1 is done.
+// Items are put in one queue in the order of a list-scheduling simulation
+// (longest remaining chain first, ready items only), so every item depends
+// only on items dequeued before it: with workgroups taking items in queue
+// order, the earliest unfinished item never waits, and the kernel drains.
+// Matrix variants (U^dagger, conj U, U^T, M + M^dagger) are computed on the
+// device from the caller's pools by a small kernel at the start of each run.
+//
+// Run = graph of: pool upload (pinned -> HBM), variant kernel, zero fill of
+// partials and counters, persistent kernel, row reduction, row download.
+#include <cstdlib>
+#include <queue>
+
+#include "vqc_tile.h"
+
+void vqc_zero(double2* v, uint64_t len, hipStream_t st);
+
+constexpr int kMaxChunks = 64;   // partial columns per row
+constexpr int kItemSmall = 0, kItemPhase = 1;
+
+struct BItem {
+    int32_t kind;
+    int32_t inst;
+    int32_t phase;       // kItemPhase: global phase index
+    int32_t chunk;       // column of the partial rows
+    int32_t need;        // items of this instance that must be done first
+    int32_t pad;
+    int64_t tile_begin;
+    int64_t tile_end;
+};
+
+struct BInst {
+    int64_t a_off;       // arena offsets (complex elements) of A and B; -1 for small
+    int64_t b_off;
+    int32_t nbits;
+    int32_t mop_begin;   // small: micro-op range and first Hamiltonian micro-op
+    int32_t mop_end;
+    int32_t ham_begin;
+    int32_t row_begin;   // first global partial row
+    int32_t nrows;
+};
+
+struct BatchArgs {
+    const BItem* items;
+    int nitems;
+    const BInst* inst;
+    const PhaseDesc* phases;
+    const MicroOp* mops;
+    const double2* pool;
+    double2* arena;
+    double2* parts;      // [row][kMaxChunks]
+    unsigned* counters;  // [0] queue head, [1] error flag, [2 + i] done items of instance i
+};
+
+struct VarEntry {
+    int64_t src, dst;
+    int32_t D, v;
+};
+
+__global__ void __launch_bounds__(256) k_variants(double2* pool, const VarEntry* tab, int n) {
+    for (int e = blockIdx.x; e < n; e += gridDim.x) {
+        const VarEntry t = tab[e];
+        const int D = t.D;
+        for (int i = threadIdx.x; i < D * D; i += blockDim.x) {
+            const int r = i % D, c = i / D;
+            const double2 m = pool[t.src + r + c * D];
+            const double2 mt = pool[t.src + c + r * D];
+            double2 o;
+            if (t.v == 4) o = make_double2(m.x + mt.x, m.y - mt.y);  // M + M^dagger
+            else {
+                o = (t.v & MF_TRANS) ? mt : m;
+                if (t.v & MF_CONJ) o.y = -o.y;
+            }
+            pool[t.dst + i] = o;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_zero_u32(unsigned* p, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
+}
+
+// Bounded wait: the queue order makes every wait finite; the bound only turns
+// a planning bug into a reported error instead of a hang.
+constexpr long long kSpinLimit = 1ll << 26;
+
+__global__ void __launch_bounds__(kTileThreads) k_batch(BatchArgs a) {
+    extern __shared__ double2 smem[];
+    __shared__ int s_item;
+    __shared__ int s_ok;
+    for (;;) {
+        if (threadIdx.x == 0) s_item = int(atomicAdd(&a.counters[0], 1u));
+        __syncthreads();
+        const int idx = s_item;
+        __syncthreads();
+        if (idx >= a.nitems) break;
+        const BItem it = a.items[idx];
+        if (threadIdx.x == 0) {
+            int ok = 1;
+            long long spins = 0;
+            while (__hip_atomic_load(&a.counters[2 + it.inst], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < unsigned(it.need)) {
+                __builtin_amdgcn_s_sleep(2);
+                if (++spins > kSpinLimit) {
+                    ok = 0;
+                    atomicOr(&a.counters[1], 1u);
+                    break;
+                }
+            }
+            s_ok = ok;
+        }
+        __syncthreads();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+        if (s_ok) {
+            const BInst in = a.inst[it.inst];
+            if (it.kind == kItemSmall) {
+                const uint32_t T = 1u << in.nbits;
+                small_eval_body(in.nbits, nullptr, nullptr, a.mops, in.mop_begin, in.mop_end, in.ham_begin, a.pool, smem,
+                                smem + T, smem + 2 * T, in.nrows, a.parts + size_t(in.row_begin) * kMaxChunks, kMaxChunks);
+            } else {
+                const PhaseDesc& p = a.phases[it.phase];
+                const uint32_t T = 1u << p.t;
+                double2* sa = smem;
+                double2* sb = p.kind == PH_FWD ? smem : smem + T;
+                double2* gacc = p.kind == PH_FWD ? smem + T : smem + 2 * T;
+                for (int i = threadIdx.x; i < p.nacc * kTileWaves; i += kTileThreads) gacc[i] = make_double2(0.0, 0.0);
+                __syncthreads();
+                double2* A = a.arena + in.a_off;
+                double2* B = a.arena + in.b_off;
+                for (int64_t tile = it.tile_begin; tile < it.tile_end; ++tile)
+                    phase_tile(p, uint64_t(tile), A, A, B, a.mops, a.pool, sa, sb, gacc);
+                for (int e = threadIdx.x; e < p.nacc; e += kTileThreads) {
+                    double2 s = make_double2(0.0, 0.0);
+                    for (int w = 0; w < kTileWaves; ++w) s = cadd(s, gacc[e * kTileWaves + w]);
+                    a.parts[size_t(p.row_begin + e) * kMaxChunks + it.chunk] = s;
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        __syncthreads();
+        if (threadIdx.x == 0) __hip_atomic_fetch_add(&a.counters[2 + it.inst], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+struct vqc_batch {
+    int ninst = 0;
+    std::vector<vqc_instance_t> inst;
+    std::vector<int> row_slot;        // global row -> local gradient slot (>= 0) or energy (< 0)
+    std::vector<int> row_inst;        // global row -> instance
+    int nrows = 0;
+    int nitems = 0;
+    int64_t pool_len = 0, hpool_len = 0, xlen = 0;
+    int64_t grad_total = 0;
+    size_t lds = 0;
+    unsigned grid = 0;
+    // device
+    void* d_items = nullptr;
+    void* d_inst = nullptr;
+    void* d_phases = nullptr;
+    void* d_mops = nullptr;
+    void* d_var = nullptr;
+    int nvar = 0;
+    double2* d_pool = nullptr;
+    double2* d_arena = nullptr;
+    double2* d_parts = nullptr;
+    double2* d_rows = nullptr;
+    unsigned* d_counters = nullptr;
+    uint64_t arena_bytes = 0;
+    // host pinned staging
+    double* h_pool = nullptr;
+    double* h_rows = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool launched = false;
+};
+
+static int batch_enqueue(vqc_batch* b, hipStream_t st) {
+    BatchArgs a{(const BItem*)b->d_items, b->nitems, (const BInst*)b->d_inst, (const PhaseDesc*)b->d_phases,
+                (const MicroOp*)b->d_mops, b->d_pool, b->d_arena, b->d_parts, b->d_counters};
+    VQC_HIP(hipMemcpyAsync(b->d_pool, b->h_pool, size_t(b->pool_len + b->hpool_len) * 16, hipMemcpyHostToDevice, st));
+    if (b->nvar) hipLaunchKernelGGL(k_variants, dim3(std::min(b->nvar, 4096)), dim3(256), 0, st, b->d_pool, (const VarEntry*)b->d_var, b->nvar);
+    vqc_zero(b->d_parts, uint64_t(std::max(b->nrows, 1)) * kMaxChunks, st);
+    hipLaunchKernelGGL(k_zero_u32, dim3(std::max(1, (b->ninst + 2 + 255) / 256)), dim3(256), 0, st, b->d_counters, b->ninst + 2);
+    hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kTileThreads), b->lds, st, a);
+    if (b->nrows) vqc_reduce_partials(b->d_parts, b->nrows, kMaxChunks, b->d_rows, st);
+    VQC_HIP(hipMemcpyAsync(b->h_rows, b->d_rows, size_t(std::max(b->nrows, 1)) * 16 + 16, hipMemcpyDeviceToHost, st));
+    VQC_HIP(hipGetLastError());
+    return VQC_OK;
+}
+
+static int batch_free(vqc_batch* b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    void* dev[] = {b->d_items, b->d_inst, b->d_phases, b->d_mops, b->d_var, b->d_pool, b->d_arena, b->d_parts, b->d_rows, b->d_counters};
+    for (void* p : dev)
+        if (p) (void)hipFree(p);
+    if (b->h_pool) (void)hipHostFree(b->h_pool);
+    if (b->h_rows) (void)hipHostFree(b->h_rows);
+    delete b;
+    return VQC_OK;
+}
+
+extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc_op_t* ops, int64_t nops_total,
+                                const vqc_term_t* terms, int64_t nterms_total, int64_t pool_len, int64_t hpool_len,
+                                int flags, vqc_batch** out) {
+    (void)flags;
+    if (!out || ninst < 1 || !inst || nops_total < 0 || nterms_total < 0 || pool_len < 0 || hpool_len < 0)
+        return vqc_fail(VQC_EINVAL, "vqc_batch_create: bad arguments");
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    const TileTuning tt = vqc_tile_tuning();
+    vqc_batch* b = new vqc_batch;
+    b->ninst = ninst;
+    b->inst.assign(inst, inst + ninst);
+    b->pool_len = pool_len;
+    b->hpool_len = hpool_len;
+    XPool xp;
+    xp.deferred = true;
+    xp.len = pool_len + hpool_len;
+    PlanBuilder pb;  // shared micro-op and phase arrays
+    pb.xp = &xp;
+    std::vector<BInst> binst(ninst);
+    struct Chain {
+        int first_phase, nphases;
+        bool small;
+        double cost;
+    };
+    std::vector<Chain> chains(ninst);
+    int64_t arena = 0;
+    auto fail = [&](int code, const char* msg, int i) {
+        batch_free(b);
+        return vqc_fail(code, "vqc_batch_create: instance %d: %s", i, msg);
+    };
+    for (int i = 0; i < ninst; ++i) {
+        const vqc_instance_t& I = inst[i];
+        const int n = I.nqubits;
+        if (n < 1 || n > 30 || I.nops < 0 || I.nterms < 0 || I.nparams < 0 || I.op_begin < 0 || I.term_begin < 0 ||
+            I.op_begin + I.nops > nops_total || I.term_begin + I.nterms > nterms_total || I.grad_begin < 0)
+            return fail(VQC_EINVAL, "bad ranges", i);
+        const vqc_op_t* iops = ops + I.op_begin;
+        const vqc_term_t* iterms = terms + I.term_begin;
+        for (int o = 0; o < I.nops; ++o) {
+            const vqc_op_t& op = iops[o];
+            if (op.kind != VQC_OP_UNITARY || vqc_check_positions(n, op.nq, op.pos, 5) != VQC_OK)
+                return fail(VQC_EINVAL, "bad op (batched instances are state vectors)", i);
+            const int64_t D2 = int64_t(1) << (2 * op.nq);
+            if (op.mat < 0 || op.mat + D2 > pool_len) return fail(VQC_EINVAL, "op matrix outside the pool", i);
+            for (int j = 0; j < op.nparams; ++j)
+                if (op.slot[j] >= I.nparams || (op.slot[j] >= 0 && (op.gen[j] < 0 || op.gen[j] + D2 > pool_len)))
+                    return fail(VQC_EINVAL, "bad parameter slot or generator", i);
+        }
+        for (int t = 0; t < I.nterms; ++t) {
+            if (vqc_check_positions(n, iterms[t].nq, iterms[t].pos, 5) != VQC_OK) return fail(VQC_EINVAL, "bad term", i);
+            if (iterms[t].mat < 0 || iterms[t].mat + (int64_t(1) << (2 * iterms[t].nq)) > hpool_len)
+                return fail(VQC_EINVAL, "term matrix outside the pool", i);
+        }
+        b->grad_total = std::max<int64_t>(b->grad_total, I.grad_begin + I.nparams);
+        BInst& bi = binst[i];
+        bi.nbits = n;
+        bi.row_begin = pb.nrows;
+        const size_t ph0 = pb.phases.size();
+        const int rows0 = pb.nrows;
+        const size_t slot0 = pb.row_slot.size();
+        bool small = false;
+        if (n <= tt.t_small) {
+            PlanBuilder sp;
+            sp.xp = &xp;
+            sp.nrows = 0;
+            vqc_plan_forward(sp, n, n, 0, I.nops, iops, n, 0);
+            const int hb = int(sp.mops.size());
+            vqc_plan_ham(sp, n, I.nterms, iterms, pool_len, n, 0);
+            vqc_plan_reverse(sp, n, n, 0, I.nops, iops, n, 0);
+            const size_t T = size_t(1) << n;
+            const size_t need = (2 * T + size_t(std::max(sp.nrows, 1)) * kTileWaves) * 16;
+            if (need <= 160 * 1024) {
+                small = true;
+                for (const PhaseDesc& p : sp.phases)
+                    for (int o = p.op_begin; o < p.op_end; ++o)
+                        for (int q = 0; q < 3; ++q) sp.mops[o].acc[q] += p.row_begin;
+                bi.mop_begin = int(pb.mops.size());
+                bi.ham_begin = bi.mop_begin + hb;
+                pb.mops.insert(pb.mops.end(), sp.mops.begin(), sp.mops.end());
+                bi.mop_end = int(pb.mops.size());
+                bi.nrows = sp.nrows;
+                for (int r : sp.row_slot) pb.row_slot.push_back(r);
+                pb.nrows += sp.nrows;
+                bi.a_off = bi.b_off = -1;
+                b->lds = std::max(b->lds, need);
+                chains[i] = {-1, 0, true, double(T) * double(bi.mop_end - bi.mop_begin)};
+            }
+        }
+        if (!small) {
+            // phases: forward (first reads |0..0>), Hamiltonian, reverse; rows are global
+            vqc_plan_forward(pb, n, n, 0, I.nops, iops, tt.t_fwd, tt.mlow);
+            if (pb.phases.size() == ph0) {  // no ops: one empty forward phase writes |0..0> into A
+                PhaseDesc p;
+                vqc_make_phase(p, PH_FWD, n, tt.t_fwd, {});
+                p.op_begin = p.op_end = int(pb.mops.size());
+                p.row_begin = pb.nrows;
+                pb.phases.push_back(p);
+            }
+            pb.phases[ph0].flags |= PF_INIT_ZERO;
+            vqc_plan_ham(pb, n, I.nterms, iterms, pool_len, tt.t_rev, tt.mlow);
+            vqc_plan_reverse(pb, n, n, 0, I.nops, iops, tt.t_rev, tt.mlow);
+            bi.nrows = pb.nrows - rows0;
+            bi.a_off = arena;
+            bi.b_off = arena + (int64_t(1) << n);
+            arena += int64_t(2) << n;
+            bi.mop_begin = bi.mop_end = bi.ham_begin = 0;
+            double cost = 0;
+            for (size_t p = ph0; p < pb.phases.size(); ++p) {
+                const PhaseDesc& ph = pb.phases[p];
+                const double nb = ph.kind == PH_FWD ? 2 : 4;
+                cost += double(int64_t(1) << n) * (nb + 0.25 * (ph.op_end - ph.op_begin));
+                b->lds = std::max(b->lds, ((ph.kind == PH_FWD ? 1 : 2) * (size_t(1) << ph.t) + size_t(std::max(ph.nacc, 1)) * kTileWaves) * 16);
+            }
+            chains[i] = {int(ph0), int(pb.phases.size() - ph0), false, cost};
+        }
+        for (size_t r = slot0; r < pb.row_slot.size(); ++r) b->row_inst.push_back(i);
+        (void)rows0;
+    }
+    b->row_slot = pb.row_slot;
+    b->nrows = pb.nrows;
+
+    // ---- items in list-scheduling order ---------------------------------
+    std::vector<BItem> items;
+    struct PhaseItems {
+        std::vector<BItem> v;
+        double cost_each;
+    };
+    const int workers = std::max(1, c->cu_count);
+    // per instance: list of phases, each a list of chunk items (need filled in order)
+    std::vector<std::vector<PhaseItems>> plist(ninst);
+    std::vector<double> remaining(ninst, 0.0);
+    for (int i = 0; i < ninst; ++i) {
+        if (chains[i].small) {
+            BItem it{kItemSmall, i, -1, 0, 0, 0, 0, 0};
+            plist[i].push_back({{it}, chains[i].cost});
+            remaining[i] = chains[i].cost;
+            continue;
+        }
+        int need = 0;
+        for (int q = 0; q < chains[i].nphases; ++q) {
+            const int pg = chains[i].first_phase + q;
+            const PhaseDesc& ph = pb.phases[pg];
+            const int64_t ntiles = int64_t(1) << (ph.nbits - ph.t);
+            const int nch = int(std::min<int64_t>(ntiles, kMaxChunks));
+            PhaseItems pi;
+            const double nbuf = ph.kind == PH_FWD ? 2 : 4;
+            const double pcost = double(int64_t(1) << ph.nbits) * (nbuf + 0.25 * (ph.op_end - ph.op_begin));
+            pi.cost_each = pcost / nch;
+            for (int ch = 0; ch < nch; ++ch) {
+                BItem it{kItemPhase, i, pg, ch, need, 0, ntiles * ch / nch, ntiles * (ch + 1) / nch};
+                pi.v.push_back(it);
+            }
+            need += nch;
+            remaining[i] += pcost;
+            plist[i].push_back(pi);
+        }
+    }
+    // simulation: workers free at times; an instance's next phase is ready when all chunks of the previous are done
+    std::priority_queue<double, std::vector<double>, std::greater<double>> free_at;
+    for (int w = 0; w < workers; ++w) free_at.push(0.0);
+    std::vector<size_t> ph_idx(ninst, 0), ch_idx(ninst, 0);
+    std::vector<double> ready(ninst, 0.0), ph_end(ninst, 0.0);
+    size_t total = 0;
+    for (int i = 0; i < ninst; ++i)
+        for (auto& p : plist[i]) total += p.v.size();
+    items.reserve(total);
+    while (items.size() < total) {
+        double t = free_at.top();
+        free_at.pop();
+        int best = -1;
+        double next_ready = 1e300;
+        for (int i = 0; i < ninst; ++i) {
+            if (ph_idx[i] >= plist[i].size()) continue;
+            if (ready[i] <= t) {
+                if (best < 0 || remaining[i] > remaining[best]) best = i;
+            } else {
+                next_ready = std::min(next_ready, ready[i]);
+            }
+        }
+        if (best < 0) {  // idle until the next phase becomes ready
+            free_at.push(next_ready);
+            continue;
+        }
+        PhaseItems& pi = plist[best][ph_idx[best]];
+        items.push_back(pi.v[ch_idx[best]]);
+        const double end = t + pi.cost_each;
+        free_at.push(end);
+        ph_end[best] = std::max(ph_end[best], end);
+        remaining[best] -= pi.cost_each;
+        if (++ch_idx[best] == pi.v.size()) {
+            ch_idx[best] = 0;
+            ++ph_idx[best];
+            ready[best] = ph_end[best];
+        }
+    }
+    // every item must depend only on items of its instance queued before it
+    {
+        std::vector<int> seen(ninst, 0);
+        for (const BItem& it : items) {
+            if (it.need > seen[it.inst]) {
+                batch_free(b);
+                return vqc_fail(VQC_EINVAL, "vqc_batch_create: internal queue order error");
+            }
+            ++seen[it.inst];
+        }
+    }
+    b->nitems = int(items.size());
+
+    // ---- device buffers -------------------------------------------------
+    b->xlen = xp.size();
+    b->nvar = int(xp.table.size());
+    b->arena_bytes = uint64_t(arena) * 16;
+    auto dalloc = [&](void** p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 64)); };
+    std::vector<VarEntry> vt;
+    for (const XPoolEntry& e : xp.table) vt.push_back({e.src, e.dst, e.D, e.v});
+    hipError_t e = hipSuccess;
+    if (e == hipSuccess) e = dalloc(&b->d_items, items.size() * sizeof(BItem));
+    if (e == hipSuccess) e = dalloc(&b->d_inst, binst.size() * sizeof(BInst));
+    if (e == hipSuccess) e = dalloc(&b->d_phases, pb.phases.size() * sizeof(PhaseDesc));
+    if (e == hipSuccess) e = dalloc(&b->d_mops, pb.mops.size() * sizeof(MicroOp));
+    if (e == hipSuccess) e = dalloc(&b->d_var, vt.size() * sizeof(VarEntry));
+    if (e == hipSuccess) e = dalloc((void**)&b->d_pool, size_t(b->xlen) * 16);
+    if (e == hipSuccess) e = dalloc((void**)&b->d_arena, size_t(arena) * 16);
+    if (e == hipSuccess) e = dalloc((void**)&b->d_parts, size_t(std::max(b->nrows, 1)) * kMaxChunks * 16);
+    if (e == hipSuccess) e = dalloc((void**)&b->d_rows, size_t(std::max(b->nrows, 1)) * 16 + 16);
+    if (e == hipSuccess) e = dalloc((void**)&b->d_counters, size_t(ninst + 2) * 4);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_pool, size_t(std::max<int64_t>(pool_len + hpool_len, 1)) * 16, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_rows, size_t(std::max(b->nrows, 1)) * 16 + 16, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        batch_free(b);
+        return vqc_fail(VQC_ENOMEM, "vqc_batch_create: device allocation failed (%s)", hipGetErrorString(e));
+    }
+    hipStream_t st = c->stream;
+    VQC_HIP(hipMemcpyAsync(b->d_items, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice, st));
+    VQC_HIP(hipMemcpyAsync(b->d_inst, binst.data(), binst.size() * sizeof(BInst), hipMemcpyHostToDevice, st));
+    if (!pb.phases.empty())
+        VQC_HIP(hipMemcpyAsync(b->d_phases, pb.phases.data(), pb.phases.size() * sizeof(PhaseDesc), hipMemcpyHostToDevice, st));
+    if (!pb.mops.empty())
+        VQC_HIP(hipMemcpyAsync(b->d_mops, pb.mops.data(), pb.mops.size() * sizeof(MicroOp), hipMemcpyHostToDevice, st));
+    if (!vt.empty()) VQC_HIP(hipMemcpyAsync(b->d_var, vt.data(), vt.size() * sizeof(VarEntry), hipMemcpyHostToDevice, st));
+    VQC_HIP(hipStreamSynchronize(st));
+
+    // persistent grid: as many workgroups as can be resident
+    b->lds = std::max<size_t>(b->lds, 16 * 1024);
+    VQC_HIP(hipFuncSetAttribute((const void*)k_batch, hipFuncAttributeMaxDynamicSharedMemorySize, int(b->lds)));
+    int per_cu = 0;
+    VQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_batch, kTileThreads, b->lds));
+    if (per_cu < 1) {
+        batch_free(b);
+        return vqc_fail(VQC_EUNSUPPORTED, "vqc_batch_create: %zu bytes of LDS per workgroup do not fit", b->lds);
+    }
+    b->grid = unsigned(std::min<int64_t>(int64_t(per_cu) * c->cu_count, std::max(b->nitems, 1)));
+
+    // capture the run as a graph (replayed by vqc_batch_run / vqc_batch_launch)
+    if (!getenv("VQC_NO_GRAPH")) {
+        hipError_t ce = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        int rc = ce == hipSuccess ? batch_enqueue(b, st) : VQC_EHIP;
+        hipGraph_t g = nullptr;
+        hipError_t ee = hipStreamEndCapture(st, &g);
+        if (ce == hipSuccess && rc == VQC_OK && ee == hipSuccess && g) {
+            if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) == hipSuccess) b->graph = g;
+            else {
+                (void)hipGraphDestroy(g);
+                b->exec = nullptr;
+            }
+        } else if (g) {
+            (void)hipGraphDestroy(g);
+        }
+        (void)hipGetLastError();
+    }
+    *out = b;
+    return VQC_OK;
+}
+
+extern "C" int vqc_batch_launch(vqc_batch* b, const double* pool, const double* hpool) {
+    if (!b) return vqc_fail(VQC_EINVAL, "null batch");
+    if ((b->pool_len && !pool) || (b->hpool_len && !hpool)) return vqc_fail(VQC_EINVAL, "null matrix pool");
+    if (b->launched) return vqc_fail(VQC_EINVAL, "vqc_batch_launch: previous launch not waited for");
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    if (b->pool_len) memcpy(b->h_pool, pool, size_t(b->pool_len) * 16);
+    if (b->hpool_len) memcpy(b->h_pool + 2 * b->pool_len, hpool, size_t(b->hpool_len) * 16);
+    if (b->exec) VQC_HIP(hipGraphLaunch(b->exec, c->stream));
+    else VQC_TRY(batch_enqueue(b, c->stream));
+    b->launched = true;
+    return VQC_OK;
+}
+
+extern "C" int vqc_batch_wait(vqc_batch* b, double* energies2, double* grads) {
+    if (!b) return vqc_fail(VQC_EINVAL, "null batch");
+    if (!b->launched) return vqc_fail(VQC_EINVAL, "vqc_batch_wait: nothing launched");
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    b->launched = false;
+    VQC_HIP(hipStreamSynchronize(c->stream));
+    unsigned flag = 0;
+    VQC_HIP(hipMemcpy(&flag, b->d_counters + 1, 4, hipMemcpyDeviceToHost));
+    if (flag) return vqc_fail(VQC_EHIP, "vqc_batch: a work item exceeded its dependency wait (planning error)");
+    if (energies2)
+        for (int i = 0; i < 2 * b->ninst; ++i) energies2[i] = 0;
+    if (grads)
+        for (int64_t i = 0; i < b->grad_total; ++i) grads[i] = 0;
+    for (int r = 0; r < b->nrows; ++r) {
+        const int i = b->row_inst[r];
+        const int s = b->row_slot[r];
+        if (s < 0) {
+            if (energies2) {
+                energies2[2 * i] += b->h_rows[2 * r];
+                energies2[2 * i + 1] += b->h_rows[2 * r + 1];
+            }
+        } else if (grads) {
+            grads[b->inst[i].grad_begin + s] += b->h_rows[2 * r];
+        }
+    }
+    return VQC_OK;
+}
+
+extern "C" int vqc_batch_run(vqc_batch* b, const double* pool, const double* hpool, double* energies2, double* grads) {
+    VQC_TRY(vqc_batch_launch(b, pool, hpool));
+    return vqc_batch_wait(b, energies2, grads);
+}
+
+extern "C" int vqc_batch_arena_bytes(const vqc_batch* b, uint64_t* bytes) {
+    if (!b || !bytes) return vqc_fail(VQC_EINVAL, "null argument");
+    *bytes = b->arena_bytes;
+    return VQC_OK;
+}
+
+extern "C" int vqc_batch_destroy(vqc_batch* b) {
+    if (!b) return VQC_OK;
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    VQC_HIP(hipStreamSynchronize(c->stream));
+    return batch_free(b);
+}

@@ -1,0 +1,872 @@
+// vqc_circuit.hip -- circuits, energies and adjoint gradients.
+//
+// Fused path: the circuit is cut into LDS tile phases (vqc_tile.h); the
+// forward pass, the Hamiltonian pass (adjoint seed and energy) and the reverse
+// adjoint sweep each cost one HBM round trip per phase instead of one per gate.
+// States of at most t_small bits run the whole evaluation in one workgroup
+// with both the state and the adjoint held in LDS.
+//
+// Plain path (VQC_FLAG_UNFUSED): one kernel per gate and per reduction, the
+// same sequence of sweeps as the reference's back_propagate loop
+// (circuitdiff.jl:23-32, 61-78); it is the per-circuit baseline the fused and
+// batched forms are measured against.
+#include <cstdlib>
+
+#include "vqc_tile.h"
+
+int vqc_dense_sweep(double2* v, int nbits, int k, const int32_t* pos, const double* U, hipStream_t st);
+int vqc_accum_sweep(const double2* in, double2* out, int nbits, int k, const int32_t* pos, const double2* Ud, hipStream_t st);
+int vqc_expec_sweep(const double2* ket, const double2* bra, int nbits, int k, const int32_t* pos, const double2* Ud,
+                    double2* parts, unsigned* grid_out, hipStream_t st);
+int vqc_dm_expec_sweep(const double2* rho, int n, int k, const int32_t* pos, const double2* Ud, double2* parts,
+                       unsigned* grid_out, hipStream_t st);
+void vqc_zero(double2* v, uint64_t len, hipStream_t st);
+
+static int env_int(const char* name, int def) {
+    const char* s = getenv(name);
+    return s && *s ? atoi(s) : def;
+}
+
+TileTuning vqc_tile_tuning() {
+    TileTuning t;
+    t.t_fwd = std::min(kMaxTileBits, std::max(6, env_int("VQC_TILE_FWD", t.t_fwd)));
+    t.t_rev = std::min(kMaxTileBits, std::max(6, env_int("VQC_TILE_REV", t.t_rev)));
+    t.t_small = std::min(kMaxTileBits, std::max(1, env_int("VQC_TILE_SMALL", t.t_small)));
+    t.mlow = std::min(8, std::max(0, env_int("VQC_TILE_MLOW", t.mlow)));
+    return t;
+}
+
+// ---------------------------------------------------------------------------
+// planning
+// ---------------------------------------------------------------------------
+void vqc_make_phase(PhaseDesc& p, int kind, int nbits, int t, const std::vector<int>& req) {
+    memset(&p, 0, sizeof p);
+    t = std::min(t, nbits);
+    std::vector<char> in(nbits, 0);
+    int cnt = 0;
+    for (int b : req)
+        if (!in[b]) {
+            in[b] = 1;
+            ++cnt;
+        }
+    for (int b = 0; b < nbits && cnt < t; ++b)
+        if (!in[b]) {
+            in[b] = 1;
+            ++cnt;
+        }
+    p.kind = kind;
+    p.nbits = nbits;
+    p.t = cnt;
+    int m = 0;
+    while (m < nbits && in[m]) ++m;
+    p.m = m;
+    p.nhb = 0;
+    for (int b = m; b < nbits; ++b)
+        if (in[b]) p.hb[p.nhb++] = b;
+    p.nob = 0;
+    for (int b = 0; b < nbits; ++b)
+        if (!in[b]) p.ob[p.nob++] = b;
+}
+
+static int local_pos(const PhaseDesc& p, int b) {
+    if (b < p.m) return b;
+    for (int i = 0; i < p.nhb; ++i)
+        if (p.hb[i] == b) return p.m + i;
+    return -1;
+}
+
+static std::vector<int> op_bits(const vqc_op_t& o, int n, int is_dm) {
+    std::vector<int> b(o.pos, o.pos + o.nq);
+    if (is_dm)
+        for (int j = 0; j < o.nq; ++j) b.push_back(o.pos[j] + n);
+    return b;
+}
+
+// can `bits` join `cur` in a tile of t bits that keeps mlow contiguous low bits?
+static bool fits(const std::vector<int>& cur, const std::vector<int>& bits, int nbits, int t, int mlow) {
+    if (nbits <= t) return true;
+    std::vector<int> u(cur);
+    for (int b : bits)
+        if (std::find(u.begin(), u.end(), b) == u.end()) u.push_back(b);
+    int high = 0;
+    for (int b : u)
+        if (b >= mlow) ++high;
+    return high <= t - std::min(mlow, t);
+}
+
+static void set_lpos(MicroOp& m, const PhaseDesc& p, const int* gbits, int kb) {
+    m.kb = kb;
+    for (int j = 0; j < 5; ++j) m.lpos[j] = 0;
+    for (int j = 0; j < kb; ++j) m.lpos[j] = local_pos(p, gbits[j]);
+}
+
+static MicroOp mop_init(int kind) {
+    MicroOp m;
+    memset(&m, 0, sizeof m);
+    m.kind = kind;
+    m.mat2 = -1;
+    for (int q = 0; q < 3; ++q) {
+        m.gen[q] = -1;
+        m.acc[q] = 0;
+    }
+    return m;
+}
+
+// Group consecutive items into phases; emit(phase, item) appends micro-ops.
+template <class BitsFn, class EmitFn>
+static void plan_groups(PlanBuilder& pb, int kind, int nbits, int nitems, int t, int mlow, int max_acc_per_item,
+                        BitsFn bits_of, EmitFn emit) {
+    int i = 0;
+    while (i < nitems) {
+        std::vector<int> cur;
+        int j = i, acc = 0;
+        while (j < nitems) {
+            std::vector<int> b = bits_of(j);
+            const int a = max_acc_per_item > 0 ? max_acc_per_item : 0;
+            if (j > i && (!fits(cur, b, nbits, t, mlow) || acc + a > kMaxAcc)) break;
+            for (int x : b)
+                if (std::find(cur.begin(), cur.end(), x) == cur.end()) cur.push_back(x);
+            acc += a;
+            ++j;
+        }
+        PhaseDesc p;
+        vqc_make_phase(p, kind, nbits, t, cur);
+        p.op_begin = int(pb.mops.size());
+        p.row_begin = pb.nrows;
+        p.nacc = 0;
+        for (int q = i; q < j; ++q) emit(p, q);
+        p.op_end = int(pb.mops.size());
+        pb.phases.push_back(p);
+        i = j;
+    }
+}
+
+int64_t XPool::variant(int64_t off, int D, int v) {
+    if (v == 0) return off;
+    for (auto& e : cache)
+        if (e.first.first == off && e.first.second == v) return e.second;
+    const int64_t out = size();
+    if (deferred) {
+        len += int64_t(D) * D;
+        table.push_back({off, out, D, v});
+        cache.push_back({{off, v}, out});
+        return out;
+    }
+    h.resize(h.size() + size_t(2) * D * D);
+    const double* M = h.data() + 2 * off;
+    double* R = h.data() + 2 * out;
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) {
+            const size_t dst = 2 * size_t(r + c * D);
+            const size_t src = (v & MF_TRANS) ? 2 * size_t(c + r * D) : dst;
+            double re = M[src], im = M[src + 1];
+            if (v == 4) {  // M + M^dagger
+                const size_t tr = 2 * size_t(c + r * D);
+                re = M[dst] + M[tr];
+                im = M[dst + 1] - M[tr + 1];
+            } else if (v & MF_CONJ) {
+                im = -im;
+            }
+            R[dst] = re;
+            R[dst + 1] = im;
+        }
+    cache.push_back({{off, v}, out});
+    return out;
+}
+
+int vqc_plan_forward(PlanBuilder& pb, int nbits, int n, int is_dm, int nops, const vqc_op_t* ops, int t, int mlow) {
+    XPool& xp = *pb.xp;
+    plan_groups(
+        pb, PH_FWD, nbits, nops, t, mlow, 0, [&](int j) { return op_bits(ops[j], n, is_dm); },
+        [&](PhaseDesc& p, int q) {
+            const vqc_op_t& o = ops[q];
+            if (!is_dm || o.kind == VQC_OP_CHANNEL) {
+                std::vector<int> b = op_bits(o, n, o.kind == VQC_OP_CHANNEL);
+                MicroOp m = mop_init(MK_APPLY);
+                set_lpos(m, p, b.data(), int(b.size()));
+                m.mat = o.mat;
+                pb.mops.push_back(m);
+            } else {
+                int cols[5];
+                for (int j = 0; j < o.nq; ++j) cols[j] = o.pos[j] + n;
+                MicroOp m = mop_init(MK_APPLY);
+                set_lpos(m, p, o.pos, o.nq);
+                m.mat = o.mat;
+                pb.mops.push_back(m);
+                m = mop_init(MK_APPLY);
+                set_lpos(m, p, cols, o.nq);
+                m.mat = xp.variant(o.mat, 1 << o.nq, MF_CONJ);
+                pb.mops.push_back(m);
+            }
+        });
+    return VQC_OK;
+}
+
+int vqc_plan_ham(PlanBuilder& pb, int nbits, int nterms, const vqc_term_t* terms, int64_t hoff, int t, int mlow) {
+    XPool& xp = *pb.xp;
+    const size_t first_phase = pb.phases.size();
+    plan_groups(
+        pb, PH_HAM, nbits, nterms, t, mlow, 1, [&](int j) { return std::vector<int>(terms[j].pos, terms[j].pos + terms[j].nq); },
+        [&](PhaseDesc& p, int q) {
+            MicroOp m = mop_init(MK_HAM);
+            set_lpos(m, p, terms[q].pos, terms[q].nq);
+            m.mat = hoff + terms[q].mat;
+            m.mat2 = xp.variant(m.mat, 1 << terms[q].nq, 4);
+            m.ng = 1;
+            m.acc[0] = p.nacc++;
+            pb.row_slot.push_back(-1 - q);
+            pb.nrows++;
+            pb.mops.push_back(m);
+        });
+    for (size_t i = first_phase; i < pb.phases.size(); ++i) pb.phases[i].ham_accum = i > first_phase;
+    return VQC_OK;
+}
+
+int vqc_plan_reverse(PlanBuilder& pb, int nbits, int n, int is_dm, int nops, const vqc_op_t* ops, int t, int mlow) {
+    XPool& xp = *pb.xp;
+    auto nact = [&](const vqc_op_t& o) {
+        int a = 0;
+        for (int q = 0; q < o.nparams; ++q) a += o.slot[q] >= 0;
+        return a;
+    };
+    int maxa = 0;
+    for (int i = 0; i < nops; ++i) maxa = std::max(maxa, nact(ops[i]));
+    // items are the ops in reverse order
+    plan_groups(
+        pb, PH_REV, nbits, nops, t, mlow, std::max(maxa, 1), [&](int j) { return op_bits(ops[nops - 1 - j], n, is_dm); },
+        [&](PhaseDesc& p, int q) {
+            const vqc_op_t& o = ops[nops - 1 - q];
+            const int D = 1 << o.nq;
+            if (is_dm && o.kind == VQC_OP_CHANNEL) {
+                std::vector<int> b = op_bits(o, n, 1);
+                MicroOp m = mop_init(MK_APPLY2X);
+                set_lpos(m, p, b.data(), int(b.size()));
+                m.mat = o.aux;                                           // state: S^-1
+                m.mat2 = xp.variant(o.mat, D * D, MF_CONJ | MF_TRANS);   // adjoint: S^dagger
+                pb.mops.push_back(m);
+                return;
+            }
+            int accs[3], na = 0;
+            int64_t gens[3];
+            for (int j = 0; j < o.nparams; ++j) {
+                if (o.slot[j] < 0) continue;
+                accs[na] = p.nacc++;
+                gens[na] = o.gen[j];
+                pb.row_slot.push_back(o.slot[j]);
+                pb.nrows++;
+                ++na;
+            }
+            auto uncompute = [&](const int* bits, int64_t mat, bool with_grads) {
+                if (o.nq <= 4) {
+                    MicroOp m = mop_init(MK_APPLY2);
+                    set_lpos(m, p, bits, o.nq);
+                    m.mat = m.mat2 = mat;
+                    if (with_grads) {
+                        m.ng = na;
+                        for (int j = 0; j < na; ++j) {
+                            m.gen[j] = gens[j];
+                            m.acc[j] = accs[j];
+                        }
+                    }
+                    pb.mops.push_back(m);
+                    return;
+                }
+                for (int buf = 0; buf < 2; ++buf) {
+                    MicroOp m = mop_init(MK_APPLY);
+                    set_lpos(m, p, bits, o.nq);
+                    m.mat = mat;
+                    m.buf = buf;
+                    pb.mops.push_back(m);
+                }
+                if (with_grads)
+                    for (int j = 0; j < na; ++j) {
+                        MicroOp g = mop_init(MK_GRAD);
+                        set_lpos(g, p, bits, o.nq);
+                        g.ng = 1;
+                        g.gen[0] = gens[j];
+                        g.acc[0] = accs[j];
+                        pb.mops.push_back(g);
+                    }
+            };
+            const int64_t udag = xp.variant(o.mat, D, MF_CONJ | MF_TRANS);
+            if (!is_dm) {
+                uncompute(o.pos, udag, true);
+                return;
+            }
+            int cols[5];
+            for (int j = 0; j < o.nq; ++j) cols[j] = o.pos[j] + n;
+            uncompute(o.pos, udag, false);                             // U^dagger on rows
+            uncompute(cols, xp.variant(o.mat, D, MF_TRANS), false);    // conj(U^dagger) = U^T on columns
+            for (int j = 0; j < na; ++j)                               // <D|G_row y> + <D|conj(G)_col y>
+                for (int side = 0; side < 2; ++side) {
+                    MicroOp g = mop_init(MK_GRAD);
+                    set_lpos(g, p, side ? cols : o.pos, o.nq);
+                    g.ng = 1;
+                    g.gen[0] = side ? xp.variant(gens[j], D, MF_CONJ) : gens[j];
+                    g.acc[0] = accs[j];
+                    pb.mops.push_back(g);
+                }
+        });
+    return VQC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+struct PhaseBufs {
+    const double2* srcA;
+    double2* A;
+    double2* B;
+    const MicroOp* ops;
+    const double2* pool;
+    double2* parts;   // [row][stride]
+    int stride;
+};
+
+__global__ void __launch_bounds__(kTileThreads) k_phase(PhaseDesc p, PhaseBufs b) {
+    extern __shared__ double2 smem[];
+    const uint32_t T = 1u << p.t;
+    double2* sa = smem;
+    double2* sb = p.kind == PH_FWD ? smem : smem + T;
+    double2* gacc = (p.kind == PH_FWD ? smem + T : smem + 2 * T);
+    for (int i = threadIdx.x; i < p.nacc * kTileWaves; i += kTileThreads) gacc[i] = make_double2(0.0, 0.0);
+    __syncthreads();
+    const uint64_t ntiles = uint64_t(1) << (p.nbits - p.t);
+    for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
+        phase_tile(p, tile, b.srcA, b.A, b.B, b.ops, b.pool, sa, sb, gacc);
+    for (int e = threadIdx.x; e < p.nacc; e += kTileThreads) {
+        double2 s = make_double2(0.0, 0.0);
+        for (int w = 0; w < kTileWaves; ++w) s = cadd(s, gacc[e * kTileWaves + w]);
+        b.parts[size_t(p.row_begin + e) * b.stride + blockIdx.x] = s;
+    }
+}
+
+// Whole evaluation of a state that fits one tile: forward, Hamiltonian, reverse.
+// Accumulator index of every micro-op is its global partial row.
+struct SmallArgs {
+    int nbits;
+    int nops;           // micro-ops
+    int ham_begin;      // first MK_HAM micro-op (B is zeroed before it)
+    int nrows;
+    const double2* src; // psi0
+    double2* A;         // final state (may be null)
+    const MicroOp* ops;
+    const double2* pool;
+    double2* rows;      // [nrows] row sums (one column)
+};
+
+__global__ void __launch_bounds__(kTileThreads) k_small_eval(SmallArgs a) {
+    extern __shared__ double2 smem[];
+    const uint32_t T = 1u << a.nbits;
+    small_eval_body(a.nbits, a.src, a.A, a.ops, 0, a.nops, a.ham_begin, a.pool, smem, smem + T, smem + 2 * T, a.nrows,
+                    a.rows, 1);
+}
+
+// ---------------------------------------------------------------------------
+// host drivers
+// ---------------------------------------------------------------------------
+struct Work {
+    void* buf = nullptr;
+    size_t cap = 0;
+};
+static Work g_wA, g_wB, g_wparts, g_wpool, g_wmops, g_wrows;
+
+static int ensure(Work& w, size_t bytes) {
+    if (bytes <= w.cap) return VQC_OK;
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    if (w.buf) {
+        VQC_HIP(hipStreamSynchronize(c->stream));
+        VQC_HIP(hipFree(w.buf));
+        w.buf = nullptr;
+        w.cap = 0;
+    }
+    VQC_HIP(hipMalloc(&w.buf, bytes));
+    w.cap = bytes;
+    return VQC_OK;
+}
+
+static int check_program(int nbits_q, int is_dm, int nops, const vqc_op_t* ops, int64_t* pool_len, int nparams) {
+    if (nops < 0 || (nops && !ops)) return vqc_fail(VQC_EINVAL, "bad op array");
+    int64_t len = 0;
+    for (int i = 0; i < nops; ++i) {
+        const vqc_op_t& o = ops[i];
+        if (o.kind != VQC_OP_UNITARY && o.kind != VQC_OP_CHANNEL) return vqc_fail(VQC_EINVAL, "op %d: unknown kind %d", i, o.kind);
+        if (o.kind == VQC_OP_CHANNEL && !is_dm) return vqc_fail(VQC_EINVAL, "op %d: channels act on density matrices only", i);
+        VQC_TRY(vqc_check_positions(nbits_q, o.nq, o.pos, o.kind == VQC_OP_CHANNEL ? 2 : 5));
+        if (o.nparams < 0 || o.nparams > 3) return vqc_fail(VQC_EINVAL, "op %d: %d parameters", i, o.nparams);
+        const int64_t D2 = o.kind == VQC_OP_CHANNEL ? (int64_t(1) << (4 * o.nq)) : (int64_t(1) << (2 * o.nq));
+        if (o.mat < 0) return vqc_fail(VQC_EINVAL, "op %d: negative matrix offset", i);
+        len = std::max(len, o.mat + D2);
+        if (o.kind == VQC_OP_CHANNEL) {
+            if (o.aux < 0) return vqc_fail(VQC_EINVAL, "op %d: channel without an inverse superoperator", i);
+            len = std::max(len, o.aux + D2);
+        }
+        for (int j = 0; j < o.nparams; ++j) {
+            if (o.slot[j] >= nparams) return vqc_fail(VQC_EINVAL, "op %d: gradient slot %d >= nparams %d", i, o.slot[j], nparams);
+            if (o.slot[j] >= 0) {
+                if (o.gen[j] < 0) return vqc_fail(VQC_EINVAL, "op %d: active parameter without generator", i);
+                len = std::max(len, o.gen[j] + D2);
+            }
+        }
+    }
+    *pool_len = len;
+    return VQC_OK;
+}
+
+static int check_terms(int nq, int nterms, const vqc_term_t* terms, int64_t* len) {
+    if (nterms < 0 || (nterms && !terms)) return vqc_fail(VQC_EINVAL, "bad term array");
+    int64_t l = 0;
+    for (int t = 0; t < nterms; ++t) {
+        VQC_TRY(vqc_check_positions(nq, terms[t].nq, terms[t].pos, 5));
+        if (terms[t].mat < 0) return vqc_fail(VQC_EINVAL, "term %d: negative matrix offset", t);
+        l = std::max(l, terms[t].mat + (int64_t(1) << (2 * terms[t].nq)));
+    }
+    *len = l;
+    return VQC_OK;
+}
+
+static size_t phase_lds(const PhaseDesc& p) {
+    const size_t T = size_t(1) << p.t;
+    return (p.kind == PH_FWD ? T : 2 * T) * 16 + size_t(std::max(p.nacc, 1)) * kTileWaves * 16;
+}
+
+static int launch_phase(const PhaseDesc& p, const PhaseBufs& b, unsigned grid, hipStream_t st) {
+    const size_t lds = phase_lds(p);
+    static bool attr = false;
+    if (!attr) {
+        VQC_HIP(hipFuncSetAttribute((const void*)k_phase, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        VQC_HIP(hipFuncSetAttribute((const void*)k_small_eval, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_phase, dim3(grid), dim3(kTileThreads), lds, st, p, b);
+    VQC_HIP(hipGetLastError());
+    return VQC_OK;
+}
+
+static unsigned phase_grid(const PhaseDesc& p, unsigned max_acc_grid) {
+    const uint64_t ntiles = uint64_t(1) << (p.nbits - p.t);
+    if (p.nacc == 0) return unsigned(std::min<uint64_t>(ntiles, 1u << 30));
+    return unsigned(std::min<uint64_t>(ntiles, max_acc_grid));
+}
+
+// Run planned FWD phases from src into A.
+static int run_forward(const PlanBuilder& pb, size_t ph_begin, size_t ph_end, const double2* src, double2* A,
+                       const MicroOp* dmops, const double2* dpool, hipStream_t st) {
+    for (size_t i = ph_begin; i < ph_end; ++i) {
+        PhaseBufs b{i == ph_begin ? src : A, A, nullptr, dmops, dpool, nullptr, 0};
+        VQC_TRY(launch_phase(pb.phases[i], b, phase_grid(pb.phases[i], 1), st));
+    }
+    return VQC_OK;
+}
+
+static int upload_program(const PlanBuilder& pb, hipStream_t st) {
+    const std::vector<double>& h = pb.xp->h;
+    VQC_TRY(ensure(g_wpool, h.size() * 8 + 16));
+    VQC_TRY(ensure(g_wmops, std::max<size_t>(pb.mops.size(), 1) * sizeof(MicroOp)));
+    if (!h.empty()) VQC_HIP(hipMemcpyAsync(g_wpool.buf, h.data(), h.size() * 8, hipMemcpyHostToDevice, st));
+    if (!pb.mops.empty())
+        VQC_HIP(hipMemcpyAsync(g_wmops.buf, pb.mops.data(), pb.mops.size() * sizeof(MicroOp), hipMemcpyHostToDevice, st));
+    return VQC_OK;
+}
+
+extern "C" int vqc_apply_circuit(vqc_state* s, int nops, const vqc_op_t* ops, const double* pool) {
+    VQC_TRY(vqc_check_state(s, "vqc_apply_circuit"));
+    int64_t plen = 0;
+    VQC_TRY(check_program(s->nqubits, s->is_dm, nops, ops, &plen, 1 << 30));
+    if (nops == 0) return VQC_OK;
+    if (!pool) return vqc_fail(VQC_EINVAL, "null matrix pool");
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    const TileTuning tt = vqc_tile_tuning();
+    XPool xp;
+    xp.h.assign(pool, pool + 2 * plen);
+    PlanBuilder pb;
+    pb.xp = &xp;
+    VQC_TRY(vqc_plan_forward(pb, s->nbits, s->nqubits, s->is_dm, nops, ops, tt.t_fwd, tt.mlow));
+    VQC_TRY(upload_program(pb, c->stream));
+    return run_forward(pb, 0, pb.phases.size(), s->d, s->d, (const MicroOp*)g_wmops.buf, (const double2*)g_wpool.buf, c->stream);
+}
+
+// Sum rows into energy and gradient (host, fixed order).
+static void finish_rows(const std::vector<double>& rows, const std::vector<int>& row_slot, int nparams, double* energy2,
+                        double* grad) {
+    double er = 0, ei = 0;
+    for (int p = 0; p < nparams; ++p) grad[p] = 0;
+    for (size_t r = 0; r < row_slot.size(); ++r) {
+        if (row_slot[r] < 0) {
+            er += rows[2 * r];
+            ei += rows[2 * r + 1];
+        } else {
+            grad[row_slot[r]] += rows[2 * r];
+        }
+    }
+    if (energy2) {
+        energy2[0] = er;
+        energy2[1] = ei;
+    }
+}
+
+// --------------------------- plain state-vector path ------------------------
+static void dagger_host(const double* U, int D, double* out) {
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) {
+            out[2 * (r + c * D)] = U[2 * (c + r * D)];
+            out[2 * (r + c * D) + 1] = -U[2 * (c + r * D) + 1];
+        }
+}
+
+static int sv_energy_grad_plain(const vqc_state* psi0, int nops, const vqc_op_t* ops, const double* pool, int64_t plen, int nterms,
+                                const vqc_term_t* terms, const double* hpool, int nparams, double* energy2, double* grad,
+                                double2* A, double2* B, hipStream_t st) {
+    const int nb = psi0->nbits;
+    const uint64_t L = psi0->len();
+    // device pool: [ops pool | for each term M, M + M^dagger]
+    std::vector<double> dp(pool, pool + 2 * plen);
+    std::vector<int64_t> toff(nterms), tdoff(nterms);
+    for (int t = 0; t < nterms; ++t) {
+        const int D = 1 << terms[t].nq;
+        const double* M = hpool + 2 * terms[t].mat;
+        toff[t] = int64_t(dp.size() / 2);
+        dp.insert(dp.end(), M, M + 2 * D * D);
+        tdoff[t] = int64_t(dp.size() / 2);
+        std::vector<double> md(2 * D * D);
+        dagger_host(M, D, md.data());
+        for (int i = 0; i < 2 * D * D; ++i) md[i] += M[i];
+        dp.insert(dp.end(), md.begin(), md.end());
+    }
+    VQC_TRY(ensure(g_wpool, dp.size() * 8));
+    VQC_HIP(hipMemcpyAsync(g_wpool.buf, dp.data(), dp.size() * 8, hipMemcpyHostToDevice, st));
+    const double2* dpool = (const double2*)g_wpool.buf;
+    // rows: energy per term, then one per active parameter
+    std::vector<int> row_slot;
+    for (int t = 0; t < nterms; ++t) row_slot.push_back(-1 - t);
+    for (int i = nops - 1; i >= 0; --i)
+        for (int j = 0; j < ops[i].nparams; ++j)
+            if (ops[i].slot[j] >= 0) row_slot.push_back(ops[i].slot[j]);
+    const int nrows = int(row_slot.size());
+    const int stride = 1024;
+    VQC_TRY(ensure(g_wparts, size_t(std::max(nrows, 1)) * (stride + 1) * 16));
+    double2* parts = (double2*)g_wparts.buf;
+    double2* rowsum = parts + size_t(std::max(nrows, 1)) * stride;
+    vqc_zero(parts, size_t(std::max(nrows, 1)) * stride, st);
+    // forward: y = circuit * psi0 (generic.jl:26-33)
+    VQC_HIP(hipMemcpyAsync(A, psi0->d, L * 16, hipMemcpyDeviceToDevice, st));
+    for (int i = 0; i < nops; ++i) VQC_TRY(vqc_dense_sweep(A, nb, ops[i].nq, ops[i].pos, pool + 2 * ops[i].mat, st));
+    // energy per key and seed Delta = (H + H^dagger) y
+    int row = 0;
+    unsigned grid;
+    for (int t = 0; t < nterms; ++t, ++row)
+        VQC_TRY(vqc_expec_sweep(A, nullptr, nb, terms[t].nq, terms[t].pos, dpool + toff[t], parts + size_t(row) * stride, &grid, st));
+    vqc_zero(B, L, st);
+    for (int t = 0; t < nterms; ++t) VQC_TRY(vqc_accum_sweep(A, B, nb, terms[t].nq, terms[t].pos, dpool + tdoff[t], st));
+    // reverse sweep (circuitdiff.jl:23-32)
+    std::vector<double> ud;
+    for (int i = nops - 1; i >= 0; --i) {
+        const vqc_op_t& o = ops[i];
+        const int D = 1 << o.nq;
+        ud.resize(2 * D * D);
+        dagger_host(pool + 2 * o.mat, D, ud.data());
+        VQC_TRY(vqc_dense_sweep(B, nb, o.nq, o.pos, ud.data(), st));
+        VQC_TRY(vqc_dense_sweep(A, nb, o.nq, o.pos, ud.data(), st));
+        for (int j = 0; j < o.nparams; ++j) {
+            if (o.slot[j] < 0) continue;
+            VQC_TRY(vqc_expec_sweep(A, B, nb, o.nq, o.pos, dpool + o.gen[j], parts + size_t(row) * stride, &grid, st));
+            ++row;
+        }
+    }
+    if (nrows) vqc_reduce_partials(parts, nrows, stride, rowsum, st);
+    VQC_HIP(hipGetLastError());
+    std::vector<double> rows(2 * size_t(std::max(nrows, 1)));
+    if (nrows) VQC_TRY(vqc_fetch(rowsum, nrows, rows.data()));
+    else VQC_HIP(hipStreamSynchronize(st));
+    finish_rows(rows, row_slot, nparams, energy2, grad);
+    return VQC_OK;
+}
+
+// --------------------------- fused state-vector path ------------------------
+static int max_small_rows(int nbits) {
+    const size_t T = size_t(1) << nbits;
+    const size_t avail = 160 * 1024 - 2 * T * 16;
+    return int(avail / (kTileWaves * 16));
+}
+
+extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_t* ops, const double* pool, int nterms,
+                                  const vqc_term_t* terms, const double* hpool, int nparams, double* energy2, double* grad,
+                                  vqc_state* psi_out, int flags) {
+    VQC_TRY(vqc_check_state(psi0, "vqc_sv_energy_grad"));
+    if (psi0->is_dm) return vqc_fail(VQC_EINVAL, "vqc_sv_energy_grad: state is a density matrix");
+    if (nparams < 0 || (nparams && !grad)) return vqc_fail(VQC_EINVAL, "bad gradient output");
+    if (psi_out && (psi_out->nbits != psi0->nbits || psi_out->is_dm)) return vqc_fail(VQC_EINVAL, "psi_out has the wrong shape");
+    int64_t plen = 0, hlen = 0;
+    VQC_TRY(check_program(psi0->nbits, 0, nops, ops, &plen, nparams));
+    VQC_TRY(check_terms(psi0->nbits, nterms, terms, &hlen));
+    if ((plen && !pool) || (hlen && !hpool)) return vqc_fail(VQC_EINVAL, "null matrix pool");
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    hipStream_t st = c->stream;
+    const int nb = psi0->nbits;
+    const uint64_t L = psi0->len();
+    VQC_TRY(ensure(g_wB, L * 16));
+    double2* A;
+    if (psi_out && psi_out->d != psi0->d) A = psi_out->d;
+    else {
+        VQC_TRY(ensure(g_wA, L * 16));
+        A = (double2*)g_wA.buf;
+    }
+    double2* B = (double2*)g_wB.buf;
+    if (flags & VQC_FLAG_UNFUSED) {
+        VQC_TRY(sv_energy_grad_plain(psi0, nops, ops, pool, plen, nterms, terms, hpool, nparams, energy2, grad, A, B, st));
+    } else {
+        const TileTuning tt = vqc_tile_tuning();
+        XPool xp;
+        xp.h.assign(pool, pool + 2 * plen);
+        xp.h.insert(xp.h.end(), hpool, hpool + 2 * hlen);
+        PlanBuilder pb;
+        pb.xp = &xp;
+        const bool small = nb <= tt.t_small;
+        if (small) {
+            VQC_TRY(vqc_plan_forward(pb, nb, nb, 0, nops, ops, nb, 0));
+            const int ham_begin = int(pb.mops.size());
+            VQC_TRY(vqc_plan_ham(pb, nb, nterms, terms, plen, nb, 0));
+            VQC_TRY(vqc_plan_reverse(pb, nb, nb, 0, nops, ops, nb, 0));
+            if (pb.nrows <= max_small_rows(nb)) {
+                // accumulator index := global row
+                for (const PhaseDesc& p : pb.phases)
+                    for (int o = p.op_begin; o < p.op_end; ++o)
+                        for (int q = 0; q < 3; ++q) pb.mops[o].acc[q] += p.row_begin;
+                VQC_TRY(upload_program(pb, st));
+                VQC_TRY(ensure(g_wrows, size_t(std::max(pb.nrows, 1)) * 16));
+                SmallArgs a{nb, int(pb.mops.size()), ham_begin, pb.nrows, psi0->d, A, (const MicroOp*)g_wmops.buf,
+                            (const double2*)g_wpool.buf, (double2*)g_wrows.buf};
+                static bool attr = false;
+                if (!attr) {
+                    VQC_HIP(hipFuncSetAttribute((const void*)k_small_eval, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+                    attr = true;
+                }
+                const size_t lds = (2 * L + size_t(std::max(pb.nrows, 1)) * kTileWaves) * 16;
+                hipLaunchKernelGGL(k_small_eval, dim3(1), dim3(kTileThreads), lds, st, a);
+                VQC_HIP(hipGetLastError());
+                std::vector<double> rows(2 * size_t(std::max(pb.nrows, 1)));
+                if (pb.nrows) VQC_TRY(vqc_fetch((double2*)g_wrows.buf, pb.nrows, rows.data()));
+                else VQC_HIP(hipStreamSynchronize(st));
+                finish_rows(rows, pb.row_slot, nparams, energy2, grad);
+                return VQC_OK;
+            }
+            pb = PlanBuilder();
+            pb.xp = &xp;
+        }
+        {
+            VQC_TRY(vqc_plan_forward(pb, nb, nb, 0, nops, ops, tt.t_fwd, tt.mlow));
+            const size_t nfwd = pb.phases.size();
+            VQC_TRY(vqc_plan_ham(pb, nb, nterms, terms, plen, tt.t_rev, tt.mlow));
+            const size_t nham_end = pb.phases.size();
+            VQC_TRY(vqc_plan_reverse(pb, nb, nb, 0, nops, ops, tt.t_rev, tt.mlow));
+            VQC_TRY(upload_program(pb, st));
+            const MicroOp* dm = (const MicroOp*)g_wmops.buf;
+            const double2* dp = (const double2*)g_wpool.buf;
+            const int stride = 1024;
+            const int nrows = std::max(pb.nrows, 1);
+            VQC_TRY(ensure(g_wparts, size_t(nrows) * (stride + 1) * 16));
+            double2* parts = (double2*)g_wparts.buf;
+            double2* rowsum = parts + size_t(nrows) * stride;
+            vqc_zero(parts, size_t(nrows) * stride, st);
+            if (nfwd == 0) VQC_HIP(hipMemcpyAsync(A, psi0->d, L * 16, hipMemcpyDeviceToDevice, st));
+            else VQC_TRY(run_forward(pb, 0, nfwd, psi0->d, A, dm, dp, st));
+            if (nham_end == nfwd) vqc_zero(B, L, st);
+            for (size_t i = nfwd; i < pb.phases.size(); ++i) {
+                PhaseBufs b{A, A, B, dm, dp, parts, stride};
+                VQC_TRY(launch_phase(pb.phases[i], b, phase_grid(pb.phases[i], stride), st));
+            }
+            if (pb.nrows) vqc_reduce_partials(parts, pb.nrows, stride, rowsum, st);
+            VQC_HIP(hipGetLastError());
+            std::vector<double> rows(2 * size_t(nrows));
+            if (pb.nrows) VQC_TRY(vqc_fetch(rowsum, pb.nrows, rows.data()));
+            else VQC_HIP(hipStreamSynchronize(st));
+            finish_rows(rows, pb.row_slot, nparams, energy2, grad);
+        }
+    }
+    return VQC_OK;
+}
+
+// ------------------------------- density matrices ---------------------------
+struct GPosDyn {
+    int p[5];
+    int k;
+};
+
+// Delta = H^dagger as a vec(matrix): zero, then each key adds conj(M[c,a]) at
+// (b + off_a, b + off_c) for every b over the other row bits (expecdiff.jl:64-65).
+template <int K>
+__global__ void __launch_bounds__(256) k_dm_seed(double2* __restrict__ Dl, int n, uint64_t ngroups, GPosDyn p,
+                                                 const double2* __restrict__ M) {
+    constexpr int D = 1 << K;
+    const uint64_t g = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+    if (g >= ngroups) return;
+    const uint64_t b = insert_zeros<K>(g, p.p);
+#pragma unroll
+    for (int a = 0; a < D; ++a)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            const uint64_t idx = (b + sub_offset<K>(a, p.p)) + ((b + sub_offset<K>(c, p.p)) << n);
+            Dl[idx] = cadd(Dl[idx], cconj(M[c + a * D]));
+        }
+}
+
+static int dm_seed(double2* Dl, int n, const vqc_term_t& t, const double2* M, hipStream_t st) {
+    GPosDyn p;
+    for (int j = 0; j < 5; ++j) p.p[j] = j < t.nq ? t.pos[j] : 0;
+    p.k = t.nq;
+    const uint64_t ng = uint64_t(1) << (n - t.nq);
+    const unsigned grid = grid_for(ng, 256);
+    switch (t.nq) {
+        case 1: hipLaunchKernelGGL(k_dm_seed<1>, dim3(grid), dim3(256), 0, st, Dl, n, ng, p, M); break;
+        case 2: hipLaunchKernelGGL(k_dm_seed<2>, dim3(grid), dim3(256), 0, st, Dl, n, ng, p, M); break;
+        case 3: hipLaunchKernelGGL(k_dm_seed<3>, dim3(grid), dim3(256), 0, st, Dl, n, ng, p, M); break;
+        case 4: hipLaunchKernelGGL(k_dm_seed<4>, dim3(grid), dim3(256), 0, st, Dl, n, ng, p, M); break;
+        case 5: hipLaunchKernelGGL(k_dm_seed<5>, dim3(grid), dim3(256), 0, st, Dl, n, ng, p, M); break;
+    }
+    VQC_HIP(hipGetLastError());
+    return VQC_OK;
+}
+
+int vqc_dm_gate_sweep(double2* v, int n, int k, const int32_t* pos, const double* U, hipStream_t st);
+
+extern "C" int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_t* ops, const double* pool, int nterms,
+                                  const vqc_term_t* terms, const double* hpool, int nparams, double* energy2, double* grad,
+                                  vqc_state* rho_out, int flags) {
+    VQC_TRY(vqc_check_state(rho0, "vqc_dm_energy_grad"));
+    if (!rho0->is_dm) return vqc_fail(VQC_EINVAL, "vqc_dm_energy_grad: state is not a density matrix");
+    if (nparams < 0 || (nparams && !grad)) return vqc_fail(VQC_EINVAL, "bad gradient output");
+    if (rho_out && (rho_out->nbits != rho0->nbits || !rho_out->is_dm)) return vqc_fail(VQC_EINVAL, "rho_out has the wrong shape");
+    const int n = rho0->nqubits, nb = rho0->nbits;
+    int64_t plen = 0, hlen = 0;
+    VQC_TRY(check_program(n, 1, nops, ops, &plen, nparams));
+    VQC_TRY(check_terms(n, nterms, terms, &hlen));
+    if ((plen && !pool) || (hlen && !hpool)) return vqc_fail(VQC_EINVAL, "null matrix pool");
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    hipStream_t st = c->stream;
+    const uint64_t L = rho0->len();
+    VQC_TRY(ensure(g_wB, L * 16));
+    double2* A;
+    if (rho_out && rho_out->d != rho0->d) A = rho_out->d;
+    else {
+        VQC_TRY(ensure(g_wA, L * 16));
+        A = (double2*)g_wA.buf;
+    }
+    double2* B = (double2*)g_wB.buf;
+    const bool plain = flags & VQC_FLAG_UNFUSED;
+    const TileTuning tt = vqc_tile_tuning();
+    // device pool [ops pool | hpool | variants]
+    XPool xp;
+    xp.h.assign(pool, pool + 2 * plen);
+    xp.h.insert(xp.h.end(), hpool, hpool + 2 * hlen);
+    PlanBuilder pb;
+    pb.xp = &xp;
+    std::vector<int64_t> conj_gen(size_t(nops) * 3, -1);
+    if (plain) {
+        for (int i = 0; i < nops; ++i)
+            for (int j = 0; j < ops[i].nparams; ++j)
+                if (ops[i].slot[j] >= 0) conj_gen[size_t(i) * 3 + j] = xp.variant(ops[i].gen[j], 1 << ops[i].nq, MF_CONJ);
+    } else {
+        VQC_TRY(vqc_plan_forward(pb, nb, n, 1, nops, ops, tt.t_fwd, tt.mlow));
+    }
+    const size_t nfwd = pb.phases.size();
+    pb.nrows = nterms;  // rows 0..nterms-1: energies (DM energies come from the trace kernels)
+    for (int t = 0; t < nterms; ++t) pb.row_slot.push_back(-1 - t);
+    if (!plain) VQC_TRY(vqc_plan_reverse(pb, nb, n, 1, nops, ops, tt.t_rev, tt.mlow));
+    else {
+        for (int i = nops - 1; i >= 0; --i)
+            for (int j = 0; j < ops[i].nparams; ++j)
+                if (ops[i].slot[j] >= 0) {
+                    pb.row_slot.push_back(ops[i].slot[j]);
+                    pb.nrows++;
+                }
+    }
+    VQC_TRY(upload_program(pb, st));
+    const MicroOp* dm = (const MicroOp*)g_wmops.buf;
+    const double2* dp = (const double2*)g_wpool.buf;
+    const int stride = 2048;  // row and column gradient halves in the plain path
+    const int nrows = std::max(pb.nrows, 1);
+    VQC_TRY(ensure(g_wparts, size_t(nrows) * (stride + 1) * 16));
+    double2* parts = (double2*)g_wparts.buf;
+    double2* rowsum = parts + size_t(nrows) * stride;
+    vqc_zero(parts, size_t(nrows) * stride, st);
+    // forward
+    if (plain) {
+        VQC_HIP(hipMemcpyAsync(A, rho0->d, L * 16, hipMemcpyDeviceToDevice, st));
+        for (int i = 0; i < nops; ++i) {
+            const vqc_op_t& o = ops[i];
+            if (o.kind == VQC_OP_CHANNEL) {
+                int32_t all[4];
+                for (int j = 0; j < o.nq; ++j) {
+                    all[j] = o.pos[j];
+                    all[j + o.nq] = o.pos[j] + n;
+                }
+                VQC_TRY(vqc_dense_sweep(A, nb, 2 * o.nq, all, pool + 2 * o.mat, st));
+            } else {
+                VQC_TRY(vqc_dm_gate_sweep(A, n, o.nq, o.pos, pool + 2 * o.mat, st));
+            }
+        }
+    } else if (nfwd == 0) {
+        VQC_HIP(hipMemcpyAsync(A, rho0->d, L * 16, hipMemcpyDeviceToDevice, st));
+    } else {
+        VQC_TRY(run_forward(pb, 0, nfwd, rho0->d, A, dm, dp, st));
+    }
+    // energies tr(M rho) and the seed Delta = H^dagger
+    unsigned grid;
+    for (int t = 0; t < nterms; ++t)
+        VQC_TRY(vqc_dm_expec_sweep(A, n, terms[t].nq, terms[t].pos, dp + plen + terms[t].mat, parts + size_t(t) * stride, &grid, st));
+    vqc_zero(B, L, st);
+    for (int t = 0; t < nterms; ++t) VQC_TRY(dm_seed(B, n, terms[t], dp + plen + terms[t].mat, st));
+    // reverse
+    if (plain) {
+        int row = nterms;
+        std::vector<double> ud;
+        for (int i = nops - 1; i >= 0; --i) {
+            const vqc_op_t& o = ops[i];
+            if (o.kind == VQC_OP_CHANNEL) {
+                int32_t all[4];
+                for (int j = 0; j < o.nq; ++j) {
+                    all[j] = o.pos[j];
+                    all[j + o.nq] = o.pos[j] + n;
+                }
+                const int D = 1 << (2 * o.nq);
+                ud.resize(2 * D * D);
+                dagger_host(pool + 2 * o.mat, D, ud.data());
+                VQC_TRY(vqc_dense_sweep(B, nb, 2 * o.nq, all, ud.data(), st));
+                VQC_TRY(vqc_dense_sweep(A, nb, 2 * o.nq, all, pool + 2 * o.aux, st));
+                continue;
+            }
+            const int D = 1 << o.nq;
+            ud.resize(2 * D * D);
+            dagger_host(pool + 2 * o.mat, D, ud.data());
+            VQC_TRY(vqc_dm_gate_sweep(B, n, o.nq, o.pos, ud.data(), st));
+            VQC_TRY(vqc_dm_gate_sweep(A, n, o.nq, o.pos, ud.data(), st));
+            int32_t cols[5];
+            for (int j = 0; j < o.nq; ++j) cols[j] = o.pos[j] + n;
+            for (int j = 0; j < o.nparams; ++j) {
+                if (o.slot[j] < 0) continue;
+                // rows and columns write their partials to two halves of the row
+                VQC_TRY(vqc_expec_sweep(A, B, nb, o.nq, o.pos, dp + o.gen[j], parts + size_t(row) * stride, &grid, st));
+                if (grid > unsigned(stride / 2)) return vqc_fail(VQC_EUNSUPPORTED, "partial buffer too small");
+                unsigned g2;
+                VQC_TRY(vqc_expec_sweep(A, B, nb, o.nq, cols, dp + conj_gen[size_t(i) * 3 + j], parts + size_t(row) * stride + grid, &g2, st));
+                ++row;
+            }
+        }
+    } else {
+        for (size_t i = nfwd; i < pb.phases.size(); ++i) {
+            PhaseBufs b{A, A, B, dm, dp, parts, stride};
+            VQC_TRY(launch_phase(pb.phases[i], b, phase_grid(pb.phases[i], stride / 2), st));
+        }
+    }
+    if (pb.nrows) vqc_reduce_partials(parts, pb.nrows, stride, rowsum, st);
+    VQC_HIP(hipGetLastError());
+    std::vector<double> rows(2 * size_t(nrows));
+    if (pb.nrows) VQC_TRY(vqc_fetch(rowsum, pb.nrows, rows.data()));
+    else VQC_HIP(hipStreamSynchronize(st));
+    finish_rows(rows, pb.row_slot, nparams, energy2, grad);
+    return VQC_OK;
+}

@@ -1,0 +1,337 @@
+// vqc_core.hip -- library context, errors, state handles and whole-state
+// elementwise kernels (StateVector / DensityMatrix storage on the device).
+#include <cstdarg>
+#include <mutex>
+
+#include "vqc_internal.h"
+
+static thread_local std::string g_err;
+
+int vqc_fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+extern "C" const char* vqc_last_error(void) { return g_err.c_str(); }
+extern "C" int vqc_version(void) { return 10000; }
+
+static VqcCtx g_ctx;
+static std::mutex g_ctx_mu;
+
+int vqc_ctx(VqcCtx** out) {
+    std::lock_guard<std::mutex> lk(g_ctx_mu);
+    if (g_ctx.device < 0) {
+        int ndev = 0;
+        hipError_t e = hipGetDeviceCount(&ndev);
+        if (e != hipSuccess || ndev < 1)
+            return vqc_fail(VQC_EHIP, "no HIP device available (%s)", hipGetErrorString(e));
+        int dev = 0;
+        VQC_HIP(hipGetDevice(&dev));
+        hipDeviceProp_t prop;
+        VQC_HIP(hipGetDeviceProperties(&prop, dev));
+        if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
+            return vqc_fail(VQC_EUNSUPPORTED, "libvqc_hip is built for gfx950, device is %s", prop.gcnArchName);
+        VQC_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
+        g_ctx.cu_count = prop.multiProcessorCount;
+        g_ctx.lds_per_cu = int(prop.maxSharedMemoryPerMultiProcessor);
+        g_ctx.hbm_bytes = prop.totalGlobalMem;
+        g_ctx.device = dev;
+    }
+    *out = &g_ctx;
+    return VQC_OK;
+}
+
+int vqc_scratch(size_t bytes, void** out) {
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    if (bytes > c->scratch_bytes) {
+        if (c->scratch) {
+            VQC_HIP(hipStreamSynchronize(c->stream));
+            VQC_HIP(hipFree(c->scratch));
+            c->scratch = nullptr;
+            c->scratch_bytes = 0;
+        }
+        size_t nb = std::max<size_t>(bytes, size_t(1) << 20);
+        VQC_HIP(hipMalloc(&c->scratch, nb));
+        c->scratch_bytes = nb;
+    }
+    *out = c->scratch;
+    return VQC_OK;
+}
+
+int vqc_pinned(size_t bytes, double** out) {
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    if (bytes > c->pinned_bytes) {
+        if (c->pinned) {
+            VQC_HIP(hipStreamSynchronize(c->stream));
+            VQC_HIP(hipHostFree(c->pinned));
+            c->pinned = nullptr;
+            c->pinned_bytes = 0;
+        }
+        size_t nb = std::max<size_t>(bytes, size_t(1) << 16);
+        VQC_HIP(hipHostMalloc((void**)&c->pinned, nb, hipHostMallocDefault));
+        c->pinned_bytes = nb;
+    }
+    *out = c->pinned;
+    return VQC_OK;
+}
+
+int vqc_upload_buffer(void** dev, size_t* cap, const void* host, size_t bytes) {
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    if (bytes > *cap) {
+        if (*dev) {
+            VQC_HIP(hipStreamSynchronize(c->stream));
+            VQC_HIP(hipFree(*dev));
+            *dev = nullptr;
+            *cap = 0;
+        }
+        size_t nb = std::max<size_t>(bytes, 4096);
+        VQC_HIP(hipMalloc(dev, nb));
+        *cap = nb;
+    }
+    if (bytes) VQC_HIP(hipMemcpyAsync(*dev, host, bytes, hipMemcpyHostToDevice, c->stream));
+    return VQC_OK;
+}
+
+extern "C" int vqc_device_info(int* cu_count, int* lds_bytes_per_cu, uint64_t* hbm_bytes) {
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    if (cu_count) *cu_count = c->cu_count;
+    if (lds_bytes_per_cu) *lds_bytes_per_cu = c->lds_per_cu;
+    if (hbm_bytes) *hbm_bytes = c->hbm_bytes;
+    return VQC_OK;
+}
+
+extern "C" int vqc_sync(void) {
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    VQC_HIP(hipStreamSynchronize(c->stream));
+    return VQC_OK;
+}
+
+extern "C" int vqc_get_stream(void** stream) {
+    if (!stream) return vqc_fail(VQC_EINVAL, "null output");
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    *stream = (void*)c->stream;
+    return VQC_OK;
+}
+
+int vqc_check_positions(int nbits, int k, const int32_t* pos, int kmax) {
+    if (!pos) return vqc_fail(VQC_EINVAL, "null position array");
+    if (k < 1 || k > kmax) return vqc_fail(VQC_EINVAL, "only 1..%d-qubit operators are supported (got %d)", kmax, k);
+    for (int j = 0; j < k; ++j) {
+        if (pos[j] < 0 || pos[j] >= nbits)
+            return vqc_fail(VQC_EINVAL, "qubit position %d out of range 0..%d", pos[j], nbits - 1);
+        if (j && pos[j] <= pos[j - 1])
+            return vqc_fail(VQC_EINVAL, "positions must be strictly ascending (got %d after %d)", pos[j], pos[j - 1]);
+    }
+    return VQC_OK;
+}
+
+int vqc_check_state(const vqc_state* s, const char* what) {
+    if (!s || !s->d) return vqc_fail(VQC_EINVAL, "%s: null state", what);
+    return VQC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// elementwise kernels
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_fill_basis(double2* v, uint64_t len, uint64_t one_at) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (; i < len; i += uint64_t(gridDim.x) * blockDim.x) v[i] = make_double2(i == one_at ? 1.0 : 0.0, 0.0);
+}
+
+__global__ void __launch_bounds__(256) k_scale(double2* v, uint64_t len, double2 a) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (; i < len; i += uint64_t(gridDim.x) * blockDim.x) v[i] = cmul(a, v[i]);
+}
+
+// per-block partials of sum conj(a_i) b_i
+__global__ void __launch_bounds__(256) k_dot(const double2* __restrict__ a, const double2* __restrict__ b,
+                                             uint64_t len, double2* parts) {
+    __shared__ double red[8];
+    double sr = 0, si = 0;
+    for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += uint64_t(gridDim.x) * blockDim.x) {
+        double2 x = a[i], y = b[i];
+        sr = fma(x.x, y.x, fma(x.y, y.y, sr));
+        si = fma(x.x, y.y, fma(-x.y, y.x, si));
+    }
+    sr = block_sum(sr, red);
+    si = block_sum(si, red);
+    if (threadIdx.x == 0) parts[blockIdx.x] = make_double2(sr, si);
+}
+
+// one block per term: out[t] = sum_p parts[t*nparts + p], summed in a fixed order
+__global__ void __launch_bounds__(256) k_reduce_parts(const double2* __restrict__ parts, int nparts, double2* out) {
+    __shared__ double red[8];
+    const double2* p = parts + size_t(blockIdx.x) * nparts;
+    double sr = 0, si = 0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        sr += p[i].x;
+        si += p[i].y;
+    }
+    sr = block_sum(sr, red);
+    si = block_sum(si, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = make_double2(sr, si);
+}
+
+void vqc_reduce_partials(const double2* parts, int nterms, int nparts, double2* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_reduce_parts, dim3(nterms), dim3(256), 0, st, parts, nparts, out);
+}
+
+int vqc_fetch(const double2* dev, int n, double* host) {
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    double* pin;
+    VQC_TRY(vqc_pinned(size_t(n) * 16, &pin));
+    VQC_HIP(hipMemcpyAsync(pin, dev, size_t(n) * 16, hipMemcpyDeviceToHost, c->stream));
+    VQC_HIP(hipStreamSynchronize(c->stream));
+    for (int i = 0; i < 2 * n; ++i) host[i] = pin[i];
+    return VQC_OK;
+}
+
+static unsigned stream_grid(uint64_t len) {
+    uint64_t b = (len + 255) / 256;
+    return unsigned(std::min<uint64_t>(b, 65536));
+}
+
+// ---------------------------------------------------------------------------
+// state handles
+// ---------------------------------------------------------------------------
+static int state_alloc(int nbits, int nq, int is_dm, vqc_state** out) {
+    if (!out) return vqc_fail(VQC_EINVAL, "null output handle");
+    if (nbits < 1 || nbits > 34) return vqc_fail(VQC_EINVAL, "unsupported state size: %d index bits", nbits);
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    vqc_state* s = new vqc_state;
+    s->nbits = nbits;
+    s->nqubits = nq;
+    s->is_dm = is_dm;
+    hipError_t e = hipMalloc((void**)&s->d, s->len() * sizeof(double2));
+    if (e != hipSuccess) {
+        delete s;
+        (void)hipGetLastError();
+        return vqc_fail(VQC_ENOMEM, "cannot allocate %llu bytes for a %d-bit state",
+                        (unsigned long long)((uint64_t(1) << nbits) * 16), nbits);
+    }
+    *out = s;
+    return vqc_state_reset(s);
+}
+
+extern "C" int vqc_sv_create(int nqubits, vqc_state** out) { return state_alloc(nqubits, nqubits, 0, out); }
+extern "C" int vqc_dm_create(int nqubits, vqc_state** out) {
+    if (nqubits < 1 || nqubits > 17) return vqc_fail(VQC_EINVAL, "unsupported density-matrix size: %d qubits", nqubits);
+    return state_alloc(2 * nqubits, nqubits, 1, out);
+}
+
+extern "C" int vqc_state_wrap(int nbits, int is_dm, void* device_ptr, vqc_state** out) {
+    if (!out || !device_ptr) return vqc_fail(VQC_EINVAL, "null pointer");
+    if (nbits < 1 || nbits > 34 || (is_dm && (nbits & 1))) return vqc_fail(VQC_EINVAL, "bad state size %d", nbits);
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    vqc_state* s = new vqc_state;
+    s->nbits = nbits;
+    s->nqubits = is_dm ? nbits / 2 : nbits;
+    s->is_dm = is_dm;
+    s->owns = 0;
+    s->d = (double2*)device_ptr;
+    *out = s;
+    return VQC_OK;
+}
+
+extern "C" int vqc_state_destroy(vqc_state* s) {
+    if (!s) return VQC_OK;
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    VQC_HIP(hipStreamSynchronize(c->stream));
+    if (s->owns && s->d) VQC_HIP(hipFree(s->d));
+    delete s;
+    return VQC_OK;
+}
+
+extern "C" int vqc_state_nbits(const vqc_state* s, int* nbits) {
+    VQC_TRY(vqc_check_state(s, "vqc_state_nbits"));
+    *nbits = s->nbits;
+    return VQC_OK;
+}
+
+extern "C" int vqc_state_device_ptr(vqc_state* s, void** out) {
+    VQC_TRY(vqc_check_state(s, "vqc_state_device_ptr"));
+    *out = s->d;
+    return VQC_OK;
+}
+
+extern "C" int vqc_state_upload(vqc_state* s, const double* host, uint64_t n) {
+    VQC_TRY(vqc_check_state(s, "vqc_state_upload"));
+    if (!host || n != s->len()) return vqc_fail(VQC_EINVAL, "upload of %llu amplitudes into a state of %llu",
+                                                (unsigned long long)n, (unsigned long long)s->len());
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    VQC_HIP(hipMemcpyAsync(s->d, host, n * 16, hipMemcpyHostToDevice, c->stream));
+    VQC_HIP(hipStreamSynchronize(c->stream));
+    return VQC_OK;
+}
+
+extern "C" int vqc_state_download(const vqc_state* s, double* host, uint64_t n) {
+    VQC_TRY(vqc_check_state(s, "vqc_state_download"));
+    if (!host || n != s->len()) return vqc_fail(VQC_EINVAL, "download of %llu amplitudes from a state of %llu",
+                                                (unsigned long long)n, (unsigned long long)s->len());
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    VQC_HIP(hipMemcpyAsync(host, s->d, n * 16, hipMemcpyDeviceToHost, c->stream));
+    VQC_HIP(hipStreamSynchronize(c->stream));
+    return VQC_OK;
+}
+
+extern "C" int vqc_state_reset(vqc_state* s) {
+    VQC_TRY(vqc_check_state(s, "vqc_state_reset"));
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    hipLaunchKernelGGL(k_fill_basis, dim3(stream_grid(s->len())), dim3(256), 0, c->stream, s->d, s->len(), uint64_t(0));
+    VQC_HIP(hipGetLastError());
+    return VQC_OK;
+}
+
+extern "C" int vqc_state_copy(vqc_state* dst, const vqc_state* src) {
+    VQC_TRY(vqc_check_state(dst, "vqc_state_copy"));
+    VQC_TRY(vqc_check_state(src, "vqc_state_copy"));
+    if (dst->nbits != src->nbits) return vqc_fail(VQC_EINVAL, "copy between states of different size");
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    VQC_HIP(hipMemcpyAsync(dst->d, src->d, src->len() * 16, hipMemcpyDeviceToDevice, c->stream));
+    return VQC_OK;
+}
+
+extern "C" int vqc_state_scale(vqc_state* s, double re, double im) {
+    VQC_TRY(vqc_check_state(s, "vqc_state_scale"));
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    hipLaunchKernelGGL(k_scale, dim3(stream_grid(s->len())), dim3(256), 0, c->stream, s->d, s->len(), make_double2(re, im));
+    VQC_HIP(hipGetLastError());
+    return VQC_OK;
+}
+
+extern "C" int vqc_state_dot(const vqc_state* a, const vqc_state* b, double* out2) {
+    VQC_TRY(vqc_check_state(a, "vqc_state_dot"));
+    VQC_TRY(vqc_check_state(b, "vqc_state_dot"));
+    if (a->nbits != b->nbits) return vqc_fail(VQC_EINVAL, "dot of states of different size");
+    if (!out2) return vqc_fail(VQC_EINVAL, "null output");
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    const unsigned nb = unsigned(std::min<uint64_t>((a->len() + 255) / 256, 2048));
+    void* scr;
+    VQC_TRY(vqc_scratch((nb + 1) * 16, &scr));
+    double2* parts = (double2*)scr;
+    hipLaunchKernelGGL(k_dot, dim3(nb), dim3(256), 0, c->stream, a->d, b->d, a->len(), parts);
+    vqc_reduce_partials(parts, 1, nb, parts + nb, c->stream);
+    VQC_HIP(hipGetLastError());
+    return vqc_fetch(parts + nb, 1, out2);
+}

@@ -1,0 +1,321 @@
+// vqc_gates.hip -- one-pass gate sweeps over a state in HBM.
+//
+// Dense k-qubit gates (k = 1..5) replace _apply_gate_threaded2!(key, U, v)
+// (threaded_short_range.jl:100,364,753; threaded_long_range.jl:51,131). The
+// reference tiles each target class differently for CPU SIMD; here one kernel
+// covers every class: thread t owns group g = t (the amplitudes whose target
+// bits run over all 2^k values), so lanes of a wavefront read consecutive base
+// indices and every one of the 2^k loads is a contiguous 1 KiB wavefront
+// access whenever the lowest target is >= 6, and stays cache-line complete
+// below that. Each amplitude is read and written exactly once (32 B per
+// amplitude of HBM traffic), which is the roofline of an unfused gate.
+#include "vqc_internal.h"
+
+template <int K>
+struct GPos {
+    int p[K];
+};
+template <int K>
+struct GMat {
+    double2 m[(1 << K) * (1 << K)];
+};
+
+constexpr int kBlock = 256;
+
+// K <= 3: the matrix travels in the kernel arguments (<= 1 KiB, read through
+// the scalar cache). Each thread applies the gate to R groups, issuing all
+// R * 2^K loads before the first multiply.
+template <int K, int R>
+__global__ void __launch_bounds__(kBlock) k_dense_small(double2* __restrict__ v, uint64_t ngroups, GPos<K> gp, GMat<K> U) {
+    constexpr int D = 1 << K;
+    uint64_t off[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) off[j] = sub_offset<K>(j, gp.p);
+    const uint64_t g0 = uint64_t(blockIdx.x) * (R * kBlock) + threadIdx.x;
+    double2 a[R][D];
+    uint64_t base[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const uint64_t g = g0 + uint64_t(r) * kBlock;
+        base[r] = insert_zeros<K>(g, gp.p);
+        if (g < ngroups) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) a[r][j] = v[base[r] + off[j]];
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        if (g0 + uint64_t(r) * kBlock >= ngroups) continue;
+#pragma unroll
+        for (int row = 0; row < D; ++row) {
+            double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int c = 0; c < D; ++c) s = cfma(U.m[row + c * D], a[r][c], s);
+            v[base[r] + off[row]] = s;
+        }
+    }
+}
+
+// K = 4, 5: the matrix (4 or 16 KiB) is staged once per block in LDS and read
+// as a broadcast; one group per thread.
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_dense_big(double2* __restrict__ v, uint64_t ngroups, GPos<K> gp,
+                                                       const double2* __restrict__ Ug) {
+    constexpr int D = 1 << K;
+    __shared__ double2 Us[D * D];
+    for (int i = threadIdx.x; i < D * D; i += kBlock) Us[i] = Ug[i];
+    __syncthreads();
+    const uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (g >= ngroups) return;
+    const uint64_t base = insert_zeros<K>(g, gp.p);
+    double2 a[D];
+#pragma unroll
+    for (int j = 0; j < D; ++j) a[j] = v[base + sub_offset<K>(j, gp.p)];
+#pragma unroll
+    for (int row = 0; row < D; ++row) {
+        double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int c = 0; c < D; ++c) s = cfma(Us[row + c * D], a[c], s);
+        v[base + sub_offset<K>(row, gp.p)] = s;
+    }
+}
+
+// Device copy of a host matrix for the K >= 4 kernels (stream ordered, so a
+// later call may reuse the buffer).
+static void* g_mat_dev = nullptr;
+static size_t g_mat_cap = 0;
+
+template <int K>
+static int launch_dense(double2* v, int nbits, const int32_t* pos, const double* Uh, hipStream_t st) {
+    constexpr int D = 1 << K;
+    GPos<K> gp;
+    for (int j = 0; j < K; ++j) gp.p[j] = pos[j];
+    const uint64_t ngroups = uint64_t(1) << (nbits - K);
+    if constexpr (K <= 3) {
+        GMat<K> U;
+        for (int i = 0; i < D * D; ++i) U.m[i] = make_double2(Uh[2 * i], Uh[2 * i + 1]);
+        constexpr int R = K == 1 ? 4 : (K == 2 ? 2 : 1);
+        hipLaunchKernelGGL((k_dense_small<K, R>), dim3(grid_for(ngroups, R * kBlock)), dim3(kBlock), 0, st, v, ngroups, gp, U);
+    } else {
+        VQC_TRY(vqc_upload_buffer(&g_mat_dev, &g_mat_cap, Uh, size_t(D) * D * 16));
+        hipLaunchKernelGGL((k_dense_big<K>), dim3(grid_for(ngroups, kBlock)), dim3(kBlock), 0, st, v, ngroups, gp,
+                           (const double2*)g_mat_dev);
+    }
+    VQC_HIP(hipGetLastError());
+    return VQC_OK;
+}
+
+int vqc_dense_sweep(double2* v, int nbits, int k, const int32_t* pos, const double* U, hipStream_t st) {
+    switch (k) {
+        case 1: return launch_dense<1>(v, nbits, pos, U, st);
+        case 2: return launch_dense<2>(v, nbits, pos, U, st);
+        case 3: return launch_dense<3>(v, nbits, pos, U, st);
+        case 4: return launch_dense<4>(v, nbits, pos, U, st);
+        case 5: return launch_dense<5>(v, nbits, pos, U, st);
+    }
+    return vqc_fail(VQC_EINVAL, "dense gate on %d qubits", k);
+}
+
+extern "C" int vqc_sv_apply_dense(vqc_state* s, int k, const int32_t* pos, const double* U) {
+    VQC_TRY(vqc_check_state(s, "vqc_sv_apply_dense"));
+    VQC_TRY(vqc_check_positions(s->nbits, k, pos, 5));
+    if (!U) return vqc_fail(VQC_EINVAL, "null gate matrix");
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    return vqc_dense_sweep(s->d, s->nbits, k, pos, U, c->stream);
+}
+
+// ---------------------------------------------------------------------------
+// Named gates (src/applygates/specific/*.jl): permutations and phases that
+// touch only the amplitudes they change.
+// ---------------------------------------------------------------------------
+template <int KIND, int K>
+__global__ void __launch_bounds__(kBlock) k_named(double2* __restrict__ v, uint64_t ngroups, GPos<K> sorted,
+                                                  GPos<K> given, double2 ph) {
+    const uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (g >= ngroups) return;
+    const uint64_t l = insert_zeros<K>(g, sorted.p);
+    const uint64_t p0 = uint64_t(1) << given.p[0];
+    const uint64_t p1 = K > 1 ? uint64_t(1) << given.p[K > 1 ? 1 : 0] : 0;
+    const uint64_t p2 = K > 2 ? uint64_t(1) << given.p[K > 2 ? 2 : 0] : 0;
+    if constexpr (KIND == VQC_GATE_X) {
+        double2 a = v[l], b = v[l + p0];
+        v[l] = b;
+        v[l + p0] = a;
+    } else if constexpr (KIND == VQC_GATE_PHASE) {
+        v[l + p0] = cmul(ph, v[l + p0]);
+    } else if constexpr (KIND == VQC_GATE_CZ) {
+        double2 a = v[l + p0 + p1];
+        v[l + p0 + p1] = make_double2(-a.x, -a.y);
+    } else if constexpr (KIND == VQC_GATE_CNOT) {
+        double2 a = v[l + p0], b = v[l + p0 + p1];
+        v[l + p0] = b;
+        v[l + p0 + p1] = a;
+    } else if constexpr (KIND == VQC_GATE_CPHASE) {
+        v[l + p0 + p1] = cmul(ph, v[l + p0 + p1]);
+    } else if constexpr (KIND == VQC_GATE_SWAP) {
+        double2 a = v[l + p0], b = v[l + p1];
+        v[l + p0] = b;
+        v[l + p1] = a;
+    } else if constexpr (KIND == VQC_GATE_ISWAP) {  // ph = +i or -i
+        double2 a = v[l + p0], b = v[l + p1];
+        v[l + p0] = cmul(ph, b);
+        v[l + p1] = cmul(ph, a);
+    } else if constexpr (KIND == VQC_GATE_FREDKIN) {
+        double2 a = v[l + p0 + p1], b = v[l + p0 + p2];
+        v[l + p0 + p1] = b;
+        v[l + p0 + p2] = a;
+    } else if constexpr (KIND == VQC_GATE_CCPHASE) {
+        v[l + p0 + p1 + p2] = cmul(ph, v[l + p0 + p1 + p2]);
+    }
+}
+
+template <int KIND, int K>
+static int launch_named(double2* v, int nbits, const int32_t* pos, double2 ph, hipStream_t st) {
+    GPos<K> sorted, given;
+    for (int j = 0; j < K; ++j) sorted.p[j] = given.p[j] = pos[j];
+    std::sort(sorted.p, sorted.p + K);
+    for (int j = 0; j < K; ++j) {
+        if (sorted.p[j] < 0 || sorted.p[j] >= nbits) return vqc_fail(VQC_EINVAL, "qubit position %d out of range", sorted.p[j]);
+        if (j && sorted.p[j] == sorted.p[j - 1]) return vqc_fail(VQC_EINVAL, "repeated qubit position %d", sorted.p[j]);
+    }
+    const uint64_t ngroups = uint64_t(1) << (nbits - K);
+    hipLaunchKernelGGL((k_named<KIND, K>), dim3(grid_for(ngroups, kBlock)), dim3(kBlock), 0, st, v, ngroups, sorted, given, ph);
+    VQC_HIP(hipGetLastError());
+    return VQC_OK;
+}
+
+extern "C" int vqc_sv_apply_named(vqc_state* s, int kind, const int32_t* pos, double phi) {
+    VQC_TRY(vqc_check_state(s, "vqc_sv_apply_named"));
+    if (!pos) return vqc_fail(VQC_EINVAL, "null position array");
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    const double2 e = make_double2(cos(phi), sin(phi));
+    hipStream_t st = c->stream;
+    switch (kind) {
+        case VQC_GATE_X: return launch_named<VQC_GATE_X, 1>(s->d, s->nbits, pos, e, st);
+        case VQC_GATE_PHASE: return launch_named<VQC_GATE_PHASE, 1>(s->d, s->nbits, pos, e, st);
+        case VQC_GATE_CZ: return launch_named<VQC_GATE_CZ, 2>(s->d, s->nbits, pos, e, st);
+        case VQC_GATE_CNOT: return launch_named<VQC_GATE_CNOT, 2>(s->d, s->nbits, pos, e, st);
+        case VQC_GATE_CPHASE: return launch_named<VQC_GATE_CPHASE, 2>(s->d, s->nbits, pos, e, st);
+        case VQC_GATE_SWAP: return launch_named<VQC_GATE_SWAP, 2>(s->d, s->nbits, pos, e, st);
+        case VQC_GATE_ISWAP: return launch_named<VQC_GATE_ISWAP, 2>(s->d, s->nbits, pos, make_double2(0, 1), st);
+        case VQC_GATE_ISWAPDAG: return launch_named<VQC_GATE_ISWAP, 2>(s->d, s->nbits, pos, make_double2(0, -1), st);
+        case VQC_GATE_FREDKIN: return launch_named<VQC_GATE_FREDKIN, 3>(s->d, s->nbits, pos, e, st);
+        case VQC_GATE_CCPHASE: return launch_named<VQC_GATE_CCPHASE, 3>(s->d, s->nbits, pos, e, st);
+    }
+    return vqc_fail(VQC_EINVAL, "unknown named gate kind %d", kind);
+}
+
+// ---------------------------------------------------------------------------
+// Density matrices. rho <- U rho U^dagger in ONE sweep: the reference runs two
+// k-qubit sweeps, U on pos then conj(U) on pos+n (generic.jl:54-64); both act
+// on the same 4^k block, so each block is loaded once, U applied to its row
+// index and conj(U) to its column index, and stored once (32 B per entry).
+// ---------------------------------------------------------------------------
+template <int K>
+__global__ void __launch_bounds__(kBlock) k_dm_gate(double2* __restrict__ v, uint64_t ngroups, GPos<2 * K> gp, GMat<K> U) {
+    constexpr int D = 1 << K;
+    const uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    if (g >= ngroups) return;
+    const uint64_t base = insert_zeros<2 * K>(g, gp.p);
+    double2 x[D][D];  // [row][col]
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) x[r][c] = v[base + sub_offset<2 * K>(r | (c << K), gp.p)];
+    double2 t[D][D];
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < D; ++q) s = cfma(U.m[r + q * D], x[q][c], s);
+            t[r][c] = s;
+        }
+#pragma unroll
+    for (int r = 0; r < D; ++r)
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int q = 0; q < D; ++q) s = cfma(cconj(U.m[c + q * D]), t[r][q], s);
+            v[base + sub_offset<2 * K>(r | (c << K), gp.p)] = s;
+        }
+}
+
+template <int K>
+static int launch_dm_gate(vqc_state* s, const int32_t* pos, const double* Uh, hipStream_t st) {
+    constexpr int D = 1 << K;
+    GPos<2 * K> gp;
+    for (int j = 0; j < K; ++j) {
+        gp.p[j] = pos[j];
+        gp.p[j + K] = pos[j] + s->nqubits;
+    }
+    GMat<K> U;
+    for (int i = 0; i < D * D; ++i) U.m[i] = make_double2(Uh[2 * i], Uh[2 * i + 1]);
+    const uint64_t ngroups = uint64_t(1) << (s->nbits - 2 * K);
+    hipLaunchKernelGGL((k_dm_gate<K>), dim3(grid_for(ngroups, kBlock)), dim3(kBlock), 0, st, s->d, ngroups, gp, U);
+    VQC_HIP(hipGetLastError());
+    return VQC_OK;
+}
+
+// rho <- U rho U^dagger on a raw vec(rho) of 2n index bits.
+int vqc_dm_gate_sweep(double2* v, int n, int k, const int32_t* pos, const double* U, hipStream_t st) {
+    vqc_state tmp;
+    tmp.nbits = 2 * n;
+    tmp.nqubits = n;
+    tmp.is_dm = 1;
+    tmp.owns = 0;
+    tmp.d = v;
+    switch (k) {
+        case 1: return launch_dm_gate<1>(&tmp, pos, U, st);
+        case 2: return launch_dm_gate<2>(&tmp, pos, U, st);
+        case 3: return launch_dm_gate<3>(&tmp, pos, U, st);
+    }
+    const int D = 1 << k;
+    std::vector<double> Uc(size_t(2) * D * D);
+    for (int i = 0; i < D * D; ++i) {
+        Uc[2 * i] = U[2 * i];
+        Uc[2 * i + 1] = -U[2 * i + 1];
+    }
+    int32_t pos2[5];
+    for (int j = 0; j < k; ++j) pos2[j] = pos[j] + n;
+    VQC_TRY(vqc_dense_sweep(v, 2 * n, k, pos, U, st));
+    return vqc_dense_sweep(v, 2 * n, k, pos2, Uc.data(), st);
+}
+
+extern "C" int vqc_dm_apply_gate(vqc_state* rho, int k, const int32_t* pos, const double* U) {
+    VQC_TRY(vqc_check_state(rho, "vqc_dm_apply_gate"));
+    if (!rho->is_dm) return vqc_fail(VQC_EINVAL, "vqc_dm_apply_gate: state is not a density matrix");
+    VQC_TRY(vqc_check_positions(rho->nqubits, k, pos, 5));
+    if (!U) return vqc_fail(VQC_EINVAL, "null gate matrix");
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    return vqc_dm_gate_sweep(rho->d, rho->nqubits, k, pos, U, c->stream);
+}
+
+extern "C" int vqc_dm_apply_superop(vqc_state* rho, int N, const int32_t* pos, const double* S, int mode) {
+    VQC_TRY(vqc_check_state(rho, "vqc_dm_apply_superop"));
+    if (!rho->is_dm) return vqc_fail(VQC_EINVAL, "vqc_dm_apply_superop: state is not a density matrix");
+    VQC_TRY(vqc_check_positions(rho->nqubits, N, pos, 2));
+    if (!S) return vqc_fail(VQC_EINVAL, "null superoperator");
+    if (mode != 0 && mode != 1) return vqc_fail(VQC_EINVAL, "superoperator mode must be 0 (S) or 1 (S^dagger)");
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    const int D = 1 << (2 * N);
+    int32_t all[4];
+    for (int j = 0; j < N; ++j) {
+        all[j] = pos[j];
+        all[j + N] = pos[j] + rho->nqubits;
+    }
+    if (mode == 0) return vqc_dense_sweep(rho->d, rho->nbits, 2 * N, all, S, c->stream);
+    std::vector<double> Sd(size_t(2) * D * D);
+    for (int r = 0; r < D; ++r)
+        for (int cc = 0; cc < D; ++cc) {
+            Sd[2 * (r + cc * D)] = S[2 * (cc + r * D)];
+            Sd[2 * (r + cc * D) + 1] = -S[2 * (cc + r * D) + 1];
+        }
+    return vqc_dense_sweep(rho->d, rho->nbits, 2 * N, all, Sd.data(), c->stream);
+}

@@ -1,0 +1,143 @@
+// vqc_internal.h -- shared definitions of libvqc_hip.so (gfx950 only).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/vqc_hip.h"
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+int vqc_fail(int code, const char* fmt, ...);
+#define VQC_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) return vqc_fail(VQC_EHIP, "%s: %s", #call, hipGetErrorString(e_)); \
+    } while (0)
+#define VQC_TRY(call)               \
+    do {                            \
+        int r_ = (call);            \
+        if (r_ != VQC_OK) return r_; \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// library context: one stream per process on the current device, a growable
+// device scratch area, and pinned host staging for scalar results.
+// ---------------------------------------------------------------------------
+struct VqcCtx {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    int lds_per_cu = 0;
+    uint64_t hbm_bytes = 0;
+    void* scratch = nullptr;      // device, grown on demand
+    size_t scratch_bytes = 0;
+    double* pinned = nullptr;     // host pinned staging
+    size_t pinned_bytes = 0;
+};
+int vqc_ctx(VqcCtx** out);
+int vqc_scratch(size_t bytes, void** out);      // device scratch, valid until the next call
+int vqc_pinned(size_t bytes, double** out);     // pinned host staging
+// Upload host data into a dedicated device buffer of at least `bytes` bytes (grown on demand).
+int vqc_upload_buffer(void** dev, size_t* cap, const void* host, size_t bytes);
+
+struct vqc_state {
+    int nbits = 0;       // log2 of the number of stored amplitudes
+    int nqubits = 0;     // n (== nbits for SV, nbits/2 for DM)
+    int is_dm = 0;
+    int owns = 1;
+    double2* d = nullptr;
+    uint64_t len() const { return uint64_t(1) << nbits; }
+};
+
+// ---------------------------------------------------------------------------
+// complex helpers (double2 = re, im)
+// ---------------------------------------------------------------------------
+__host__ __device__ __forceinline__ double2 cmk(double a, double b) { return make_double2(a, b); }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+    return make_double2(fma(a.x, b.x, -a.y * b.y), fma(a.x, b.y, a.y * b.x));
+}
+// acc + a*b
+__device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 acc) {
+    acc.x = fma(a.x, b.x, fma(-a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(a.y, b.x, acc.y));
+    return acc;
+}
+// acc + conj(a)*b
+__device__ __forceinline__ double2 cfmaj(double2 a, double2 b, double2 acc) {
+    acc.x = fma(a.x, b.x, fma(a.y, b.y, acc.x));
+    acc.y = fma(a.x, b.y, fma(-a.y, b.x, acc.y));
+    return acc;
+}
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+
+// insert zero bits at ascending positions p[0] < p[1] < ...
+template <int K>
+__device__ __forceinline__ uint64_t insert_zeros(uint64_t i, const int* p) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        uint64_t lo = i & ((uint64_t(1) << p[j]) - 1);
+        i = ((i - lo) << 1) | lo;
+    }
+    return i;
+}
+__device__ __forceinline__ uint64_t insert_zeros_dyn(uint64_t i, const int* p, int k) {
+    for (int j = 0; j < k; ++j) {
+        uint64_t lo = i & ((uint64_t(1) << p[j]) - 1);
+        i = ((i - lo) << 1) | lo;
+    }
+    return i;
+}
+template <int K>
+__device__ __forceinline__ uint64_t sub_offset(int j, const int* p) {
+    uint64_t o = 0;
+#pragma unroll
+    for (int b = 0; b < K; ++b)
+        if (j >> b & 1) o |= uint64_t(1) << p[b];
+    return o;
+}
+
+// ---------------------------------------------------------------------------
+// wavefront / block reductions (64-wide wavefronts)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+// Sum over a block of blockDim.x threads (multiple of 64); every thread gets
+// the result. `red` holds at least blockDim.x/64 doubles of LDS.
+__device__ __forceinline__ double block_sum(double v, double* red) {
+    v = wave_sum(v);
+    const int w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) red[w] = v;
+    __syncthreads();
+    double s = 0;
+    for (int i = 0; i < nw; ++i) s += red[i];
+    return s;
+}
+
+// ---------------------------------------------------------------------------
+// launch helpers
+// ---------------------------------------------------------------------------
+inline unsigned grid_for(uint64_t work, unsigned per_block) {
+    uint64_t b = (work + per_block - 1) / per_block;
+    return unsigned(b ? b : 1);
+}
+int vqc_check_positions(int nbits, int k, const int32_t* pos, int kmax);
+int vqc_check_state(const vqc_state* s, const char* what);
+
+// Reduce `nparts` per-block complex partials for each of `nterms` terms
+// (layout [term][part]) into `out` (device, nterms complex), in a fixed order.
+void vqc_reduce_partials(const double2* parts, int nterms, int nparts, double2* out, hipStream_t st);
+// Copy n complex device results to the host and wait.
+int vqc_fetch(const double2* dev, int n, double* host);

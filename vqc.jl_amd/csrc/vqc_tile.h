@@ -1,0 +1,390 @@
+// vqc_tile.h -- the LDS tile engine shared by single-circuit and batched
+// evaluation.
+//
+// A state of `nbits` index bits is cut into tiles: a tile fixes every index
+// bit outside a chosen set S of t "tile bits" and holds the 2^t amplitudes
+// that vary over S. S always contains the contiguous low bits 0..m-1 (so a
+// wavefront's 64 loads cover consecutive addresses) plus the targets of a run
+// of consecutive circuit operations; every operation whose targets lie in S
+// can then be applied to the tile in LDS, so a whole run of gates costs one
+// HBM round trip instead of one per gate. A "phase" is one such run with its
+// tile bits; a phase is executed by streaming all 2^(nbits-t) tiles through LDS.
+//
+// Micro-operations (MicroOp) are expressed in local tile coordinates:
+//   MK_APPLY   A <- M A                       (forward gate / channel sweep)
+//   MK_APPLY2  A <- M A, B <- M B, then for each generator G_q:
+//              acc[q] += Re sum conj(B) (G_q A)   (adjoint reverse of a gate)
+//   MK_APPLY2X A <- M A, B <- M2 B                (channel reverse: S^-1, S^dagger)
+//   MK_GRAD    acc += Re sum conj(B) (G A)        (density-matrix gradient terms)
+//   MK_HAM     B += (M + M^dagger) A, acc += sum conj(A) (M A)   (adjoint seed, energy)
+// Matrices are read from a device pool in which the planner has already stored
+// every variant a micro-op needs (U, U^dagger, conj(U), U^T, M + M^dagger), so
+// the inner loops are plain complex matrix-vector products.
+#pragma once
+#include "vqc_internal.h"
+
+enum { MK_APPLY = 0, MK_APPLY2 = 1, MK_APPLY2X = 2, MK_GRAD = 3, MK_HAM = 4 };
+enum { MF_CONJ = 1, MF_TRANS = 2 };
+enum { PH_FWD = 0, PH_REV = 1, PH_HAM = 2 };
+
+constexpr int kTileThreads = 256;
+constexpr int kTileWaves = kTileThreads / 64;
+constexpr int kMaxTileBits = 12;
+constexpr int kMaxAcc = 48;  // accumulator entries per phase
+
+struct MicroOp {
+    int32_t kind;
+    int32_t kb;         // local bits touched (1..5)
+    int32_t buf;        // MK_APPLY: 0 = A, 1 = B
+    int32_t pad0;
+    int32_t lpos[5];    // ascending local bit positions
+    int32_t ng;         // generators evaluated (MK_APPLY2: 0..3, MK_GRAD / MK_HAM: 1)
+    int32_t acc[3];     // accumulator index within the phase
+    int64_t mat;
+    int64_t mat2;
+    int64_t gen[3];
+};
+
+struct PhaseDesc {
+    int32_t kind;        // PH_FWD / PH_REV / PH_HAM
+    int32_t nbits;
+    int32_t t;           // tile bits
+    int32_t m;           // contiguous low bits in the tile
+    int32_t nhb;         // tile bits above the low run
+    int32_t nob;         // index bits outside the tile
+    int32_t hb[kMaxTileBits];
+    int32_t ob[40];
+    int32_t op_begin;
+    int32_t op_end;
+    int32_t nacc;        // accumulator entries used by this phase
+    int32_t row_begin;   // first partial row of those entries
+    int32_t ham_accum;   // PH_HAM: 1 = add onto B already in HBM, 0 = overwrite
+    int32_t flags;       // PF_INIT_ZERO: the phase reads |0...0> instead of srcA
+};
+enum { PF_INIT_ZERO = 1 };
+
+template <int K>
+__device__ __forceinline__ uint32_t ins32(uint32_t i, const int32_t* p) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        uint32_t lo = i & ((1u << p[j]) - 1);
+        i = ((i - lo) << 1) | lo;
+    }
+    return i;
+}
+template <int K>
+__device__ __forceinline__ uint32_t off32(int j, const int32_t* p) {
+    uint32_t o = 0;
+#pragma unroll
+    for (int b = 0; b < K; ++b)
+        if (j >> b & 1) o |= 1u << p[b];
+    return o;
+}
+
+__device__ __forceinline__ uint64_t tile_base(uint64_t tile, const PhaseDesc& p) {
+    uint64_t b = 0;
+    for (int i = 0; i < p.nob; ++i) b |= ((tile >> i) & 1) << p.ob[i];
+    return b;
+}
+__device__ __forceinline__ uint64_t tile_global(uint32_t j, const PhaseDesc& p, uint64_t base) {
+    uint64_t idx = base | (j & ((1u << p.m) - 1));
+    const uint32_t h = j >> p.m;
+    for (int i = 0; i < p.nhb; ++i) idx |= uint64_t((h >> i) & 1) << p.hb[i];
+    return idx;
+}
+
+// lane 0 of each wave adds v to its own accumulator column (no races, fixed order later)
+__device__ __forceinline__ void wave_acc(double2* gacc, int e, double re, double im) {
+    re = wave_sum(re);
+    im = wave_sum(im);
+    if ((threadIdx.x & 63) == 0) {
+        double2* p = gacc + e * kTileWaves + (threadIdx.x >> 6);
+        p->x += re;
+        p->y += im;
+    }
+}
+
+// Row loops are unrolled for up to 3 local bits; for 4 and 5 bits only the
+// inner product is unrolled (it indexes the register-held group).
+template <int KB>
+__device__ __forceinline__ void mop_apply(double2* __restrict__ s, int t, const MicroOp& op, const double2* __restrict__ pool) {
+    constexpr int D = 1 << KB;
+    constexpr int RU = KB <= 3 ? D : 1;
+    const double2* __restrict__ M = pool + op.mat;
+    const uint32_t ng = 1u << (t - KB);
+    for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
+        const uint32_t base = ins32<KB>(g, op.lpos);
+        double2 a[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) a[j] = s[base + off32<KB>(j, op.lpos)];
+#pragma unroll RU
+        for (int r = 0; r < D; ++r) {
+            double2 v = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int c = 0; c < D; ++c) v = cfma(M[r + c * D], a[c], v);
+            s[base + off32<KB>(r, op.lpos)] = v;
+        }
+    }
+}
+
+// MK_APPLY2 / MK_APPLY2X (KB <= 4) and MK_GRAD (KB <= 5): A <- M A, B <- M2 B,
+// then acc[q] += Re sum conj(B) (G_q A). New values go to LDS as soon as a row
+// is done (the group's inputs are in registers and groups are disjoint between
+// threads) and are re-read for the generator terms.
+template <int KB, bool APPLY>
+__device__ __forceinline__ void mop_pair(double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
+                                         const double2* __restrict__ pool, double2* gacc) {
+    constexpr int D = 1 << KB;
+    constexpr int RU = KB <= 3 ? D : 1;
+    const double2* __restrict__ M = pool + op.mat;
+    const double2* __restrict__ M2 = pool + op.mat2;
+    const int ngen = op.ng;
+    double gs[3] = {0.0, 0.0, 0.0};
+    const uint32_t ng = 1u << (t - KB);
+    for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
+        const uint32_t base = ins32<KB>(g, op.lpos);
+        double2 a[D], b[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            a[j] = sa[base + off32<KB>(j, op.lpos)];
+            b[j] = sb[base + off32<KB>(j, op.lpos)];
+        }
+        if constexpr (APPLY) {
+#pragma unroll RU
+            for (int r = 0; r < D; ++r) {
+                double2 s = make_double2(0.0, 0.0), u = make_double2(0.0, 0.0);
+#pragma unroll
+                for (int c = 0; c < D; ++c) {
+                    s = cfma(M[r + c * D], a[c], s);
+                    u = cfma(M2[r + c * D], b[c], u);
+                }
+                sa[base + off32<KB>(r, op.lpos)] = s;
+                sb[base + off32<KB>(r, op.lpos)] = u;
+            }
+            if (ngen) {
+#pragma unroll
+                for (int j = 0; j < D; ++j) {
+                    a[j] = sa[base + off32<KB>(j, op.lpos)];
+                    b[j] = sb[base + off32<KB>(j, op.lpos)];
+                }
+            }
+        }
+        for (int q = 0; q < ngen; ++q) {
+            const double2* __restrict__ G = pool + op.gen[q];
+            double acc = 0.0;
+#pragma unroll RU
+            for (int r = 0; r < D; ++r) {
+                double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+                for (int c = 0; c < D; ++c) s = cfma(G[r + c * D], a[c], s);
+                double2 br = b[0];
+#pragma unroll
+                for (int j = 1; j < D; ++j)
+                    if (j == r) br = b[j];
+                acc = fma(br.x, s.x, fma(br.y, s.y, acc));  // Re(conj(b_r) s)
+            }
+            gs[q] += acc;
+        }
+    }
+    for (int q = 0; q < ngen; ++q) wave_acc(gacc, op.acc[q], gs[q], 0.0);
+}
+
+// MK_HAM: B += Dm A with Dm = M + M^dagger (op.mat2); acc += sum conj(A) (M A)
+template <int KB>
+__device__ __forceinline__ void mop_ham(const double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
+                                        const double2* __restrict__ pool, double2* gacc) {
+    constexpr int D = 1 << KB;
+    constexpr int RU = KB <= 3 ? D : 1;
+    const double2* __restrict__ M = pool + op.mat;
+    const double2* __restrict__ Dm = pool + op.mat2;
+    double er = 0.0, ei = 0.0;
+    const uint32_t ng = 1u << (t - KB);
+    for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
+        const uint32_t base = ins32<KB>(g, op.lpos);
+        double2 a[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) a[j] = sa[base + off32<KB>(j, op.lpos)];
+#pragma unroll RU
+        for (int r = 0; r < D; ++r) {
+            double2 s = make_double2(0.0, 0.0), h = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                s = cfma(M[r + c * D], a[c], s);
+                h = cfma(Dm[r + c * D], a[c], h);
+            }
+            const uint32_t o = base + off32<KB>(r, op.lpos);
+            const double2 ar = sa[o];
+            er = fma(ar.x, s.x, fma(ar.y, s.y, er));
+            ei = fma(ar.x, s.y, fma(-ar.y, s.x, ei));
+            sb[o] = cadd(sb[o], h);
+        }
+    }
+    wave_acc(gacc, op.acc[0], er, ei);
+}
+
+__device__ __noinline__ void mop_apply4(double2* s, int t, const MicroOp& op, const double2* pool) { mop_apply<4>(s, t, op, pool); }
+__device__ __noinline__ void mop_apply5(double2* s, int t, const MicroOp& op, const double2* pool) { mop_apply<5>(s, t, op, pool); }
+__device__ __noinline__ void mop_pair4(double2* sa, double2* sb, int t, const MicroOp& op, const double2* pool, double2* gacc) {
+    mop_pair<4, true>(sa, sb, t, op, pool, gacc);
+}
+__device__ __noinline__ void mop_grad4(double2* sa, double2* sb, int t, const MicroOp& op, const double2* pool, double2* gacc) {
+    mop_pair<4, false>(sa, sb, t, op, pool, gacc);
+}
+__device__ __noinline__ void mop_grad5(double2* sa, double2* sb, int t, const MicroOp& op, const double2* pool, double2* gacc) {
+    mop_pair<5, false>(sa, sb, t, op, pool, gacc);
+}
+__device__ __noinline__ void mop_ham4(double2* sa, double2* sb, int t, const MicroOp& op, const double2* pool, double2* gacc) {
+    mop_ham<4>(sa, sb, t, op, pool, gacc);
+}
+__device__ __noinline__ void mop_ham5(double2* sa, double2* sb, int t, const MicroOp& op, const double2* pool, double2* gacc) {
+    mop_ham<5>(sa, sb, t, op, pool, gacc);
+}
+
+__device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const MicroOp& op, const double2* pool, double2* gacc) {
+    const int kb = op.kb;
+    switch (op.kind) {
+        case MK_APPLY: {
+            double2* s = op.buf ? sb : sa;
+            if (kb == 1) mop_apply<1>(s, t, op, pool);
+            else if (kb == 2) mop_apply<2>(s, t, op, pool);
+            else if (kb == 3) mop_apply<3>(s, t, op, pool);
+            else if (kb == 4) mop_apply4(s, t, op, pool);
+            else mop_apply5(s, t, op, pool);
+            break;
+        }
+        case MK_APPLY2:
+        case MK_APPLY2X:
+            if (kb == 1) mop_pair<1, true>(sa, sb, t, op, pool, gacc);
+            else if (kb == 2) mop_pair<2, true>(sa, sb, t, op, pool, gacc);
+            else if (kb == 3) mop_pair<3, true>(sa, sb, t, op, pool, gacc);
+            else mop_pair4(sa, sb, t, op, pool, gacc);
+            break;
+        case MK_GRAD:
+            if (kb == 1) mop_pair<1, false>(sa, sb, t, op, pool, gacc);
+            else if (kb == 2) mop_pair<2, false>(sa, sb, t, op, pool, gacc);
+            else if (kb == 3) mop_pair<3, false>(sa, sb, t, op, pool, gacc);
+            else if (kb == 4) mop_grad4(sa, sb, t, op, pool, gacc);
+            else mop_grad5(sa, sb, t, op, pool, gacc);
+            break;
+        case MK_HAM:
+            if (kb == 1) mop_ham<1>(sa, sb, t, op, pool, gacc);
+            else if (kb == 2) mop_ham<2>(sa, sb, t, op, pool, gacc);
+            else if (kb == 3) mop_ham<3>(sa, sb, t, op, pool, gacc);
+            else if (kb == 4) mop_ham4(sa, sb, t, op, pool, gacc);
+            else mop_ham5(sa, sb, t, op, pool, gacc);
+            break;
+    }
+    __syncthreads();
+}
+
+// Stream one tile of phase p through LDS.
+//  PH_FWD: sa <- srcA tile; ops; dstA tile <- sa
+//  PH_REV: sa <- A, sb <- B; ops; A <- sa, B <- sb
+//  PH_HAM: sa <- A, sb <- (ham_accum ? B : 0); ops; B <- sb
+__device__ __forceinline__ void phase_tile(const PhaseDesc& p, uint64_t tile, const double2* srcA, double2* A, double2* B,
+                                           const MicroOp* __restrict__ ops, const double2* __restrict__ pool, double2* sa,
+                                           double2* sb, double2* gacc) {
+    const uint64_t base = tile_base(tile, p);
+    const uint32_t T = 1u << p.t;
+    for (uint32_t j = threadIdx.x; j < T; j += kTileThreads) {
+        const uint64_t gi = tile_global(j, p, base);
+        sa[j] = (p.flags & PF_INIT_ZERO) ? make_double2(gi == 0 ? 1.0 : 0.0, 0.0) : srcA[gi];
+        if (p.kind == PH_REV || (p.kind == PH_HAM && p.ham_accum)) sb[j] = B[gi];
+        else if (p.kind == PH_HAM) sb[j] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int o = p.op_begin; o < p.op_end; ++o) run_mop(sa, sb, p.t, ops[o], pool, gacc);
+    for (uint32_t j = threadIdx.x; j < T; j += kTileThreads) {
+        const uint64_t gi = tile_global(j, p, base);
+        if (p.kind != PH_HAM) A[gi] = sa[j];
+        if (p.kind != PH_FWD) B[gi] = sb[j];
+    }
+    __syncthreads();
+}
+
+// Whole evaluation of a state held in one tile: sa <- src (|0...0> when src is
+// null), micro-ops [mop_begin, mop_end) with sb zeroed first; A receives the
+// state as it stands at ham_begin (the forward result). The accumulator index
+// of every micro-op is a local row; row e is written to rows[e * row_stride].
+__device__ __forceinline__ void small_eval_body(int nbits, const double2* src, double2* A, const MicroOp* __restrict__ ops,
+                                                int mop_begin, int mop_end, int ham_begin, const double2* __restrict__ pool,
+                                                double2* sa, double2* sb, double2* gacc, int nrows, double2* rows,
+                                                size_t row_stride) {
+    const uint32_t T = 1u << nbits;
+    for (int i = threadIdx.x; i < nrows * kTileWaves; i += kTileThreads) gacc[i] = make_double2(0.0, 0.0);
+    for (uint32_t j = threadIdx.x; j < T; j += kTileThreads) {
+        sa[j] = src ? src[j] : make_double2(j == 0 ? 1.0 : 0.0, 0.0);
+        sb[j] = make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    for (int o = mop_begin; o < mop_end; ++o) {
+        if (o == ham_begin && A) {
+            for (uint32_t j = threadIdx.x; j < T; j += kTileThreads) A[j] = sa[j];
+        }
+        run_mop(sa, sb, nbits, ops[o], pool, gacc);
+    }
+    if (ham_begin >= mop_end && A) {
+        for (uint32_t j = threadIdx.x; j < T; j += kTileThreads) A[j] = sa[j];
+    }
+    __syncthreads();
+    for (int e = threadIdx.x; e < nrows; e += kTileThreads) {
+        double2 s = make_double2(0.0, 0.0);
+        for (int w = 0; w < kTileWaves; ++w) s = cadd(s, gacc[e * kTileWaves + w]);
+        rows[size_t(e) * row_stride] = s;
+    }
+    __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// host-side planning
+// ---------------------------------------------------------------------------
+struct TileTuning {
+    int t_fwd = 12;     // tile bits of forward phases
+    int t_rev = 11;     // tile bits of reverse / Hamiltonian phases (two buffers)
+    int t_small = 12;   // whole state in one tile up to this many bits (both buffers)
+    int mlow = 6;       // contiguous low bits kept in every tile
+};
+TileTuning vqc_tile_tuning();
+
+struct OpBits {
+    std::vector<int> bits;  // global index bits the op touches
+};
+
+// Host copy of the device matrix pool: the caller's matrices first, then the
+// variants the micro-ops use (cached per (offset, variant)).
+struct XPoolEntry {
+    int64_t src, dst;
+    int32_t D, v;
+};
+struct XPool {
+    std::vector<double> h;  // interleaved complex
+    std::vector<std::pair<std::pair<int64_t, int>, int64_t>> cache;
+    // deferred: only offsets are assigned and each variant is recorded in
+    // `table`, to be computed on the device once the caller's matrices arrive
+    bool deferred = false;
+    int64_t len = 0;
+    std::vector<XPoolEntry> table;
+    int64_t size() const { return deferred ? len : int64_t(h.size() / 2); }
+    // variant: MF_CONJ / MF_TRANS combination, or 4 for M + M^dagger
+    int64_t variant(int64_t off, int D, int v);
+};
+
+struct PlanBuilder {
+    XPool* xp = nullptr;
+    std::vector<MicroOp> mops;
+    std::vector<PhaseDesc> phases;
+    int nrows = 0;
+    // row -> (gradient slot, or -1 for an energy row)
+    std::vector<int> row_slot;
+};
+
+// Choose tile bits S for a set of required bits; fills the contiguous low bits.
+void vqc_make_phase(PhaseDesc& p, int kind, int nbits, int t, const std::vector<int>& req);
+
+// Plan the forward phases of ops (SV when !is_dm) onto nbits with tile bits t.
+// pool offsets are used as given.
+int vqc_plan_forward(PlanBuilder& pb, int nbits, int n, int is_dm, int nops, const vqc_op_t* ops, int t, int mlow);
+// Plan H phases: MK_HAM micro-ops for each term; one accumulator (energy row) per term.
+int vqc_plan_ham(PlanBuilder& pb, int nbits, int nterms, const vqc_term_t* terms, int64_t hoff, int t, int mlow);
+// Plan the reverse (adjoint) phases.
+int vqc_plan_reverse(PlanBuilder& pb, int nbits, int n, int is_dm, int nops, const vqc_op_t* ops, int t, int mlow);

@@ -1,0 +1,11 @@
+"""vqchip: host-side mirror of VQC.jl's state / circuit / Hamiltonian API over
+libvqc_hip.so, the MI355X (gfx950) kernels behind include/vqc_hip.h."""
+from . import _lib
+from ._lib import VQCError, load
+from .api import apply_, apply_operator, energy_and_gradient, expectation
+from .circuit import Program, QCircuit
+from .gates import (AmplitudeDamping, CCPHASEGate, CNOTGate, CPHASEGate, CRxGate, CRyGate, CRzGate, CZGate,
+                    Depolarizing, FREDKINGate, FSIMGate, Gate, HGate, PHASEGate, QuantumGate, QuantumMap, RxGate,
+                    RyGate, RzGate, SGate, SWAPGate, TGate, TOFFOLIGate, XGate, YGate, ZGate, iSWAPGate)
+from .operators import QubitsOperator, QubitsTerm, pack_pauli, pack_terms
+from .state import DensityMatrix, StateVector, onehot_encoding, qubit_encoding, rand_state

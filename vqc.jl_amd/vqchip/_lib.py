@@ -1,0 +1,147 @@
+"""ctypes binding of libvqc_hip.so (include/vqc_hip.h).
+
+The library is the only compute path: if it cannot be loaded, or no gfx950
+device is present, every call raises. Nothing here falls back to the CPU.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(_PKG_DIR, "libvqc_hip.so")
+
+VQC_OK = 0
+VQC_FLAG_UNFUSED = 1
+
+GATE_X, GATE_PHASE, GATE_CZ, GATE_CNOT, GATE_CPHASE = 1, 2, 3, 4, 5
+GATE_SWAP, GATE_ISWAP, GATE_ISWAPDAG, GATE_FREDKIN, GATE_CCPHASE = 6, 7, 8, 9, 10
+OP_UNITARY, OP_CHANNEL = 0, 1
+
+
+class VQCError(RuntimeError):
+    """A libvqc_hip call failed (the reference raises a Julia exception here)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"libvqc_hip error {code}: {msg}")
+        self.code = code
+
+
+class Op(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int32),
+        ("nq", ctypes.c_int32),
+        ("pos", ctypes.c_int32 * 5),
+        ("nparams", ctypes.c_int32),
+        ("slot", ctypes.c_int32 * 3),
+        ("reserved", ctypes.c_int32),
+        ("mat", ctypes.c_int64),
+        ("aux", ctypes.c_int64),
+        ("gen", ctypes.c_int64 * 3),
+    ]
+
+
+class Term(ctypes.Structure):
+    _fields_ = [("nq", ctypes.c_int32), ("pos", ctypes.c_int32 * 5), ("mat", ctypes.c_int64)]
+
+
+class Pauli(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_uint64), ("z", ctypes.c_uint64), ("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
+class Instance(ctypes.Structure):
+    _fields_ = [
+        ("nqubits", ctypes.c_int32),
+        ("nops", ctypes.c_int32),
+        ("op_begin", ctypes.c_int64),
+        ("nterms", ctypes.c_int32),
+        ("nparams", ctypes.c_int32),
+        ("term_begin", ctypes.c_int64),
+        ("grad_begin", ctypes.c_int64),
+    ]
+
+
+_P = ctypes.c_void_p
+_I = ctypes.c_int
+_D = ctypes.POINTER(ctypes.c_double)
+_I32 = ctypes.POINTER(ctypes.c_int32)
+_U64 = ctypes.c_uint64
+
+# name -> argtypes (every function returns int)
+SIGNATURES = {
+    "vqc_version": [],
+    "vqc_device_info": [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint64)],
+    "vqc_sync": [],
+    "vqc_get_stream": [ctypes.POINTER(_P)],
+    "vqc_sv_create": [_I, ctypes.POINTER(_P)],
+    "vqc_dm_create": [_I, ctypes.POINTER(_P)],
+    "vqc_state_wrap": [_I, _I, _P, ctypes.POINTER(_P)],
+    "vqc_state_destroy": [_P],
+    "vqc_state_nbits": [_P, ctypes.POINTER(ctypes.c_int)],
+    "vqc_state_device_ptr": [_P, ctypes.POINTER(_P)],
+    "vqc_state_upload": [_P, _D, _U64],
+    "vqc_state_download": [_P, _D, _U64],
+    "vqc_state_reset": [_P],
+    "vqc_state_copy": [_P, _P],
+    "vqc_state_scale": [_P, ctypes.c_double, ctypes.c_double],
+    "vqc_state_dot": [_P, _P, _D],
+    "vqc_sv_apply_dense": [_P, _I, _I32, _D],
+    "vqc_sv_apply_named": [_P, _I, _I32, ctypes.c_double],
+    "vqc_sv_accum_dense": [_P, _P, _I, _I32, _D],
+    "vqc_sv_expec_dense": [_P, _P, _I, _I32, _D, _D],
+    "vqc_sv_expec_pauli": [_P, _P, _I, ctypes.POINTER(Pauli), _D, _D],
+    "vqc_sv_apply_pauli_sum": [_P, _P, _I, ctypes.POINTER(Pauli)],
+    "vqc_sv_apply_terms": [_P, _P, _I, ctypes.POINTER(Term), _D, _D],
+    "vqc_dm_apply_gate": [_P, _I, _I32, _D],
+    "vqc_dm_apply_superop": [_P, _I, _I32, _D, _I],
+    "vqc_dm_expec_dense": [_P, _I, _I32, _D, _D],
+    "vqc_apply_circuit": [_P, _I, ctypes.POINTER(Op), _D],
+    "vqc_sv_energy_grad": [_P, _I, ctypes.POINTER(Op), _D, _I, ctypes.POINTER(Term), _D, _I, _D, _D, _P, _I],
+    "vqc_dm_energy_grad": [_P, _I, ctypes.POINTER(Op), _D, _I, ctypes.POINTER(Term), _D, _I, _D, _D, _P, _I],
+    "vqc_batch_create": [_I, ctypes.POINTER(Instance), ctypes.POINTER(Op), ctypes.c_int64, ctypes.POINTER(Term),
+                         ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _I, ctypes.POINTER(_P)],
+    "vqc_batch_run": [_P, _D, _D, _D, _D],
+    "vqc_batch_launch": [_P, _D, _D],
+    "vqc_batch_wait": [_P, _D, _D],
+    "vqc_batch_arena_bytes": [_P, ctypes.POINTER(ctypes.c_uint64)],
+    "vqc_batch_destroy": [_P],
+}
+
+_lib = None
+
+
+def load():
+    """Load libvqc_hip.so (raises if it is missing: there is no fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise VQCError(-4, f"{LIB_PATH} is not built; run __graft_entry__.build()")
+    lib = ctypes.CDLL(LIB_PATH)
+    lib.vqc_last_error.restype = ctypes.c_char_p
+    lib.vqc_last_error.argtypes = []
+    for name, args in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = ctypes.c_int
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def call(name, *args):
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc != VQC_OK:
+        raise VQCError(rc, lib.vqc_last_error().decode(errors="replace"))
+    return rc
+
+
+def dptr(a):
+    """double* view of a contiguous float64 or complex128 numpy array."""
+    assert a.flags.c_contiguous and a.dtype in (np.float64, np.complex128)
+    return a.ctypes.data_as(_D)
+
+
+def i32ptr(seq):
+    arr = (ctypes.c_int32 * len(seq))(*[int(x) for x in seq])
+    return arr

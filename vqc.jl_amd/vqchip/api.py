@@ -1,0 +1,162 @@
+"""apply!, expectation and the adjoint gradient, over libvqc_hip.
+
+    apply_(gate | circuit | channel, state)      apply!(x, state)        generic.jl:6-31, apply_qmaps.jl:8-16
+    circuit * state                              Base.:*                  generic.jl:33
+    expectation(H, state)                        expectation(m, state)    expecs.jl:12-42, expecs_dm.jl:7-27
+    expectation(bra, H, state)                   expectation(c, m, s)     expecs.jl:45-74
+    H * state                                    (m::QubitsOperator)(v)   apply_qterms.jl:13-28
+    energy_and_gradient(H, circuit, state0)      Zygote.gradient(c -> real(expectation(H, c * state0)), c)
+                                                 through circuitdiff.jl:2-8,23-78 and expecdiff.jl:30-68
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from .circuit import Program, QCircuit
+from .gates import Gate, QuantumMap
+from .operators import LARGEST_SUPPORTED_NTERMS, QubitsOperator, QubitsTerm, pack_pauli, pack_terms
+from .state import DensityMatrix, StateVector
+
+
+def _mat_ptr(m):
+    m = np.ascontiguousarray(m, dtype=np.complex128).reshape(-1, order="F")
+    return m, _lib.dptr(np.ascontiguousarray(m))
+
+
+def _check_pos_range(pos, n):
+    if len(pos) > 5:
+        raise ValueError("only implement 5-qubit gates and less currently.")  # generic.jl:37
+    if pos[0] < 1 or pos[-1] > n:
+        raise ValueError(f"qubit positions {pos} outside 1..{n}")  # generic.jl:7 @assert
+
+
+def apply_(x, state, named=True):
+    """apply!(x, state): in place; returns the state."""
+    if isinstance(x, (list, QCircuit)):
+        if len(x) == 0:
+            return state
+        prog = Program(x, n=state.nqubits(), dm=bool(state.is_dm))
+        ops = prog.op_array()
+        pool = prog.pool()
+        _lib.call("vqc_apply_circuit", state.handle, len(prog.ops), ops, _lib.dptr(pool))
+        return state
+    if isinstance(x, QuantumMap):
+        if not state.is_dm:
+            state = DensityMatrix(state)  # apply_qmaps.jl:19
+        pos = x.ordered_positions()
+        _check_pos_range(pos, state.nqubits())
+        S = np.ascontiguousarray(x.ordered_supermat().reshape(-1, order="F"))
+        p = _lib.i32ptr([q - 1 for q in pos])
+        _lib.call("vqc_dm_apply_superop", state.handle, len(pos), p, _lib.dptr(S), 0)
+        return state
+    if not isinstance(x, Gate):
+        raise TypeError(f"cannot apply {type(x)}")
+    pos = x.ordered_positions()
+    _check_pos_range(pos, state.nqubits())
+    if state.is_dm:
+        U = np.ascontiguousarray(x.ordered_mat().reshape(-1, order="F"))
+        _lib.call("vqc_dm_apply_gate", state.handle, len(pos), _lib.i32ptr([q - 1 for q in pos]), _lib.dptr(U))
+        return state
+    if named and x.named_kind is not None:
+        given = _lib.i32ptr([q - 1 for q in x.positions])
+        phi = x.params[0] if x.params else 0.0
+        _lib.call("vqc_sv_apply_named", state.handle, x.named_kind, given, float(phi))
+        return state
+    U = np.ascontiguousarray(x.ordered_mat().reshape(-1, order="F"))
+    _lib.call("vqc_sv_apply_dense", state.handle, len(pos), _lib.i32ptr([q - 1 for q in pos]), _lib.dptr(U))
+    return state
+
+
+def _mul(circuit, state):
+    return apply_(circuit, state.copy())
+
+
+QCircuit.__mul__ = _mul
+Gate.__mul__ = _mul
+
+
+def _as_operator(m):
+    return QubitsOperator([m]) if isinstance(m, QubitsTerm) else m
+
+
+def expectation(*args):
+    """expectation(H, state) or expectation(bra, H, state); returns a complex."""
+    if len(args) == 2:
+        m, state, bra = args[0], args[1], None
+    else:
+        bra, m, state = args
+    if isinstance(m, Gate):
+        pos = m.ordered_positions()
+        M = np.ascontiguousarray(m.ordered_mat().reshape(-1, order="F"))
+        out = np.zeros(2)
+        _lib.call("vqc_sv_expec_dense", state.handle, bra.handle if bra else None, len(pos),
+                  _lib.i32ptr([q - 1 for q in pos]), _lib.dptr(M), _lib.dptr(out))
+        return complex(out[0], out[1])
+    m = _as_operator(m)
+    if state.is_dm:
+        if bra is not None:
+            raise NotImplementedError("bra-ket expectation on density matrices")
+        total = 0j
+        for key, M in m.data().items():
+            Mf = np.ascontiguousarray(M.reshape(-1, order="F"))
+            out = np.zeros(2)
+            _lib.call("vqc_dm_expec_dense", state.handle, len(key), _lib.i32ptr([q - 1 for q in key]), _lib.dptr(Mf),
+                      _lib.dptr(out))
+            total += complex(out[0], out[1])
+        return total
+    if m.largest_nterm() <= LARGEST_SUPPORTED_NTERMS:
+        total = 0j
+        for key, M in m.data().items():
+            Mf = np.ascontiguousarray(M.reshape(-1, order="F"))
+            out = np.zeros(2)
+            _lib.call("vqc_sv_expec_dense", state.handle, bra.handle if bra else None, len(key),
+                      _lib.i32ptr([q - 1 for q in key]), _lib.dptr(Mf), _lib.dptr(out))
+            total += complex(out[0], out[1])
+        return total
+    arr, nt = pack_pauli(m)
+    s2 = np.zeros(2)
+    _lib.call("vqc_sv_expec_pauli", state.handle, bra.handle if bra else None, nt, arr, None, _lib.dptr(s2))
+    return complex(s2[0], s2[1])
+
+
+def apply_operator(m, state):
+    """H * state as a new StateVector ((m::QubitsOperator)(v), apply_qterms.jl:13-28)."""
+    m = _as_operator(m)
+    out = StateVector(state.nqubits())
+    if m.largest_nterm() <= LARGEST_SUPPORTED_NTERMS:
+        terms, nt, pool = pack_terms(m)
+        _lib.call("vqc_sv_apply_terms", state.handle, out.handle, nt, terms, _lib.dptr(pool), None)
+    else:
+        arr, nt = pack_pauli(m)
+        _lib.call("vqc_sv_apply_pauli_sum", state.handle, out.handle, nt, arr)
+    return out
+
+
+def _qop_mul(m, state):
+    return apply_operator(m, state)
+
+
+QubitsOperator.__mul__ = _qop_mul
+QubitsTerm.__mul__ = _qop_mul
+
+
+def energy_and_gradient(H, circuit, state0, fused=True, return_state=False):
+    """E = expectation(H, circuit * state0) and dRe(E)/dtheta for every active
+    parameter of the circuit, in forward order (circuitdiff.jl:61-75)."""
+    H = _as_operator(H)
+    dm = bool(state0.is_dm)
+    if H.largest_nterm() > LARGEST_SUPPORTED_NTERMS:
+        raise NotImplementedError("adjoint with terms longer than 5 sites")
+    prog = Program(circuit, n=state0.nqubits(), dm=dm)
+    pool = prog.pool()
+    terms, nt, hpool = pack_terms(H)
+    grad = np.zeros(max(prog.nparams, 1))
+    e2 = np.zeros(2)
+    out = (DensityMatrix(state0.nqubits()) if dm else StateVector(state0.nqubits())) if return_state else None
+    fn = "vqc_dm_energy_grad" if dm else "vqc_sv_energy_grad"
+    _lib.call(fn, state0.handle, len(prog.ops), prog.op_array(), _lib.dptr(pool), nt, terms, _lib.dptr(hpool),
+              prog.nparams, _lib.dptr(e2), _lib.dptr(grad), out.handle if out else None,
+              0 if fused else _lib.VQC_FLAG_UNFUSED)
+    res = (complex(e2[0], e2[1]), grad[: prog.nparams])
+    return res + (out,) if return_state else res

@@ -1,0 +1,110 @@
+"""QCircuit and its packing into the C-ABI program format (vqc_op_t + matrix pool)."""
+import numpy as np
+
+from . import _lib
+from .gates import Gate, QuantumMap
+
+
+class QCircuit(list):
+    """An ordered list of gates and channels (QuantumCircuits.QCircuit)."""
+
+    def nparameters(self):
+        return sum(g.nparameters() for g in self)
+
+    def active_parameters(self):
+        out = []
+        for g in self:
+            if isinstance(g, Gate):
+                out.extend(g.active_parameters())
+        return out
+
+    def reset_parameters(self, values):
+        """reset_parameters!(circuit, values) (test/parameter.jl:39-43)."""
+        values = list(values)
+        if len(values) != self.nparameters():
+            raise ValueError("wrong number of parameters")
+        i = 0
+        for g in self:
+            n = g.nparameters()
+            if n:
+                g.reset_parameters(values[i:i + n])
+                i += n
+        return self
+
+    def adjoint(self):
+        return QCircuit(g.adjoint() for g in reversed(self))
+
+
+class Program:
+    """A circuit packed as vqc_op_t records plus one complex matrix pool.
+
+    The gradient slot of each active parameter is its index in
+    active_parameters(circuit): forward circuit order, as the reference
+    flattens gradients (circuitdiff.jl:61-75).
+    """
+
+    def __init__(self, circuit, n=None, dm=False, slot_base=0, pool=None):
+        self.ops = []
+        chunks = [] if pool is None else pool
+        self._chunks = chunks
+        self._len = sum(len(c) for c in chunks)
+        slot = slot_base
+        for g in circuit:
+            op = _lib.Op()
+            if isinstance(g, QuantumMap):
+                if not dm:
+                    raise ValueError("channels act on density matrices only (apply_qmaps.jl:19)")
+                S = g.ordered_supermat()
+                op.kind = _lib.OP_CHANNEL
+                op.mat = self._add(S)
+                op.aux = self._add(np.linalg.inv(S))
+                op.nparams = 0
+                pos = g.ordered_positions()
+            else:
+                op.kind = _lib.OP_UNITARY
+                op.mat = self._add(g.ordered_mat())
+                op.aux = -1
+                pos = g.ordered_positions()
+                gens = g.differentiate()
+                op.nparams = len(gens)
+                if len(gens) > 3:
+                    raise ValueError("at most 3 parameters per gate")
+                for j, G in enumerate(gens):
+                    op.gen[j] = self._add(G)
+                    op.slot[j] = slot
+                    slot += 1
+                for j in range(len(gens), 3):
+                    op.slot[j] = -1
+                    op.gen[j] = -1
+            if len(pos) > 5:
+                raise ValueError("only implement 5-qubit gates and less currently.")  # generic.jl:37
+            if n is not None and (pos[0] < 1 or pos[-1] > n):
+                raise ValueError(f"qubit positions {pos} outside 1..{n}")
+            op.nq = len(pos)
+            for j, p in enumerate(pos):
+                op.pos[j] = p - 1
+            if op.kind == _lib.OP_CHANNEL:
+                for j in range(3):
+                    op.slot[j] = -1
+                    op.gen[j] = -1
+            self.ops.append(op)
+        self.nparams = slot - slot_base
+        self.slot_end = slot
+
+    def _add(self, m):
+        m = np.ascontiguousarray(m, dtype=np.complex128).reshape(-1, order="F")
+        off = self._len
+        self._chunks.append(m)
+        self._len += len(m)
+        return off
+
+    def pool(self):
+        if not self._chunks:
+            return np.zeros(1, dtype=np.complex128)
+        return np.ascontiguousarray(np.concatenate(self._chunks))
+
+    def op_array(self):
+        arr = (_lib.Op * max(len(self.ops), 1))()
+        for i, o in enumerate(self.ops):
+            arr[i] = o
+        return arr
