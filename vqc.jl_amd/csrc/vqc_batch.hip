@@ -88,37 +88,49 @@ __global__ void __launch_bounds__(256) k_zero_u32(unsigned* p, int n) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
 
-// Bounded wait: the queue order makes every wait finite; the bound only turns
-// a planning bug into a reported error instead of a hang.
-constexpr long long kSpinLimit = 1ll << 26;
+// Bounded wait: the queue order makes every wait finite; the bound (5 s of the
+// 100 MHz s_memrealtime clock) only turns a planning bug into a reported error.
+constexpr uint64_t kWaitTicks = 500000000ull;
 
 __global__ void __launch_bounds__(kTileThreads) k_batch(BatchArgs a) {
     extern __shared__ double2 smem[];
     __shared__ int s_item;
     __shared__ int s_ok;
+    // Every branch of this loop is wave-uniform. The queue head and the done
+    // counters are touched by all 64 lanes of wave 0 (lane 0 adds 1, the others
+    // add 0) instead of by thread 0 alone: a lane-divergent region next to the
+    // loop's back-edge lets the compiler rotate the loop so that lane 0 (the
+    // only lane that would dequeue and signal) leaves it while the other lanes
+    // keep re-running the body with a stale item.
+    const bool wave0 = (__builtin_amdgcn_readfirstlane(threadIdx.x) >> 6) == 0;
+    const unsigned one = (threadIdx.x & 63) == 0 ? 1u : 0u;
     for (;;) {
-        if (threadIdx.x == 0) s_item = int(atomicAdd(&a.counters[0], 1u));
+        if (wave0) s_item = int(__builtin_amdgcn_readfirstlane(atomicAdd(&a.counters[0], one)));
         __syncthreads();
-        const int idx = s_item;
+        const int idx = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
         if (idx >= a.nitems) break;
         const BItem it = a.items[idx];
-        if (threadIdx.x == 0) {
+        if (wave0) {
             int ok = 1;
-            long long spins = 0;
-            while (__hip_atomic_load(&a.counters[2 + it.inst], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < unsigned(it.need)) {
-                __builtin_amdgcn_s_sleep(2);
-                if (++spins > kSpinLimit) {
+            const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            for (;;) {
+                const unsigned done = __builtin_amdgcn_readfirstlane(
+                    __hip_atomic_load(&a.counters[2 + it.inst], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+                if (done >= unsigned(it.need)) break;
+                if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
                     ok = 0;
-                    atomicOr(&a.counters[1], 1u);
+                    atomicOr(&a.counters[1], one);
                     break;
                 }
+                __builtin_amdgcn_s_sleep(2);
             }
             s_ok = ok;
         }
         __syncthreads();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-        if (s_ok) {
+        const int ok = __builtin_amdgcn_readfirstlane(s_ok);
+        if (ok) {
             const BInst in = a.inst[it.inst];
             if (it.kind == kItemSmall) {
                 const uint32_t T = 1u << in.nbits;
@@ -145,7 +157,8 @@ __global__ void __launch_bounds__(kTileThreads) k_batch(BatchArgs a) {
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
         __syncthreads();
-        if (threadIdx.x == 0) __hip_atomic_fetch_add(&a.counters[2 + it.inst], 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (wave0) __hip_atomic_fetch_add(&a.counters[2 + it.inst], one, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
     }
 }
 

@@ -518,7 +518,7 @@ static void dagger_host(const double* U, int D, double* out) {
 
 static int sv_energy_grad_plain(const vqc_state* psi0, int nops, const vqc_op_t* ops, const double* pool, int64_t plen, int nterms,
                                 const vqc_term_t* terms, const double* hpool, int nparams, double* energy2, double* grad,
-                                double2* A, double2* B, hipStream_t st) {
+                                double2* A, double2* B, double2* fwd_out, hipStream_t st) {
     const int nb = psi0->nbits;
     const uint64_t L = psi0->len();
     // device pool: [ops pool | for each term M, M + M^dagger]
@@ -553,6 +553,7 @@ static int sv_energy_grad_plain(const vqc_state* psi0, int nops, const vqc_op_t*
     // forward: y = circuit * psi0 (generic.jl:26-33)
     VQC_HIP(hipMemcpyAsync(A, psi0->d, L * 16, hipMemcpyDeviceToDevice, st));
     for (int i = 0; i < nops; ++i) VQC_TRY(vqc_dense_sweep(A, nb, ops[i].nq, ops[i].pos, pool + 2 * ops[i].mat, st));
+    if (fwd_out) VQC_HIP(hipMemcpyAsync(fwd_out, A, L * 16, hipMemcpyDeviceToDevice, st));
     // energy per key and seed Delta = (H + H^dagger) y
     int row = 0;
     unsigned grid;
@@ -608,15 +609,12 @@ extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_
     const int nb = psi0->nbits;
     const uint64_t L = psi0->len();
     VQC_TRY(ensure(g_wB, L * 16));
-    double2* A;
-    if (psi_out && psi_out->d != psi0->d) A = psi_out->d;
-    else {
-        VQC_TRY(ensure(g_wA, L * 16));
-        A = (double2*)g_wA.buf;
-    }
+    VQC_TRY(ensure(g_wA, L * 16));
+    double2* A = (double2*)g_wA.buf;  // the reverse sweep uncomputes A back to psi0
     double2* B = (double2*)g_wB.buf;
+    double2* fwd_out = psi_out ? psi_out->d : nullptr;
     if (flags & VQC_FLAG_UNFUSED) {
-        VQC_TRY(sv_energy_grad_plain(psi0, nops, ops, pool, plen, nterms, terms, hpool, nparams, energy2, grad, A, B, st));
+        VQC_TRY(sv_energy_grad_plain(psi0, nops, ops, pool, plen, nterms, terms, hpool, nparams, energy2, grad, A, B, fwd_out, st));
     } else {
         const TileTuning tt = vqc_tile_tuning();
         XPool xp;
@@ -637,7 +635,7 @@ extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_
                         for (int q = 0; q < 3; ++q) pb.mops[o].acc[q] += p.row_begin;
                 VQC_TRY(upload_program(pb, st));
                 VQC_TRY(ensure(g_wrows, size_t(std::max(pb.nrows, 1)) * 16));
-                SmallArgs a{nb, int(pb.mops.size()), ham_begin, pb.nrows, psi0->d, A, (const MicroOp*)g_wmops.buf,
+                SmallArgs a{nb, int(pb.mops.size()), ham_begin, pb.nrows, psi0->d, fwd_out, (const MicroOp*)g_wmops.buf,
                             (const double2*)g_wpool.buf, (double2*)g_wrows.buf};
                 static bool attr = false;
                 if (!attr) {
@@ -673,6 +671,7 @@ extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_
             vqc_zero(parts, size_t(nrows) * stride, st);
             if (nfwd == 0) VQC_HIP(hipMemcpyAsync(A, psi0->d, L * 16, hipMemcpyDeviceToDevice, st));
             else VQC_TRY(run_forward(pb, 0, nfwd, psi0->d, A, dm, dp, st));
+            if (fwd_out) VQC_HIP(hipMemcpyAsync(fwd_out, A, L * 16, hipMemcpyDeviceToDevice, st));
             if (nham_end == nfwd) vqc_zero(B, L, st);
             for (size_t i = nfwd; i < pb.phases.size(); ++i) {
                 PhaseBufs b{A, A, B, dm, dp, parts, stride};
@@ -749,12 +748,8 @@ extern "C" int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_
     hipStream_t st = c->stream;
     const uint64_t L = rho0->len();
     VQC_TRY(ensure(g_wB, L * 16));
-    double2* A;
-    if (rho_out && rho_out->d != rho0->d) A = rho_out->d;
-    else {
-        VQC_TRY(ensure(g_wA, L * 16));
-        A = (double2*)g_wA.buf;
-    }
+    VQC_TRY(ensure(g_wA, L * 16));
+    double2* A = (double2*)g_wA.buf;  // the reverse sweep takes A back to rho0
     double2* B = (double2*)g_wB.buf;
     const bool plain = flags & VQC_FLAG_UNFUSED;
     const TileTuning tt = vqc_tile_tuning();
@@ -814,6 +809,7 @@ extern "C" int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_
     } else {
         VQC_TRY(run_forward(pb, 0, nfwd, rho0->d, A, dm, dp, st));
     }
+    if (rho_out) VQC_HIP(hipMemcpyAsync(rho_out->d, A, L * 16, hipMemcpyDeviceToDevice, st));
     // energies tr(M rho) and the seed Delta = H^dagger
     unsigned grid;
     for (int t = 0; t < nterms; ++t)
