@@ -57,6 +57,149 @@ def unfused_sweep(vq, _lib, state, mats):
         _lib.call("vqc_sv_apply_dense", state.handle, k, pos, U)
 
 
+def _prog_args(vq, circuit, H, n, dm=False):
+    from vqchip.operators import pack_terms
+    prog = vq.Program(circuit, n=n, dm=dm)
+    terms, nt, hpool = pack_terms(H)
+    return prog, prog.op_array(), prog.pool(), terms, nt, hpool
+
+
+def _eval(vq, _lib, state, args, fused):
+    prog, ops, pool, terms, nt, hpool = args
+    g = np.zeros(max(prog.nparams, 1))
+    e = np.zeros(2)
+    fn = "vqc_dm_energy_grad" if state.is_dm else "vqc_sv_energy_grad"
+    _lib.call(fn, state.handle, len(prog.ops), ops, _lib.dptr(pool), nt, terms, _lib.dptr(hpool), prog.nparams,
+              _lib.dptr(e), _lib.dptr(g), None, 0 if fused else 1)
+    return complex(e[0], e[1]), g[:prog.nparams]
+
+
+def _time_evals(fn, min_s=1.0, max_n=200):
+    fn()  # warm
+    n, t0 = 0, time.perf_counter()
+    while n < max_n and (n < 3 or time.perf_counter() - t0 < min_s):
+        fn()
+        n += 1
+    return n / (time.perf_counter() - t0)
+
+
+def bench_config0(vq, _lib, oracle, timer):
+    """12 qubits, 4 HEA layers (144 angles), TFIM: energy + full adjoint gradient."""
+    from vqchip.workloads import config0
+    c, H, _ = config0()
+    n = 12
+    args = _prog_args(vq, c, H, n)
+    psi0 = vq.StateVector(n)
+    Ef, gf = _eval(vq, _lib, psi0, args, True)
+    Ep, gp = _eval(vq, _lib, psi0, args, False)
+    v0 = np.zeros(1 << n, dtype=complex)
+    v0[0] = 1
+    prog, ops, pool, terms, nt, hpool = args
+    Er, gr, _ = oracle.energy_grad(v0, n, 0, ops, len(prog.ops), pool, terms, nt, hpool, prog.nparams)
+    out = {
+        "workload": "12 qubits, 4 layers Rx/Ry/Rz + CNOT ladder, 144 angles, TFIM (23 keys)",
+        "fused_evals_per_s": round(_time_evals(lambda: _eval(vq, _lib, psi0, args, True)), 1),
+        "plain_evals_per_s": round(_time_evals(lambda: _eval(vq, _lib, psi0, args, False)), 1),
+        "cpu_evals_per_s": round(_time_evals(lambda: oracle.energy_grad(v0, n, 0, ops, len(prog.ops), pool, terms, nt,
+                                                                        hpool, prog.nparams)), 1),
+        "energy": Ef.real,
+        "max_abs_err_energy_vs_oracle": abs(Ef - Er),
+        "max_abs_err_grad_vs_oracle": float(np.max(np.abs(gf - gr))),
+        "max_abs_err_grad_plain_vs_oracle": float(np.max(np.abs(gp - gr))),
+    }
+    log(f"config0: {out}")
+    return out
+
+
+def bench_config2(vq, _lib, oracle, timer):
+    """256 instances, n ~ U{8..20}, depth ~ U{2..12}, Heisenberg: batched graph vs plain per-instance loop."""
+    from vqchip.batch import Batch
+    from vqchip.workloads import config2_instances
+    inst = config2_instances()
+    t0 = time.perf_counter()
+    b = Batch(inst)
+    plan_s = time.perf_counter() - t0
+    E_b, g_b = b.run()
+    reps = 5
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        b.run()
+    t_batch = (time.perf_counter() - t0) / reps
+    # plain: one circuit per call, one kernel per gate and per reduction
+    prepared = [(_prog_args(vq, c, H, n), n) for c, H, n in inst]
+    states = {}
+    t0 = time.perf_counter()
+    E_p, g_p = [], []
+    for a, n in prepared:
+        if n not in states:
+            states[n] = vq.StateVector(n)
+        E, g = _eval(vq, _lib, states[n], a, False)
+        E_p.append(E)
+        g_p.append(g)
+    t_plain = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for a, n in prepared:
+        _eval(vq, _lib, states[n], a, True)
+    t_fused = time.perf_counter() - t0
+    err_e = max(abs(x - y) for x, y in zip(E_b, E_p))
+    err_g = max(float(np.max(np.abs(x - y))) if x.size else 0.0 for x, y in zip(g_b, g_p))
+    # oracle on the small instances (n <= 14) of the batch
+    err_o, n_o, t_o, cnt = 0.0, 0, 0.0, 0
+    for i, ((a, n), (c, H, _)) in enumerate(zip(prepared, inst)):
+        if n > 14:
+            continue
+        prog, ops, pool, terms, nt, hpool = a
+        v0 = np.zeros(1 << n, dtype=complex)
+        v0[0] = 1
+        t1 = time.perf_counter()
+        Er, gr, _ = oracle.energy_grad(v0, n, 0, ops, len(prog.ops), pool, terms, nt, hpool, prog.nparams)
+        t_o += time.perf_counter() - t1
+        cnt += 1
+        err_o = max(err_o, abs(E_b[i] - Er), float(np.max(np.abs(g_b[i] - gr))) if gr.size else 0.0)
+    out = {
+        "workload": "256 instances, n~U{8..20}, depth~U{2..12}, Heisenberg J~N(0,1)",
+        "batched_evals_per_s": round(len(inst) / t_batch, 1),
+        "batched_ms": round(t_batch * 1e3, 2),
+        "plain_evals_per_s": round(len(inst) / t_plain, 1),
+        "plain_ms": round(t_plain * 1e3, 1),
+        "fused_per_instance_evals_per_s": round(len(inst) / t_fused, 1),
+        "speedup_batched_vs_plain": round(t_plain / t_batch, 2),
+        "plan_ms": round(plan_s * 1e3, 1),
+        "arena_MiB": round(b.arena_bytes() / 2**20, 1),
+        "max_abs_diff_batched_vs_plain": max(err_e, err_g),
+        "max_abs_err_vs_oracle_n_le_14": err_o,
+        "oracle_checked_instances": cnt,
+        "cpu_evals_per_s_n_le_14_sample": round(cnt / t_o, 1) if t_o else None,
+    }
+    log(f"config2: {out}")
+    return out
+
+
+def bench_config3(vq, _lib, oracle, timer):
+    """12-qubit density matrix, 6 noisy layers, sum Z_i: energy + 216-angle gradient."""
+    from vqchip.workloads import config3
+    c, H, _ = config3()
+    n = 12
+    args = _prog_args(vq, c, H, n, dm=True)
+    rho0 = vq.DensityMatrix(n)
+    Ef, gf = _eval(vq, _lib, rho0, args, True)
+    Ep, gp = _eval(vq, _lib, rho0, args, False)
+    tf = 1.0 / _time_evals(lambda: _eval(vq, _lib, rho0, args, True), min_s=1.0, max_n=20)
+    tp = 1.0 / _time_evals(lambda: _eval(vq, _lib, rho0, args, False), min_s=1.0, max_n=5)
+    out = {
+        "workload": "12-qubit DM (2^24 entries), 6 layers, depolarizing p=0.01 after each CNOT, "
+                    "amplitude damping 0.005 per qubit per layer, sum Z_i, 216 angles",
+        "fused_evals_per_s": round(1 / tf, 3),
+        "plain_evals_per_s": round(1 / tp, 3),
+        "fused_ms": round(tf * 1e3, 2),
+        "plain_ms": round(tp * 1e3, 2),
+        "energy": Ef.real,
+        "max_abs_diff_fused_vs_plain": max(abs(Ef - Ep), float(np.max(np.abs(gf - gp)))),
+    }
+    log(f"config3: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,6 +209,7 @@ def main():
     ap.add_argument("--layers", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--skip-extra", action="store_true")
+    ap.add_argument("--only", default="", help="comma list of parts to run: c0,c1,c2,c3 (default all)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -125,6 +269,16 @@ def main():
     log(f"per-class GB/s {per_class}")
 
     extra = {"norm_error": abs(norm - 1.0), "per_gate_class_GBps": per_class}
+    # ceiling reference: a plain device-to-device copy of the same 2^n amplitudes
+    other = vq.StateVector(n)
+    _lib.call("vqc_state_copy", other.handle, state.handle)
+    timer.start()
+    for _ in range(3):
+        _lib.call("vqc_state_copy", other.handle, state.handle)
+    ms_copy = timer.stop_ms() / 3
+    extra["copy_GBps"] = round(2 * 16 * (1 << n) / (ms_copy * 1e-3) / 1e9, 1)
+    del other
+    log(f"copy {extra['copy_GBps']} GB/s")
     # fused (LDS tile phases)
     prog = vq.Program(circuit, n=n)
     ops, pool = prog.op_array(), prog.pool()
@@ -150,11 +304,20 @@ def main():
         extra["pauli200_read_GBps"] = round(16 * (1 << n) * nt / (mse * 1e-3) / 1e9, 1)
         log(f"pauli energy {s2[0]:.6f} in {mse:.1f} ms")
 
-    # CPU baseline: the oracle (C++/OpenMP restatement of the reference's
-    # threaded kernels) on the host cores, on the first gates of the same circuit
+    parts = set(args.only.split(",")) if args.only else {"c0", "c2", "c3"}
     import oracle
     cores = int(os.environ.get("VQC_CPU_THREADS", min(16, os.cpu_count() or 1)))
     oracle.set_threads(cores)
+    del state  # free the 28-qubit state before the other workloads
+    if "c0" in parts:
+        extra["config0"] = bench_config0(vq, _lib, oracle, timer)
+    if "c2" in parts:
+        extra["config2"] = bench_config2(vq, _lib, oracle, timer)
+    if "c3" in parts:
+        extra["config3"] = bench_config3(vq, _lib, oracle, timer)
+
+    # CPU baseline: the oracle (C++/OpenMP restatement of the reference's
+    # threaded kernels) on the host cores, on the first gates of the same circuit
     v = np.zeros(1 << n, dtype=np.complex128)
     v[0] = 1
     done, t0 = 0, time.perf_counter()

@@ -44,8 +44,10 @@ struct BInst {
     int32_t mop_begin;   // small: micro-op range and first Hamiltonian micro-op
     int32_t mop_end;
     int32_t ham_begin;
+    int32_t rev_begin;
     int32_t row_begin;   // first global partial row
     int32_t nrows;
+    int32_t pad;
 };
 
 struct BatchArgs {
@@ -92,6 +94,7 @@ __global__ void __launch_bounds__(256) k_zero_u32(unsigned* p, int n) {
 // 100 MHz s_memrealtime clock) only turns a planning bug into a reported error.
 constexpr uint64_t kWaitTicks = 500000000ull;
 
+template <int MAXKB, int RKS>
 __global__ void __launch_bounds__(kTileThreads) k_batch(BatchArgs a) {
     extern __shared__ double2 smem[];
     __shared__ int s_item;
@@ -134,25 +137,35 @@ __global__ void __launch_bounds__(kTileThreads) k_batch(BatchArgs a) {
             const BInst in = a.inst[it.inst];
             if (it.kind == kItemSmall) {
                 const uint32_t T = 1u << in.nbits;
-                small_eval_body(in.nbits, nullptr, nullptr, a.mops, in.mop_begin, in.mop_end, in.ham_begin, a.pool, smem,
-                                smem + T, smem + 2 * T, in.nrows, a.parts + size_t(in.row_begin) * kMaxChunks, kMaxChunks);
+                small_eval_body<MAXKB, RKS>(in.nbits, nullptr, nullptr, a.mops, in.mop_begin, in.ham_begin, in.rev_begin, in.mop_end, a.pool, smem,
+                                smem + T, smem + 2 * T, smem + 2 * T + kRedDoubles2, in.nrows,
+                                a.parts + size_t(in.row_begin) * kMaxChunks, kMaxChunks);
             } else {
                 const PhaseDesc& p = a.phases[it.phase];
                 const uint32_t T = 1u << p.t;
                 double2* sa = smem;
                 double2* sb = p.kind == PH_FWD ? smem : smem + T;
-                double2* gacc = p.kind == PH_FWD ? smem + T : smem + 2 * T;
-                for (int i = threadIdx.x; i < p.nacc * kTileWaves; i += kTileThreads) gacc[i] = make_double2(0.0, 0.0);
+                double2* red = p.kind == PH_FWD ? smem + T : smem + 2 * T;
+                double2* gacc = red + kRedDoubles2;
+                for (int i = threadIdx.x; i < p.nacc; i += kTileThreads) gacc[i] = make_double2(0.0, 0.0);
                 __syncthreads();
+                AccState st;
+                acc_init(st);
                 double2* A = a.arena + in.a_off;
                 double2* B = a.arena + in.b_off;
-                for (int64_t tile = it.tile_begin; tile < it.tile_end; ++tile)
-                    phase_tile(p, uint64_t(tile), A, A, B, a.mops, a.pool, sa, sb, gacc);
-                for (int e = threadIdx.x; e < p.nacc; e += kTileThreads) {
-                    double2 s = make_double2(0.0, 0.0);
-                    for (int w = 0; w < kTileWaves; ++w) s = cadd(s, gacc[e * kTileWaves + w]);
-                    a.parts[size_t(p.row_begin + e) * kMaxChunks + it.chunk] = s;
+                if (p.kind == PH_FWD) {
+                    for (int64_t tile = it.tile_begin; tile < it.tile_end; ++tile)
+                        phase_tile<MAXKB, KS_FWD>(p, uint64_t(tile), A, A, B, a.mops, a.pool, sa, sb, red, gacc, st);
+                } else if (p.kind == PH_HAM) {
+                    for (int64_t tile = it.tile_begin; tile < it.tile_end; ++tile)
+                        phase_tile<MAXKB, KS_HAM>(p, uint64_t(tile), A, A, B, a.mops, a.pool, sa, sb, red, gacc, st);
+                } else {
+                    for (int64_t tile = it.tile_begin; tile < it.tile_end; ++tile)
+                        phase_tile<MAXKB, RKS>(p, uint64_t(tile), A, A, B, a.mops, a.pool, sa, sb, red, gacc, st);
                 }
+                acc_finish(st, red, gacc);
+                for (int e = threadIdx.x; e < p.nacc; e += kTileThreads)
+                    a.parts[size_t(p.row_begin + e) * kMaxChunks + it.chunk] = gacc[e];
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
@@ -176,6 +189,8 @@ struct vqc_batch {
     int64_t grad_total = 0;
     size_t lds = 0;
     unsigned grid = 0;
+    int maxkb = 1;
+    bool rev_simple = true;  // every reverse micro-op is MK_APPLY2
     // device
     void* d_items = nullptr;
     void* d_inst = nullptr;
@@ -197,6 +212,13 @@ struct vqc_batch {
     bool launched = false;
 };
 
+typedef void (*BatchKernel)(BatchArgs);
+static BatchKernel batch_kernel(const vqc_batch* b) {
+    if (b->maxkb <= 2) return b->rev_simple ? k_batch<2, KS_REV> : k_batch<2, KS_REVX>;
+    if (b->maxkb <= 3) return b->rev_simple ? k_batch<3, KS_REV> : k_batch<3, KS_REVX>;
+    return b->rev_simple ? k_batch<5, KS_REV> : k_batch<5, KS_REVX>;
+}
+
 static int batch_enqueue(vqc_batch* b, hipStream_t st) {
     BatchArgs a{(const BItem*)b->d_items, b->nitems, (const BInst*)b->d_inst, (const PhaseDesc*)b->d_phases,
                 (const MicroOp*)b->d_mops, b->d_pool, b->d_arena, b->d_parts, b->d_counters};
@@ -204,7 +226,7 @@ static int batch_enqueue(vqc_batch* b, hipStream_t st) {
     if (b->nvar) hipLaunchKernelGGL(k_variants, dim3(std::min(b->nvar, 4096)), dim3(256), 0, st, b->d_pool, (const VarEntry*)b->d_var, b->nvar);
     vqc_zero(b->d_parts, uint64_t(std::max(b->nrows, 1)) * kMaxChunks, st);
     hipLaunchKernelGGL(k_zero_u32, dim3(std::max(1, (b->ninst + 2 + 255) / 256)), dim3(256), 0, st, b->d_counters, b->ninst + 2);
-    hipLaunchKernelGGL(k_batch, dim3(b->grid), dim3(kTileThreads), b->lds, st, a);
+    hipLaunchKernelGGL(batch_kernel(b), dim3(b->grid), dim3(kTileThreads), b->lds, st, a);
     if (b->nrows) vqc_reduce_partials(b->d_parts, b->nrows, kMaxChunks, b->d_rows, st);
     VQC_HIP(hipMemcpyAsync(b->h_rows, b->d_rows, size_t(std::max(b->nrows, 1)) * 16 + 16, hipMemcpyDeviceToHost, st));
     VQC_HIP(hipGetLastError());
@@ -292,9 +314,10 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
             vqc_plan_forward(sp, n, n, 0, I.nops, iops, n, 0);
             const int hb = int(sp.mops.size());
             vqc_plan_ham(sp, n, I.nterms, iterms, pool_len, n, 0);
+            const int rb = int(sp.mops.size());
             vqc_plan_reverse(sp, n, n, 0, I.nops, iops, n, 0);
             const size_t T = size_t(1) << n;
-            const size_t need = (2 * T + size_t(std::max(sp.nrows, 1)) * kTileWaves) * 16;
+            const size_t need = (2 * T + kRedDoubles2 + size_t(std::max(sp.nrows, 1))) * 16;
             if (need <= 160 * 1024) {
                 small = true;
                 for (const PhaseDesc& p : sp.phases)
@@ -302,6 +325,7 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
                         for (int q = 0; q < 3; ++q) sp.mops[o].acc[q] += p.row_begin;
                 bi.mop_begin = int(pb.mops.size());
                 bi.ham_begin = bi.mop_begin + hb;
+                bi.rev_begin = bi.mop_begin + rb;
                 pb.mops.insert(pb.mops.end(), sp.mops.begin(), sp.mops.end());
                 bi.mop_end = int(pb.mops.size());
                 bi.nrows = sp.nrows;
@@ -329,13 +353,13 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
             bi.a_off = arena;
             bi.b_off = arena + (int64_t(1) << n);
             arena += int64_t(2) << n;
-            bi.mop_begin = bi.mop_end = bi.ham_begin = 0;
+            bi.mop_begin = bi.mop_end = bi.ham_begin = bi.rev_begin = 0;
             double cost = 0;
             for (size_t p = ph0; p < pb.phases.size(); ++p) {
                 const PhaseDesc& ph = pb.phases[p];
                 const double nb = ph.kind == PH_FWD ? 2 : 4;
                 cost += double(int64_t(1) << n) * (nb + 0.25 * (ph.op_end - ph.op_begin));
-                b->lds = std::max(b->lds, ((ph.kind == PH_FWD ? 1 : 2) * (size_t(1) << ph.t) + size_t(std::max(ph.nacc, 1)) * kTileWaves) * 16);
+                b->lds = std::max(b->lds, ((ph.kind == PH_FWD ? 1 : 2) * (size_t(1) << ph.t) + kRedDoubles2 + size_t(std::max(ph.nacc, 1))) * 16);
             }
             chains[i] = {int(ph0), int(pb.phases.size() - ph0), false, cost};
         }
@@ -344,6 +368,13 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
     }
     b->row_slot = pb.row_slot;
     b->nrows = pb.nrows;
+    for (const MicroOp& m : pb.mops) b->maxkb = std::max(b->maxkb, int(m.kb));
+    // reverse micro-ops: small instances [rev_begin, mop_end), large instances their PH_REV phases
+    for (const BInst& bi : binst)
+        for (int o = bi.rev_begin; o < bi.mop_end; ++o) b->rev_simple = b->rev_simple && pb.mops[o].kind == MK_APPLY2;
+    for (const PhaseDesc& ph : pb.phases)
+        if (ph.kind == PH_REV)
+            for (int o = ph.op_begin; o < ph.op_end; ++o) b->rev_simple = b->rev_simple && pb.mops[o].kind == MK_APPLY2;
 
     // ---- items in list-scheduling order ---------------------------------
     std::vector<BItem> items;
@@ -469,9 +500,10 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
 
     // persistent grid: as many workgroups as can be resident
     b->lds = std::max<size_t>(b->lds, 16 * 1024);
-    VQC_HIP(hipFuncSetAttribute((const void*)k_batch, hipFuncAttributeMaxDynamicSharedMemorySize, int(b->lds)));
+    const void* kfn = (const void*)batch_kernel(b);
+    VQC_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, int(b->lds)));
     int per_cu = 0;
-    VQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void*)k_batch, kTileThreads, b->lds));
+    VQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kTileThreads, b->lds));
     if (per_cu < 1) {
         batch_free(b);
         return vqc_fail(VQC_EUNSUPPORTED, "vqc_batch_create: %zu bytes of LDS per workgroup do not fit", b->lds);

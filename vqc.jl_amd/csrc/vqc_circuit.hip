@@ -323,22 +323,24 @@ struct PhaseBufs {
     int stride;
 };
 
-__global__ void __launch_bounds__(kTileThreads) k_phase(PhaseDesc p, PhaseBufs b) {
+template <int MAXKB, int KS>
+__global__ void __launch_bounds__(kTileThreads) k_phase(const PhaseDesc* __restrict__ pd, PhaseBufs b) {
     extern __shared__ double2 smem[];
+    const PhaseDesc& p = *pd;
     const uint32_t T = 1u << p.t;
     double2* sa = smem;
     double2* sb = p.kind == PH_FWD ? smem : smem + T;
-    double2* gacc = (p.kind == PH_FWD ? smem + T : smem + 2 * T);
-    for (int i = threadIdx.x; i < p.nacc * kTileWaves; i += kTileThreads) gacc[i] = make_double2(0.0, 0.0);
+    double2* red = p.kind == PH_FWD ? smem + T : smem + 2 * T;
+    double2* gacc = red + kRedDoubles2;
+    for (int i = threadIdx.x; i < p.nacc; i += kTileThreads) gacc[i] = make_double2(0.0, 0.0);
     __syncthreads();
+    AccState st;
+    acc_init(st);
     const uint64_t ntiles = uint64_t(1) << (p.nbits - p.t);
     for (uint64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x)
-        phase_tile(p, tile, b.srcA, b.A, b.B, b.ops, b.pool, sa, sb, gacc);
-    for (int e = threadIdx.x; e < p.nacc; e += kTileThreads) {
-        double2 s = make_double2(0.0, 0.0);
-        for (int w = 0; w < kTileWaves; ++w) s = cadd(s, gacc[e * kTileWaves + w]);
-        b.parts[size_t(p.row_begin + e) * b.stride + blockIdx.x] = s;
-    }
+        phase_tile<MAXKB, KS>(p, tile, b.srcA, b.A, b.B, b.ops, b.pool, sa, sb, red, gacc, st);
+    acc_finish(st, red, gacc);
+    for (int e = threadIdx.x; e < p.nacc; e += kTileThreads) b.parts[size_t(p.row_begin + e) * b.stride + blockIdx.x] = gacc[e];
 }
 
 // Whole evaluation of a state that fits one tile: forward, Hamiltonian, reverse.
@@ -346,7 +348,8 @@ __global__ void __launch_bounds__(kTileThreads) k_phase(PhaseDesc p, PhaseBufs b
 struct SmallArgs {
     int nbits;
     int nops;           // micro-ops
-    int ham_begin;      // first MK_HAM micro-op (B is zeroed before it)
+    int ham_begin;      // first MK_HAM micro-op
+    int rev_begin;      // first reverse micro-op
     int nrows;
     const double2* src; // psi0
     double2* A;         // final state (may be null)
@@ -355,11 +358,12 @@ struct SmallArgs {
     double2* rows;      // [nrows] row sums (one column)
 };
 
+template <int MAXKB, int RKS>
 __global__ void __launch_bounds__(kTileThreads) k_small_eval(SmallArgs a) {
     extern __shared__ double2 smem[];
     const uint32_t T = 1u << a.nbits;
-    small_eval_body(a.nbits, a.src, a.A, a.ops, 0, a.nops, a.ham_begin, a.pool, smem, smem + T, smem + 2 * T, a.nrows,
-                    a.rows, 1);
+    small_eval_body<MAXKB, RKS>(a.nbits, a.src, a.A, a.ops, 0, a.ham_begin, a.rev_begin, a.nops, a.pool, smem, smem + T, smem + 2 * T,
+                    smem + 2 * T + kRedDoubles2, a.nrows, a.rows, 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -369,7 +373,7 @@ struct Work {
     void* buf = nullptr;
     size_t cap = 0;
 };
-static Work g_wA, g_wB, g_wparts, g_wpool, g_wmops, g_wrows;
+static Work g_wA, g_wB, g_wparts, g_wpool, g_wmops, g_wrows, g_wphases;
 
 static int ensure(Work& w, size_t bytes) {
     if (bytes <= w.cap) return VQC_OK;
@@ -428,18 +432,55 @@ static int check_terms(int nq, int nterms, const vqc_term_t* terms, int64_t* len
 
 static size_t phase_lds(const PhaseDesc& p) {
     const size_t T = size_t(1) << p.t;
-    return (p.kind == PH_FWD ? T : 2 * T) * 16 + size_t(std::max(p.nacc, 1)) * kTileWaves * 16;
+    return ((p.kind == PH_FWD ? T : 2 * T) + kRedDoubles2 + size_t(std::max(p.nacc, 1))) * 16;
 }
 
-static int launch_phase(const PhaseDesc& p, const PhaseBufs& b, unsigned grid, hipStream_t st) {
+int vqc_max_kb(const std::vector<MicroOp>& mops, int begin, int end) {
+    int m = 1;
+    for (int o = begin; o < end; ++o) m = std::max(m, int(mops[o].kb));
+    return m;
+}
+
+#define VQC_FOR_KERNELS(X)                                                                                     \
+    X(2, KS_FWD) X(2, KS_HAM) X(2, KS_REV) X(2, KS_REVX) X(3, KS_FWD) X(3, KS_HAM) X(3, KS_REV) X(3, KS_REVX) \
+    X(5, KS_FWD) X(5, KS_HAM) X(5, KS_REV) X(5, KS_REVX)
+
+static int set_lds_attrs() {
+    static bool done = false;
+    if (done) return VQC_OK;
+#define VQC_ATTR(M, K) VQC_HIP(hipFuncSetAttribute((const void*)k_phase<M, K>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    VQC_FOR_KERNELS(VQC_ATTR)
+#undef VQC_ATTR
+#define VQC_ATTR2(M, K) VQC_HIP(hipFuncSetAttribute((const void*)k_small_eval<M, K>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+    VQC_ATTR2(2, KS_REV) VQC_ATTR2(2, KS_REVX) VQC_ATTR2(3, KS_REV) VQC_ATTR2(3, KS_REVX) VQC_ATTR2(5, KS_REV)
+    VQC_ATTR2(5, KS_REVX)
+#undef VQC_ATTR2
+    done = true;
+    return VQC_OK;
+}
+
+// kind set of the micro-ops of phase i
+int vqc_phase_kindset(const PlanBuilder& pb, size_t i) {
+    const PhaseDesc& p = pb.phases[i];
+    if (p.kind == PH_FWD) return KS_FWD;
+    if (p.kind == PH_HAM) return KS_HAM;
+    for (int o = p.op_begin; o < p.op_end; ++o)
+        if (pb.mops[o].kind != MK_APPLY2) return KS_REVX;
+    return KS_REV;
+}
+
+// dpd: device copy of pb.phases
+static int launch_phase(const PlanBuilder& pb, size_t i, const PhaseDesc* dpd, const PhaseBufs& b, unsigned grid, hipStream_t st) {
+    const PhaseDesc& p = pb.phases[i];
     const size_t lds = phase_lds(p);
-    static bool attr = false;
-    if (!attr) {
-        VQC_HIP(hipFuncSetAttribute((const void*)k_phase, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        VQC_HIP(hipFuncSetAttribute((const void*)k_small_eval, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-        attr = true;
-    }
-    hipLaunchKernelGGL(k_phase, dim3(grid), dim3(kTileThreads), lds, st, p, b);
+    VQC_TRY(set_lds_attrs());
+    const int mkb = vqc_max_kb(pb.mops, p.op_begin, p.op_end);
+    const int mk = mkb <= 2 ? 2 : (mkb <= 3 ? 3 : 5);
+    const int ks = vqc_phase_kindset(pb, i);
+#define VQC_LAUNCH(M, K) \
+    if (mk == M && ks == K) hipLaunchKernelGGL((k_phase<M, K>), dim3(grid), dim3(kTileThreads), lds, st, dpd + i, b);
+    VQC_FOR_KERNELS(VQC_LAUNCH)
+#undef VQC_LAUNCH
     VQC_HIP(hipGetLastError());
     return VQC_OK;
 }
@@ -452,10 +493,10 @@ static unsigned phase_grid(const PhaseDesc& p, unsigned max_acc_grid) {
 
 // Run planned FWD phases from src into A.
 static int run_forward(const PlanBuilder& pb, size_t ph_begin, size_t ph_end, const double2* src, double2* A,
-                       const MicroOp* dmops, const double2* dpool, hipStream_t st) {
+                       const MicroOp* dmops, const double2* dpool, const PhaseDesc* dpd, hipStream_t st) {
     for (size_t i = ph_begin; i < ph_end; ++i) {
         PhaseBufs b{i == ph_begin ? src : A, A, nullptr, dmops, dpool, nullptr, 0};
-        VQC_TRY(launch_phase(pb.phases[i], b, phase_grid(pb.phases[i], 1), st));
+        VQC_TRY(launch_phase(pb, i, dpd, b, phase_grid(pb.phases[i], 1), st));
     }
     return VQC_OK;
 }
@@ -467,6 +508,9 @@ static int upload_program(const PlanBuilder& pb, hipStream_t st) {
     if (!h.empty()) VQC_HIP(hipMemcpyAsync(g_wpool.buf, h.data(), h.size() * 8, hipMemcpyHostToDevice, st));
     if (!pb.mops.empty())
         VQC_HIP(hipMemcpyAsync(g_wmops.buf, pb.mops.data(), pb.mops.size() * sizeof(MicroOp), hipMemcpyHostToDevice, st));
+    VQC_TRY(ensure(g_wphases, std::max<size_t>(pb.phases.size(), 1) * sizeof(PhaseDesc)));
+    if (!pb.phases.empty())
+        VQC_HIP(hipMemcpyAsync(g_wphases.buf, pb.phases.data(), pb.phases.size() * sizeof(PhaseDesc), hipMemcpyHostToDevice, st));
     return VQC_OK;
 }
 
@@ -485,7 +529,8 @@ extern "C" int vqc_apply_circuit(vqc_state* s, int nops, const vqc_op_t* ops, co
     pb.xp = &xp;
     VQC_TRY(vqc_plan_forward(pb, s->nbits, s->nqubits, s->is_dm, nops, ops, tt.t_fwd, tt.mlow));
     VQC_TRY(upload_program(pb, c->stream));
-    return run_forward(pb, 0, pb.phases.size(), s->d, s->d, (const MicroOp*)g_wmops.buf, (const double2*)g_wpool.buf, c->stream);
+    return run_forward(pb, 0, pb.phases.size(), s->d, s->d, (const MicroOp*)g_wmops.buf, (const double2*)g_wpool.buf,
+                       (const PhaseDesc*)g_wphases.buf, c->stream);
 }
 
 // Sum rows into energy and gradient (host, fixed order).
@@ -588,8 +633,8 @@ static int sv_energy_grad_plain(const vqc_state* psi0, int nops, const vqc_op_t*
 // --------------------------- fused state-vector path ------------------------
 static int max_small_rows(int nbits) {
     const size_t T = size_t(1) << nbits;
-    const size_t avail = 160 * 1024 - 2 * T * 16;
-    return int(avail / (kTileWaves * 16));
+    const size_t used = (2 * T + kRedDoubles2) * 16;
+    return used >= 160 * 1024 ? 0 : int((160 * 1024 - used) / 16);
 }
 
 extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_t* ops, const double* pool, int nterms,
@@ -627,6 +672,7 @@ extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_
             VQC_TRY(vqc_plan_forward(pb, nb, nb, 0, nops, ops, nb, 0));
             const int ham_begin = int(pb.mops.size());
             VQC_TRY(vqc_plan_ham(pb, nb, nterms, terms, plen, nb, 0));
+            const int rev_begin = int(pb.mops.size());
             VQC_TRY(vqc_plan_reverse(pb, nb, nb, 0, nops, ops, nb, 0));
             if (pb.nrows <= max_small_rows(nb)) {
                 // accumulator index := global row
@@ -635,15 +681,20 @@ extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_
                         for (int q = 0; q < 3; ++q) pb.mops[o].acc[q] += p.row_begin;
                 VQC_TRY(upload_program(pb, st));
                 VQC_TRY(ensure(g_wrows, size_t(std::max(pb.nrows, 1)) * 16));
-                SmallArgs a{nb, int(pb.mops.size()), ham_begin, pb.nrows, psi0->d, fwd_out, (const MicroOp*)g_wmops.buf,
+                SmallArgs a{nb, int(pb.mops.size()), ham_begin, rev_begin, pb.nrows, psi0->d, fwd_out, (const MicroOp*)g_wmops.buf,
                             (const double2*)g_wpool.buf, (double2*)g_wrows.buf};
-                static bool attr = false;
-                if (!attr) {
-                    VQC_HIP(hipFuncSetAttribute((const void*)k_small_eval, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-                    attr = true;
-                }
-                const size_t lds = (2 * L + size_t(std::max(pb.nrows, 1)) * kTileWaves) * 16;
-                hipLaunchKernelGGL(k_small_eval, dim3(1), dim3(kTileThreads), lds, st, a);
+                VQC_TRY(set_lds_attrs());
+                const size_t lds = (2 * L + kRedDoubles2 + size_t(std::max(pb.nrows, 1))) * 16;
+                const int mkb = vqc_max_kb(pb.mops, 0, int(pb.mops.size()));
+                const int mk = mkb <= 2 ? 2 : (mkb <= 3 ? 3 : 5);
+                bool rev_simple = true;
+                for (size_t o = size_t(rev_begin); o < pb.mops.size(); ++o) rev_simple &= pb.mops[o].kind == MK_APPLY2;
+                const int ks = rev_simple ? KS_REV : KS_REVX;
+#define VQC_SMALL(M, K) \
+    if (mk == M && ks == K) hipLaunchKernelGGL((k_small_eval<M, K>), dim3(1), dim3(kTileThreads), lds, st, a);
+                VQC_SMALL(2, KS_REV) VQC_SMALL(2, KS_REVX) VQC_SMALL(3, KS_REV) VQC_SMALL(3, KS_REVX) VQC_SMALL(5, KS_REV)
+                VQC_SMALL(5, KS_REVX)
+#undef VQC_SMALL
                 VQC_HIP(hipGetLastError());
                 std::vector<double> rows(2 * size_t(std::max(pb.nrows, 1)));
                 if (pb.nrows) VQC_TRY(vqc_fetch((double2*)g_wrows.buf, pb.nrows, rows.data()));
@@ -670,12 +721,12 @@ extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_
             double2* rowsum = parts + size_t(nrows) * stride;
             vqc_zero(parts, size_t(nrows) * stride, st);
             if (nfwd == 0) VQC_HIP(hipMemcpyAsync(A, psi0->d, L * 16, hipMemcpyDeviceToDevice, st));
-            else VQC_TRY(run_forward(pb, 0, nfwd, psi0->d, A, dm, dp, st));
+            else VQC_TRY(run_forward(pb, 0, nfwd, psi0->d, A, dm, dp, (const PhaseDesc*)g_wphases.buf, st));
             if (fwd_out) VQC_HIP(hipMemcpyAsync(fwd_out, A, L * 16, hipMemcpyDeviceToDevice, st));
             if (nham_end == nfwd) vqc_zero(B, L, st);
             for (size_t i = nfwd; i < pb.phases.size(); ++i) {
                 PhaseBufs b{A, A, B, dm, dp, parts, stride};
-                VQC_TRY(launch_phase(pb.phases[i], b, phase_grid(pb.phases[i], stride), st));
+                VQC_TRY(launch_phase(pb, i, (const PhaseDesc*)g_wphases.buf, b, phase_grid(pb.phases[i], stride), st));
             }
             if (pb.nrows) vqc_reduce_partials(parts, pb.nrows, stride, rowsum, st);
             VQC_HIP(hipGetLastError());
@@ -807,7 +858,7 @@ extern "C" int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_
     } else if (nfwd == 0) {
         VQC_HIP(hipMemcpyAsync(A, rho0->d, L * 16, hipMemcpyDeviceToDevice, st));
     } else {
-        VQC_TRY(run_forward(pb, 0, nfwd, rho0->d, A, dm, dp, st));
+        VQC_TRY(run_forward(pb, 0, nfwd, rho0->d, A, dm, dp, (const PhaseDesc*)g_wphases.buf, st));
     }
     if (rho_out) VQC_HIP(hipMemcpyAsync(rho_out->d, A, L * 16, hipMemcpyDeviceToDevice, st));
     // energies tr(M rho) and the seed Delta = H^dagger
@@ -855,7 +906,7 @@ extern "C" int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_
     } else {
         for (size_t i = nfwd; i < pb.phases.size(); ++i) {
             PhaseBufs b{A, A, B, dm, dp, parts, stride};
-            VQC_TRY(launch_phase(pb.phases[i], b, phase_grid(pb.phases[i], stride / 2), st));
+            VQC_TRY(launch_phase(pb, i, (const PhaseDesc*)g_wphases.buf, b, phase_grid(pb.phases[i], stride / 2), st));
         }
     }
     if (pb.nrows) vqc_reduce_partials(parts, pb.nrows, stride, rowsum, st);

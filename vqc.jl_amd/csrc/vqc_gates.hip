@@ -9,6 +9,8 @@
 // access whenever the lowest target is >= 6, and stays cache-line complete
 // below that. Each amplitude is read and written exactly once (32 B per
 // amplitude of HBM traffic), which is the roofline of an unfused gate.
+#include <cstdlib>
+
 #include "vqc_internal.h"
 
 template <int K>
@@ -22,10 +24,34 @@ struct GMat {
 
 constexpr int kBlock = 256;
 
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+template <bool NT>
+__device__ __forceinline__ double2 ld(const double2* p) {
+    if constexpr (NT) {
+        const d2v v = __builtin_nontemporal_load(reinterpret_cast<const d2v*>(p));
+        return make_double2(v.x, v.y);
+    } else {
+        return *p;
+    }
+}
+template <bool NT>
+__device__ __forceinline__ void st(double2* p, double2 x) {
+    if constexpr (NT) {
+        d2v v;
+        v.x = x.x;
+        v.y = x.y;
+        __builtin_nontemporal_store(v, reinterpret_cast<d2v*>(p));
+    } else {
+        *p = x;
+    }
+}
+
 // K <= 3: the matrix travels in the kernel arguments (<= 1 KiB, read through
 // the scalar cache). Each thread applies the gate to R groups, issuing all
-// R * 2^K loads before the first multiply.
-template <int K, int R>
+// R * 2^K loads before the first multiply; NT marks the streaming accesses
+// non-temporal (each amplitude is touched once per sweep).
+template <int K, int R, bool NT>
 __global__ void __launch_bounds__(kBlock) k_dense_small(double2* __restrict__ v, uint64_t ngroups, GPos<K> gp, GMat<K> U) {
     constexpr int D = 1 << K;
     uint64_t off[D];
@@ -40,7 +66,7 @@ __global__ void __launch_bounds__(kBlock) k_dense_small(double2* __restrict__ v,
         base[r] = insert_zeros<K>(g, gp.p);
         if (g < ngroups) {
 #pragma unroll
-            for (int j = 0; j < D; ++j) a[r][j] = v[base[r] + off[j]];
+            for (int j = 0; j < D; ++j) a[r][j] = ld<NT>(v + base[r] + off[j]);
         }
     }
 #pragma unroll
@@ -51,7 +77,7 @@ __global__ void __launch_bounds__(kBlock) k_dense_small(double2* __restrict__ v,
             double2 s = make_double2(0.0, 0.0);
 #pragma unroll
             for (int c = 0; c < D; ++c) s = cfma(U.m[row + c * D], a[r][c], s);
-            v[base[r] + off[row]] = s;
+            st<NT>(v + base[r] + off[row], s);
         }
     }
 }
@@ -94,8 +120,25 @@ static int launch_dense(double2* v, int nbits, const int32_t* pos, const double*
     if constexpr (K <= 3) {
         GMat<K> U;
         for (int i = 0; i < D * D; ++i) U.m[i] = make_double2(Uh[2 * i], Uh[2 * i + 1]);
-        constexpr int R = K == 1 ? 4 : (K == 2 ? 2 : 1);
-        hipLaunchKernelGGL((k_dense_small<K, R>), dim3(grid_for(ngroups, R * kBlock)), dim3(kBlock), 0, st, v, ngroups, gp, U);
+        // Groups per thread and non-temporal hints. Measured on MI355X at 28
+        // qubits: one group per thread is best for every K; non-temporal
+        // accesses help the 2- and 4-stream sweeps (K = 1, 2: 5.9 / 5.6 TB/s)
+        // and hurt the 8-stream K = 3 sweep. VQC_DENSE_R / VQC_DENSE_NT override.
+        static const int r_env = getenv("VQC_DENSE_R") ? atoi(getenv("VQC_DENSE_R")) : 0;
+        static const int nt_env = getenv("VQC_DENSE_NT") ? atoi(getenv("VQC_DENSE_NT")) : -1;
+        const int R = r_env > 0 ? r_env : 1;
+        const bool NT = nt_env >= 0 ? nt_env != 0 : K <= 2;
+        const dim3 blk(kBlock);
+#define VQC_DENSE_LAUNCH(RR)                                                                                             \
+    if (R == RR) {                                                                                                      \
+        if (NT) hipLaunchKernelGGL((k_dense_small<K, RR, true>), dim3(grid_for(ngroups, RR * kBlock)), blk, 0, st, v,     \
+                                   ngroups, gp, U);                                                                       \
+        else hipLaunchKernelGGL((k_dense_small<K, RR, false>), dim3(grid_for(ngroups, RR * kBlock)), blk, 0, st, v,      \
+                                ngroups, gp, U);                                                                          \
+    }
+        VQC_DENSE_LAUNCH(1) else VQC_DENSE_LAUNCH(2) else VQC_DENSE_LAUNCH(4) else VQC_DENSE_LAUNCH(8) else return vqc_fail(
+            VQC_EINVAL, "VQC_DENSE_R must be 1, 2, 4 or 8");
+#undef VQC_DENSE_LAUNCH
     } else {
         VQC_TRY(vqc_upload_buffer(&g_mat_dev, &g_mat_cap, Uh, size_t(D) * D * 16));
         hipLaunchKernelGGL((k_dense_big<K>), dim3(grid_for(ngroups, kBlock)), dim3(kBlock), 0, st, v, ngroups, gp,

@@ -26,9 +26,17 @@
 enum { MK_APPLY = 0, MK_APPLY2 = 1, MK_APPLY2X = 2, MK_GRAD = 3, MK_HAM = 4 };
 enum { MF_CONJ = 1, MF_TRANS = 2 };
 enum { PH_FWD = 0, PH_REV = 1, PH_HAM = 2 };
+// kind sets a kernel is built for (bit k = micro-op kind k)
+enum {
+    KS_FWD = 1 << MK_APPLY,
+    KS_HAM = 1 << MK_HAM,
+    KS_REV = 1 << MK_APPLY2,                                                         // state-vector reverse
+    KS_REVX = (1 << MK_APPLY) | (1 << MK_APPLY2) | (1 << MK_APPLY2X) | (1 << MK_GRAD),  // density matrices, 5-bit ops
+};
 
-constexpr int kTileThreads = 256;
+constexpr int kTileThreads = 512;
 constexpr int kTileWaves = kTileThreads / 64;
+constexpr int kLoadBatch = 8;      // global loads in flight per thread when a tile is read
 constexpr int kMaxTileBits = 12;
 constexpr int kMaxAcc = 48;  // accumulator entries per phase
 
@@ -93,15 +101,54 @@ __device__ __forceinline__ uint64_t tile_global(uint32_t j, const PhaseDesc& p, 
     return idx;
 }
 
-// lane 0 of each wave adds v to its own accumulator column (no races, fixed order later)
-__device__ __forceinline__ void wave_acc(double2* gacc, int e, double re, double im) {
+// Reductions of the gradient / energy terms. A micro-op leaves one partial sum
+// per wave in red[buf][q][wave] (a plain store by lane 0 of each wave); at the
+// start of the NEXT micro-op, wave 0 adds those partials, in wave order, onto
+// gacc[row]. Two red buffers alternate, so no extra barrier is needed and the
+// sums are the same on every run.
+constexpr int kRedDoubles2 = 2 * 3 * kTileWaves;  // double2 entries of red[]
+
+struct AccState {
+    int pend_n;      // partials waiting in red[pend_buf] (0..3)
+    int pend_buf;
+    int e0, e1, e2;  // accumulator rows they belong to
+    int cur_buf;     // red buffer the next micro-op writes
+};
+
+__device__ __forceinline__ void acc_init(AccState& st) {
+    st.pend_n = 0;
+    st.pend_buf = 0;
+    st.cur_buf = 0;
+    st.e0 = st.e1 = st.e2 = 0;
+}
+
+__device__ __forceinline__ void wave_partial(double2* red, int buf, int q, double re, double im) {
     re = wave_sum(re);
     im = wave_sum(im);
-    if ((threadIdx.x & 63) == 0) {
-        double2* p = gacc + e * kTileWaves + (threadIdx.x >> 6);
-        p->x += re;
-        p->y += im;
+    if ((threadIdx.x & 63) == 0) red[(buf * 3 + q) * kTileWaves + (threadIdx.x >> 6)] = make_double2(re, im);
+}
+
+// fold pending partials into gacc (wave 0 only; callers order it with barriers)
+__device__ __forceinline__ void acc_fold(AccState& st, const double2* red, double2* gacc) {
+    if (st.pend_n && threadIdx.x < uint32_t(st.pend_n)) {
+        const int q = threadIdx.x;
+        const int e = q == 0 ? st.e0 : (q == 1 ? st.e1 : st.e2);
+        double2 s = gacc[e];
+        for (int w = 0; w < kTileWaves; ++w) s = cadd(s, red[(st.pend_buf * 3 + q) * kTileWaves + w]);
+        gacc[e] = s;
     }
+    st.pend_n = 0;
+}
+
+// after a micro-op that wrote n partials into red[st.cur_buf]
+__device__ __forceinline__ void acc_push(AccState& st, int n, const int32_t* e) {
+    if (!n) return;
+    st.pend_n = n;
+    st.pend_buf = st.cur_buf;
+    st.e0 = e[0];
+    st.e1 = n > 1 ? e[1] : 0;
+    st.e2 = n > 2 ? e[2] : 0;
+    st.cur_buf ^= 1;
 }
 
 // Row loops are unrolled for up to 3 local bits; for 4 and 5 bits only the
@@ -130,16 +177,21 @@ __device__ __forceinline__ void mop_apply(double2* __restrict__ s, int t, const 
 // MK_APPLY2 / MK_APPLY2X (KB <= 4) and MK_GRAD (KB <= 5): A <- M A, B <- M2 B,
 // then acc[q] += Re sum conj(B) (G_q A). New values go to LDS as soon as a row
 // is done (the group's inputs are in registers and groups are disjoint between
-// threads) and are re-read for the generator terms.
-template <int KB, bool APPLY>
+// threads) and are re-read for the generator terms. MK_APPLY2X uses a second
+// matrix (mat2) for B (channels), MK_GRAD applies nothing. Row
+// loops of 2 and more bits stay rolled: the matrix rows are read through the
+// scalar cache one at a time instead of all being hoisted into registers.
+template <int KB>
 __device__ __forceinline__ void mop_pair(double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
-                                         const double2* __restrict__ pool, double2* gacc) {
+                                         const double2* __restrict__ pool, double2* red, int buf) {
     constexpr int D = 1 << KB;
-    constexpr int RU = KB <= 3 ? D : 1;
+    constexpr int RU = KB == 1 ? D : 1;  // unrolled rows would hoist every matrix out of the group loop
+    const bool APPLY = op.kind != MK_GRAD;
+    const bool X = op.kind == MK_APPLY2X;
     const double2* __restrict__ M = pool + op.mat;
-    const double2* __restrict__ M2 = pool + op.mat2;
+    const double2* __restrict__ M2 = X ? pool + op.mat2 : M;
     const int ngen = op.ng;
-    double gs[3] = {0.0, 0.0, 0.0};
+    double gs0 = 0.0, gs1 = 0.0, gs2 = 0.0;
     const uint32_t ng = 1u << (t - KB);
     for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
         const uint32_t base = ins32<KB>(g, op.lpos);
@@ -149,7 +201,7 @@ __device__ __forceinline__ void mop_pair(double2* __restrict__ sa, double2* __re
             a[j] = sa[base + off32<KB>(j, op.lpos)];
             b[j] = sb[base + off32<KB>(j, op.lpos)];
         }
-        if constexpr (APPLY) {
+        if (APPLY) {
 #pragma unroll RU
             for (int r = 0; r < D; ++r) {
                 double2 s = make_double2(0.0, 0.0), u = make_double2(0.0, 0.0);
@@ -169,7 +221,9 @@ __device__ __forceinline__ void mop_pair(double2* __restrict__ sa, double2* __re
                 }
             }
         }
-        for (int q = 0; q < ngen; ++q) {
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            if (q >= ngen) break;
             const double2* __restrict__ G = pool + op.gen[q];
             double acc = 0.0;
 #pragma unroll RU
@@ -183,18 +237,22 @@ __device__ __forceinline__ void mop_pair(double2* __restrict__ sa, double2* __re
                     if (j == r) br = b[j];
                 acc = fma(br.x, s.x, fma(br.y, s.y, acc));  // Re(conj(b_r) s)
             }
-            gs[q] += acc;
+            if (q == 0) gs0 += acc;
+            else if (q == 1) gs1 += acc;
+            else gs2 += acc;
         }
     }
-    for (int q = 0; q < ngen; ++q) wave_acc(gacc, op.acc[q], gs[q], 0.0);
+    if (ngen > 0) wave_partial(red, buf, 0, gs0, 0.0);
+    if (ngen > 1) wave_partial(red, buf, 1, gs1, 0.0);
+    if (ngen > 2) wave_partial(red, buf, 2, gs2, 0.0);
 }
 
 // MK_HAM: B += Dm A with Dm = M + M^dagger (op.mat2); acc += sum conj(A) (M A)
 template <int KB>
 __device__ __forceinline__ void mop_ham(const double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
-                                        const double2* __restrict__ pool, double2* gacc) {
+                                        const double2* __restrict__ pool, double2* red, int buf) {
     constexpr int D = 1 << KB;
-    constexpr int RU = KB <= 3 ? D : 1;
+    constexpr int RU = KB == 1 ? D : 1;
     const double2* __restrict__ M = pool + op.mat;
     const double2* __restrict__ Dm = pool + op.mat2;
     double er = 0.0, ei = 0.0;
@@ -219,61 +277,73 @@ __device__ __forceinline__ void mop_ham(const double2* __restrict__ sa, double2*
             sb[o] = cadd(sb[o], h);
         }
     }
-    wave_acc(gacc, op.acc[0], er, ei);
+    wave_partial(red, buf, 0, er, ei);
 }
 
-__device__ __noinline__ void mop_apply4(double2* s, int t, const MicroOp& op, const double2* pool) { mop_apply<4>(s, t, op, pool); }
-__device__ __noinline__ void mop_apply5(double2* s, int t, const MicroOp& op, const double2* pool) { mop_apply<5>(s, t, op, pool); }
-__device__ __noinline__ void mop_pair4(double2* sa, double2* sb, int t, const MicroOp& op, const double2* pool, double2* gacc) {
-    mop_pair<4, true>(sa, sb, t, op, pool, gacc);
-}
-__device__ __noinline__ void mop_grad4(double2* sa, double2* sb, int t, const MicroOp& op, const double2* pool, double2* gacc) {
-    mop_pair<4, false>(sa, sb, t, op, pool, gacc);
-}
-__device__ __noinline__ void mop_grad5(double2* sa, double2* sb, int t, const MicroOp& op, const double2* pool, double2* gacc) {
-    mop_pair<5, false>(sa, sb, t, op, pool, gacc);
-}
-__device__ __noinline__ void mop_ham4(double2* sa, double2* sb, int t, const MicroOp& op, const double2* pool, double2* gacc) {
-    mop_ham<4>(sa, sb, t, op, pool, gacc);
-}
-__device__ __noinline__ void mop_ham5(double2* sa, double2* sb, int t, const MicroOp& op, const double2* pool, double2* gacc) {
-    mop_ham<5>(sa, sb, t, op, pool, gacc);
-}
-
-__device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const MicroOp& op, const double2* pool, double2* gacc) {
+// One micro-op followed by a barrier. `red` holds kRedDoubles2 entries and
+// `gacc` the accumulator rows addressed by op.acc[]. MAXKB is the widest
+// micro-op the calling kernel is built for: kernels for phases of at most 3
+// local bits carry no 4/5-bit code, which keeps them free of register spills.
+template <int MAXKB, int KS>
+__device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const MicroOp& op, const double2* pool, double2* red,
+                                        double2* gacc, AccState& st) {
+    acc_fold(st, red, gacc);
     const int kb = op.kb;
+    const int buf = st.cur_buf;
+    int nred = 0;
     switch (op.kind) {
-        case MK_APPLY: {
+        case MK_APPLY:
+            if constexpr ((KS & (1 << MK_APPLY)) != 0) {
             double2* s = op.buf ? sb : sa;
             if (kb == 1) mop_apply<1>(s, t, op, pool);
             else if (kb == 2) mop_apply<2>(s, t, op, pool);
-            else if (kb == 3) mop_apply<3>(s, t, op, pool);
-            else if (kb == 4) mop_apply4(s, t, op, pool);
-            else mop_apply5(s, t, op, pool);
+            else if constexpr (MAXKB >= 3) {
+                if (kb == 3) mop_apply<3>(s, t, op, pool);
+                else if constexpr (MAXKB >= 5) {
+                    if (kb == 4) mop_apply<4>(s, t, op, pool);
+                    else mop_apply<5>(s, t, op, pool);
+                }
+            }
+            }
             break;
-        }
         case MK_APPLY2:
         case MK_APPLY2X:
-            if (kb == 1) mop_pair<1, true>(sa, sb, t, op, pool, gacc);
-            else if (kb == 2) mop_pair<2, true>(sa, sb, t, op, pool, gacc);
-            else if (kb == 3) mop_pair<3, true>(sa, sb, t, op, pool, gacc);
-            else mop_pair4(sa, sb, t, op, pool, gacc);
-            break;
         case MK_GRAD:
-            if (kb == 1) mop_pair<1, false>(sa, sb, t, op, pool, gacc);
-            else if (kb == 2) mop_pair<2, false>(sa, sb, t, op, pool, gacc);
-            else if (kb == 3) mop_pair<3, false>(sa, sb, t, op, pool, gacc);
-            else if (kb == 4) mop_grad4(sa, sb, t, op, pool, gacc);
-            else mop_grad5(sa, sb, t, op, pool, gacc);
+            if constexpr ((KS & ((1 << MK_APPLY2) | (1 << MK_APPLY2X) | (1 << MK_GRAD))) != 0) {
+                if (kb == 1) mop_pair<1>(sa, sb, t, op, pool, red, buf);
+                else if (kb == 2) mop_pair<2>(sa, sb, t, op, pool, red, buf);
+                else if constexpr (MAXKB >= 3) {
+                    if (kb == 3) mop_pair<3>(sa, sb, t, op, pool, red, buf);
+                    else if constexpr (MAXKB >= 5) {
+                        if (kb == 4) mop_pair<4>(sa, sb, t, op, pool, red, buf);
+                        else mop_pair<5>(sa, sb, t, op, pool, red, buf);
+                    }
+                }
+                nred = op.ng;
+            }
             break;
         case MK_HAM:
-            if (kb == 1) mop_ham<1>(sa, sb, t, op, pool, gacc);
-            else if (kb == 2) mop_ham<2>(sa, sb, t, op, pool, gacc);
-            else if (kb == 3) mop_ham<3>(sa, sb, t, op, pool, gacc);
-            else if (kb == 4) mop_ham4(sa, sb, t, op, pool, gacc);
-            else mop_ham5(sa, sb, t, op, pool, gacc);
+            if constexpr ((KS & (1 << MK_HAM)) != 0) {
+            if (kb == 1) mop_ham<1>(sa, sb, t, op, pool, red, buf);
+            else if (kb == 2) mop_ham<2>(sa, sb, t, op, pool, red, buf);
+            else if constexpr (MAXKB >= 3) {
+                if (kb == 3) mop_ham<3>(sa, sb, t, op, pool, red, buf);
+                else if constexpr (MAXKB >= 5) {
+                    if (kb == 4) mop_ham<4>(sa, sb, t, op, pool, red, buf);
+                    else mop_ham<5>(sa, sb, t, op, pool, red, buf);
+                }
+            }
+            nred = 1;
+            }
             break;
     }
+    __syncthreads();
+    acc_push(st, nred, op.acc);
+}
+
+// Fold the last pending partials; ends with a barrier so gacc is complete.
+__device__ __forceinline__ void acc_finish(AccState& st, const double2* red, double2* gacc) {
+    acc_fold(st, red, gacc);
     __syncthreads();
 }
 
@@ -281,19 +351,44 @@ __device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const M
 //  PH_FWD: sa <- srcA tile; ops; dstA tile <- sa
 //  PH_REV: sa <- A, sb <- B; ops; A <- sa, B <- sb
 //  PH_HAM: sa <- A, sb <- (ham_accum ? B : 0); ops; B <- sb
+// Global loads are issued kLoadBatch at a time per thread before any LDS store
+// so that each wave keeps several 1 KiB requests in flight.
+template <int MAXKB, int KS>
 __device__ __forceinline__ void phase_tile(const PhaseDesc& p, uint64_t tile, const double2* srcA, double2* A, double2* B,
                                            const MicroOp* __restrict__ ops, const double2* __restrict__ pool, double2* sa,
-                                           double2* sb, double2* gacc) {
+                                           double2* sb, double2* red, double2* gacc, AccState& st) {
     const uint64_t base = tile_base(tile, p);
     const uint32_t T = 1u << p.t;
-    for (uint32_t j = threadIdx.x; j < T; j += kTileThreads) {
-        const uint64_t gi = tile_global(j, p, base);
-        sa[j] = (p.flags & PF_INIT_ZERO) ? make_double2(gi == 0 ? 1.0 : 0.0, 0.0) : srcA[gi];
-        if (p.kind == PH_REV || (p.kind == PH_HAM && p.ham_accum)) sb[j] = B[gi];
-        else if (p.kind == PH_HAM) sb[j] = make_double2(0.0, 0.0);
+    const bool loadB = p.kind == PH_REV || (p.kind == PH_HAM && p.ham_accum);
+    const bool zeroB = p.kind == PH_HAM && !p.ham_accum;
+    const bool init = p.flags & PF_INIT_ZERO;
+    // per thread: T / kTileThreads amplitudes (uniform), read kLoadBatch at a time
+    const uint32_t per = (T + kTileThreads - 1) / kTileThreads;
+    for (uint32_t u0 = 0; u0 < per; u0 += kLoadBatch) {
+        double2 va[kLoadBatch], vb[kLoadBatch];
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            va[u] = make_double2(0.0, 0.0);
+            vb[u] = make_double2(0.0, 0.0);
+            const uint32_t j = (u0 + u) * kTileThreads + threadIdx.x;
+            if (u0 + u < per && j < T) {
+                const uint64_t gi = tile_global(j, p, base);
+                if (init) va[u].x = gi == 0 ? 1.0 : 0.0;
+                else va[u] = srcA[gi];
+                if (loadB) vb[u] = B[gi];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kLoadBatch; ++u) {
+            const uint32_t j = (u0 + u) * kTileThreads + threadIdx.x;
+            if (u0 + u < per && j < T) {
+                sa[j] = va[u];
+                if (loadB || zeroB) sb[j] = vb[u];
+            }
+        }
     }
     __syncthreads();
-    for (int o = p.op_begin; o < p.op_end; ++o) run_mop(sa, sb, p.t, ops[o], pool, gacc);
+    for (int o = p.op_begin; o < p.op_end; ++o) run_mop<MAXKB, KS>(sa, sb, p.t, ops[o], pool, red, gacc, st);
     for (uint32_t j = threadIdx.x; j < T; j += kTileThreads) {
         const uint64_t gi = tile_global(j, p, base);
         if (p.kind != PH_HAM) A[gi] = sa[j];
@@ -303,35 +398,33 @@ __device__ __forceinline__ void phase_tile(const PhaseDesc& p, uint64_t tile, co
 }
 
 // Whole evaluation of a state held in one tile: sa <- src (|0...0> when src is
-// null), micro-ops [mop_begin, mop_end) with sb zeroed first; A receives the
-// state as it stands at ham_begin (the forward result). The accumulator index
+// null), then the forward micro-ops [mop_begin, ham_begin), the Hamiltonian
+// ones [ham_begin, rev_begin) into sb (zeroed first) and the reverse ones
+// [rev_begin, mop_end), each stage in its own loop; A receives the forward state. The accumulator index
 // of every micro-op is a local row; row e is written to rows[e * row_stride].
+// LDS: sa, sb (2^nbits each), red (kRedDoubles2), gacc (nrows).
+template <int MAXKB, int RKS>
 __device__ __forceinline__ void small_eval_body(int nbits, const double2* src, double2* A, const MicroOp* __restrict__ ops,
-                                                int mop_begin, int mop_end, int ham_begin, const double2* __restrict__ pool,
-                                                double2* sa, double2* sb, double2* gacc, int nrows, double2* rows,
-                                                size_t row_stride) {
+                                                int mop_begin, int ham_begin, int rev_begin, int mop_end,
+                                                const double2* __restrict__ pool, double2* sa, double2* sb, double2* red,
+                                                double2* gacc, int nrows, double2* rows, size_t row_stride) {
     const uint32_t T = 1u << nbits;
-    for (int i = threadIdx.x; i < nrows * kTileWaves; i += kTileThreads) gacc[i] = make_double2(0.0, 0.0);
+    for (int i = threadIdx.x; i < nrows; i += kTileThreads) gacc[i] = make_double2(0.0, 0.0);
     for (uint32_t j = threadIdx.x; j < T; j += kTileThreads) {
         sa[j] = src ? src[j] : make_double2(j == 0 ? 1.0 : 0.0, 0.0);
         sb[j] = make_double2(0.0, 0.0);
     }
     __syncthreads();
-    for (int o = mop_begin; o < mop_end; ++o) {
-        if (o == ham_begin && A) {
-            for (uint32_t j = threadIdx.x; j < T; j += kTileThreads) A[j] = sa[j];
-        }
-        run_mop(sa, sb, nbits, ops[o], pool, gacc);
-    }
-    if (ham_begin >= mop_end && A) {
+    AccState st;
+    acc_init(st);
+    for (int o = mop_begin; o < ham_begin; ++o) run_mop<MAXKB, KS_FWD>(sa, sb, nbits, ops[o], pool, red, gacc, st);
+    if (A) {
         for (uint32_t j = threadIdx.x; j < T; j += kTileThreads) A[j] = sa[j];
     }
-    __syncthreads();
-    for (int e = threadIdx.x; e < nrows; e += kTileThreads) {
-        double2 s = make_double2(0.0, 0.0);
-        for (int w = 0; w < kTileWaves; ++w) s = cadd(s, gacc[e * kTileWaves + w]);
-        rows[size_t(e) * row_stride] = s;
-    }
+    for (int o = ham_begin; o < rev_begin; ++o) run_mop<MAXKB, KS_HAM>(sa, sb, nbits, ops[o], pool, red, gacc, st);
+    for (int o = rev_begin; o < mop_end; ++o) run_mop<MAXKB, RKS>(sa, sb, nbits, ops[o], pool, red, gacc, st);
+    acc_finish(st, red, gacc);
+    for (int e = threadIdx.x; e < nrows; e += kTileThreads) rows[size_t(e) * row_stride] = gacc[e];
     __syncthreads();
 }
 
