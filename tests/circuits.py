@@ -1,0 +1,24 @@
+"""Circuit generators shared by the GPU parity tests."""
+import vqchip as vq
+from brute import rand_unitary
+
+
+def chain_circuit(n, rng, layers=2, value_rng=None):
+    """Runs of 1..7 one-qubit gates on the same qubit (fixed, active and inactive
+    rotations, dense unitaries), separated by CNOT layers, so that the fusion of
+    same-qubit runs meets every run length and the four-step cap.
+
+    rng fixes the structure; value_rng (default rng) draws angles and matrices, so
+    two calls with equal rng seeds and different value_rng share one structure."""
+    vr = rng if value_rng is None else value_rng
+    kinds = [lambda q, a: vq.RxGate(q, a), lambda q, a: vq.RyGate(q, a), lambda q, a: vq.RzGate(q, a),
+             lambda q, a: vq.HGate(q), lambda q, a: vq.TGate(q), lambda q, a: vq.RzGate(q, a, isparas=False),
+             lambda q, a: vq.PHASEGate(q, a), lambda q, a: vq.QuantumGate((q,), rand_unitary(1, vr))]
+    c = vq.QCircuit()
+    for layer in range(layers):
+        for q in range(1, n + 1):
+            for _ in range(1 + (q + layer) % 7):
+                c.append(kinds[int(rng.integers(len(kinds)))](q, float(vr.uniform(0, 6.3))))
+        for q in range(1 + layer % 2, n, 2):
+            c.append(vq.CNOTGate(q, q + 1))
+    return c

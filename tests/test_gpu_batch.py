@@ -7,6 +7,7 @@ import pytest
 
 import oracle
 from brute import rand_unitary
+from circuits import chain_circuit
 import vqchip as vq
 from vqchip.batch import Batch
 from vqchip.circuit import Program
@@ -89,3 +90,16 @@ def test_instance_without_ops_or_terms():
     inst = [(vq.QCircuit(), tfim(5), 5), (vq.QCircuit(), tfim(14), 14), (hea_circuit(6, 1, np.ones(18)), vq.QubitsOperator(), 6)]
     E, grads = Batch(inst).run()
     check(inst, E, grads)
+
+
+def test_same_qubit_runs_in_batch():
+    ns = (4, 9, 12, 14, 16)
+    vr = np.random.default_rng(10)
+    inst = [(chain_circuit(n, np.random.default_rng(n), value_rng=vr), tfim(n, 0.8), n) for n in ns]
+    b = Batch(inst)
+    E, grads = b.run()
+    check(inst, E, grads)
+    inst2 = [(chain_circuit(n, np.random.default_rng(n), value_rng=vr), H, n) for _, H, n in inst]
+    b.update([c for c, _, _ in inst2])
+    E2, g2 = b.run()
+    check(inst2, E2, g2)

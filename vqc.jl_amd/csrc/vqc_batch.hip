@@ -23,7 +23,7 @@
 
 void vqc_zero(double2* v, uint64_t len, hipStream_t st);
 
-constexpr int kMaxChunks = 64;   // partial columns per row
+constexpr int kMaxChunks = 256;  // tile chunks (and partial columns) per phase
 constexpr int kItemSmall = 0, kItemPhase = 1;
 
 struct BItem {
@@ -58,7 +58,8 @@ struct BatchArgs {
     const MicroOp* mops;
     const double2* pool;
     double2* arena;
-    double2* parts;      // [row][kMaxChunks]
+    double2* parts;      // row r's partials start at row_off[r] (one per chunk of its phase)
+    const int64_t* row_off;
     unsigned* counters;  // [0] queue head, [1] error flag, [2 + i] done items of instance i
 };
 
@@ -66,6 +67,35 @@ struct VarEntry {
     int64_t src, dst;
     int32_t D, v;
 };
+
+// one 2x2 chain product U_n ... U_1 per entry (its sources are pool matrices or
+// variants written by k_variants before)
+struct ProdEntry {
+    int64_t dst;
+    int64_t srcs[4];
+    int32_t n, pad;
+};
+
+__global__ void __launch_bounds__(256) k_products(double2* pool, const ProdEntry* tab, int n) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const ProdEntry t = tab[e];
+    double2 p00 = pool[t.srcs[0]], p10 = pool[t.srcs[0] + 1], p01 = pool[t.srcs[0] + 2], p11 = pool[t.srcs[0] + 3];
+    for (int f = 1; f < t.n; ++f) {
+        const double2* U = pool + t.srcs[f];
+        const double2 u00 = U[0], u10 = U[1], u01 = U[2], u11 = U[3];
+        const double2 q00 = cfma(u01, p10, cmul(u00, p00)), q10 = cfma(u11, p10, cmul(u10, p00));
+        const double2 q01 = cfma(u01, p11, cmul(u00, p01)), q11 = cfma(u11, p11, cmul(u10, p01));
+        p00 = q00;
+        p10 = q10;
+        p01 = q01;
+        p11 = q11;
+    }
+    pool[t.dst] = p00;
+    pool[t.dst + 1] = p10;
+    pool[t.dst + 2] = p01;
+    pool[t.dst + 3] = p11;
+}
 
 __global__ void __launch_bounds__(256) k_variants(double2* pool, const VarEntry* tab, int n) {
     for (int e = blockIdx.x; e < n; e += gridDim.x) {
@@ -126,12 +156,14 @@ __global__ void __launch_bounds__(kTileThreads) k_batch(BatchArgs a) {
                     atomicOr(&a.counters[1], one);
                     break;
                 }
-                __builtin_amdgcn_s_sleep(2);
+                __builtin_amdgcn_s_sleep(32);  // ~2k cycles between polls of the counter
             }
+            // one wave's acquire invalidates the CU's L1 and this XCD's stale L2
+            // lines for the whole workgroup; the barrier below orders the rest
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
             s_ok = ok;
         }
         __syncthreads();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
         const int ok = __builtin_amdgcn_readfirstlane(s_ok);
         if (ok) {
             const BInst in = a.inst[it.inst];
@@ -139,7 +171,7 @@ __global__ void __launch_bounds__(kTileThreads) k_batch(BatchArgs a) {
                 const uint32_t T = 1u << in.nbits;
                 small_eval_body<MAXKB, RKS>(in.nbits, nullptr, nullptr, a.mops, in.mop_begin, in.ham_begin, in.rev_begin, in.mop_end, a.pool, smem,
                                 smem + T, smem + 2 * T, smem + 2 * T + kRedDoubles2, in.nrows,
-                                a.parts + size_t(in.row_begin) * kMaxChunks, kMaxChunks);
+                                a.parts + a.row_off[in.row_begin], 1);
             } else {
                 const PhaseDesc& p = a.phases[it.phase];
                 const uint32_t T = 1u << p.t;
@@ -165,14 +197,34 @@ __global__ void __launch_bounds__(kTileThreads) k_batch(BatchArgs a) {
                 }
                 acc_finish(st, red, gacc);
                 for (int e = threadIdx.x; e < p.nacc; e += kTileThreads)
-                    a.parts[size_t(p.row_begin + e) * kMaxChunks + it.chunk] = gacc[e];
+                    a.parts[a.row_off[p.row_begin + e] + it.chunk] = gacc[e];
             }
         }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+        // the barrier makes every wave's stores complete in L2; one agent-scope
+        // release (L2 write-back) by wave 0 then publishes them to other XCDs
         __syncthreads();
-        if (wave0) __hip_atomic_fetch_add(&a.counters[2 + it.inst], one, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        if (wave0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+            __hip_atomic_fetch_add(&a.counters[2 + it.inst], one, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
         __syncthreads();
     }
+}
+
+// out[r] = sum of row r's partials, in a fixed order
+__global__ void __launch_bounds__(256) k_reduce_rows(const double2* __restrict__ parts, const int64_t* __restrict__ off,
+                                                     const int32_t* __restrict__ cnt, double2* out) {
+    __shared__ double red[4];
+    const double2* p = parts + off[blockIdx.x];
+    const int n = cnt[blockIdx.x];
+    double sr = 0, si = 0;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        sr += p[i].x;
+        si += p[i].y;
+    }
+    sr = block_sum(sr, red);
+    si = block_sum(si, red);
+    if (threadIdx.x == 0) out[blockIdx.x] = make_double2(sr, si);
 }
 
 // ---------------------------------------------------------------------------
@@ -198,10 +250,15 @@ struct vqc_batch {
     void* d_mops = nullptr;
     void* d_var = nullptr;
     int nvar = 0;
+    void* d_prod = nullptr;
+    int nprod = 0;
     double2* d_pool = nullptr;
     double2* d_arena = nullptr;
     double2* d_parts = nullptr;
     double2* d_rows = nullptr;
+    void* d_row_off = nullptr;
+    void* d_row_cnt = nullptr;
+    int64_t nparts = 0;
     unsigned* d_counters = nullptr;
     uint64_t arena_bytes = 0;
     // host pinned staging
@@ -221,13 +278,17 @@ static BatchKernel batch_kernel(const vqc_batch* b) {
 
 static int batch_enqueue(vqc_batch* b, hipStream_t st) {
     BatchArgs a{(const BItem*)b->d_items, b->nitems, (const BInst*)b->d_inst, (const PhaseDesc*)b->d_phases,
-                (const MicroOp*)b->d_mops, b->d_pool, b->d_arena, b->d_parts, b->d_counters};
+                (const MicroOp*)b->d_mops, b->d_pool, b->d_arena, b->d_parts, (const int64_t*)b->d_row_off, b->d_counters};
     VQC_HIP(hipMemcpyAsync(b->d_pool, b->h_pool, size_t(b->pool_len + b->hpool_len) * 16, hipMemcpyHostToDevice, st));
     if (b->nvar) hipLaunchKernelGGL(k_variants, dim3(std::min(b->nvar, 4096)), dim3(256), 0, st, b->d_pool, (const VarEntry*)b->d_var, b->nvar);
-    vqc_zero(b->d_parts, uint64_t(std::max(b->nrows, 1)) * kMaxChunks, st);
+    if (b->nprod)
+        hipLaunchKernelGGL(k_products, dim3((b->nprod + 255) / 256), dim3(256), 0, st, b->d_pool, (const ProdEntry*)b->d_prod, b->nprod);
+    vqc_zero(b->d_parts, uint64_t(std::max<int64_t>(b->nparts, 1)), st);
     hipLaunchKernelGGL(k_zero_u32, dim3(std::max(1, (b->ninst + 2 + 255) / 256)), dim3(256), 0, st, b->d_counters, b->ninst + 2);
     hipLaunchKernelGGL(batch_kernel(b), dim3(b->grid), dim3(kTileThreads), b->lds, st, a);
-    if (b->nrows) vqc_reduce_partials(b->d_parts, b->nrows, kMaxChunks, b->d_rows, st);
+    if (b->nrows)
+        hipLaunchKernelGGL(k_reduce_rows, dim3(b->nrows), dim3(256), 0, st, b->d_parts, (const int64_t*)b->d_row_off,
+                           (const int32_t*)b->d_row_cnt, b->d_rows);
     VQC_HIP(hipMemcpyAsync(b->h_rows, b->d_rows, size_t(std::max(b->nrows, 1)) * 16 + 16, hipMemcpyDeviceToHost, st));
     VQC_HIP(hipGetLastError());
     return VQC_OK;
@@ -236,7 +297,8 @@ static int batch_enqueue(vqc_batch* b, hipStream_t st) {
 static int batch_free(vqc_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    void* dev[] = {b->d_items, b->d_inst, b->d_phases, b->d_mops, b->d_var, b->d_pool, b->d_arena, b->d_parts, b->d_rows, b->d_counters};
+    void* dev[] = {b->d_items, b->d_inst, b->d_phases, b->d_mops, b->d_var, b->d_prod, b->d_pool, b->d_arena, b->d_parts, b->d_rows, b->d_counters,
+                   b->d_row_off, b->d_row_cnt};
     for (void* p : dev)
         if (p) (void)hipFree(p);
     if (b->h_pool) (void)hipHostFree(b->h_pool);
@@ -312,17 +374,20 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
             sp.xp = &xp;
             sp.nrows = 0;
             vqc_plan_forward(sp, n, n, 0, I.nops, iops, n, 0);
-            const int hb = int(sp.mops.size());
+            const size_t nf = sp.phases.size();
             vqc_plan_ham(sp, n, I.nterms, iterms, pool_len, n, 0);
-            const int rb = int(sp.mops.size());
+            const size_t nh = sp.phases.size();
             vqc_plan_reverse(sp, n, n, 0, I.nops, iops, n, 0);
+            vqc_fuse_chains(sp, 0, sp.phases.size());
+            const int hb = vqc_phase_op_start(sp, nf);
+            const int rb = vqc_phase_op_start(sp, nh);
             const size_t T = size_t(1) << n;
             const size_t need = (2 * T + kRedDoubles2 + size_t(std::max(sp.nrows, 1))) * 16;
             if (need <= 160 * 1024) {
                 small = true;
                 for (const PhaseDesc& p : sp.phases)
                     for (int o = p.op_begin; o < p.op_end; ++o)
-                        for (int q = 0; q < 3; ++q) sp.mops[o].acc[q] += p.row_begin;
+                        for (int q = 0; q < 4; ++q) sp.mops[o].acc[q] += p.row_begin;
                 bi.mop_begin = int(pb.mops.size());
                 bi.ham_begin = bi.mop_begin + hb;
                 bi.rev_begin = bi.mop_begin + rb;
@@ -349,6 +414,7 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
             pb.phases[ph0].flags |= PF_INIT_ZERO;
             vqc_plan_ham(pb, n, I.nterms, iterms, pool_len, tt.t_rev, tt.mlow);
             vqc_plan_reverse(pb, n, n, 0, I.nops, iops, tt.t_rev, tt.mlow);
+            vqc_fuse_chains(pb, ph0, pb.phases.size());
             bi.nrows = pb.nrows - rows0;
             bi.a_off = arena;
             bi.b_off = arena + (int64_t(1) << n);
@@ -371,10 +437,10 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
     for (const MicroOp& m : pb.mops) b->maxkb = std::max(b->maxkb, int(m.kb));
     // reverse micro-ops: small instances [rev_begin, mop_end), large instances their PH_REV phases
     for (const BInst& bi : binst)
-        for (int o = bi.rev_begin; o < bi.mop_end; ++o) b->rev_simple = b->rev_simple && pb.mops[o].kind == MK_APPLY2;
+        for (int o = bi.rev_begin; o < bi.mop_end; ++o) b->rev_simple = b->rev_simple && vqc_rev_simple(pb.mops[o]);
     for (const PhaseDesc& ph : pb.phases)
         if (ph.kind == PH_REV)
-            for (int o = ph.op_begin; o < ph.op_end; ++o) b->rev_simple = b->rev_simple && pb.mops[o].kind == MK_APPLY2;
+            for (int o = ph.op_begin; o < ph.op_end; ++o) b->rev_simple = b->rev_simple && vqc_rev_simple(pb.mops[o]);
 
     // ---- items in list-scheduling order ---------------------------------
     std::vector<BItem> items;
@@ -463,22 +529,54 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
     }
     b->nitems = int(items.size());
 
+    // ---- partial layout: one slot per (row, chunk of the row's phase) -----
+    std::vector<int32_t> row_cnt(size_t(std::max(b->nrows, 1)), 1);
+    for (int i = 0; i < ninst; ++i) {
+        if (chains[i].small) continue;
+        for (int q = 0; q < chains[i].nphases; ++q) {
+            const PhaseDesc& ph = pb.phases[chains[i].first_phase + q];
+            const int nch = int(std::min<int64_t>(int64_t(1) << (ph.nbits - ph.t), kMaxChunks));
+            for (int e = 0; e < ph.nacc; ++e) row_cnt[ph.row_begin + e] = nch;
+        }
+    }
+    std::vector<int64_t> row_off(row_cnt.size());
+    int64_t np = 0;
+    for (size_t r = 0; r < row_cnt.size(); ++r) {
+        row_off[r] = np;
+        np += row_cnt[r];
+    }
+    b->nparts = np;
+
     // ---- device buffers -------------------------------------------------
     b->xlen = xp.size();
-    b->nvar = int(xp.table.size());
     b->arena_bytes = uint64_t(arena) * 16;
     auto dalloc = [&](void** p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 64)); };
+    // variants first (one launch), then the chain products that may read them
     std::vector<VarEntry> vt;
-    for (const XPoolEntry& e : xp.table) vt.push_back({e.src, e.dst, e.D, e.v});
+    std::vector<ProdEntry> pt;
+    for (const XPoolEntry& e : xp.table) {
+        if (e.v == 8) {
+            ProdEntry q{e.dst, {-1, -1, -1, -1}, e.nsrc, 0};
+            for (int j = 0; j < e.nsrc; ++j) q.srcs[j] = e.srcs[j];
+            pt.push_back(q);
+        } else {
+            vt.push_back({e.src, e.dst, e.D, e.v});
+        }
+    }
+    b->nvar = int(vt.size());
+    b->nprod = int(pt.size());
     hipError_t e = hipSuccess;
     if (e == hipSuccess) e = dalloc(&b->d_items, items.size() * sizeof(BItem));
     if (e == hipSuccess) e = dalloc(&b->d_inst, binst.size() * sizeof(BInst));
     if (e == hipSuccess) e = dalloc(&b->d_phases, pb.phases.size() * sizeof(PhaseDesc));
     if (e == hipSuccess) e = dalloc(&b->d_mops, pb.mops.size() * sizeof(MicroOp));
     if (e == hipSuccess) e = dalloc(&b->d_var, vt.size() * sizeof(VarEntry));
+    if (e == hipSuccess) e = dalloc(&b->d_prod, pt.size() * sizeof(ProdEntry));
     if (e == hipSuccess) e = dalloc((void**)&b->d_pool, size_t(b->xlen) * 16);
     if (e == hipSuccess) e = dalloc((void**)&b->d_arena, size_t(arena) * 16);
-    if (e == hipSuccess) e = dalloc((void**)&b->d_parts, size_t(std::max(b->nrows, 1)) * kMaxChunks * 16);
+    if (e == hipSuccess) e = dalloc((void**)&b->d_parts, size_t(std::max<int64_t>(b->nparts, 1)) * 16);
+    if (e == hipSuccess) e = dalloc(&b->d_row_off, row_off.size() * sizeof(int64_t));
+    if (e == hipSuccess) e = dalloc(&b->d_row_cnt, row_cnt.size() * sizeof(int32_t));
     if (e == hipSuccess) e = dalloc((void**)&b->d_rows, size_t(std::max(b->nrows, 1)) * 16 + 16);
     if (e == hipSuccess) e = dalloc((void**)&b->d_counters, size_t(ninst + 2) * 4);
     if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_pool, size_t(std::max<int64_t>(pool_len + hpool_len, 1)) * 16, hipHostMallocDefault);
@@ -496,6 +594,9 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
     if (!pb.mops.empty())
         VQC_HIP(hipMemcpyAsync(b->d_mops, pb.mops.data(), pb.mops.size() * sizeof(MicroOp), hipMemcpyHostToDevice, st));
     if (!vt.empty()) VQC_HIP(hipMemcpyAsync(b->d_var, vt.data(), vt.size() * sizeof(VarEntry), hipMemcpyHostToDevice, st));
+    if (!pt.empty()) VQC_HIP(hipMemcpyAsync(b->d_prod, pt.data(), pt.size() * sizeof(ProdEntry), hipMemcpyHostToDevice, st));
+    VQC_HIP(hipMemcpyAsync(b->d_row_off, row_off.data(), row_off.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    VQC_HIP(hipMemcpyAsync(b->d_row_cnt, row_cnt.data(), row_cnt.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
     VQC_HIP(hipStreamSynchronize(st));
 
     // persistent grid: as many workgroups as can be resident

@@ -105,9 +105,10 @@ static MicroOp mop_init(int kind) {
     memset(&m, 0, sizeof m);
     m.kind = kind;
     m.mat2 = -1;
-    for (int q = 0; q < 3; ++q) {
+    for (int q = 0; q < 4; ++q) {
         m.gen[q] = -1;
         m.acc[q] = 0;
+        m.cmat[q] = -1;
     }
     return m;
 }
@@ -172,6 +173,114 @@ int64_t XPool::variant(int64_t off, int D, int v) {
         }
     cache.push_back({{off, v}, out});
     return out;
+}
+
+int64_t XPool::product(const int64_t* srcs, int n, int D) {
+    const int64_t out = size();
+    if (deferred) {
+        len += int64_t(D) * D;
+        XPoolEntry e{};
+        e.src = srcs[0];
+        e.dst = out;
+        e.D = D;
+        e.v = 8;
+        e.nsrc = n;
+        for (int i = 0; i < 4; ++i) e.srcs[i] = i < n ? srcs[i] : -1;
+        table.push_back(e);
+        return out;
+    }
+    std::vector<double> P(size_t(2) * D * D, 0.0), T(size_t(2) * D * D);
+    for (int i = 0; i < D; ++i) P[2 * (i + i * D)] = 1.0;
+    for (int f = 0; f < n; ++f) {  // P <- U_f P
+        const double* U = h.data() + 2 * srcs[f];
+        for (int r = 0; r < D; ++r)
+            for (int c = 0; c < D; ++c) {
+                double re = 0, im = 0;
+                for (int k = 0; k < D; ++k) {
+                    const double ur = U[2 * (r + k * D)], ui = U[2 * (r + k * D) + 1];
+                    const double pr = P[2 * (k + c * D)], pi = P[2 * (k + c * D) + 1];
+                    re += ur * pr - ui * pi;
+                    im += ur * pi + ui * pr;
+                }
+                T[2 * (r + c * D)] = re;
+                T[2 * (r + c * D) + 1] = im;
+            }
+        P.swap(T);
+    }
+    h.insert(h.end(), P.begin(), P.end());
+    return out;
+}
+
+void vqc_fuse_chains(PlanBuilder& pb, size_t phase_begin, size_t phase_end) {
+    // Phases own contiguous, ascending micro-op ranges; the range's phases are
+    // rewritten in place and later phases are shifted.
+    if (phase_begin >= phase_end || phase_end > pb.phases.size()) return;
+    XPool& xp = *pb.xp;
+    const int first = pb.phases[phase_begin].op_begin;
+    const int last = pb.phases[phase_end - 1].op_end;
+    std::vector<MicroOp> out(pb.mops.begin(), pb.mops.begin() + first);
+    out.reserve(pb.mops.size());
+    for (size_t i = phase_begin; i < phase_end; ++i) {
+        PhaseDesc& p = pb.phases[i];
+        const int b = int(out.size());
+        for (int o = p.op_begin; o < p.op_end;) {
+            const MicroOp& m = pb.mops[o];
+            int run = 1;
+            if (p.kind == PH_FWD && m.kind == MK_APPLY && m.kb == 1) {
+                // U_k ... U_1 on one bit of one buffer -> one 2x2 product
+                while (o + run < p.op_end && run < 4) {
+                    const MicroOp& x = pb.mops[o + run];
+                    if (x.kind != MK_APPLY || x.kb != 1 || x.lpos[0] != m.lpos[0] || x.buf != m.buf) break;
+                    ++run;
+                }
+                if (run > 1) {
+                    int64_t srcs[4];
+                    for (int j = 0; j < run; ++j) srcs[j] = pb.mops[o + j].mat;
+                    MicroOp f = m;
+                    f.mat = xp.product(srcs, run, 2);
+                    out.push_back(f);
+                    o += run;
+                    continue;
+                }
+            } else if (p.kind == PH_REV && m.kind == MK_APPLY2 && m.kb == 1 && m.ng <= 1) {
+                // reverse steps on one bit stay separate (each gradient needs its
+                // intermediate pair) but share one pass over the tile
+                while (o + run < p.op_end && run < 4) {
+                    const MicroOp& x = pb.mops[o + run];
+                    if (x.kind != MK_APPLY2 || x.kb != 1 || x.lpos[0] != m.lpos[0] || x.ng > 1) break;
+                    ++run;
+                }
+                if (run > 1) {
+                    MicroOp f = mop_init(MK_CHAIN);
+                    f.kb = 1;
+                    f.buf = m.buf;
+                    f.lpos[0] = m.lpos[0];
+                    f.nchain = run;
+                    f.ng = 0;
+                    for (int j = 0; j < run; ++j) {
+                        const MicroOp& x = pb.mops[o + j];
+                        f.cmat[j] = x.mat;
+                        f.gen[j] = x.ng ? x.gen[0] : -1;
+                        if (x.ng) f.acc[f.ng++] = x.acc[0];
+                    }
+                    out.push_back(f);
+                    o += run;
+                    continue;
+                }
+            }
+            out.push_back(m);
+            ++o;
+        }
+        p.op_begin = b;
+        p.op_end = int(out.size());
+    }
+    const int shift = int(out.size()) - last;
+    out.insert(out.end(), pb.mops.begin() + last, pb.mops.end());
+    for (size_t i = phase_end; i < pb.phases.size(); ++i) {
+        pb.phases[i].op_begin += shift;
+        pb.phases[i].op_end += shift;
+    }
+    pb.mops.swap(out);
 }
 
 int vqc_plan_forward(PlanBuilder& pb, int nbits, int n, int is_dm, int nops, const vqc_op_t* ops, int t, int mlow) {
@@ -465,7 +574,7 @@ int vqc_phase_kindset(const PlanBuilder& pb, size_t i) {
     if (p.kind == PH_FWD) return KS_FWD;
     if (p.kind == PH_HAM) return KS_HAM;
     for (int o = p.op_begin; o < p.op_end; ++o)
-        if (pb.mops[o].kind != MK_APPLY2) return KS_REVX;
+        if (!vqc_rev_simple(pb.mops[o])) return KS_REVX;
     return KS_REV;
 }
 
@@ -528,6 +637,7 @@ extern "C" int vqc_apply_circuit(vqc_state* s, int nops, const vqc_op_t* ops, co
     PlanBuilder pb;
     pb.xp = &xp;
     VQC_TRY(vqc_plan_forward(pb, s->nbits, s->nqubits, s->is_dm, nops, ops, tt.t_fwd, tt.mlow));
+    vqc_fuse_chains(pb, 0, pb.phases.size());
     VQC_TRY(upload_program(pb, c->stream));
     return run_forward(pb, 0, pb.phases.size(), s->d, s->d, (const MicroOp*)g_wmops.buf, (const double2*)g_wpool.buf,
                        (const PhaseDesc*)g_wphases.buf, c->stream);
@@ -670,15 +780,18 @@ extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_
         const bool small = nb <= tt.t_small;
         if (small) {
             VQC_TRY(vqc_plan_forward(pb, nb, nb, 0, nops, ops, nb, 0));
-            const int ham_begin = int(pb.mops.size());
+            const size_t nf = pb.phases.size();
             VQC_TRY(vqc_plan_ham(pb, nb, nterms, terms, plen, nb, 0));
-            const int rev_begin = int(pb.mops.size());
+            const size_t nh = pb.phases.size();
             VQC_TRY(vqc_plan_reverse(pb, nb, nb, 0, nops, ops, nb, 0));
+            vqc_fuse_chains(pb, 0, pb.phases.size());
+            const int ham_begin = vqc_phase_op_start(pb, nf);
+            const int rev_begin = vqc_phase_op_start(pb, nh);
             if (pb.nrows <= max_small_rows(nb)) {
                 // accumulator index := global row
                 for (const PhaseDesc& p : pb.phases)
                     for (int o = p.op_begin; o < p.op_end; ++o)
-                        for (int q = 0; q < 3; ++q) pb.mops[o].acc[q] += p.row_begin;
+                        for (int q = 0; q < 4; ++q) pb.mops[o].acc[q] += p.row_begin;
                 VQC_TRY(upload_program(pb, st));
                 VQC_TRY(ensure(g_wrows, size_t(std::max(pb.nrows, 1)) * 16));
                 SmallArgs a{nb, int(pb.mops.size()), ham_begin, rev_begin, pb.nrows, psi0->d, fwd_out, (const MicroOp*)g_wmops.buf,
@@ -688,7 +801,7 @@ extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_
                 const int mkb = vqc_max_kb(pb.mops, 0, int(pb.mops.size()));
                 const int mk = mkb <= 2 ? 2 : (mkb <= 3 ? 3 : 5);
                 bool rev_simple = true;
-                for (size_t o = size_t(rev_begin); o < pb.mops.size(); ++o) rev_simple &= pb.mops[o].kind == MK_APPLY2;
+                for (size_t o = size_t(rev_begin); o < pb.mops.size(); ++o) rev_simple &= vqc_rev_simple(pb.mops[o]);
                 const int ks = rev_simple ? KS_REV : KS_REVX;
 #define VQC_SMALL(M, K) \
     if (mk == M && ks == K) hipLaunchKernelGGL((k_small_eval<M, K>), dim3(1), dim3(kTileThreads), lds, st, a);
@@ -711,6 +824,7 @@ extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_
             VQC_TRY(vqc_plan_ham(pb, nb, nterms, terms, plen, tt.t_rev, tt.mlow));
             const size_t nham_end = pb.phases.size();
             VQC_TRY(vqc_plan_reverse(pb, nb, nb, 0, nops, ops, tt.t_rev, tt.mlow));
+            vqc_fuse_chains(pb, 0, pb.phases.size());
             VQC_TRY(upload_program(pb, st));
             const MicroOp* dm = (const MicroOp*)g_wmops.buf;
             const double2* dp = (const double2*)g_wpool.buf;
@@ -817,6 +931,7 @@ extern "C" int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_
                 if (ops[i].slot[j] >= 0) conj_gen[size_t(i) * 3 + j] = xp.variant(ops[i].gen[j], 1 << ops[i].nq, MF_CONJ);
     } else {
         VQC_TRY(vqc_plan_forward(pb, nb, n, 1, nops, ops, tt.t_fwd, tt.mlow));
+        vqc_fuse_chains(pb, 0, pb.phases.size());
     }
     const size_t nfwd = pb.phases.size();
     pb.nrows = nterms;  // rows 0..nterms-1: energies (DM energies come from the trace kernels)

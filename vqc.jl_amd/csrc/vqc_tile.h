@@ -17,21 +17,23 @@
 //   MK_APPLY2X A <- M A, B <- M2 B                (channel reverse: S^-1, S^dagger)
 //   MK_GRAD    acc += Re sum conj(B) (G A)        (density-matrix gradient terms)
 //   MK_HAM     B += (M + M^dagger) A, acc += sum conj(A) (M A)   (adjoint seed, energy)
+//   MK_CHAIN   up to 4 one-bit MK_APPLY2 steps on the same bit, done in registers
+//              (one LDS round trip for e.g. the Rz, Ry, Rx of one qubit)
 // Matrices are read from a device pool in which the planner has already stored
 // every variant a micro-op needs (U, U^dagger, conj(U), U^T, M + M^dagger), so
 // the inner loops are plain complex matrix-vector products.
 #pragma once
 #include "vqc_internal.h"
 
-enum { MK_APPLY = 0, MK_APPLY2 = 1, MK_APPLY2X = 2, MK_GRAD = 3, MK_HAM = 4 };
+enum { MK_APPLY = 0, MK_APPLY2 = 1, MK_APPLY2X = 2, MK_GRAD = 3, MK_HAM = 4, MK_CHAIN = 5 };
 enum { MF_CONJ = 1, MF_TRANS = 2 };
 enum { PH_FWD = 0, PH_REV = 1, PH_HAM = 2 };
 // kind sets a kernel is built for (bit k = micro-op kind k)
 enum {
     KS_FWD = 1 << MK_APPLY,
     KS_HAM = 1 << MK_HAM,
-    KS_REV = 1 << MK_APPLY2,                                                         // state-vector reverse
-    KS_REVX = (1 << MK_APPLY) | (1 << MK_APPLY2) | (1 << MK_APPLY2X) | (1 << MK_GRAD),  // density matrices, 5-bit ops
+    KS_REV = (1 << MK_APPLY2) | (1 << MK_CHAIN),  // state-vector reverse
+    KS_REVX = (1 << MK_APPLY) | (1 << MK_APPLY2) | (1 << MK_APPLY2X) | (1 << MK_GRAD) | (1 << MK_CHAIN),
 };
 
 constexpr int kTileThreads = 512;
@@ -44,13 +46,14 @@ struct MicroOp {
     int32_t kind;
     int32_t kb;         // local bits touched (1..5)
     int32_t buf;        // MK_APPLY: 0 = A, 1 = B
-    int32_t pad0;
+    int32_t nchain;     // MK_CHAIN: steps (2..4)
     int32_t lpos[5];    // ascending local bit positions
-    int32_t ng;         // generators evaluated (MK_APPLY2: 0..3, MK_GRAD / MK_HAM: 1)
-    int32_t acc[3];     // accumulator index within the phase
+    int32_t ng;         // partial sums produced (MK_APPLY2: 0..3, MK_GRAD / MK_HAM: 1, MK_CHAIN: 0..4)
+    int32_t acc[4];     // accumulator index of each partial
     int64_t mat;
     int64_t mat2;
-    int64_t gen[3];
+    int64_t gen[4];     // generators; MK_CHAIN: one per step, -1 when the step has none
+    int64_t cmat[4];    // MK_CHAIN: the steps' matrices (U^dagger), in order
 };
 
 struct PhaseDesc {
@@ -106,35 +109,36 @@ __device__ __forceinline__ uint64_t tile_global(uint32_t j, const PhaseDesc& p, 
 // start of the NEXT micro-op, wave 0 adds those partials, in wave order, onto
 // gacc[row]. Two red buffers alternate, so no extra barrier is needed and the
 // sums are the same on every run.
-constexpr int kRedDoubles2 = 2 * 3 * kTileWaves;  // double2 entries of red[]
+constexpr int kRedSlots = 4;                                // partials per micro-op
+constexpr int kRedDoubles2 = 2 * kRedSlots * kTileWaves;  // double2 entries of red[]
 
 struct AccState {
-    int pend_n;      // partials waiting in red[pend_buf] (0..3)
+    int pend_n;          // partials waiting in red[pend_buf] (0..4)
     int pend_buf;
-    int e0, e1, e2;  // accumulator rows they belong to
-    int cur_buf;     // red buffer the next micro-op writes
+    int e0, e1, e2, e3;  // accumulator rows they belong to
+    int cur_buf;         // red buffer the next micro-op writes
 };
 
 __device__ __forceinline__ void acc_init(AccState& st) {
     st.pend_n = 0;
     st.pend_buf = 0;
     st.cur_buf = 0;
-    st.e0 = st.e1 = st.e2 = 0;
+    st.e0 = st.e1 = st.e2 = st.e3 = 0;
 }
 
 __device__ __forceinline__ void wave_partial(double2* red, int buf, int q, double re, double im) {
     re = wave_sum(re);
     im = wave_sum(im);
-    if ((threadIdx.x & 63) == 0) red[(buf * 3 + q) * kTileWaves + (threadIdx.x >> 6)] = make_double2(re, im);
+    if ((threadIdx.x & 63) == 0) red[(buf * kRedSlots + q) * kTileWaves + (threadIdx.x >> 6)] = make_double2(re, im);
 }
 
 // fold pending partials into gacc (wave 0 only; callers order it with barriers)
 __device__ __forceinline__ void acc_fold(AccState& st, const double2* red, double2* gacc) {
     if (st.pend_n && threadIdx.x < uint32_t(st.pend_n)) {
         const int q = threadIdx.x;
-        const int e = q == 0 ? st.e0 : (q == 1 ? st.e1 : st.e2);
+        const int e = q == 0 ? st.e0 : (q == 1 ? st.e1 : (q == 2 ? st.e2 : st.e3));
         double2 s = gacc[e];
-        for (int w = 0; w < kTileWaves; ++w) s = cadd(s, red[(st.pend_buf * 3 + q) * kTileWaves + w]);
+        for (int w = 0; w < kTileWaves; ++w) s = cadd(s, red[(st.pend_buf * kRedSlots + q) * kTileWaves + w]);
         gacc[e] = s;
     }
     st.pend_n = 0;
@@ -148,6 +152,7 @@ __device__ __forceinline__ void acc_push(AccState& st, int n, const int32_t* e) 
     st.e0 = e[0];
     st.e1 = n > 1 ? e[1] : 0;
     st.e2 = n > 2 ? e[2] : 0;
+    st.e3 = n > 3 ? e[3] : 0;
     st.cur_buf ^= 1;
 }
 
@@ -247,6 +252,51 @@ __device__ __forceinline__ void mop_pair(double2* __restrict__ sa, double2* __re
     if (ngen > 2) wave_partial(red, buf, 2, gs2, 0.0);
 }
 
+// MK_CHAIN: for each step j, (A, B) <- U_j^dagger (A, B) on one bit and, when the
+// step has a generator, partial += Re conj(B) (G_j A); the pair stays in
+// registers across the steps.
+__device__ __forceinline__ void mop_chain(double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
+                                          const double2* __restrict__ pool, double2* red, int buf) {
+    const int nch = op.nchain;
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+    const uint32_t ng = 1u << (t - 1);
+    const uint32_t o1 = 1u << op.lpos[0];
+    for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
+        const uint32_t base = ins32<1>(g, op.lpos);
+        double2 a0 = sa[base], a1 = sa[base + o1], b0 = sb[base], b1 = sb[base + o1];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            if (j >= nch) break;
+            const double2* __restrict__ M = pool + op.cmat[j];
+            const double2 m00 = M[0], m10 = M[1], m01 = M[2], m11 = M[3];
+            double2 x0 = cfma(m01, a1, cmul(m00, a0)), x1 = cfma(m11, a1, cmul(m10, a0));
+            double2 y0 = cfma(m01, b1, cmul(m00, b0)), y1 = cfma(m11, b1, cmul(m10, b0));
+            a0 = x0;
+            a1 = x1;
+            b0 = y0;
+            b1 = y1;
+            if (op.gen[j] >= 0) {
+                const double2* __restrict__ G = pool + op.gen[j];
+                const double2 s0 = cfma(G[2], a1, cmul(G[0], a0)), s1 = cfma(G[3], a1, cmul(G[1], a0));
+                const double v = fma(b0.x, s0.x, fma(b0.y, s0.y, fma(b1.x, s1.x, b1.y * s1.y)));
+                if (j == 0) g0 += v;
+                else if (j == 1) g1 += v;
+                else if (j == 2) g2 += v;
+                else g3 += v;
+            }
+        }
+        sa[base] = a0;
+        sa[base + o1] = a1;
+        sb[base] = b0;
+        sb[base + o1] = b1;
+    }
+    int q = 0;
+    if (nch > 0 && op.gen[0] >= 0) wave_partial(red, buf, q++, g0, 0.0);
+    if (nch > 1 && op.gen[1] >= 0) wave_partial(red, buf, q++, g1, 0.0);
+    if (nch > 2 && op.gen[2] >= 0) wave_partial(red, buf, q++, g2, 0.0);
+    if (nch > 3 && op.gen[3] >= 0) wave_partial(red, buf, q++, g3, 0.0);
+}
+
 // MK_HAM: B += Dm A with Dm = M + M^dagger (op.mat2); acc += sum conj(A) (M A)
 template <int KB>
 __device__ __forceinline__ void mop_ham(const double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
@@ -319,6 +369,12 @@ __device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const M
                         else mop_pair<5>(sa, sb, t, op, pool, red, buf);
                     }
                 }
+                nred = op.ng;
+            }
+            break;
+        case MK_CHAIN:
+            if constexpr ((KS & (1 << MK_CHAIN)) != 0) {
+                mop_chain(sa, sb, t, op, pool, red, buf);
                 nred = op.ng;
             }
             break;
@@ -447,7 +503,10 @@ struct OpBits {
 // variants the micro-ops use (cached per (offset, variant)).
 struct XPoolEntry {
     int64_t src, dst;
-    int32_t D, v;
+    int32_t D, v;        // v: MF_CONJ|MF_TRANS bits, 4 = M + M^dagger, 8 = product of srcs
+    int64_t srcs[4];     // v = 8: factors in application order (first applied first)
+    int32_t nsrc;
+    int32_t pad;
 };
 struct XPool {
     std::vector<double> h;  // interleaved complex
@@ -460,6 +519,8 @@ struct XPool {
     int64_t size() const { return deferred ? len : int64_t(h.size() / 2); }
     // variant: MF_CONJ / MF_TRANS combination, or 4 for M + M^dagger
     int64_t variant(int64_t off, int D, int v);
+    // U_n ... U_2 U_1 for D x D factors given in application order (n <= 4)
+    int64_t product(const int64_t* srcs, int n, int D);
 };
 
 struct PlanBuilder {
@@ -481,3 +542,12 @@ int vqc_plan_forward(PlanBuilder& pb, int nbits, int n, int is_dm, int nops, con
 int vqc_plan_ham(PlanBuilder& pb, int nbits, int nterms, const vqc_term_t* terms, int64_t hoff, int t, int mlow);
 // Plan the reverse (adjoint) phases.
 int vqc_plan_reverse(PlanBuilder& pb, int nbits, int n, int is_dm, int nops, const vqc_op_t* ops, int t, int mlow);
+// Merge runs of one-bit micro-ops on the same bit: forward MK_APPLY runs become one
+// MK_APPLY with the product matrix, reverse MK_APPLY2 runs become MK_CHAIN.
+void vqc_fuse_chains(PlanBuilder& pb, size_t phase_begin, size_t phase_end);
+// first micro-op of phase i (the micro-op count when i is past the last phase)
+inline int vqc_phase_op_start(const PlanBuilder& pb, size_t i) {
+    return i < pb.phases.size() ? pb.phases[i].op_begin : int(pb.mops.size());
+}
+// micro-ops a KS_REV kernel runs
+inline bool vqc_rev_simple(const MicroOp& m) { return m.kind == MK_APPLY2 || m.kind == MK_CHAIN; }
