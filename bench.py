@@ -200,6 +200,64 @@ def bench_config3(vq, _lib, oracle, timer):
     return out
 
 
+def bench_config4(vq, _lib, count, seed=20261203):
+    """Serving: open-loop Poisson stream of energy + gradient requests, n log-uniform
+    in 10..26, depth U{4..40}, TFIM h ~ U(0.5, 1.5). The rate is set so that the
+    plain path (one request per call, one kernel per gate) is ~70% busy; the
+    continuous-batching server gets the same arrival trace."""
+    from vqchip.serve import Request, Server, poisson_arrivals, run_plain_fifo, run_server_open_loop, plain_eval
+    from vqchip.workloads import config4_request
+    rng = np.random.default_rng(seed)
+    t0 = time.perf_counter()
+    reqs = [Request(*config4_request(rng)) for _ in range(count)]
+    pack_s = time.perf_counter() - t0
+    # service time of every request on the plain path, back to back (also its closed-loop throughput)
+    states = {}
+    svc = np.zeros(count)
+    plain_res = []
+    for i, r in enumerate(reqs):
+        if r.n not in states:
+            states[r.n] = vq.StateVector(r.n)
+        s0 = time.perf_counter()
+        plain_res.append(plain_eval(r, states[r.n], fused=False))
+        svc[i] = time.perf_counter() - s0
+    del states
+    rate = 0.7 / float(np.mean(svc))
+    arr = poisson_arrivals(rate, count, np.random.default_rng(seed + 1))
+    lat_p, busy_p, span_p = run_plain_fifo(reqs, arr, fused=False)
+    log(f"config4: plain open loop done ({span_p:.1f} s)")
+    with Server(hbm_budget=288e9) as srv:
+        lat_s, span_s, res_s = run_server_open_loop(srv, reqs, arr)
+        st = srv.stats()
+    err = 0.0
+    for (E, g), (Ep, gp) in zip(res_s, plain_res):
+        err = max(err, abs(E - Ep), float(np.max(np.abs(g - gp))) if gp.size else 0.0)
+    pct = lambda v, q: round(float(np.percentile(v, q)) * 1e3, 2)
+    out = {
+        "workload": f"{count} requests, n=round(exp(U(ln10, ln26))), depth~U{{4..40}}, HEA layer, TFIM h~U(0.5,1.5), "
+                    "Poisson arrivals",
+        "rate_per_s": round(rate, 3),
+        "plain_mean_service_ms": round(float(np.mean(svc)) * 1e3, 2),
+        "plain_utilisation": round(busy_p / span_p, 3),
+        "plain_p50_ms": pct(lat_p, 50), "plain_p99_ms": pct(lat_p, 99), "plain_mean_ms": round(float(np.mean(lat_p)) * 1e3, 2),
+        "plain_evals_per_s": round(count / span_p, 3),
+        "plain_closed_loop_evals_per_s": round(count / float(np.sum(svc)), 3),
+        "served_p50_ms": pct(lat_s, 50), "served_p99_ms": pct(lat_s, 99), "served_mean_ms": round(float(np.mean(lat_s)) * 1e3, 2),
+        "served_evals_per_s": round(count / span_s, 3),
+        "p99_speedup": round(float(np.percentile(lat_p, 99) / np.percentile(lat_s, 99)), 2),
+        "p50_speedup": round(float(np.percentile(lat_p, 50) / np.percentile(lat_s, 50)), 2),
+        "server_batches": st["batches"],
+        "mean_batch_size": round(st["batched_requests"] / max(st["batches"], 1), 2),
+        "peak_live_GiB": round(st["peak_live_bytes"] / 2**30, 3),
+        "budget_GiB": round(st["budget_bytes"] / 2**30, 1),
+        "max_abs_diff_served_vs_plain": err,
+        "qubits_histogram": {int(k): int(v) for k, v in zip(*np.unique([r.n for r in reqs], return_counts=True))},
+        "pack_s": round(pack_s, 2),
+    }
+    log(f"config4: {out}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -209,7 +267,8 @@ def main():
     ap.add_argument("--layers", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--skip-extra", action="store_true")
-    ap.add_argument("--only", default="", help="comma list of parts to run: c0,c1,c2,c3 (default all)")
+    ap.add_argument("--only", default="", help="comma list of parts to run: c0,c1,c2,c3,c4 (default all)")
+    ap.add_argument("--serve-requests", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -304,7 +363,7 @@ def main():
         extra["pauli200_read_GBps"] = round(16 * (1 << n) * nt / (mse * 1e-3) / 1e9, 1)
         log(f"pauli energy {s2[0]:.6f} in {mse:.1f} ms")
 
-    parts = set(args.only.split(",")) if args.only else {"c0", "c2", "c3"}
+    parts = set(args.only.split(",")) if args.only else {"c0", "c2", "c3", "c4"}
     import oracle
     cores = int(os.environ.get("VQC_CPU_THREADS", min(16, os.cpu_count() or 1)))
     oracle.set_threads(cores)
@@ -315,6 +374,8 @@ def main():
         extra["config2"] = bench_config2(vq, _lib, oracle, timer)
     if "c3" in parts:
         extra["config3"] = bench_config3(vq, _lib, oracle, timer)
+    if "c4" in parts:
+        extra["config4"] = bench_config4(vq, _lib, args.serve_requests)
 
     # CPU baseline: the oracle (C++/OpenMP restatement of the reference's
     # threaded kernels) on the host cores, on the first gates of the same circuit

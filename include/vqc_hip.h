@@ -207,6 +207,48 @@ int vqc_batch_wait(vqc_batch* b, double* energies2, double* grads);
 int vqc_batch_arena_bytes(const vqc_batch* b, uint64_t* bytes);
 int vqc_batch_destroy(vqc_batch* b);
 
+/* ---- serving loop (no counterpart in the reference) ------------------------
+ * Continuous batching of energy + adjoint-gradient requests of mixed size:
+ * requests queue on submit; a scheduler thread packs pending requests of a
+ * similar cost that fit the device-memory budget into persistent-kernel batches
+ * on one of `lanes` HIP streams, several batches in flight at once, and hands
+ * each result to its waiter when its batch completes. A request large enough
+ * to fill the chip by itself runs alone on its lane through the tiled phase
+ * kernels. Each request is one
+ * vqc_instance_t whose op_begin / term_begin index the arrays given with it and
+ * whose matrices index the pools given with it (grad_begin is ignored).
+ * Results are identical to the same request evaluated alone. */
+typedef struct vqc_server_config {
+    uint64_t hbm_budget;  /* bytes of device memory the loop may hold (capped by free memory) */
+    int32_t lanes;        /* concurrent batches (HIP streams), 1..8 */
+    int32_t max_batch;    /* requests per batch */
+    double grid_frac;     /* share of the resident-workgroup capacity one batch may take (0, 1] */
+    double cost_ratio;    /* largest/smallest work estimate allowed in one batch (<= 1: no limit) */
+    int32_t poll_us;      /* completion polling interval while batches run */
+    int32_t solo_qubits;  /* requests of at least this many qubits run alone (0: never) */
+} vqc_server_config_t;
+
+typedef struct vqc_server_stats {
+    int64_t completed, batches, batched_requests, pending, running;
+    uint64_t budget_bytes, reserved_bytes, peak_live_bytes;
+} vqc_server_stats_t;
+
+typedef struct vqc_server vqc_server;
+int vqc_server_default_config(vqc_server_config_t* cfg);
+int vqc_server_create(const vqc_server_config_t* cfg, vqc_server** out);
+/* Copy one request in and queue it; *ticket names it in poll / wait. */
+int vqc_server_submit(vqc_server* s, const vqc_instance_t* inst, const vqc_op_t* ops, const vqc_term_t* terms,
+                      const double* pool, int64_t pool_len, const double* hpool, int64_t hpool_len,
+                      int64_t* ticket);
+/* 1 while the request runs; 0 (and results) once done; < 0 on its failure.
+ * times3 = (submit, launch, done) in seconds since the server started. A
+ * finished ticket is released by the call that returns its result. */
+int vqc_server_poll(vqc_server* s, int64_t ticket, double* energy2, double* grad, double* times3);
+int vqc_server_wait(vqc_server* s, int64_t ticket, double* energy2, double* grad, double* times3);
+int vqc_server_stats(vqc_server* s, vqc_server_stats_t* out);
+/* Finishes every submitted request, then frees the server. */
+int vqc_server_destroy(vqc_server* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -19,11 +19,12 @@
 #include <cstdlib>
 #include <queue>
 
+#include "vqc_batch_internal.h"
 #include "vqc_tile.h"
 
 void vqc_zero(double2* v, uint64_t len, hipStream_t st);
 
-constexpr int kMaxChunks = 256;  // tile chunks (and partial columns) per phase
+constexpr int kMaxChunks = 1024;  // tile chunks (and partial columns) per phase
 constexpr int kItemSmall = 0, kItemPhase = 1;
 
 struct BItem {
@@ -125,10 +126,8 @@ __global__ void __launch_bounds__(256) k_zero_u32(unsigned* p, int n) {
 constexpr uint64_t kWaitTicks = 500000000ull;
 
 template <int MAXKB, int RKS>
-__global__ void __launch_bounds__(kTileThreads) k_batch(BatchArgs a) {
+__device__ __forceinline__ void batch_loop(const BatchArgs& a, int& s_item, int& s_ok) {
     extern __shared__ double2 smem[];
-    __shared__ int s_item;
-    __shared__ int s_ok;
     // Every branch of this loop is wave-uniform. The queue head and the done
     // counters are touched by all 64 lanes of wave 0 (lane 0 adds 1, the others
     // add 0) instead of by thread 0 alone: a lane-divergent region next to the
@@ -156,7 +155,20 @@ __global__ void __launch_bounds__(kTileThreads) k_batch(BatchArgs a) {
                     atomicOr(&a.counters[1], one);
                     break;
                 }
-                __builtin_amdgcn_s_sleep(32);  // ~2k cycles between polls of the counter
+                // back off by the number of items still ahead: many workgroups
+                // can wait on one instance, and their polls of its counter must
+                // not crowd out the increments of the workgroups that finish
+                const unsigned rem = unsigned(it.need) - done;
+                if (rem > 32) {
+                    __builtin_amdgcn_s_sleep(127);
+                    __builtin_amdgcn_s_sleep(127);
+                    __builtin_amdgcn_s_sleep(127);
+                    __builtin_amdgcn_s_sleep(127);
+                } else if (rem > 4) {
+                    __builtin_amdgcn_s_sleep(127);
+                } else {
+                    __builtin_amdgcn_s_sleep(16);
+                }
             }
             // one wave's acquire invalidates the CU's L1 and this XCD's stale L2
             // lines for the whole workgroup; the barrier below orders the rest
@@ -211,6 +223,23 @@ __global__ void __launch_bounds__(kTileThreads) k_batch(BatchArgs a) {
     }
 }
 
+// One workgroup per CU, any register count (5-qubit micro-ops need them).
+template <int MAXKB, int RKS>
+__global__ void __launch_bounds__(kTileThreads) k_batch(BatchArgs a) {
+    __shared__ int s_item;
+    __shared__ int s_ok;
+    batch_loop<MAXKB, RKS>(a, s_item, s_ok);
+}
+
+// At most 128 VGPRs, so two 512-thread workgroups fit a CU when their LDS does
+// (four waves per SIMD).
+template <int MAXKB, int RKS>
+__global__ void __launch_bounds__(kTileThreads, 4) k_batch2(BatchArgs a) {
+    __shared__ int s_item;
+    __shared__ int s_ok;
+    batch_loop<MAXKB, RKS>(a, s_item, s_ok);
+}
+
 // out[r] = sum of row r's partials, in a fixed order
 __global__ void __launch_bounds__(256) k_reduce_rows(const double2* __restrict__ parts, const int64_t* __restrict__ off,
                                                      const int32_t* __restrict__ cnt, double2* out) {
@@ -225,6 +254,11 @@ __global__ void __launch_bounds__(256) k_reduce_rows(const double2* __restrict__
     sr = block_sum(sr, red);
     si = block_sum(si, red);
     if (threadIdx.x == 0) out[blockIdx.x] = make_double2(sr, si);
+}
+
+// rows[nrows].x = the error flag, so one download brings results and status
+__global__ void k_finish(const unsigned* counters, double2* flag_slot) {
+    if (threadIdx.x == 0) *flag_slot = make_double2(double(counters[1]), 0.0);
 }
 
 // ---------------------------------------------------------------------------
@@ -242,7 +276,11 @@ struct vqc_batch {
     size_t lds = 0;
     unsigned grid = 0;
     int maxkb = 1;
-    bool rev_simple = true;  // every reverse micro-op is MK_APPLY2
+    bool rev_simple = true;  // every reverse micro-op is MK_APPLY2 or MK_CHAIN
+    BatchOpts opts;
+    hipStream_t stream = nullptr;
+    void* d_block = nullptr;  // every device buffer below is carved from this block
+    size_t block_bytes = 0;
     // device
     void* d_items = nullptr;
     void* d_inst = nullptr;
@@ -271,15 +309,20 @@ struct vqc_batch {
 
 typedef void (*BatchKernel)(BatchArgs);
 static BatchKernel batch_kernel(const vqc_batch* b) {
-    if (b->maxkb <= 2) return b->rev_simple ? k_batch<2, KS_REV> : k_batch<2, KS_REVX>;
+    if (b->maxkb <= 2) {
+        static const bool occ1 = getenv("VQC_BATCH_OCC1") != nullptr;  // one workgroup per CU
+        if (occ1) return b->rev_simple ? k_batch<2, KS_REV> : k_batch<2, KS_REVX>;
+        return b->rev_simple ? k_batch2<2, KS_REV> : k_batch2<2, KS_REVX>;
+    }
     if (b->maxkb <= 3) return b->rev_simple ? k_batch<3, KS_REV> : k_batch<3, KS_REVX>;
     return b->rev_simple ? k_batch<5, KS_REV> : k_batch<5, KS_REVX>;
 }
 
-static int batch_enqueue(vqc_batch* b, hipStream_t st) {
+static int batch_enqueue(vqc_batch* b, hipStream_t st, const double* h_pool, double* h_rows) {
     BatchArgs a{(const BItem*)b->d_items, b->nitems, (const BInst*)b->d_inst, (const PhaseDesc*)b->d_phases,
                 (const MicroOp*)b->d_mops, b->d_pool, b->d_arena, b->d_parts, (const int64_t*)b->d_row_off, b->d_counters};
-    VQC_HIP(hipMemcpyAsync(b->d_pool, b->h_pool, size_t(b->pool_len + b->hpool_len) * 16, hipMemcpyHostToDevice, st));
+    if (b->pool_len + b->hpool_len)
+        VQC_HIP(hipMemcpyAsync(b->d_pool, h_pool, size_t(b->pool_len + b->hpool_len) * 16, hipMemcpyHostToDevice, st));
     if (b->nvar) hipLaunchKernelGGL(k_variants, dim3(std::min(b->nvar, 4096)), dim3(256), 0, st, b->d_pool, (const VarEntry*)b->d_var, b->nvar);
     if (b->nprod)
         hipLaunchKernelGGL(k_products, dim3((b->nprod + 255) / 256), dim3(256), 0, st, b->d_pool, (const ProdEntry*)b->d_prod, b->nprod);
@@ -289,7 +332,8 @@ static int batch_enqueue(vqc_batch* b, hipStream_t st) {
     if (b->nrows)
         hipLaunchKernelGGL(k_reduce_rows, dim3(b->nrows), dim3(256), 0, st, b->d_parts, (const int64_t*)b->d_row_off,
                            (const int32_t*)b->d_row_cnt, b->d_rows);
-    VQC_HIP(hipMemcpyAsync(b->h_rows, b->d_rows, size_t(std::max(b->nrows, 1)) * 16 + 16, hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(k_finish, dim3(1), dim3(64), 0, st, b->d_counters, b->d_rows + b->nrows);
+    VQC_HIP(hipMemcpyAsync(h_rows, b->d_rows, size_t(b->nrows + 1) * 16, hipMemcpyDeviceToHost, st));
     VQC_HIP(hipGetLastError());
     return VQC_OK;
 }
@@ -297,26 +341,44 @@ static int batch_enqueue(vqc_batch* b, hipStream_t st) {
 static int batch_free(vqc_batch* b) {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    void* dev[] = {b->d_items, b->d_inst, b->d_phases, b->d_mops, b->d_var, b->d_prod, b->d_pool, b->d_arena, b->d_parts, b->d_rows, b->d_counters,
-                   b->d_row_off, b->d_row_cnt};
-    for (void* p : dev)
-        if (p) (void)hipFree(p);
+    if (b->d_block) {
+        if (b->opts.dev_free) b->opts.dev_free(b->opts.alloc_ctx, b->d_block);
+        else (void)hipFree(b->d_block);
+    }
     if (b->h_pool) (void)hipHostFree(b->h_pool);
     if (b->h_rows) (void)hipHostFree(b->h_rows);
     delete b;
     return VQC_OK;
 }
 
+int vqc_batch_release(vqc_batch* b) { return b ? batch_free(b) : VQC_OK; }
+
+void vqc_batch_sizes(const vqc_batch* b, size_t* device_bytes, size_t* pool_staging_bytes, size_t* rows_staging_bytes) {
+    if (device_bytes) *device_bytes = b->block_bytes;
+    if (pool_staging_bytes) *pool_staging_bytes = size_t(std::max<int64_t>(b->pool_len + b->hpool_len, 1)) * 16;
+    if (rows_staging_bytes) *rows_staging_bytes = size_t(b->nrows + 1) * 16;
+}
+
 extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc_op_t* ops, int64_t nops_total,
                                 const vqc_term_t* terms, int64_t nterms_total, int64_t pool_len, int64_t hpool_len,
                                 int flags, vqc_batch** out) {
     (void)flags;
+    BatchOpts o;
+    o.graph = getenv("VQC_NO_GRAPH") == nullptr;
+    return vqc_batch_build(ninst, inst, ops, nops_total, terms, nterms_total, pool_len, hpool_len, o, out);
+}
+
+int vqc_batch_build(int ninst, const vqc_instance_t* inst, const vqc_op_t* ops, int64_t nops_total,
+                    const vqc_term_t* terms, int64_t nterms_total, int64_t pool_len, int64_t hpool_len,
+                    const BatchOpts& opts, vqc_batch** out) {
     if (!out || ninst < 1 || !inst || nops_total < 0 || nterms_total < 0 || pool_len < 0 || hpool_len < 0)
         return vqc_fail(VQC_EINVAL, "vqc_batch_create: bad arguments");
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));
     const TileTuning tt = vqc_tile_tuning();
     vqc_batch* b = new vqc_batch;
+    b->opts = opts;
+    b->stream = opts.stream ? opts.stream : c->stream;
     b->ninst = ninst;
     b->inst.assign(inst, inst + ninst);
     b->pool_len = pool_len;
@@ -442,13 +504,29 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
         if (ph.kind == PH_REV)
             for (int o = ph.op_begin; o < ph.op_end; ++o) b->rev_simple = b->rev_simple && vqc_rev_simple(pb.mops[o]);
 
+    // ---- persistent grid: as many workgroups as can be resident (times grid_frac)
+    b->lds = std::max<size_t>(b->lds, 16 * 1024);
+    const void* kfn = (const void*)batch_kernel(b);
+    VQC_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, int(b->lds)));
+    int per_cu = 0;
+    VQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kTileThreads, b->lds));
+    if (per_cu < 1) {
+        batch_free(b);
+        return vqc_fail(VQC_EUNSUPPORTED, "vqc_batch_create: %zu bytes of LDS per workgroup do not fit", b->lds);
+    }
+    const int64_t resident = int64_t(per_cu) * c->cu_count;
+    int64_t grid_cap = std::min<int64_t>(
+        kMaxChunks, std::max<int64_t>(1, int64_t(double(resident) * std::min(1.0, std::max(opts.grid_frac, 0.0)))));
+    if (const char* e = getenv("VQC_BATCH_GRID")) grid_cap = std::max<int64_t>(1, std::min<int64_t>(grid_cap, atoll(e)));
+
     // ---- items in list-scheduling order ---------------------------------
+    // a phase is cut into one chunk per workgroup of the grid (or per tile when
+    // it has fewer), so a phase that fills the chip ends in one round
     std::vector<BItem> items;
     struct PhaseItems {
         std::vector<BItem> v;
         double cost_each;
     };
-    const int workers = std::max(1, c->cu_count);
     // per instance: list of phases, each a list of chunk items (need filled in order)
     std::vector<std::vector<PhaseItems>> plist(ninst);
     std::vector<double> remaining(ninst, 0.0);
@@ -464,7 +542,7 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
             const int pg = chains[i].first_phase + q;
             const PhaseDesc& ph = pb.phases[pg];
             const int64_t ntiles = int64_t(1) << (ph.nbits - ph.t);
-            const int nch = int(std::min<int64_t>(ntiles, kMaxChunks));
+            const int nch = int(std::min<int64_t>(ntiles, grid_cap));
             PhaseItems pi;
             const double nbuf = ph.kind == PH_FWD ? 2 : 4;
             const double pcost = double(int64_t(1) << ph.nbits) * (nbuf + 0.25 * (ph.op_end - ph.op_begin));
@@ -478,6 +556,15 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
             plist[i].push_back(pi);
         }
     }
+    // no more workgroups than the instances can keep busy at once (the widest
+    // phase of each); extra ones would only wait
+    int64_t width = 0;
+    for (int i = 0; i < ninst; ++i) {
+        int64_t w = 1;
+        for (const PhaseItems& pi : plist[i]) w = std::max<int64_t>(w, int64_t(pi.v.size()));
+        width += w;
+    }
+    const int workers = int(std::max<int64_t>(1, std::min(grid_cap, width)));
     // simulation: workers free at times; an instance's next phase is ready when all chunks of the previous are done
     std::priority_queue<double, std::vector<double>, std::greater<double>> free_at;
     for (int w = 0; w < workers; ++w) free_at.push(0.0);
@@ -535,7 +622,7 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
         if (chains[i].small) continue;
         for (int q = 0; q < chains[i].nphases; ++q) {
             const PhaseDesc& ph = pb.phases[chains[i].first_phase + q];
-            const int nch = int(std::min<int64_t>(int64_t(1) << (ph.nbits - ph.t), kMaxChunks));
+            const int nch = int(std::min<int64_t>(int64_t(1) << (ph.nbits - ph.t), grid_cap));
             for (int e = 0; e < ph.nacc; ++e) row_cnt[ph.row_begin + e] = nch;
         }
     }
@@ -550,7 +637,6 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
     // ---- device buffers -------------------------------------------------
     b->xlen = xp.size();
     b->arena_bytes = uint64_t(arena) * 16;
-    auto dalloc = [&](void** p, size_t bytes) { return hipMalloc(p, std::max<size_t>(bytes, 64)); };
     // variants first (one launch), then the chain products that may read them
     std::vector<VarEntry> vt;
     std::vector<ProdEntry> pt;
@@ -565,28 +651,62 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
     }
     b->nvar = int(vt.size());
     b->nprod = int(pt.size());
+    // one device block, carved into 256-byte aligned buffers
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t o = off;
+        off += (std::max<size_t>(bytes, 16) + 255) & ~size_t(255);
+        return o;
+    };
+    const size_t o_items = carve(items.size() * sizeof(BItem));
+    const size_t o_inst = carve(binst.size() * sizeof(BInst));
+    const size_t o_phases = carve(pb.phases.size() * sizeof(PhaseDesc));
+    const size_t o_mops = carve(pb.mops.size() * sizeof(MicroOp));
+    const size_t o_var = carve(vt.size() * sizeof(VarEntry));
+    const size_t o_prod = carve(pt.size() * sizeof(ProdEntry));
+    const size_t o_pool = carve(size_t(b->xlen) * 16);
+    const size_t o_arena = carve(size_t(arena) * 16);
+    const size_t o_parts = carve(size_t(std::max<int64_t>(b->nparts, 1)) * 16);
+    const size_t o_row_off = carve(row_off.size() * sizeof(int64_t));
+    const size_t o_row_cnt = carve(row_cnt.size() * sizeof(int32_t));
+    const size_t o_rows = carve(size_t(b->nrows + 1) * 16);
+    const size_t o_counters = carve(size_t(ninst + 2) * 4);
+    b->block_bytes = off;
     hipError_t e = hipSuccess;
-    if (e == hipSuccess) e = dalloc(&b->d_items, items.size() * sizeof(BItem));
-    if (e == hipSuccess) e = dalloc(&b->d_inst, binst.size() * sizeof(BInst));
-    if (e == hipSuccess) e = dalloc(&b->d_phases, pb.phases.size() * sizeof(PhaseDesc));
-    if (e == hipSuccess) e = dalloc(&b->d_mops, pb.mops.size() * sizeof(MicroOp));
-    if (e == hipSuccess) e = dalloc(&b->d_var, vt.size() * sizeof(VarEntry));
-    if (e == hipSuccess) e = dalloc(&b->d_prod, pt.size() * sizeof(ProdEntry));
-    if (e == hipSuccess) e = dalloc((void**)&b->d_pool, size_t(b->xlen) * 16);
-    if (e == hipSuccess) e = dalloc((void**)&b->d_arena, size_t(arena) * 16);
-    if (e == hipSuccess) e = dalloc((void**)&b->d_parts, size_t(std::max<int64_t>(b->nparts, 1)) * 16);
-    if (e == hipSuccess) e = dalloc(&b->d_row_off, row_off.size() * sizeof(int64_t));
-    if (e == hipSuccess) e = dalloc(&b->d_row_cnt, row_cnt.size() * sizeof(int32_t));
-    if (e == hipSuccess) e = dalloc((void**)&b->d_rows, size_t(std::max(b->nrows, 1)) * 16 + 16);
-    if (e == hipSuccess) e = dalloc((void**)&b->d_counters, size_t(ninst + 2) * 4);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_pool, size_t(std::max<int64_t>(pool_len + hpool_len, 1)) * 16, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_rows, size_t(std::max(b->nrows, 1)) * 16 + 16, hipHostMallocDefault);
+    if (opts.dev_alloc) {
+        b->d_block = opts.dev_alloc(opts.alloc_ctx, off);
+        if (!b->d_block) e = hipErrorOutOfMemory;
+    } else {
+        e = hipMalloc(&b->d_block, off);
+    }
+    if (e == hipSuccess) {
+        char* base = (char*)b->d_block;
+        b->d_items = base + o_items;
+        b->d_inst = base + o_inst;
+        b->d_phases = base + o_phases;
+        b->d_mops = base + o_mops;
+        b->d_var = base + o_var;
+        b->d_prod = base + o_prod;
+        b->d_pool = (double2*)(base + o_pool);
+        b->d_arena = (double2*)(base + o_arena);
+        b->d_parts = (double2*)(base + o_parts);
+        b->d_row_off = base + o_row_off;
+        b->d_row_cnt = base + o_row_cnt;
+        b->d_rows = (double2*)(base + o_rows);
+        b->d_counters = (unsigned*)(base + o_counters);
+    } else {
+        b->d_block = nullptr;
+    }
+    if (e == hipSuccess && opts.pinned) {
+        e = hipHostMalloc((void**)&b->h_pool, size_t(std::max<int64_t>(pool_len + hpool_len, 1)) * 16, hipHostMallocDefault);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_rows, size_t(b->nrows + 1) * 16, hipHostMallocDefault);
+    }
     if (e != hipSuccess) {
         (void)hipGetLastError();
         batch_free(b);
-        return vqc_fail(VQC_ENOMEM, "vqc_batch_create: device allocation failed (%s)", hipGetErrorString(e));
+        return vqc_fail(VQC_ENOMEM, "vqc_batch_create: device allocation of %zu bytes failed (%s)", off, hipGetErrorString(e));
     }
-    hipStream_t st = c->stream;
+    hipStream_t st = b->stream;
     VQC_HIP(hipMemcpyAsync(b->d_items, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice, st));
     VQC_HIP(hipMemcpyAsync(b->d_inst, binst.data(), binst.size() * sizeof(BInst), hipMemcpyHostToDevice, st));
     if (!pb.phases.empty())
@@ -599,22 +719,12 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
     VQC_HIP(hipMemcpyAsync(b->d_row_cnt, row_cnt.data(), row_cnt.size() * sizeof(int32_t), hipMemcpyHostToDevice, st));
     VQC_HIP(hipStreamSynchronize(st));
 
-    // persistent grid: as many workgroups as can be resident
-    b->lds = std::max<size_t>(b->lds, 16 * 1024);
-    const void* kfn = (const void*)batch_kernel(b);
-    VQC_HIP(hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, int(b->lds)));
-    int per_cu = 0;
-    VQC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kTileThreads, b->lds));
-    if (per_cu < 1) {
-        batch_free(b);
-        return vqc_fail(VQC_EUNSUPPORTED, "vqc_batch_create: %zu bytes of LDS per workgroup do not fit", b->lds);
-    }
-    b->grid = unsigned(std::min<int64_t>(int64_t(per_cu) * c->cu_count, std::max(b->nitems, 1)));
+    b->grid = unsigned(std::min<int64_t>({grid_cap, std::max(b->nitems, 1), width}));
 
     // capture the run as a graph (replayed by vqc_batch_run / vqc_batch_launch)
-    if (!getenv("VQC_NO_GRAPH")) {
+    if (opts.graph && opts.pinned) {
         hipError_t ce = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-        int rc = ce == hipSuccess ? batch_enqueue(b, st) : VQC_EHIP;
+        int rc = ce == hipSuccess ? batch_enqueue(b, st, b->h_pool, b->h_rows) : VQC_EHIP;
         hipGraph_t g = nullptr;
         hipError_t ee = hipStreamEndCapture(st, &g);
         if (ce == hipSuccess && rc == VQC_OK && ee == hipSuccess && g) {
@@ -632,30 +742,22 @@ extern "C" int vqc_batch_create(int ninst, const vqc_instance_t* inst, const vqc
     return VQC_OK;
 }
 
-extern "C" int vqc_batch_launch(vqc_batch* b, const double* pool, const double* hpool) {
-    if (!b) return vqc_fail(VQC_EINVAL, "null batch");
+int vqc_batch_enqueue_run(vqc_batch* b, const double* pool, const double* hpool, double* h_pool, double* h_rows) {
     if ((b->pool_len && !pool) || (b->hpool_len && !hpool)) return vqc_fail(VQC_EINVAL, "null matrix pool");
-    if (b->launched) return vqc_fail(VQC_EINVAL, "vqc_batch_launch: previous launch not waited for");
-    VqcCtx* c;
-    VQC_TRY(vqc_ctx(&c));
-    if (b->pool_len) memcpy(b->h_pool, pool, size_t(b->pool_len) * 16);
-    if (b->hpool_len) memcpy(b->h_pool + 2 * b->pool_len, hpool, size_t(b->hpool_len) * 16);
-    if (b->exec) VQC_HIP(hipGraphLaunch(b->exec, c->stream));
-    else VQC_TRY(batch_enqueue(b, c->stream));
-    b->launched = true;
+    const bool own = h_pool == nullptr;
+    double* hp = own ? b->h_pool : h_pool;
+    double* hr = h_rows ? h_rows : b->h_rows;
+    if (!hp || !hr) return vqc_fail(VQC_EINVAL, "vqc_batch: no staging buffers");
+    if (b->pool_len && hp != pool) memcpy(hp, pool, size_t(b->pool_len) * 16);
+    if (b->hpool_len && hp + 2 * b->pool_len != hpool) memcpy(hp + 2 * b->pool_len, hpool, size_t(b->hpool_len) * 16);
+    if (b->exec && own && hr == b->h_rows) VQC_HIP(hipGraphLaunch(b->exec, b->stream));
+    else VQC_TRY(batch_enqueue(b, b->stream, hp, hr));
     return VQC_OK;
 }
 
-extern "C" int vqc_batch_wait(vqc_batch* b, double* energies2, double* grads) {
-    if (!b) return vqc_fail(VQC_EINVAL, "null batch");
-    if (!b->launched) return vqc_fail(VQC_EINVAL, "vqc_batch_wait: nothing launched");
-    VqcCtx* c;
-    VQC_TRY(vqc_ctx(&c));
-    b->launched = false;
-    VQC_HIP(hipStreamSynchronize(c->stream));
-    unsigned flag = 0;
-    VQC_HIP(hipMemcpy(&flag, b->d_counters + 1, 4, hipMemcpyDeviceToHost));
-    if (flag) return vqc_fail(VQC_EHIP, "vqc_batch: a work item exceeded its dependency wait (planning error)");
+int vqc_batch_collect(const vqc_batch* b, const double* h_rows, double* energies2, double* grads) {
+    const double* rows = h_rows ? h_rows : b->h_rows;
+    if (rows[2 * b->nrows] != 0.0) return vqc_fail(VQC_EHIP, "vqc_batch: a work item exceeded its dependency wait (planning error)");
     if (energies2)
         for (int i = 0; i < 2 * b->ninst; ++i) energies2[i] = 0;
     if (grads)
@@ -665,14 +767,30 @@ extern "C" int vqc_batch_wait(vqc_batch* b, double* energies2, double* grads) {
         const int s = b->row_slot[r];
         if (s < 0) {
             if (energies2) {
-                energies2[2 * i] += b->h_rows[2 * r];
-                energies2[2 * i + 1] += b->h_rows[2 * r + 1];
+                energies2[2 * i] += rows[2 * r];
+                energies2[2 * i + 1] += rows[2 * r + 1];
             }
         } else if (grads) {
-            grads[b->inst[i].grad_begin + s] += b->h_rows[2 * r];
+            grads[b->inst[i].grad_begin + s] += rows[2 * r];
         }
     }
     return VQC_OK;
+}
+
+extern "C" int vqc_batch_launch(vqc_batch* b, const double* pool, const double* hpool) {
+    if (!b) return vqc_fail(VQC_EINVAL, "null batch");
+    if (b->launched) return vqc_fail(VQC_EINVAL, "vqc_batch_launch: previous launch not waited for");
+    VQC_TRY(vqc_batch_enqueue_run(b, pool, hpool, nullptr, nullptr));
+    b->launched = true;
+    return VQC_OK;
+}
+
+extern "C" int vqc_batch_wait(vqc_batch* b, double* energies2, double* grads) {
+    if (!b) return vqc_fail(VQC_EINVAL, "null batch");
+    if (!b->launched) return vqc_fail(VQC_EINVAL, "vqc_batch_wait: nothing launched");
+    b->launched = false;
+    VQC_HIP(hipStreamSynchronize(b->stream));
+    return vqc_batch_collect(b, nullptr, energies2, grads);
 }
 
 extern "C" int vqc_batch_run(vqc_batch* b, const double* pool, const double* hpool, double* energies2, double* grads) {
@@ -688,8 +806,6 @@ extern "C" int vqc_batch_arena_bytes(const vqc_batch* b, uint64_t* bytes) {
 
 extern "C" int vqc_batch_destroy(vqc_batch* b) {
     if (!b) return VQC_OK;
-    VqcCtx* c;
-    VQC_TRY(vqc_ctx(&c));
-    VQC_HIP(hipStreamSynchronize(c->stream));
+    VQC_HIP(hipStreamSynchronize(b->stream));
     return batch_free(b);
 }

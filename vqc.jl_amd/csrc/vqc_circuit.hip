@@ -144,13 +144,14 @@ static void plan_groups(PlanBuilder& pb, int kind, int nbits, int nitems, int t,
 
 int64_t XPool::variant(int64_t off, int D, int v) {
     if (v == 0) return off;
-    for (auto& e : cache)
-        if (e.first.first == off && e.first.second == v) return e.second;
+    const uint64_t key = (uint64_t(off) << 4) | uint64_t(v);
+    const auto hit = cache.find(key);
+    if (hit != cache.end()) return hit->second;
     const int64_t out = size();
+    cache.emplace(key, out);
     if (deferred) {
         len += int64_t(D) * D;
         table.push_back({off, out, D, v});
-        cache.push_back({{off, v}, out});
         return out;
     }
     h.resize(h.size() + size_t(2) * D * D);
@@ -171,7 +172,6 @@ int64_t XPool::variant(int64_t off, int D, int v) {
             R[dst] = re;
             R[dst + 1] = im;
         }
-    cache.push_back({{off, v}, out});
     return out;
 }
 
@@ -1032,3 +1032,114 @@ extern "C" int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_
     finish_rows(rows, pb.row_slot, nparams, energy2, grad);
     return VQC_OK;
 }
+
+// ------------------------- one evaluation on a caller's stream ---------------
+// The tiled energy + gradient path of vqc_sv_energy_grad for |0..0>, with every
+// device buffer in one caller-owned block and every launch on a caller's
+// stream, so that several evaluations can be in flight at once (serving loop).
+struct vqc_solo {
+    XPool xp;
+    PlanBuilder pb;
+    size_t nfwd = 0;
+    int nbits = 0, nparams = 0, nrows = 0;
+    static constexpr int kStride = 1024;
+    size_t o_A = 0, o_B = 0, o_pool = 0, o_mops = 0, o_phases = 0, o_parts = 0, o_rowsum = 0, bytes = 0;
+};
+
+int vqc_solo_build(int n, int nops, const vqc_op_t* ops, const double* pool, int64_t pool_len, int nterms,
+                   const vqc_term_t* terms, const double* hpool, int64_t hpool_len, int nparams, vqc_solo** out) {
+    if (!out || n < 1 || n > 34 || nops < 0 || nterms < 0 || nparams < 0) return vqc_fail(VQC_EINVAL, "vqc_solo: bad arguments");
+    int64_t plen = 0, hlen = 0;
+    VQC_TRY(check_program(n, 0, nops, ops, &plen, nparams));
+    VQC_TRY(check_terms(n, nterms, terms, &hlen));
+    if (plen > pool_len || hlen > hpool_len) return vqc_fail(VQC_EINVAL, "vqc_solo: matrix offsets outside the pools");
+    if ((plen && !pool) || (hlen && !hpool)) return vqc_fail(VQC_EINVAL, "vqc_solo: null matrix pool");
+    const TileTuning tt = vqc_tile_tuning();
+    if (n <= tt.t_fwd) return vqc_fail(VQC_EINVAL, "vqc_solo: %d qubits fit one tile; batch them instead", n);
+    vqc_solo* s = new vqc_solo;
+    s->nbits = n;
+    s->nparams = nparams;
+    s->xp.h.assign(pool, pool + 2 * plen);
+    s->xp.h.insert(s->xp.h.end(), hpool, hpool + 2 * hlen);
+    PlanBuilder& pb = s->pb;
+    pb.xp = &s->xp;
+    int rc = vqc_plan_forward(pb, n, n, 0, nops, ops, tt.t_fwd, tt.mlow);
+    if (rc == VQC_OK && pb.phases.empty()) {  // no ops: one empty phase writes |0..0> into A
+        PhaseDesc p;
+        vqc_make_phase(p, PH_FWD, n, tt.t_fwd, {});
+        p.op_begin = p.op_end = 0;
+        p.row_begin = pb.nrows;
+        pb.phases.push_back(p);
+    }
+    if (rc == VQC_OK) {
+        pb.phases[0].flags |= PF_INIT_ZERO;
+        s->nfwd = pb.phases.size();
+        rc = vqc_plan_ham(pb, n, nterms, terms, plen, tt.t_rev, tt.mlow);
+    }
+    if (rc == VQC_OK) rc = vqc_plan_reverse(pb, n, n, 0, nops, ops, tt.t_rev, tt.mlow);
+    if (rc != VQC_OK) {
+        delete s;
+        return rc;
+    }
+    vqc_fuse_chains(pb, 0, pb.phases.size());
+    s->nrows = pb.nrows;
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t o = off;
+        off += (std::max<size_t>(bytes, 16) + 255) & ~size_t(255);
+        return o;
+    };
+    const size_t L = size_t(1) << n;
+    s->o_A = carve(L * 16);
+    s->o_B = carve(L * 16);
+    s->o_pool = carve(s->xp.h.size() * 8);
+    s->o_mops = carve(pb.mops.size() * sizeof(MicroOp));
+    s->o_phases = carve(pb.phases.size() * sizeof(PhaseDesc));
+    s->o_parts = carve(size_t(std::max(s->nrows, 1)) * vqc_solo::kStride * 16);
+    s->o_rowsum = carve(size_t(s->nrows + 1) * 16);
+    s->bytes = off;
+    *out = s;
+    return VQC_OK;
+}
+
+size_t vqc_solo_device_bytes(const vqc_solo* s) { return s->bytes; }
+size_t vqc_solo_rows_bytes(const vqc_solo* s) { return size_t(s->nrows + 1) * 16; }
+
+int vqc_solo_enqueue(vqc_solo* s, void* block, hipStream_t st, double* h_rows) {
+    char* base = (char*)block;
+    double2* A = (double2*)(base + s->o_A);
+    double2* B = (double2*)(base + s->o_B);
+    const double2* dp = (const double2*)(base + s->o_pool);
+    const MicroOp* dm = (const MicroOp*)(base + s->o_mops);
+    const PhaseDesc* dpd = (const PhaseDesc*)(base + s->o_phases);
+    double2* parts = (double2*)(base + s->o_parts);
+    double2* rowsum = (double2*)(base + s->o_rowsum);
+    const PlanBuilder& pb = s->pb;
+    if (!s->xp.h.empty())
+        VQC_HIP(hipMemcpyAsync(base + s->o_pool, s->xp.h.data(), s->xp.h.size() * 8, hipMemcpyHostToDevice, st));
+    if (!pb.mops.empty())
+        VQC_HIP(hipMemcpyAsync(base + s->o_mops, pb.mops.data(), pb.mops.size() * sizeof(MicroOp), hipMemcpyHostToDevice, st));
+    VQC_HIP(hipMemcpyAsync(base + s->o_phases, pb.phases.data(), pb.phases.size() * sizeof(PhaseDesc), hipMemcpyHostToDevice, st));
+    const int nrows = std::max(s->nrows, 1);
+    vqc_zero(parts, size_t(nrows) * vqc_solo::kStride, st);
+    VQC_TRY(run_forward(pb, 0, s->nfwd, nullptr, A, dm, dp, dpd, st));
+    if (pb.phases.size() == s->nfwd || pb.phases[s->nfwd].kind != PH_HAM) vqc_zero(B, size_t(1) << s->nbits, st);
+    for (size_t i = s->nfwd; i < pb.phases.size(); ++i) {
+        PhaseBufs b{A, A, B, dm, dp, parts, vqc_solo::kStride};
+        VQC_TRY(launch_phase(pb, i, dpd, b, phase_grid(pb.phases[i], vqc_solo::kStride), st));
+    }
+    if (s->nrows) vqc_reduce_partials(parts, s->nrows, vqc_solo::kStride, rowsum, st);
+    VQC_HIP(hipGetLastError());
+    VQC_HIP(hipMemcpyAsync(h_rows, rowsum, size_t(s->nrows + 1) * 16, hipMemcpyDeviceToHost, st));
+    return VQC_OK;
+}
+
+int vqc_solo_collect(const vqc_solo* s, const double* h_rows, double* energy2, double* grad) {
+    std::vector<double> rows(h_rows, h_rows + 2 * size_t(std::max(s->nrows, 1)));
+    std::vector<double> g(size_t(std::max(s->nparams, 1)));
+    finish_rows(rows, s->pb.row_slot, s->nparams, energy2, g.data());
+    if (grad && s->nparams) memcpy(grad, g.data(), size_t(s->nparams) * sizeof(double));
+    return VQC_OK;
+}
+
+void vqc_solo_free(vqc_solo* s) { delete s; }

@@ -23,6 +23,8 @@
 // every variant a micro-op needs (U, U^dagger, conj(U), U^T, M + M^dagger), so
 // the inner loops are plain complex matrix-vector products.
 #pragma once
+#include <unordered_map>
+
 #include "vqc_internal.h"
 
 enum { MK_APPLY = 0, MK_APPLY2 = 1, MK_APPLY2X = 2, MK_GRAD = 3, MK_HAM = 4, MK_CHAIN = 5 };
@@ -510,7 +512,7 @@ struct XPoolEntry {
 };
 struct XPool {
     std::vector<double> h;  // interleaved complex
-    std::vector<std::pair<std::pair<int64_t, int>, int64_t>> cache;
+    std::unordered_map<uint64_t, int64_t> cache;  // (source offset, variant) -> offset
     // deferred: only offsets are assigned and each variant is recorded in
     // `table`, to be computed on the device once the caller's matrices arrive
     bool deferred = false;

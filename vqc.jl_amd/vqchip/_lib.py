@@ -61,6 +61,23 @@ class Instance(ctypes.Structure):
     ]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [
+        ("hbm_budget", ctypes.c_uint64),
+        ("lanes", ctypes.c_int32),
+        ("max_batch", ctypes.c_int32),
+        ("grid_frac", ctypes.c_double),
+        ("cost_ratio", ctypes.c_double),
+        ("poll_us", ctypes.c_int32),
+        ("solo_qubits", ctypes.c_int32),
+    ]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [(k, ctypes.c_int64) for k in ("completed", "batches", "batched_requests", "pending", "running")] + [
+        (k, ctypes.c_uint64) for k in ("budget_bytes", "reserved_bytes", "peak_live_bytes")]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.POINTER(ctypes.c_double)
@@ -105,6 +122,14 @@ SIGNATURES = {
     "vqc_batch_wait": [_P, _D, _D],
     "vqc_batch_arena_bytes": [_P, ctypes.POINTER(ctypes.c_uint64)],
     "vqc_batch_destroy": [_P],
+    "vqc_server_default_config": [ctypes.POINTER(ServerConfig)],
+    "vqc_server_create": [ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)],
+    "vqc_server_submit": [_P, ctypes.POINTER(Instance), ctypes.POINTER(Op), ctypes.POINTER(Term), _D, ctypes.c_int64, _D,
+                          ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)],
+    "vqc_server_poll": [_P, ctypes.c_int64, _D, _D, _D],
+    "vqc_server_wait": [_P, ctypes.c_int64, _D, _D, _D],
+    "vqc_server_stats": [_P, ctypes.POINTER(ServerStats)],
+    "vqc_server_destroy": [_P],
 }
 
 _lib = None
@@ -132,6 +157,15 @@ def call(name, *args):
     lib = load()
     rc = getattr(lib, name)(*args)
     if rc != VQC_OK:
+        raise VQCError(rc, lib.vqc_last_error().decode(errors="replace"))
+    return rc
+
+
+def call_status(name, *args):
+    """Like call, for entry points whose positive return values are statuses."""
+    lib = load()
+    rc = getattr(lib, name)(*args)
+    if rc < 0:
         raise VQCError(rc, lib.vqc_last_error().decode(errors="replace"))
     return rc
 
