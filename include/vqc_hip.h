@@ -148,12 +148,32 @@ int vqc_sv_apply_terms(const vqc_state* in, vqc_state* out, int nterms, const vq
 /* ---- density matrices ----------------------------------------------------
  * rho <- U rho U^dagger: U on pos, conj(U) on pos+n (generic.jl:54-64). */
 int vqc_dm_apply_gate(vqc_state* rho, int k, const int32_t* pos, const double* U);
-/* Superoperator S (4^N square) on (pos..., pos.+n...) (apply_qmaps.jl:52-86);
- * mode 0 applies S (apply!), mode 1 applies S^dagger (apply_dagger!). For
- * apply_inverse! the caller passes inv(S) with mode 0. */
+/* Superoperator S (4^N square, N = 1, 2) on (pos..., pos.+n...) (apply_qmaps.jl:52-86);
+ * mode 0 applies S (apply!, :8-16), mode 1 S^dagger (apply_dagger!, :25-33), mode 2
+ * S^-1 (apply_inverse!, :40-48; VQC_EINVAL when S is singular). */
 int vqc_dm_apply_superop(vqc_state* rho, int N, const int32_t* pos, const double* S, int mode);
 /* tr(M rho) for a k-site M (expecs_dm.jl:7-27, expec_dm_serial.jl:2-129). */
 int vqc_dm_expec_dense(const vqc_state* rho, int k, const int32_t* pos, const double* M, double* out2);
+
+/* ---- measurement and post-selection (src/measure.jl, src/postselect.jl) ----
+ * Qubit positions are 0-based. */
+/* P(qubit pos == value): sum of |psi_b|^2 (SV) or of rho[l,l] (DM) over the
+ * indices with that bit (compute_probability_zero, measure.jl:46-58, 70-82). */
+int vqc_prob(const vqc_state* s, int pos, int value, double* p);
+/* measure! (measure.jl:4-25): outcome 0 when u < P(0), else 1 (discrete_sample,
+ * auxiliary/sampling.jl:2-18, with the caller's uniform u in [0,1) standing for
+ * rand()); the state collapses in place by |0><i|/sqrt(p) (auto_reset) or
+ * |i><i|/sqrt(p) (on a density matrix: M rho M^dagger). */
+int vqc_measure(vqc_state* s, int pos, double u, int auto_reset, int* outcome, double* prob);
+/* post_select! (postselect.jl:3-8): project qubit pos on |value>, divide by the
+ * norm; *norm receives that norm (the value post_select! returns). SV only. */
+int vqc_post_select(vqc_state* s, int pos, int value, double* norm);
+/* The amplitudes with bit pos == value, divided by their norm, as a state of one
+ * qubit less (measure, measure.jl:28-44; post_select with keep=false,
+ * postselect.jl:11-15, 47-53). The remaining qubits come in the reference's
+ * order: qubit 0 takes the place of the measured one (its swap!(v, 1, pos)).
+ * *norm receives that norm. SV only. */
+int vqc_sv_select(const vqc_state* s, int pos, int value, vqc_state* out, double* norm);
 
 /* ---- circuits -----------------------------------------------------------
  * apply!(circuit, state) for SV or DM (generic.jl:26-31, apply_qmaps.jl:8-16),

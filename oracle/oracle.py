@@ -132,6 +132,36 @@ def dm_expec(rho, n, k, pos0, M):
     return complex(out[0], out[1])
 
 
+def prob_zero(v, n, key0, is_dm=False):
+    """compute_probability_zero (measure.jl:46-58, 70-82); v is vec(rho) for a DM."""
+    f = lib().orc_prob_zero
+    f.restype = ctypes.c_double
+    return f(n, _c(v), int(is_dm), key0)
+
+
+def measure_inplace(v, n, key0, u, auto_reset=True, is_dm=False):
+    """measure! (measure.jl:4-25) with rand() = u; returns (outcome, prob)."""
+    i = ctypes.c_int()
+    p = ctypes.c_double()
+    _rc(lib().orc_measure(n, _c(v), int(is_dm), key0, ctypes.c_double(u), int(auto_reset), ctypes.byref(i),
+                          ctypes.byref(p)))
+    return i.value, p.value
+
+
+def post_select_inplace(v, n, key0, value):
+    """post_select! (postselect.jl:3-8); returns the norm it divided by."""
+    s = ctypes.c_double()
+    _rc(lib().orc_post_select(n, _c(v), key0, value, ctypes.byref(s)))
+    return s.value
+
+
+def select(v, n, key0, value):
+    """The unnormalised row `value` of the measured qubit (postselect.jl:47-53)."""
+    out = np.zeros(1 << (n - 1), dtype=np.complex128)
+    _rc(lib().orc_select(n, _c(v), key0, value, _c(out)))
+    return out
+
+
 def apply_circuit(v, n, is_dm, ops, nops, pool):
     _rc(lib().orc_apply_circuit(n, int(is_dm), _c(v), nops, ops, _c(pool)))
     return v

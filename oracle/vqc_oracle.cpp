@@ -555,3 +555,92 @@ extern "C" int orc_dm_energy_grad(int n, const cd* rho0, int nops, const vqc_op_
     }
     return 0;
 }
+
+// ---------------------------------------------------------------------------
+// Measurement and post-selection (src/measure.jl, src/postselect.jl). Serial,
+// like the reference's loops.
+// ---------------------------------------------------------------------------
+// compute_probability_zero (measure.jl:46-58 for a vector, :70-82 for a
+// density matrix): sum over the blocks i = 0, stri, 2 stri, ... of the first
+// `pos` entries (|v|^2, or the diagonal of rho).
+extern "C" double orc_prob_zero(int n, const cd* v, int is_dm, int key0) {
+    const uint64_t L = uint64_t(1) << n;
+    const uint64_t pos = uint64_t(1) << key0, stri = pos * 2;
+    double r = 0.0;
+    for (uint64_t i = 0; i < L; i += stri)
+        for (uint64_t j = 0; j < pos; ++j) {
+            const uint64_t l = i + j;
+            r += is_dm ? v[l + l * L].real() : std::norm(v[l]);
+        }
+    return r;
+}
+
+// discrete_sample(l) with rand() = u (auxiliary/sampling.jl:2-18): the first i
+// with cumulative(l)[i] <= u < cumulative(l)[i+1], over l normalised by its sum.
+static int discrete_sample(const double* l, int L, double u) {
+    double s = 0.0;
+    for (int i = 0; i < L; ++i) s += l[i];
+    double c = 0.0;
+    for (int i = 0; i < L; ++i) {
+        const double c1 = c + l[i] / s;
+        if (u >= c && u < c1) return i;
+        c = c1;
+    }
+    return -1;  // the reference errors here
+}
+
+// measure! (measure.jl:4-25): sample the outcome, then apply the 1-qubit
+// matrix reshape(kron(e_i, ZERO) or kron(e_i, e_i), 2, 2) / sqrt(p) as a gate
+// (U rho U^dagger on a density matrix).
+extern "C" int orc_measure(int n, cd* v, int is_dm, int key0, double u, int auto_reset, int* outcome, double* prob) {
+    if (key0 < 0 || key0 >= n) return -1;
+    const double p0 = orc_prob_zero(n, v, is_dm, key0);
+    const double l[2] = {p0, 1.0 - p0};
+    const int i = discrete_sample(l, 2, u);
+    if (i < 0) return -1;
+    const double p = l[i];
+    cd e[2] = {0.0, 0.0}, z[2] = {1.0, 0.0};
+    e[i] = 1.0;
+    const cd* second = auto_reset ? z : e;
+    cd k4[4];  // kron(e, second): k4[a*2 + b] = e[a] * second[b]
+    for (int a = 0; a < 2; ++a)
+        for (int b = 0; b < 2; ++b) k4[a * 2 + b] = e[a] * second[b] / std::sqrt(p);
+    // reshape(k4, (2, 2)) is column-major: m[r, c] = k4[r + 2c]
+    cd m[4] = {k4[0], k4[1], k4[2], k4[3]};
+    const int pos[1] = {key0};
+    if (is_dm) orc_dm_apply_gate(n, v, 1, pos, m);
+    else orc_apply_dense(n, v, 1, pos, m);
+    *outcome = i;
+    *prob = p;
+    return 0;
+}
+
+// post_select! (postselect.jl:3-8): the UP or DOWN projector as a gate, then
+// division by the norm, which is returned.
+extern "C" int orc_post_select(int n, cd* v, int key0, int value, double* norm) {
+    if (key0 < 0 || key0 >= n || (value != 0 && value != 1)) return -1;
+    const cd P[4] = {value == 0 ? 1.0 : 0.0, 0.0, 0.0, value == 1 ? 1.0 : 0.0};
+    const int pos[1] = {key0};
+    orc_apply_dense(n, v, 1, pos, P);
+    const uint64_t L = uint64_t(1) << n;
+    double s = 0.0;
+    for (uint64_t b = 0; b < L; ++b) s += std::norm(v[b]);
+    s = std::sqrt(s);
+    for (uint64_t b = 0; b < L; ++b) v[b] /= s;
+    *norm = s;
+    return 0;
+}
+
+// _apply_throw_impl (postselect.jl:47-53) and the collapse of measure
+// (measure.jl:28-39): swap qubit 1 with the measured qubit, take row `value`
+// of reshape(v, 2, L/2), swap back. out gets 2^(n-1) unnormalised entries.
+extern "C" int orc_select(int n, cd* v, int key0, int value, cd* out) {
+    if (key0 < 0 || key0 >= n || n < 2 || (value != 0 && value != 1)) return -1;
+    const cd S[16] = {1, 0, 0, 0, 0, 0, 1, 0, 0, 1, 0, 0, 0, 0, 0, 1};  // SWAP, column-major
+    const int sw[2] = {0, key0};
+    if (key0 != 0) orc_apply_dense(n, v, 2, sw, S);
+    const uint64_t half = uint64_t(1) << (n - 1);
+    for (uint64_t j = 0; j < half; ++j) out[j] = v[2 * j + value];
+    if (key0 != 0) orc_apply_dense(n, v, 2, sw, S);
+    return 0;
+}

@@ -12,7 +12,7 @@ import pytest
 
 import oracle
 from circuits import chain_circuit
-from brute import rand_density, rand_matrix, rand_state, rand_unitary, unvec, vec
+from brute import full_op, rand_density, rand_matrix, rand_state, rand_unitary, unvec, vec
 import vqchip as vq
 from vqchip import _lib
 from vqchip.circuit import Program
@@ -162,6 +162,16 @@ def test_density_matrix_gate_channel_expectation():
                       _lib.dptr(np.ascontiguousarray(S.reshape(-1, order="F"))), mode)
             ref = oracle.dm_apply_superop(rho.copy(), n, N, pos, S, mode)
             assert np.max(np.abs(vec(d.storage()) - ref)) < 1e-11
+        # mode 2 is S^-1: it undoes mode 0 (apply_inverse!, apply_qmaps.jl:40-48)
+        d = vq.DensityMatrix(rho)
+        Sf = _lib.dptr(np.ascontiguousarray(S.reshape(-1, order="F")))
+        _lib.call("vqc_dm_apply_superop", d.handle, N, _lib.i32ptr(pos), Sf, 0)
+        _lib.call("vqc_dm_apply_superop", d.handle, N, _lib.i32ptr(pos), Sf, 2)
+        assert np.max(np.abs(vec(d.storage()) - rho)) < 1e-10
+        ref = oracle.dm_apply_superop(rho.copy(), n, N, pos, np.linalg.inv(S), 0)
+        d = vq.DensityMatrix(rho)
+        _lib.call("vqc_dm_apply_superop", d.handle, N, _lib.i32ptr(pos), Sf, 2)
+        assert np.max(np.abs(vec(d.storage()) - ref)) < 1e-10
     rho = vec(rand_density(n, rng))
     d = vq.DensityMatrix(rho)
     for k, pos in [(1, (2,)), (2, (0, 5)), (3, (1, 3, 4))]:
@@ -170,6 +180,25 @@ def test_density_matrix_gate_channel_expectation():
         _lib.call("vqc_dm_expec_dense", d.handle, k, _lib.i32ptr(pos), _lib.dptr(np.ascontiguousarray(M.reshape(-1, order="F"))),
                   _lib.dptr(out))
         assert abs(complex(*out) - oracle.dm_expec(rho, n, k, pos, M)) < AMP_TOL
+
+
+def test_hamiltonian_times_density_matrix():
+    # (H * rho): each key acts on the row bits of vec(rho) (apply_qterms_dm.jl:11-27)
+    rng = np.random.default_rng(29)
+    n = 5
+    H = tfim(n, 0.8) + vq.QubitsOperator([vq.QubitsTerm({1: "+", 4: "-"}, coeff=0.3),
+                                          vq.QubitsTerm({2: "X", 3: "Y", 5: "Z"}, coeff=0.6)])
+    terms, nt, hpool = pack_terms(H)
+    Hf = np.zeros((1 << n, 1 << n), dtype=complex)
+    for j in range(nt):
+        k = terms[j].nq
+        D = 1 << k
+        M = hpool[terms[j].mat:terms[j].mat + D * D].reshape(D, D, order="F")
+        Hf += full_op(n, [terms[j].pos[q] for q in range(k)], M)
+    rho = rand_density(n, rng)
+    out = vq.apply_operator(H, vq.DensityMatrix(vec(rho)))
+    assert out.is_dm
+    assert np.max(np.abs(out.storage() - Hf @ rho)) < AMP_TOL
 
 
 @pytest.mark.parametrize("n,tiles", [(10, "small"), (16, "tiled"), (18, "tiled")])
@@ -273,3 +302,11 @@ def test_errors_are_reported_not_crashes():
         vq.apply_(vq.RxGate(6, 0.1), s)
     # the state is untouched by rejected calls
     assert np.array_equal(s.storage(), np.eye(1, 32, 0, dtype=complex).reshape(-1))
+    d = vq.DensityMatrix(3)
+    S0 = np.zeros(16, dtype=complex)
+    with pytest.raises(vq.VQCError, match="singular"):
+        _lib.call("vqc_dm_apply_superop", d.handle, 1, _lib.i32ptr([1]), _lib.dptr(S0), 2)
+    with pytest.raises(vq.VQCError, match="mode"):
+        _lib.call("vqc_dm_apply_superop", d.handle, 1, _lib.i32ptr([1]), _lib.dptr(S0), 3)
+    with pytest.raises(vq.VQCError, match="density matrix"):
+        _lib.call("vqc_dm_apply_superop", s.handle, 1, _lib.i32ptr([1]), _lib.dptr(S0), 0)

@@ -311,3 +311,104 @@ def test_adjoint_threads_agree():
     E4, g4, _ = _energy(circuit, H, 10, psi0)
     oracle.set_threads(1)
     assert abs(E1 - E4) < 1e-12 and np.max(np.abs(g1 - g4)) < 1e-12
+
+
+# ---- measurement and post-selection (measure.jl, postselect.jl) -------------
+def _bits(n, key0):
+    return (np.arange(1 << n) >> key0) & 1
+
+
+def test_measurement_probability_brute_force():
+    rng = np.random.default_rng(61)
+    for n, key0 in [(1, 0), (3, 0), (3, 2), (7, 4)]:
+        v = rand_state(n, rng)
+        b = _bits(n, key0)
+        assert abs(oracle.prob_zero(v, n, key0) - np.sum(np.abs(v[b == 0]) ** 2)) < 1e-14
+        rho = rand_density(n, rng)
+        P0 = np.diag((b == 0).astype(float))
+        assert abs(oracle.prob_zero(vec(rho), n, key0, is_dm=True) - np.trace(P0 @ rho).real) < 1e-14
+
+
+def test_measure_collapse_brute_force():
+    rng = np.random.default_rng(62)
+    n = 5
+    for key0 in (0, 2, 4):
+        for auto_reset in (True, False):
+            for u in (0.05, 0.95):
+                v = rand_state(n, rng)
+                b = _bits(n, key0)
+                p0 = np.sum(np.abs(v[b == 0]) ** 2)
+                i_exp = 0 if u < p0 else 1
+                w = v.copy()
+                i, p = oracle.measure_inplace(w, n, key0, u, auto_reset)
+                assert i == i_exp and abs(p - (p0 if i == 0 else 1 - p0)) < 1e-14
+                exp = np.zeros_like(v)
+                src = v[b == i] / np.sqrt(p)
+                exp[b == (0 if auto_reset else i)] = src
+                assert np.max(np.abs(w - exp)) < 1e-14
+    # density matrix: M rho M^dagger with M = |0><i| / sqrt(p)
+    rho = rand_density(4, rng)
+    r = vec(rho)
+    i, p = oracle.measure_inplace(r, 4, 1, 0.5, True, is_dm=True)
+    M = np.zeros((2, 2), complex)
+    M[0, i] = 1 / np.sqrt(p)
+    F = full_op(4, [1], M)
+    assert np.max(np.abs(unvec(r) - F @ rho @ F.conj().T)) < 1e-14
+    assert abs(np.trace(unvec(r)) - 1) < 1e-13
+
+
+def test_measure_and_select_known_answers():
+    # check_measure.jl: after measure! the same qubit is certain (p = 1)
+    rng = np.random.default_rng(63)
+    for key0, auto_reset in [(0, True), (1, False)]:
+        v = rand_state(3, rng)
+        i, p = oracle.measure_inplace(v, 3, key0, float(rng.random()), auto_reset)
+        p0 = oracle.prob_zero(v, 3, key0)
+        assert abs((p0 if (auto_reset or i == 0) else 1 - p0) - 1) < 1e-10
+    # check_select_1/2: post_select! then measure gives the selected value with p = 1
+    for key0, val in [(1, 0), (2, 1)]:
+        v = rand_state(3, rng)
+        oracle.post_select_inplace(v, 3, key0, val)
+        p0 = oracle.prob_zero(v, 3, key0)
+        assert abs((p0 if val == 0 else 1 - p0) - 1) < 1e-10
+    # check_select_3/4: (|00> + |11>)/sqrt(2) post-selected on qubit 1 -> ZERO / ONE
+    bell = np.zeros(4, complex)
+    bell[0] = bell[3] = np.sqrt(0.5)
+    for val, exp in [(0, [1, 0]), (1, [0, 1])]:
+        r = oracle.select(bell.copy(), 2, 0, val)
+        r = r / np.sqrt(np.vdot(r, r))
+        assert np.allclose(r, exp, atol=1e-15)
+
+
+def select_brute(v, n, key0, val):
+    """Row `val` of the measured qubit with the reference's qubit order: the
+    measured qubit leaves, and qubit 1 takes its place (the swap!(v, 1, pos) of
+    measure.jl:30 / postselect.jl:48): new qubit j holds original qubit
+    order[j]."""
+    if key0 == 0:
+        order = list(range(1, n))
+    else:
+        order = list(range(1, key0)) + [0] + list(range(key0 + 1, n))
+    out = np.zeros(1 << (n - 1), dtype=complex)
+    for c in range(1 << (n - 1)):
+        src = val << key0
+        for j, q in enumerate(order):
+            src |= ((c >> j) & 1) << q
+        out[c] = v[src]
+    return out
+
+
+def test_select_brute_force():
+    rng = np.random.default_rng(64)
+    n = 6
+    for key0 in range(n):
+        v = rand_state(n, rng)
+        for val in (0, 1):
+            w = v.copy()
+            assert np.array_equal(oracle.select(w, n, key0, val), select_brute(v, n, key0, val))
+            assert np.array_equal(w, v)  # the swaps are undone
+    # the permutation matters: a product state |q1=1, q2=0, q3=0> selected on q3=0
+    # gives |1, 0> in the reference's order (q2, q1), i.e. index 2, not index 1
+    v = np.zeros(8, complex)
+    v[1] = 1.0
+    assert np.array_equal(oracle.select(v, 3, 2, 0), np.eye(1, 4, 2, dtype=complex).reshape(-1))

@@ -9,6 +9,7 @@
 // access whenever the lowest target is >= 6, and stays cache-line complete
 // below that. Each amplitude is read and written exactly once (32 B per
 // amplitude of HBM traffic), which is the roofline of an unfused gate.
+#include <cmath>
 #include <cstdlib>
 
 #include "vqc_internal.h"
@@ -344,7 +345,7 @@ extern "C" int vqc_dm_apply_superop(vqc_state* rho, int N, const int32_t* pos, c
     if (!rho->is_dm) return vqc_fail(VQC_EINVAL, "vqc_dm_apply_superop: state is not a density matrix");
     VQC_TRY(vqc_check_positions(rho->nqubits, N, pos, 2));
     if (!S) return vqc_fail(VQC_EINVAL, "null superoperator");
-    if (mode != 0 && mode != 1) return vqc_fail(VQC_EINVAL, "superoperator mode must be 0 (S) or 1 (S^dagger)");
+    if (mode < 0 || mode > 2) return vqc_fail(VQC_EINVAL, "superoperator mode must be 0 (S), 1 (S^dagger) or 2 (S^-1)");
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));
     const int D = 1 << (2 * N);
@@ -354,6 +355,53 @@ extern "C" int vqc_dm_apply_superop(vqc_state* rho, int N, const int32_t* pos, c
         all[j + N] = pos[j] + rho->nqubits;
     }
     if (mode == 0) return vqc_dense_sweep(rho->d, rho->nbits, 2 * N, all, S, c->stream);
+    if (mode == 2) {
+        // apply_inverse! (apply_qmaps.jl:40-48): inv(S) by Gauss-Jordan elimination
+        // with partial pivoting on the host (S is at most 16 x 16)
+        std::vector<double> a(S, S + size_t(2) * D * D), inv(size_t(2) * D * D, 0.0);
+        auto A = [&](int r, int cc) -> double* { return &a[2 * size_t(r + cc * D)]; };
+        auto I = [&](int r, int cc) -> double* { return &inv[2 * size_t(r + cc * D)]; };
+        for (int i = 0; i < D; ++i) I(i, i)[0] = 1.0;
+        double scale = 0;
+        for (int i = 0; i < D * D; ++i) scale = std::max(scale, std::hypot(a[2 * i], a[2 * i + 1]));
+        for (int col = 0; col < D; ++col) {
+            int piv = col;
+            for (int r = col + 1; r < D; ++r)
+                if (std::hypot(A(r, col)[0], A(r, col)[1]) > std::hypot(A(piv, col)[0], A(piv, col)[1])) piv = r;
+            const double pr = A(piv, col)[0], pi = A(piv, col)[1];
+            const double m2 = pr * pr + pi * pi;
+            if (!(std::sqrt(m2) > 1e-13 * std::max(scale, 1e-300)))
+                return vqc_fail(VQC_EINVAL, "vqc_dm_apply_superop: the superoperator is singular; it has no inverse");
+            for (int cc = 0; cc < D; ++cc) {
+                std::swap(A(piv, cc)[0], A(col, cc)[0]);
+                std::swap(A(piv, cc)[1], A(col, cc)[1]);
+                std::swap(I(piv, cc)[0], I(col, cc)[0]);
+                std::swap(I(piv, cc)[1], I(col, cc)[1]);
+            }
+            const double ir = pr / m2, ii = -pi / m2;  // 1 / pivot
+            for (int cc = 0; cc < D; ++cc) {
+                double* x = A(col, cc);
+                double* y = I(col, cc);
+                const double xr = x[0] * ir - x[1] * ii, xi = x[0] * ii + x[1] * ir;
+                const double yr = y[0] * ir - y[1] * ii, yi = y[0] * ii + y[1] * ir;
+                x[0] = xr, x[1] = xi, y[0] = yr, y[1] = yi;
+            }
+            for (int r = 0; r < D; ++r) {
+                if (r == col) continue;
+                const double fr = A(r, col)[0], fi = A(r, col)[1];
+                if (fr == 0.0 && fi == 0.0) continue;
+                for (int cc = 0; cc < D; ++cc) {
+                    const double* x = A(col, cc);
+                    const double* y = I(col, cc);
+                    A(r, cc)[0] -= fr * x[0] - fi * x[1];
+                    A(r, cc)[1] -= fr * x[1] + fi * x[0];
+                    I(r, cc)[0] -= fr * y[0] - fi * y[1];
+                    I(r, cc)[1] -= fr * y[1] + fi * y[0];
+                }
+            }
+        }
+        return vqc_dense_sweep(rho->d, rho->nbits, 2 * N, all, inv.data(), c->stream);
+    }
     std::vector<double> Sd(size_t(2) * D * D);
     for (int r = 0; r < D; ++r)
         for (int cc = 0; cc < D; ++cc) {

@@ -98,6 +98,22 @@ function apply!(g::Gate, s::GPUState)
     return s
 end
 
+# named kernels (src/applygates/specific/*.jl): permutations and phases, no matrix
+const NAMED = Dict{DataType,Tuple{Cint,Bool}}(  # kind, has angle
+    XGate => (1, false), PHASEGate => (2, true), CZGate => (3, false), CNOTGate => (4, false),
+    CPHASEGate => (5, true), SWAPGate => (6, false), iSWAPGate => (7, false), FREDKINGate => (9, false),
+    CCPHASEGate => (10, true))
+
+function apply!(g::Union{XGate,PHASEGate,CZGate,CNOTGate,CPHASEGate,SWAPGate,iSWAPGate,FREDKINGate,CCPHASEGate},
+                s::GPUState)
+    s.isdm && return invoke(apply!, Tuple{Gate,GPUState}, g, s)
+    kind, has_angle = NAMED[typeof(g)]
+    pos = Int32.(collect(positions(g)) .- 1)   # the gate's own order: control first (czgate.jl:30-40)
+    phi = has_angle ? Float64(parameters(g)[1]) : 0.0
+    check(ccall((:vqc_sv_apply_named, libvqc), Cint, (Ptr{Cvoid}, Cint, Ptr{Int32}, Cdouble), s.h, kind, pos, phi))
+    return s
+end
+
 function apply!(x::QuantumMap, s::GPUState)
     s.isdm || throw(ArgumentError("channels act on density matrices"))
     pos = Int32.(collect(ordered_positions(x)) .- 1)
@@ -176,6 +192,43 @@ function VQC.expectation(h::QubitsOperator, s::GPUState)
     return r
 end
 
+# H * psi ((m::QubitsOperator)(v), apply_qterms.jl:13-28; on a density matrix the
+# keys act on the row bits, apply_qterms_dm.jl:11-27)
+function Base.:*(h::QubitsOperator, s::GPUState)
+    terms, pool = pack(h)
+    out = GPUState(s.nqubits, s.isdm)
+    check(ccall((:vqc_sv_apply_terms, libvqc), Cint,
+                (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{CTerm}, Ptr{ComplexF64}, Ptr{Float64}),
+                s.h, out.h, length(terms), terms, pool, C_NULL))
+    return out
+end
+
+# Pauli strings beyond 5 sites (the dot(psi, H*psi) fallback of expecs.jl:40)
+struct CPauli
+    x::UInt64
+    z::UInt64
+    re::Float64
+    im::Float64
+end
+
+function pauli_expectation(strings::Vector{Tuple{Dict{Int,Char},ComplexF64}}, s::GPUState)
+    ps = CPauli[]
+    for (ops, c) in strings
+        x = UInt64(0); z = UInt64(0)
+        for (q, p) in ops
+            b = UInt64(1) << (q - 1)
+            p in ('X', 'Y') && (x |= b)
+            p in ('Z', 'Y') && (z |= b)
+        end
+        push!(ps, CPauli(x, z, real(c), imag(c)))
+    end
+    per = zeros(Float64, 2 * length(ps))
+    tot = zeros(Float64, 2)
+    check(ccall((:vqc_sv_expec_pauli, libvqc), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{CPauli}, Ptr{Float64}, Ptr{Float64}),
+                s.h, C_NULL, length(ps), ps, per, tot))
+    return complex(tot[1], tot[2])
+end
+
 # ---- energy and adjoint gradient -----------------------------------------------
 "E = expectation(h, circuit * s0) and dRe(E)/dθ for the active parameters, in circuit order."
 function energy_and_gradient(h::QubitsOperator, circuit::QCircuit, s0::GPUState)
@@ -200,6 +253,45 @@ function ChainRulesCore.rrule(::typeof(energy), h::QubitsOperator, circuit::QCir
     E, g = energy_and_gradient(h, circuit, s0)
     pullback(Δ) = (NoTangent(), NoTangent(), Δ .* g, NoTangent())
     return real(E), pullback
+end
+
+# ---- batches and serving (no counterpart in the reference) ---------------------
+struct CInstance
+    nqubits::Int32
+    nops::Int32
+    op_begin::Int64
+    nterms::Int32
+    nparams::Int32
+    term_begin::Int64
+    grad_begin::Int64
+end
+
+"Energies and gradients of many (circuit, H, n) instances in one persistent-kernel launch."
+function batch_energy_and_gradient(instances::Vector{<:Tuple{QCircuit,QubitsOperator,Int}})
+    ops = COp[]; terms = CTerm[]; pool = ComplexF64[]; hpool = ComplexF64[]; inst = CInstance[]
+    g = 0
+    for (c, h, n) in instances
+        o, p, np = pack(c, false)
+        t, hp = pack(h)
+        shift(x) = x < 0 ? x : x + length(pool)
+        append!(ops, [COp(a.kind, a.nq, a.pos, a.nparams, a.slot, a.reserved, shift(a.mat), shift(a.aux), map(shift, a.gen)) for a in o])
+        append!(terms, [CTerm(a.nq, a.pos, a.mat + length(hpool)) for a in t])
+        push!(inst, CInstance(n, length(o), length(ops) - length(o), length(t), np, length(terms) - length(t), g))
+        append!(pool, p); append!(hpool, hp)
+        g += np
+    end
+    b = Ref{Ptr{Cvoid}}(C_NULL)
+    check(ccall((:vqc_batch_create, libvqc), Cint,
+                (Cint, Ptr{CInstance}, Ptr{COp}, Int64, Ptr{CTerm}, Int64, Int64, Int64, Cint, Ptr{Ptr{Cvoid}}),
+                length(inst), inst, ops, length(ops), terms, length(terms), length(pool), length(hpool), 0, b))
+    e = zeros(Float64, 2 * length(inst)); gr = zeros(Float64, max(g, 1))
+    try
+        check(ccall((:vqc_batch_run, libvqc), Cint, (Ptr{Cvoid}, Ptr{ComplexF64}, Ptr{ComplexF64}, Ptr{Float64}, Ptr{Float64}),
+                    b[], pool, hpool, e, gr))
+    finally
+        ccall((:vqc_batch_destroy, libvqc), Cint, (Ptr{Cvoid},), b[])
+    end
+    return [complex(e[2i-1], e[2i]) for i in 1:length(inst)], gr[1:g]
 end
 
 end # module

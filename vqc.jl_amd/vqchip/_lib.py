@@ -112,6 +112,10 @@ SIGNATURES = {
     "vqc_dm_apply_gate": [_P, _I, _I32, _D],
     "vqc_dm_apply_superop": [_P, _I, _I32, _D, _I],
     "vqc_dm_expec_dense": [_P, _I, _I32, _D, _D],
+    "vqc_prob": [_P, _I, _I, _D],
+    "vqc_measure": [_P, _I, ctypes.c_double, _I, ctypes.POINTER(ctypes.c_int), _D],
+    "vqc_post_select": [_P, _I, _I, _D],
+    "vqc_sv_select": [_P, _I, _I, _P, _D],
     "vqc_apply_circuit": [_P, _I, ctypes.POINTER(Op), _D],
     "vqc_sv_energy_grad": [_P, _I, ctypes.POINTER(Op), _D, _I, ctypes.POINTER(Term), _D, _I, _D, _D, _P, _I],
     "vqc_dm_energy_grad": [_P, _I, ctypes.POINTER(Op), _D, _I, ctypes.POINTER(Term), _D, _I, _D, _D, _P, _I],

@@ -121,9 +121,10 @@ def expectation(*args):
 
 
 def apply_operator(m, state):
-    """H * state as a new StateVector ((m::QubitsOperator)(v), apply_qterms.jl:13-28)."""
+    """H * state as a new state ((m::QubitsOperator)(v), apply_qterms.jl:13-28). On a
+    density matrix the keys act on the row bits of vec(rho) (apply_qterms_dm.jl:11-27)."""
     m = _as_operator(m)
-    out = StateVector(state.nqubits())
+    out = DensityMatrix(state.nqubits()) if state.is_dm else StateVector(state.nqubits())
     if m.largest_nterm() <= LARGEST_SUPPORTED_NTERMS:
         terms, nt, pool = pack_terms(m)
         _lib.call("vqc_sv_apply_terms", state.handle, out.handle, nt, terms, _lib.dptr(pool), None)
