@@ -200,6 +200,13 @@ def bench_config3(vq, _lib, oracle, timer):
     return out
 
 
+# HBM bytes one k_dense_small<1, 1, true> launch moves at 28 qubits, from two
+# rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a
+# copy of known size reads; profiles/hbm_traffic_dense_28q.md). Algorithmic:
+# 2 * 16 * 2^28 = 8.590e9.
+TRAFFIC_1Q_BYTES = 8.628e9
+
+
 def bench_config4(vq, _lib, count, seed=20261203):
     """Serving: open-loop Poisson stream of energy + gradient requests, n log-uniform
     in 10..26, depth U{4..40}, TFIM h ~ U(0.5, 1.5). The rate is set so that the
@@ -408,8 +415,8 @@ def main():
                                "gates, one kernel per gate",
                    "qubits": n, "gates": G, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(ach1, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach1 / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "k_dense_small<1, 4> (1-qubit dense sweep)",
+                     "frac": round(ach1 / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_1Q_BYTES if n == 28 else None,
+                     "kernel": "k_dense_small<1, 1, true> (1-qubit dense sweep)",
                      "avg_launch_ms": round(avg1_ms, 4), "bytes_per_launch": bytes_per_gate},
         "cpu_baseline": {"value": round(cpu_gbs, 2), "unit": "GB/s", "cores": cores, "kind": "port",
                          "sample": f"first {done} gates of the same {n}-qubit circuit, oracle restatement"},
