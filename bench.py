@@ -182,19 +182,32 @@ def bench_config3(vq, _lib, oracle, timer):
     n = 12
     args = _prog_args(vq, c, H, n, dm=True)
     rho0 = vq.DensityMatrix(n)
+    from vqchip import autotune
     Ef, gf = _eval(vq, _lib, rho0, args, True)
     Ep, gp = _eval(vq, _lib, rho0, args, False)
-    tf = 1.0 / _time_evals(lambda: _eval(vq, _lib, rho0, args, True), min_s=1.0, max_n=20)
+    tf_default = 1.0 / _time_evals(lambda: _eval(vq, _lib, rho0, args, True), min_s=1.0, max_n=20)
     tp = 1.0 / _time_evals(lambda: _eval(vq, _lib, rho0, args, False), min_s=1.0, max_n=5)
+    # the autotuner picks the tile shape for this workload (tile bits, contiguous low bits)
+    before = autotune.get_tuning()
+    t0 = time.perf_counter()
+    tuned = autotune.tune_energy_grad(H, c, rho0, fwd=(11, 12), rev=(10, 11, 12), low=(2, 3, 4, 6), reps=2)
+    tune_s = time.perf_counter() - t0
+    Et, gt = _eval(vq, _lib, rho0, args, True)
+    tf = 1.0 / _time_evals(lambda: _eval(vq, _lib, rho0, args, True), min_s=1.0, max_n=20)
+    autotune.set_tuning(**{k: before[k] for k in autotune.FIELDS})
     out = {
         "workload": "12-qubit DM (2^24 entries), 6 layers, depolarizing p=0.01 after each CNOT, "
                     "amplitude damping 0.005 per qubit per layer, sum Z_i, 216 angles",
         "fused_evals_per_s": round(1 / tf, 3),
-        "plain_evals_per_s": round(1 / tp, 3),
         "fused_ms": round(tf * 1e3, 2),
+        "fused_ms_default_tiles": round(tf_default * 1e3, 2),
+        "autotuned_tiles": tuned["best"],
+        "autotune_s": round(tune_s, 2),
+        "plain_evals_per_s": round(1 / tp, 3),
         "plain_ms": round(tp * 1e3, 2),
         "energy": Ef.real,
         "max_abs_diff_fused_vs_plain": max(abs(Ef - Ep), float(np.max(np.abs(gf - gp)))),
+        "max_abs_diff_autotuned_vs_plain": max(abs(Et - Ep), float(np.max(np.abs(gt - gp)))),
     }
     log(f"config3: {out}")
     return out

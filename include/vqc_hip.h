@@ -98,6 +98,21 @@ int vqc_sync(void);
  * time or order their own work against the library's (no reference counterpart). */
 int vqc_get_stream(void** stream);
 
+/* ---- tuning (no counterpart in the reference) ----------------------------
+ * Launch and fusion parameters, process-wide. A field set to -1 keeps its
+ * current value; vqc_get_tuning reports the values in force. The VQC_TILE_* /
+ * VQC_DENSE_* environment variables give the initial values. */
+typedef struct vqc_tuning {
+    int32_t tile_fwd_bits;          /* tile bits of forward phases, 6..12 (default 12) */
+    int32_t tile_rev_bits;          /* tile bits of Hamiltonian / reverse phases, 6..12 (11) */
+    int32_t tile_small_bits;        /* whole evaluation in one workgroup up to this many bits, 1..12 (12) */
+    int32_t tile_low_bits;          /* contiguous low index bits in every tile, 0..8 (6) */
+    int32_t dense_groups[3];        /* amplitude groups per thread, 1-, 2-, 3-qubit sweeps: 1, 2, 4, 8 (1) */
+    int32_t dense_nontemporal[3];   /* non-temporal loads/stores, 1-, 2-, 3-qubit sweeps: 0/1 (1, 1, 0) */
+} vqc_tuning_t;
+int vqc_set_tuning(const vqc_tuning_t* t);
+int vqc_get_tuning(vqc_tuning_t* t);
+
 /* ---- state handles (StateVector / DensityMatrix storage) ----------------
  * src/statevector.jl:1-21 (StateVector{T}(n) = |0..0>), src/densitymatrix.jl:38-42. */
 typedef struct vqc_state vqc_state;

@@ -114,3 +114,25 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert set(_lib.SIGNATURES) | {"vqc_last_error"} >= set(names) - set()
+
+
+def test_tuning_roundtrip_and_validation():
+    # vqc_set_tuning / vqc_get_tuning are host-only: no device is touched
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libvqc_hip.so not built")
+    from vqchip import autotune
+    before = autotune.get_tuning()
+    try:
+        autotune.set_tuning(tile_fwd_bits=11, tile_low_bits=4, dense_groups=[2, -1, -1])
+        t = autotune.get_tuning()
+        assert t["tile_fwd_bits"] == 11 and t["tile_low_bits"] == 4 and t["dense_groups"][0] == 2
+        assert t["tile_rev_bits"] == before["tile_rev_bits"]  # -1 keeps the value
+        for bad in ({"tile_fwd_bits": 13}, {"tile_low_bits": 12}, {"dense_groups": [3, -1, -1]},
+                    {"dense_nontemporal": [2, -1, -1]}, {"tile_small_bits": 0}):
+            with pytest.raises(_lib.VQCError):
+                autotune.set_tuning(**bad)
+        assert autotune.get_tuning()["tile_fwd_bits"] == 11  # a rejected call changes nothing
+    finally:
+        autotune.set_tuning(**{k: before[k] for k in autotune.FIELDS}, dense_groups=before["dense_groups"],
+                            dense_nontemporal=before["dense_nontemporal"])
+    assert autotune.get_tuning() == before

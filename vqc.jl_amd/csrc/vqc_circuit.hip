@@ -22,17 +22,13 @@ int vqc_dm_expec_sweep(const double2* rho, int n, int k, const int32_t* pos, con
                        unsigned* grid_out, hipStream_t st);
 void vqc_zero(double2* v, uint64_t len, hipStream_t st);
 
-static int env_int(const char* name, int def) {
-    const char* s = getenv(name);
-    return s && *s ? atoi(s) : def;
-}
-
 TileTuning vqc_tile_tuning() {
+    const vqc_tuning_t g = vqc_tuning_now();
     TileTuning t;
-    t.t_fwd = std::min(kMaxTileBits, std::max(6, env_int("VQC_TILE_FWD", t.t_fwd)));
-    t.t_rev = std::min(kMaxTileBits, std::max(6, env_int("VQC_TILE_REV", t.t_rev)));
-    t.t_small = std::min(kMaxTileBits, std::max(1, env_int("VQC_TILE_SMALL", t.t_small)));
-    t.mlow = std::min(8, std::max(0, env_int("VQC_TILE_MLOW", t.mlow)));
+    t.t_fwd = std::min(kMaxTileBits, std::max(6, int(g.tile_fwd_bits)));
+    t.t_rev = std::min(kMaxTileBits, std::max(6, int(g.tile_rev_bits)));
+    t.t_small = std::min(kMaxTileBits, std::max(1, int(g.tile_small_bits)));
+    t.mlow = std::min(8, std::max(0, int(g.tile_low_bits)));
     return t;
 }
 

@@ -335,3 +335,73 @@ extern "C" int vqc_state_dot(const vqc_state* a, const vqc_state* b, double* out
     VQC_HIP(hipGetLastError());
     return vqc_fetch(parts + nb, 1, out2);
 }
+
+// ---------------------------------------------------------------------------
+// tuning
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_tune_mu;
+bool g_tune_init = false;
+vqc_tuning_t g_tune;
+
+int env_or(const char* name, int def) {
+    const char* v = getenv(name);
+    return v && *v ? atoi(v) : def;
+}
+
+void tune_init_locked() {
+    if (g_tune_init) return;
+    g_tune.tile_fwd_bits = env_or("VQC_TILE_FWD", 12);
+    g_tune.tile_rev_bits = env_or("VQC_TILE_REV", 11);
+    g_tune.tile_small_bits = env_or("VQC_TILE_SMALL", 12);
+    g_tune.tile_low_bits = env_or("VQC_TILE_MLOW", 6);
+    const int r = env_or("VQC_DENSE_R", 1), nt = env_or("VQC_DENSE_NT", -1);
+    for (int k = 0; k < 3; ++k) {
+        g_tune.dense_groups[k] = r;
+        // measured at 28 qubits: non-temporal helps the 2- and 4-stream sweeps, hurts the 8-stream one
+        g_tune.dense_nontemporal[k] = nt >= 0 ? (nt != 0) : (k < 2);
+    }
+    g_tune_init = true;
+}
+
+bool valid_groups(int r) { return r == 1 || r == 2 || r == 4 || r == 8; }
+}  // namespace
+
+vqc_tuning_t vqc_tuning_now() {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    tune_init_locked();
+    return g_tune;
+}
+
+extern "C" int vqc_set_tuning(const vqc_tuning_t* t) {
+    if (!t) return vqc_fail(VQC_EINVAL, "null tuning");
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    tune_init_locked();
+    vqc_tuning_t n = g_tune;
+    auto take = [](int32_t v, int32_t& dst) {
+        if (v != -1) dst = v;
+    };
+    take(t->tile_fwd_bits, n.tile_fwd_bits);
+    take(t->tile_rev_bits, n.tile_rev_bits);
+    take(t->tile_small_bits, n.tile_small_bits);
+    take(t->tile_low_bits, n.tile_low_bits);
+    for (int k = 0; k < 3; ++k) {
+        take(t->dense_groups[k], n.dense_groups[k]);
+        take(t->dense_nontemporal[k], n.dense_nontemporal[k]);
+    }
+    if (n.tile_fwd_bits < 6 || n.tile_fwd_bits > 12 || n.tile_rev_bits < 6 || n.tile_rev_bits > 12 ||
+        n.tile_small_bits < 1 || n.tile_small_bits > 12 || n.tile_low_bits < 0 || n.tile_low_bits > 8 ||
+        n.tile_low_bits > std::min(n.tile_fwd_bits, n.tile_rev_bits))
+        return vqc_fail(VQC_EINVAL, "vqc_set_tuning: tile bits out of range");
+    for (int k = 0; k < 3; ++k)
+        if (!valid_groups(n.dense_groups[k]) || (n.dense_nontemporal[k] != 0 && n.dense_nontemporal[k] != 1))
+            return vqc_fail(VQC_EINVAL, "vqc_set_tuning: dense groups must be 1, 2, 4 or 8 and non-temporal 0 or 1");
+    g_tune = n;
+    return VQC_OK;
+}
+
+extern "C" int vqc_get_tuning(vqc_tuning_t* t) {
+    if (!t) return vqc_fail(VQC_EINVAL, "null tuning");
+    *t = vqc_tuning_now();
+    return VQC_OK;
+}

@@ -121,14 +121,13 @@ static int launch_dense(double2* v, int nbits, const int32_t* pos, const double*
     if constexpr (K <= 3) {
         GMat<K> U;
         for (int i = 0; i < D * D; ++i) U.m[i] = make_double2(Uh[2 * i], Uh[2 * i + 1]);
-        // Groups per thread and non-temporal hints. Measured on MI355X at 28
-        // qubits: one group per thread is best for every K; non-temporal
-        // accesses help the 2- and 4-stream sweeps (K = 1, 2: 5.9 / 5.6 TB/s)
-        // and hurt the 8-stream K = 3 sweep. VQC_DENSE_R / VQC_DENSE_NT override.
-        static const int r_env = getenv("VQC_DENSE_R") ? atoi(getenv("VQC_DENSE_R")) : 0;
-        static const int nt_env = getenv("VQC_DENSE_NT") ? atoi(getenv("VQC_DENSE_NT")) : -1;
-        const int R = r_env > 0 ? r_env : 1;
-        const bool NT = nt_env >= 0 ? nt_env != 0 : K <= 2;
+        // Groups per thread and non-temporal hints (vqc_set_tuning). Measured on
+        // MI355X at 28 qubits: one group per thread is best for every K;
+        // non-temporal accesses help the 2- and 4-stream sweeps (K = 1, 2:
+        // 5.9 / 5.6 TB/s) and hurt the 8-stream K = 3 sweep.
+        const vqc_tuning_t tu = vqc_tuning_now();
+        const int R = tu.dense_groups[K - 1];
+        const bool NT = tu.dense_nontemporal[K - 1] != 0;
         const dim3 blk(kBlock);
 #define VQC_DENSE_LAUNCH(RR)                                                                                             \
     if (R == RR) {                                                                                                      \
@@ -138,7 +137,7 @@ static int launch_dense(double2* v, int nbits, const int32_t* pos, const double*
                                 ngroups, gp, U);                                                                          \
     }
         VQC_DENSE_LAUNCH(1) else VQC_DENSE_LAUNCH(2) else VQC_DENSE_LAUNCH(4) else VQC_DENSE_LAUNCH(8) else return vqc_fail(
-            VQC_EINVAL, "VQC_DENSE_R must be 1, 2, 4 or 8");
+            VQC_EINVAL, "dense groups per thread must be 1, 2, 4 or 8");
 #undef VQC_DENSE_LAUNCH
     } else {
         VQC_TRY(vqc_upload_buffer(&g_mat_dev, &g_mat_cap, Uh, size_t(D) * D * 16));

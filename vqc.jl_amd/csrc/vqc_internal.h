@@ -43,6 +43,8 @@ struct VqcCtx {
     size_t pinned_bytes = 0;
 };
 int vqc_ctx(VqcCtx** out);
+// the tuning in force (vqc_set_tuning; initial values from the environment)
+vqc_tuning_t vqc_tuning_now();
 int vqc_scratch(size_t bytes, void** out);      // device scratch, valid until the next call
 int vqc_pinned(size_t bytes, double** out);     // pinned host staging
 // Upload host data into a dedicated device buffer of at least `bytes` bytes (grown on demand).

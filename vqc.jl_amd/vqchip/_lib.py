@@ -78,6 +78,17 @@ class ServerStats(ctypes.Structure):
         (k, ctypes.c_uint64) for k in ("budget_bytes", "reserved_bytes", "peak_live_bytes")]
 
 
+class Tuning(ctypes.Structure):
+    _fields_ = [
+        ("tile_fwd_bits", ctypes.c_int32),
+        ("tile_rev_bits", ctypes.c_int32),
+        ("tile_small_bits", ctypes.c_int32),
+        ("tile_low_bits", ctypes.c_int32),
+        ("dense_groups", ctypes.c_int32 * 3),
+        ("dense_nontemporal", ctypes.c_int32 * 3),
+    ]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.POINTER(ctypes.c_double)
@@ -90,6 +101,8 @@ SIGNATURES = {
     "vqc_device_info": [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint64)],
     "vqc_sync": [],
     "vqc_get_stream": [ctypes.POINTER(_P)],
+    "vqc_set_tuning": [ctypes.POINTER(Tuning)],
+    "vqc_get_tuning": [ctypes.POINTER(Tuning)],
     "vqc_sv_create": [_I, ctypes.POINTER(_P)],
     "vqc_dm_create": [_I, ctypes.POINTER(_P)],
     "vqc_state_wrap": [_I, _I, _P, ctypes.POINTER(_P)],
