@@ -129,6 +129,9 @@ int vqc_state_download(const vqc_state* s, double* host, uint64_t ncomplex);
 int vqc_state_reset(vqc_state* s);                            /* reset!, statevector.jl:111-115 */
 int vqc_state_copy(vqc_state* dst, const vqc_state* src);     /* Base.copy, statevector.jl:35 */
 int vqc_state_scale(vqc_state* s, double re, double im);      /* x * y, statevector.jl:50 */
+/* y += a x (x + y and x - y, statevector.jl:44-45; the vector updates of the Krylov solvers
+ * that groundstate.jl delegates to KrylovKit). */
+int vqc_state_axpy(vqc_state* y, double re, double im, const vqc_state* x);
 /* <a|b> = sum conj(a_i) b_i (LinearAlgebra.dot, statevector.jl:58). */
 int vqc_state_dot(const vqc_state* a, const vqc_state* b, double* out2);
 
@@ -189,6 +192,16 @@ int vqc_post_select(vqc_state* s, int pos, int value, double* norm);
  * order: qubit 0 takes the place of the measured one (its swap!(v, 1, pos)).
  * *norm receives that norm. SV only. */
 int vqc_sv_select(const vqc_state* s, int pos, int value, vqc_state* out, double* norm);
+
+/* ---- reductions around the path -----------------------------------------
+ * partial_tr (ptrace.jl:4-53): trace out `sites` (0-based, distinct, fewer than
+ * all n) of a state vector (out = A A^dagger, ptrace.jl:13-30) or a density
+ * matrix (ptrace.jl:42-53). out is a density matrix of n - nsites qubits, which
+ * are the kept qubits in ascending order (move_selected_index_backward,
+ * auxiliary/indexop.jl:46-70). A trace over every qubit is tr(rho) or <psi|psi>. */
+int vqc_partial_trace(const vqc_state* s, int nsites, const int32_t* sites, vqc_state* out);
+/* <psi|rho|psi>: fidelity(::DensityMatrix, ::StateVector) (densitymatrix.jl:70). */
+int vqc_dm_sandwich(const vqc_state* rho, const vqc_state* psi, double* out2);
 
 /* ---- circuits -----------------------------------------------------------
  * apply!(circuit, state) for SV or DM (generic.jl:26-31, apply_qmaps.jl:8-16),

@@ -136,3 +136,58 @@ def test_tuning_roundtrip_and_validation():
         autotune.set_tuning(**{k: before[k] for k in autotune.FIELDS}, dense_groups=before["dense_groups"],
                             dense_nontemporal=before["dense_nontemporal"])
     assert autotune.get_tuning() == before
+
+
+def _dense(H, n):
+    terms, nt, hpool = pack_terms(H)
+    M = np.zeros((1 << n, 1 << n), complex)
+    for j in range(nt):
+        k = terms[j].nq
+        D = 1 << k
+        M += full_op(n, [terms[j].pos[q] for q in range(k)], hpool[terms[j].mat:terms[j].mat + D * D].reshape(D, D, order="F"))
+    return M
+
+
+def test_spin_hamiltonian_builders():
+    # spin_hamiltonians.jl: 2J(s+ s- + h.c.) = J(XX + YY) with s+- = (X +- iY)/2
+    from vqchip.utility import heisenberg_2d, heisenberg_chain, ising_2d, ising_chain
+    X = np.array([[0, 1], [1, 0]], complex)
+    Y = np.array([[0, -1j], [1j, 0]])
+    Z = np.diag([1.0 + 0j, -1.0])
+    L, J, Jzz, hz = 4, 0.7, -0.4, 0.3
+    ref = np.zeros((16, 16), complex)
+    for i in range(L):
+        ref += hz * full_op(L, [i], Z)
+    for i in range(L - 1):
+        for P, c in ((X, J), (Y, J), (Z, Jzz)):
+            ref += c * full_op(L, [i, i + 1], np.kron(P, P))
+    assert np.allclose(_dense(heisenberg_chain(L, J=J, Jzz=Jzz, hz=hz), L), ref, atol=1e-14)
+    ref = sum(1.2 * full_op(3, [i], Z) for i in range(3)) + sum(0.5 * full_op(3, [i, i + 1], np.kron(X, X)) for i in range(2))
+    assert np.allclose(_dense(ising_chain(3, J=0.5, hz=1.2), 3), ref, atol=1e-14)
+    # 2 x 3 lattice, LinearIndices column-major: bonds (1,3),(3,5),(2,4),(4,6) and (1,2),(3,4),(5,6)
+    H = ising_2d(2, 3, J=1.0, hz=0.0)
+    keys = sorted(tuple(k) for k in H.data() if len(k) == 2)
+    assert keys == [(1, 2), (1, 3), (2, 4), (3, 4), (3, 5), (4, 6), (5, 6)]
+    assert len(heisenberg_2d(2, 3).data()) == 6 + 7
+
+
+def test_variational_circuit_builders():
+    from vqchip.utility import nparas, real_variational_circuit_1d, variational_circuit_1d
+    th = np.arange(nparas(3, 2), dtype=float)
+    c = variational_circuit_1d(3, 2, th)
+    names = [type(g).__name__ for g in c]
+    assert names[:9] == ["RzGate", "RyGate", "RzGate"] * 3
+    cn = [g.positions for g in c if type(g).__name__ == "CNOTGate"]
+    assert cn == [(1, 2), (2, 3), (2, 3), (1, 2)]  # odd layers upwards, even layers downwards
+    assert Program(c, n=3).nparams == 27
+    r = real_variational_circuit_1d(4, 3, np.zeros(nparas(4, 3, real=True)))
+    assert sum(type(g).__name__ == "RyGate" for g in r) == 16
+    with pytest.raises(ValueError):
+        variational_circuit_1d(3, 2, np.zeros(5))
+
+
+def test_krylov_rejects_non_hermitian_before_any_device_call():
+    from vqchip import krylov
+    H = QubitsOperator([QubitsTerm({1: "+", 2: "-"}, coeff=1.0)])  # s+ s- alone is not Hermitian
+    with pytest.raises(ValueError, match="hermitian"):
+        krylov.ground_state(H, 2)

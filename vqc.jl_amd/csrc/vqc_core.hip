@@ -310,6 +310,22 @@ extern "C" int vqc_state_copy(vqc_state* dst, const vqc_state* src) {
     return VQC_OK;
 }
 
+__global__ void __launch_bounds__(256) k_axpy(double2* __restrict__ y, const double2* __restrict__ x, uint64_t len, double2 a) {
+    uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    for (; i < len; i += uint64_t(gridDim.x) * blockDim.x) y[i] = cfma(a, x[i], y[i]);
+}
+
+extern "C" int vqc_state_axpy(vqc_state* y, double re, double im, const vqc_state* x) {
+    VQC_TRY(vqc_check_state(y, "vqc_state_axpy"));
+    VQC_TRY(vqc_check_state(x, "vqc_state_axpy"));
+    if (x->nbits != y->nbits) return vqc_fail(VQC_EINVAL, "vqc_state_axpy: states differ in size");
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    hipLaunchKernelGGL(k_axpy, dim3(stream_grid(y->len())), dim3(256), 0, c->stream, y->d, x->d, y->len(), make_double2(re, im));
+    VQC_HIP(hipGetLastError());
+    return VQC_OK;
+}
+
 extern "C" int vqc_state_scale(vqc_state* s, double re, double im) {
     VQC_TRY(vqc_check_state(s, "vqc_state_scale"));
     VqcCtx* c;

@@ -114,6 +114,7 @@ SIGNATURES = {
     "vqc_state_reset": [_P],
     "vqc_state_copy": [_P, _P],
     "vqc_state_scale": [_P, ctypes.c_double, ctypes.c_double],
+    "vqc_state_axpy": [_P, ctypes.c_double, ctypes.c_double, _P],
     "vqc_state_dot": [_P, _P, _D],
     "vqc_sv_apply_dense": [_P, _I, _I32, _D],
     "vqc_sv_apply_named": [_P, _I, _I32, ctypes.c_double],
@@ -129,6 +130,8 @@ SIGNATURES = {
     "vqc_measure": [_P, _I, ctypes.c_double, _I, ctypes.POINTER(ctypes.c_int), _D],
     "vqc_post_select": [_P, _I, _I, _D],
     "vqc_sv_select": [_P, _I, _I, _P, _D],
+    "vqc_partial_trace": [_P, _I, _I32, _P],
+    "vqc_dm_sandwich": [_P, _P, _D],
     "vqc_apply_circuit": [_P, _I, ctypes.POINTER(Op), _D],
     "vqc_sv_energy_grad": [_P, _I, ctypes.POINTER(Op), _D, _I, ctypes.POINTER(Term), _D, _I, _D, _D, _P, _I],
     "vqc_dm_energy_grad": [_P, _I, ctypes.POINTER(Op), _D, _I, ctypes.POINTER(Term), _D, _I, _D, _D, _P, _I],
