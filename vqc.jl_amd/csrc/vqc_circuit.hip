@@ -109,32 +109,68 @@ static MicroOp mop_init(int kind) {
     return m;
 }
 
-// Group consecutive items into phases; emit(phase, item) appends micro-ops.
+// Group items into phases; emit(phase, item) appends micro-ops.
+//
+// Lookahead: scanning the unplaced items in order, an item joins the current
+// phase when its bits fit the tile and none of its bits belongs to an item
+// skipped earlier in this scan. Such an item acts on qubits disjoint from every
+// item it overtakes, so it commutes with them exactly, and so do the generators
+// of a gradient (U_j G_k U_j^dagger = G_k for disjoint supports): states,
+// energies and gradients are unchanged. all_commute (Hamiltonian terms, which
+// are summed) drops the dependence test.
+constexpr int kLookahead = 128;
+
 template <class BitsFn, class EmitFn>
 static void plan_groups(PlanBuilder& pb, int kind, int nbits, int nitems, int t, int mlow, int max_acc_per_item,
-                        BitsFn bits_of, EmitFn emit) {
-    int i = 0;
-    while (i < nitems) {
-        std::vector<int> cur;
-        int j = i, acc = 0;
-        while (j < nitems) {
-            std::vector<int> b = bits_of(j);
-            const int a = max_acc_per_item > 0 ? max_acc_per_item : 0;
-            if (j > i && (!fits(cur, b, nbits, t, mlow) || acc + a > kMaxAcc)) break;
-            for (int x : b)
-                if (std::find(cur.begin(), cur.end(), x) == cur.end()) cur.push_back(x);
-            acc += a;
-            ++j;
+                        BitsFn bits_of, EmitFn emit, bool all_commute = false) {
+    std::vector<std::vector<int>> bits(nitems);
+    for (int j = 0; j < nitems; ++j) bits[j] = bits_of(j);
+    const int a = std::max(max_acc_per_item, 0);
+    std::vector<int> rest(nitems);
+    for (int j = 0; j < nitems; ++j) rest[j] = j;
+    std::vector<char> blocked(size_t(std::max(nbits, 1)), 0);
+    while (!rest.empty()) {
+        std::vector<int> cur, take, keep;
+        std::fill(blocked.begin(), blocked.end(), 0);
+        int nblocked = 0, acc = 0;
+        size_t idx = 0;
+        for (; idx < rest.size() && int(idx) < kLookahead && nblocked < nbits; ++idx) {
+            const int j = rest[idx];
+            const std::vector<int>& b = bits[j];
+            bool ok = true;
+            if (!all_commute)
+                for (int x : b)
+                    if (blocked[x]) ok = false;
+            if (ok && !take.empty()) ok = fits(cur, b, nbits, t, mlow) && acc + a <= kMaxAcc;
+            if (ok) {
+                take.push_back(j);
+                for (int x : b)
+                    if (std::find(cur.begin(), cur.end(), x) == cur.end()) cur.push_back(x);
+                acc += a;
+                if (a > 0 && acc + a > kMaxAcc) {
+                    ++idx;
+                    break;
+                }
+            } else {
+                keep.push_back(j);
+                if (!all_commute)
+                    for (int x : b)
+                        if (!blocked[x]) {
+                            blocked[x] = 1;
+                            ++nblocked;
+                        }
+            }
         }
+        keep.insert(keep.end(), rest.begin() + idx, rest.end());
+        rest.swap(keep);
         PhaseDesc p;
         vqc_make_phase(p, kind, nbits, t, cur);
         p.op_begin = int(pb.mops.size());
         p.row_begin = pb.nrows;
         p.nacc = 0;
-        for (int q = i; q < j; ++q) emit(p, q);
+        for (int q : take) emit(p, q);
         p.op_end = int(pb.mops.size());
         pb.phases.push_back(p);
-        i = j;
     }
 }
 
@@ -322,7 +358,8 @@ int vqc_plan_ham(PlanBuilder& pb, int nbits, int nterms, const vqc_term_t* terms
             pb.row_slot.push_back(-1 - q);
             pb.nrows++;
             pb.mops.push_back(m);
-        });
+        },
+        true);
     for (size_t i = first_phase; i < pb.phases.size(); ++i) pb.phases[i].ham_accum = i > first_phase;
     return VQC_OK;
 }
