@@ -103,7 +103,7 @@ int vqc_get_stream(void** stream);
  * current value; vqc_get_tuning reports the values in force. The VQC_TILE_* /
  * VQC_DENSE_* environment variables give the initial values. */
 typedef struct vqc_tuning {
-    int32_t tile_fwd_bits;          /* tile bits of forward phases, 6..12 (default 12) */
+    int32_t tile_fwd_bits;          /* tile bits of forward phases, 6..13 (default 12) */
     int32_t tile_rev_bits;          /* tile bits of Hamiltonian / reverse phases, 6..12 (11) */
     int32_t tile_small_bits;        /* whole evaluation in one workgroup up to this many bits, 1..12 (12) */
     int32_t tile_low_bits;          /* contiguous low index bits in every tile, 0..8 (6) */

@@ -122,3 +122,19 @@ def test_bad_request_is_rejected():
     with Server(hbm_budget=1 << 30) as srv:
         with pytest.raises(vq.VQCError):
             srv.submit(bad)
+
+
+def test_submissions_racing_the_idle_scheduler():
+    # a request queued while the scheduler finishes an idle pass must still run
+    # (regression: it used to be failed as "does not fit" without being tried)
+    reqs = make_requests(61, [4, 5, 6], depth_max=1)
+    rng = np.random.default_rng(62)
+    with Server(hbm_budget=1 << 30, lanes=2) as srv:
+        for i in range(300):
+            r = reqs[i % 3]
+            E, g, _ = srv.wait(srv.submit(r))
+            if i < 3:
+                Er, gr = oracle_eval(r)
+                assert close(E, Er, 1e-10)
+            time.sleep(float(rng.uniform(0, 2e-4)))
+        assert srv.stats()["completed"] == 300

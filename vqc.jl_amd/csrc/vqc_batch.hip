@@ -310,8 +310,11 @@ struct vqc_batch {
 typedef void (*BatchKernel)(BatchArgs);
 static BatchKernel batch_kernel(const vqc_batch* b) {
     if (b->maxkb <= 2) {
-        static const bool occ1 = getenv("VQC_BATCH_OCC1") != nullptr;  // one workgroup per CU
-        if (occ1) return b->rev_simple ? k_batch<2, KS_REV> : k_batch<2, KS_REVX>;
+        // the 128-VGPR variant only when the LDS of two workgroups fits a CU;
+        // otherwise one workgroup per CU, which needs no register cap
+        static const bool occ1 = getenv("VQC_BATCH_OCC1") != nullptr;
+        if (occ1 || 2 * std::max<size_t>(b->lds, 16 * 1024) > 160 * 1024)
+            return b->rev_simple ? k_batch<2, KS_REV> : k_batch<2, KS_REVX>;
         return b->rev_simple ? k_batch2<2, KS_REV> : k_batch2<2, KS_REVX>;
     }
     if (b->maxkb <= 3) return b->rev_simple ? k_batch<3, KS_REV> : k_batch<3, KS_REVX>;

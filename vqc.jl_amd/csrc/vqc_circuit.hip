@@ -26,8 +26,8 @@ TileTuning vqc_tile_tuning() {
     const vqc_tuning_t g = vqc_tuning_now();
     TileTuning t;
     t.t_fwd = std::min(kMaxTileBits, std::max(6, int(g.tile_fwd_bits)));
-    t.t_rev = std::min(kMaxTileBits, std::max(6, int(g.tile_rev_bits)));
-    t.t_small = std::min(kMaxTileBits, std::max(1, int(g.tile_small_bits)));
+    t.t_rev = std::min(kMaxTileBits2, std::max(6, int(g.tile_rev_bits)));
+    t.t_small = std::min(kMaxTileBits2, std::max(1, int(g.tile_small_bits)));
     t.mlow = std::min(8, std::max(0, int(g.tile_low_bits)));
     return t;
 }
@@ -243,6 +243,122 @@ int64_t XPool::product(const int64_t* srcs, int n, int D) {
     return out;
 }
 
+// Register blocks on a pair of bits (MK_BLOCK). Within one phase's micro-ops,
+// a one-bit op C on bit a joins a two-bit op P on {a, b} when no op between
+// them touches a, so C commutes with everything it is moved across:
+//   reverse: P is the last op before C touching a (an uncompute without
+//            gradients or with one, e.g. CNOT^dagger); C's steps run after P's;
+//   forward: P is the next op after C touching a (else the last one before);
+//            C's step runs before (after) P's.
+// A block holds at most 4 steps.
+namespace {
+struct BStep {
+    int tgt;
+    int64_t mat, gen;
+    int32_t acc;
+};
+bool touches(const MicroOp& m, int bit) {
+    for (int j = 0; j < m.kb; ++j)
+        if (m.lpos[j] == bit) return true;
+    return false;
+}
+void form_blocks(std::vector<MicroOp>& out, size_t b, int phase_kind) {
+    const size_t L = out.size() - b;
+    if (L < 2 || (phase_kind != PH_FWD && phase_kind != PH_REV)) return;
+    std::vector<MicroOp> seg(out.begin() + b, out.end());
+    std::vector<std::vector<BStep>> steps(L);
+    std::vector<char> dead(L, 0), isblk(L, 0);
+    const bool rev = phase_kind == PH_REV;
+    auto anchor = [&](size_t q) -> bool {  // can seg[q] become or grow a block?
+        const MicroOp& P = seg[q];
+        if (isblk[q]) return steps[q].size() < 4;
+        if (P.kb != 2) return false;
+        if (rev) return P.kind == MK_APPLY2 && P.ng <= 1;
+        return P.kind == MK_APPLY;
+    };
+    auto make_block = [&](size_t q) {
+        if (isblk[q]) return;
+        const MicroOp& P = seg[q];
+        steps[q].push_back({BS_BOTH, P.mat, rev && P.ng ? P.gen[0] : -1, rev && P.ng ? P.acc[0] : 0});
+        isblk[q] = 1;
+    };
+    for (size_t j = 0; j < L; ++j) {
+        const MicroOp& C = seg[j];
+        std::vector<BStep> cs;
+        if (rev) {
+            if (C.kind == MK_CHAIN) {
+                int k = 0;
+                for (int s2 = 0; s2 < C.nchain; ++s2) {
+                    const bool g = C.gen[s2] >= 0;
+                    cs.push_back({0, C.cmat[s2], g ? C.gen[s2] : -1, g ? C.acc[k] : 0});
+                    k += g;
+                }
+            } else if (C.kind == MK_APPLY2 && C.kb == 1 && C.ng <= 1) {
+                cs.push_back({0, C.mat, C.ng ? C.gen[0] : -1, C.ng ? C.acc[0] : 0});
+            } else {
+                continue;
+            }
+        } else {
+            if (C.kind != MK_APPLY || C.kb != 1) continue;
+            cs.push_back({0, C.mat, -1, 0});
+        }
+        const int a = C.lpos[0];
+        size_t q = L;
+        bool after = true;  // the block runs at q, C's steps after P's
+        if (!rev) {
+            for (size_t k = j + 1; k < L; ++k)
+                if (!dead[k] && touches(seg[k], a)) {
+                    q = k;
+                    break;
+                }
+            if (q < L && anchor(q) && steps[q].size() + cs.size() + (isblk[q] ? 0 : 1) <= 4 && seg[q].buf == C.buf) {
+                after = false;
+            } else {
+                q = L;
+            }
+        }
+        if (q == L)
+            for (size_t k = j; k-- > 0;)
+                if (!dead[k] && touches(seg[k], a)) {
+                    q = k;
+                    break;
+                }
+        if (q == L || !anchor(q) || seg[q].buf != C.buf) continue;
+        if (steps[q].size() + cs.size() + (isblk[q] ? 0 : 1) > 4) continue;
+        make_block(q);
+        const int tgt = seg[q].lpos[0] == a ? BS_LO : BS_HI;
+        for (BStep& st : cs) st.tgt = tgt;
+        if (after) steps[q].insert(steps[q].end(), cs.begin(), cs.end());
+        else steps[q].insert(steps[q].begin(), cs.begin(), cs.end());
+        dead[j] = 1;
+    }
+    out.resize(b);
+    for (size_t k = 0; k < L; ++k) {
+        if (dead[k]) continue;
+        if (!isblk[k]) {
+            out.push_back(seg[k]);
+            continue;
+        }
+        MicroOp m = mop_init(MK_BLOCK);
+        m.kb = 2;
+        m.buf = seg[k].buf;
+        m.lpos[0] = seg[k].lpos[0];
+        m.lpos[1] = seg[k].lpos[1];
+        m.nchain = int(steps[k].size());
+        m.ng = 0;
+        m.smask = 0;
+        for (int j = 0; j < m.nchain; ++j) {
+            const BStep& st = steps[k][j];
+            m.smask |= st.tgt << (2 * j);
+            m.cmat[j] = st.mat;
+            m.gen[j] = st.gen;
+            if (st.gen >= 0) m.acc[m.ng++] = st.acc;
+        }
+        out.push_back(m);
+    }
+}
+}  // namespace
+
 void vqc_fuse_chains(PlanBuilder& pb, size_t phase_begin, size_t phase_end) {
     // Phases own contiguous, ascending micro-op ranges; the range's phases are
     // rewritten in place and later phases are shifted.
@@ -303,6 +419,7 @@ void vqc_fuse_chains(PlanBuilder& pb, size_t phase_begin, size_t phase_end) {
             out.push_back(m);
             ++o;
         }
+        if (!getenv("VQC_NO_BLOCKS")) form_blocks(out, size_t(b), p.kind);
         p.op_begin = b;
         p.op_end = int(out.size());
     }
@@ -1088,7 +1205,6 @@ int vqc_solo_build(int n, int nops, const vqc_op_t* ops, const double* pool, int
     if (plen > pool_len || hlen > hpool_len) return vqc_fail(VQC_EINVAL, "vqc_solo: matrix offsets outside the pools");
     if ((plen && !pool) || (hlen && !hpool)) return vqc_fail(VQC_EINVAL, "vqc_solo: null matrix pool");
     const TileTuning tt = vqc_tile_tuning();
-    if (n <= tt.t_fwd) return vqc_fail(VQC_EINVAL, "vqc_solo: %d qubits fit one tile; batch them instead", n);
     vqc_solo* s = new vqc_solo;
     s->nbits = n;
     s->nparams = nparams;

@@ -405,7 +405,7 @@ extern "C" int vqc_set_tuning(const vqc_tuning_t* t) {
         take(t->dense_groups[k], n.dense_groups[k]);
         take(t->dense_nontemporal[k], n.dense_nontemporal[k]);
     }
-    if (n.tile_fwd_bits < 6 || n.tile_fwd_bits > 12 || n.tile_rev_bits < 6 || n.tile_rev_bits > 12 ||
+    if (n.tile_fwd_bits < 6 || n.tile_fwd_bits > 13 || n.tile_rev_bits < 6 || n.tile_rev_bits > 12 ||
         n.tile_small_bits < 1 || n.tile_small_bits > 12 || n.tile_low_bits < 0 || n.tile_low_bits > 8 ||
         n.tile_low_bits > std::min(n.tile_fwd_bits, n.tile_rev_bits))
         return vqc_fail(VQC_EINVAL, "vqc_set_tuning: tile bits out of range");

@@ -17,6 +17,7 @@
 // Device memory comes from a caching allocator bounded by the budget (blocks
 // are reused, not freed, so no hipFree stalls the other lanes).
 #include <algorithm>
+#include <cstdio>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -143,6 +144,7 @@ struct vqc_server {
     std::deque<Request*> pending;
     std::unordered_map<int64_t, Request*> reqs;
     int64_t next_id = 1;
+    uint64_t submits = 0;  // requests queued so far (the scheduler's view of new work)
     bool stop = false;
     int running = 0;  // requests on the device
     DevCache mem;
@@ -277,6 +279,9 @@ bool vqc_server::launch(Lane& L) {
         rc = vqc_batch_build(ni, inst.data(), ops.empty() ? nullptr : ops.data(), int64_t(ops.size()),
                              terms.empty() ? nullptr : terms.data(), int64_t(terms.size()), plen, hlen, o, &b);
     }
+    if (rc == VQC_ENOMEM && getenv("VQC_SERVE_DEBUG"))
+        fprintf(stderr, "[serve] batch build ENOMEM: %s (busy=%d live=%llu)\n", vqc_last_error(), busy,
+                (unsigned long long)mem.live);
     if (rc == VQC_ENOMEM && busy > 0) {
         // the estimate admitted it but the allocator could not place it yet:
         // back to the head of the queue, in order, until a running lane finishes
@@ -338,6 +343,10 @@ bool vqc_server::launch_solo(Lane& L, Request* r) {
         vqc_solo_free(so);
         int busy = 0;
         for (const Lane& x : lanes) busy += x.batch != nullptr || x.solo != nullptr;
+        if (getenv("VQC_SERVE_DEBUG"))
+            fprintf(stderr, "[serve] solo alloc failed: bytes=%zu budget=%llu live=%llu reserved=%llu busy=%d\n",
+                    vqc_solo_device_bytes(so), (unsigned long long)mem.budget, (unsigned long long)mem.live,
+                    (unsigned long long)mem.reserved, busy);
         if (busy) {  // wait for running work to return memory
             std::lock_guard<std::mutex> g(mu);
             pending.push_front(r);
@@ -418,6 +427,11 @@ void vqc_server::finish(Lane& L) {
 void vqc_server::run() {
     (void)hipSetDevice(device);
     for (;;) {
+        uint64_t seen;
+        {
+            std::lock_guard<std::mutex> g(mu);
+            seen = submits;
+        }
         bool progress = false;
         for (Lane& L : lanes) {
             if (!L.batch && !L.solo) continue;
@@ -447,6 +461,7 @@ void vqc_server::run() {
         for (const Lane& L : lanes) any_running |= L.batch != nullptr || L.solo != nullptr;
         std::unique_lock<std::mutex> g(mu);
         if (stop && pending.empty() && !any_running) break;
+        if (submits != seen) continue;  // queued after the launch attempts above: try those first
         if (any_running) cv_work.wait_for(g, std::chrono::microseconds(cfg.poll_us > 0 ? cfg.poll_us : 50));
         else if (pending.empty() && !stop) cv_work.wait(g);
         else if (!pending.empty()) {
@@ -454,7 +469,10 @@ void vqc_server::run() {
             Request* head = pending.front();
             pending.pop_front();
             head->status = VQC_ENOMEM;
-            head->msg = "request does not fit the serving budget";
+            char buf[192];
+            snprintf(buf, sizeof buf, "request of %llu device bytes does not fit the serving budget (%llu bytes, %llu held)",
+                     (unsigned long long)head->bytes, (unsigned long long)mem.budget, (unsigned long long)mem.live);
+            head->msg = buf;
             head->t_done = now();
             cv_done.notify_all();
         }
@@ -542,6 +560,7 @@ extern "C" int vqc_server_submit(vqc_server* s, const vqc_instance_t* inst, cons
             return vqc_fail(VQC_EINVAL, "vqc_server_submit: server is shutting down");
         }
         r->id = s->next_id++;
+        ++s->submits;
         r->t_submit = s->now();
         s->reqs[r->id] = r;
         s->pending.push_back(r);

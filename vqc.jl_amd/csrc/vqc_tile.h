@@ -19,6 +19,10 @@
 //   MK_HAM     B += (M + M^dagger) A, acc += sum conj(A) (M A)   (adjoint seed, energy)
 //   MK_CHAIN   up to 4 one-bit MK_APPLY2 steps on the same bit, done in registers
 //              (one LDS round trip for e.g. the Rz, Ry, Rx of one qubit)
+//   MK_BLOCK   up to 4 steps on one pair of bits, done in registers: each step a
+//              4x4 on both bits or a 2x2 on either bit, on A alone (forward
+//              phases) or on A and B with an optional generator per step
+//              (reverse phases), e.g. a CNOT and the rotations of one of its qubits
 // Matrices are read from a device pool in which the planner has already stored
 // every variant a micro-op needs (U, U^dagger, conj(U), U^T, M + M^dagger), so
 // the inner loops are plain complex matrix-vector products.
@@ -27,35 +31,38 @@
 
 #include "vqc_internal.h"
 
-enum { MK_APPLY = 0, MK_APPLY2 = 1, MK_APPLY2X = 2, MK_GRAD = 3, MK_HAM = 4, MK_CHAIN = 5 };
+enum { MK_APPLY = 0, MK_APPLY2 = 1, MK_APPLY2X = 2, MK_GRAD = 3, MK_HAM = 4, MK_CHAIN = 5, MK_BLOCK = 6 };
+enum { BS_BOTH = 0, BS_LO = 1, BS_HI = 2 };  // MK_BLOCK step targets
 enum { MF_CONJ = 1, MF_TRANS = 2 };
 enum { PH_FWD = 0, PH_REV = 1, PH_HAM = 2 };
 // kind sets a kernel is built for (bit k = micro-op kind k)
 enum {
-    KS_FWD = 1 << MK_APPLY,
+    KS_FWD = (1 << MK_APPLY) | (1 << MK_BLOCK),
     KS_HAM = 1 << MK_HAM,
-    KS_REV = (1 << MK_APPLY2) | (1 << MK_CHAIN),  // state-vector reverse
-    KS_REVX = (1 << MK_APPLY) | (1 << MK_APPLY2) | (1 << MK_APPLY2X) | (1 << MK_GRAD) | (1 << MK_CHAIN),
+    KS_REV = (1 << MK_APPLY2) | (1 << MK_CHAIN) | (1 << MK_BLOCK),  // state-vector reverse
+    KS_REVX = (1 << MK_APPLY) | (1 << MK_APPLY2) | (1 << MK_APPLY2X) | (1 << MK_GRAD) | (1 << MK_CHAIN) | (1 << MK_BLOCK),
 };
 
 constexpr int kTileThreads = 512;
 constexpr int kTileWaves = kTileThreads / 64;
 constexpr int kLoadBatch = 8;      // global loads in flight per thread when a tile is read
-constexpr int kMaxTileBits = 12;
+constexpr int kMaxTileBits = 13;   // one tile buffer in LDS (forward phases): 128 KiB
+constexpr int kMaxTileBits2 = 12;  // two tile buffers (Hamiltonian / reverse phases, whole small evaluations)
 constexpr int kMaxAcc = 48;  // accumulator entries per phase
 
 struct MicroOp {
     int32_t kind;
     int32_t kb;         // local bits touched (1..5)
     int32_t buf;        // MK_APPLY: 0 = A, 1 = B
-    int32_t nchain;     // MK_CHAIN: steps (2..4)
+    int32_t nchain;     // MK_CHAIN / MK_BLOCK: steps (2..4)
     int32_t lpos[5];    // ascending local bit positions
     int32_t ng;         // partial sums produced (MK_APPLY2: 0..3, MK_GRAD / MK_HAM: 1, MK_CHAIN: 0..4)
     int32_t acc[4];     // accumulator index of each partial
+    int32_t smask;      // MK_BLOCK: target of step j in bits 2j..2j+1 (BS_BOTH / BS_LO / BS_HI)
     int64_t mat;
     int64_t mat2;
-    int64_t gen[4];     // generators; MK_CHAIN: one per step, -1 when the step has none
-    int64_t cmat[4];    // MK_CHAIN: the steps' matrices (U^dagger), in order
+    int64_t gen[4];     // generators; MK_CHAIN / MK_BLOCK: one per step, -1 when the step has none
+    int64_t cmat[4];    // MK_CHAIN / MK_BLOCK: the steps' matrices, in order
 };
 
 struct PhaseDesc {
@@ -299,6 +306,114 @@ __device__ __forceinline__ void mop_chain(double2* __restrict__ sa, double2* __r
     if (nch > 3 && op.gen[3] >= 0) wave_partial(red, buf, q++, g3, 0.0);
 }
 
+// MK_BLOCK: the amplitudes of one pair of bits (lo, hi) in registers, index
+// j = bit_lo + 2 bit_hi; each step updates them in place, and a reverse step
+// with a generator adds Re sum conj(b) (G a) over the pairs it touched.
+__device__ __forceinline__ void blk_2x2(const double2* __restrict__ M, double2& x0, double2& x1) {
+    const double2 m00 = M[0], m10 = M[1], m01 = M[2], m11 = M[3];
+    const double2 y0 = cfma(m01, x1, cmul(m00, x0)), y1 = cfma(m11, x1, cmul(m10, x0));
+    x0 = y0;
+    x1 = y1;
+}
+__device__ __forceinline__ void blk_4x4(const double2* __restrict__ M, double2* x) {
+    double2 y[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc = cfma(M[r + 4 * c], x[c], acc);
+        y[r] = acc;
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r) x[r] = y[r];
+}
+__device__ __forceinline__ double blk_grad2(const double2* __restrict__ G, double2 a0, double2 a1, double2 b0, double2 b1) {
+    const double2 s0 = cfma(G[2], a1, cmul(G[0], a0)), s1 = cfma(G[3], a1, cmul(G[1], a0));
+    return fma(b0.x, s0.x, fma(b0.y, s0.y, fma(b1.x, s1.x, b1.y * s1.y)));
+}
+__device__ __forceinline__ void mop_block(double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
+                                          const double2* __restrict__ pool, double2* red, int buf, bool two) {
+    const int nst = op.nchain;
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+    const uint32_t ng = 1u << (t - 2);
+    const uint32_t olo = 1u << op.lpos[0], ohi = 1u << op.lpos[1];
+    for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
+        const uint32_t base = ins32<2>(g, op.lpos);
+        double2 a[4], b[4];
+        a[0] = sa[base];
+        a[1] = sa[base + olo];
+        a[2] = sa[base + ohi];
+        a[3] = sa[base + olo + ohi];
+        if (two) {
+            b[0] = sb[base];
+            b[1] = sb[base + olo];
+            b[2] = sb[base + ohi];
+            b[3] = sb[base + olo + ohi];
+        }
+        for (int j = 0; j < 4; ++j) {
+            if (j >= nst) break;
+            const double2* __restrict__ M = pool + op.cmat[j];
+            const int tgt = (op.smask >> (2 * j)) & 3;
+            if (tgt == BS_BOTH) {
+                blk_4x4(M, a);
+                if (two) blk_4x4(M, b);
+            } else if (tgt == BS_LO) {
+                blk_2x2(M, a[0], a[1]);
+                blk_2x2(M, a[2], a[3]);
+                if (two) {
+                    blk_2x2(M, b[0], b[1]);
+                    blk_2x2(M, b[2], b[3]);
+                }
+            } else {
+                blk_2x2(M, a[0], a[2]);
+                blk_2x2(M, a[1], a[3]);
+                if (two) {
+                    blk_2x2(M, b[0], b[2]);
+                    blk_2x2(M, b[1], b[3]);
+                }
+            }
+            if (two && op.gen[j] >= 0) {
+                const double2* __restrict__ G = pool + op.gen[j];
+                double v = 0.0;
+                if (tgt == BS_BOTH) {
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        double2 sg = make_double2(0.0, 0.0);
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) sg = cfma(G[r + 4 * c], a[c], sg);
+                        v = fma(b[r].x, sg.x, fma(b[r].y, sg.y, v));
+                    }
+                } else if (tgt == BS_LO) {
+                    v = blk_grad2(G, a[0], a[1], b[0], b[1]) + blk_grad2(G, a[2], a[3], b[2], b[3]);
+                } else {
+                    v = blk_grad2(G, a[0], a[2], b[0], b[2]) + blk_grad2(G, a[1], a[3], b[1], b[3]);
+                }
+                if (j == 0) g0 += v;
+                else if (j == 1) g1 += v;
+                else if (j == 2) g2 += v;
+                else g3 += v;
+            }
+        }
+        sa[base] = a[0];
+        sa[base + olo] = a[1];
+        sa[base + ohi] = a[2];
+        sa[base + olo + ohi] = a[3];
+        if (two) {
+            sb[base] = b[0];
+            sb[base + olo] = b[1];
+            sb[base + ohi] = b[2];
+            sb[base + olo + ohi] = b[3];
+        }
+    }
+    if (two) {
+        int q = 0;
+        if (nst > 0 && op.gen[0] >= 0) wave_partial(red, buf, q++, g0, 0.0);
+        if (nst > 1 && op.gen[1] >= 0) wave_partial(red, buf, q++, g1, 0.0);
+        if (nst > 2 && op.gen[2] >= 0) wave_partial(red, buf, q++, g2, 0.0);
+        if (nst > 3 && op.gen[3] >= 0) wave_partial(red, buf, q++, g3, 0.0);
+    }
+}
+
 // MK_HAM: B += Dm A with Dm = M + M^dagger (op.mat2); acc += sum conj(A) (M A)
 template <int KB>
 __device__ __forceinline__ void mop_ham(const double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
@@ -378,6 +493,14 @@ __device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const M
             if constexpr ((KS & (1 << MK_CHAIN)) != 0) {
                 mop_chain(sa, sb, t, op, pool, red, buf);
                 nred = op.ng;
+            }
+            break;
+        case MK_BLOCK:
+            if constexpr ((KS & (1 << MK_BLOCK)) != 0) {
+                // forward micro-ops act on A alone; reverse ones on A and B
+                constexpr bool two = KS != KS_FWD;
+                mop_block(sa, sb, t, op, pool, red, buf, two);
+                nred = two ? op.ng : 0;
             }
             break;
         case MK_HAM:
@@ -552,4 +675,4 @@ inline int vqc_phase_op_start(const PlanBuilder& pb, size_t i) {
     return i < pb.phases.size() ? pb.phases[i].op_begin : int(pb.mops.size());
 }
 // micro-ops a KS_REV kernel runs
-inline bool vqc_rev_simple(const MicroOp& m) { return m.kind == MK_APPLY2 || m.kind == MK_CHAIN; }
+inline bool vqc_rev_simple(const MicroOp& m) { return m.kind == MK_APPLY2 || m.kind == MK_CHAIN || m.kind == MK_BLOCK; }
