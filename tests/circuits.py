@@ -22,3 +22,21 @@ def chain_circuit(n, rng, layers=2, value_rng=None):
         for q in range(1 + layer % 2, n, 2):
             c.append(vq.CNOTGate(q, q + 1))
     return c
+
+
+def structured_circuit(n, rng, layers=3, value_rng=None):
+    """Diagonal gates (Rz, PHASE, S, T, Z) next to dense ones and CNOTs with the
+    control on either side, so that every structure fast path (diagonal steps and
+    generators, CNOT as a register swap, control low or high) is exercised."""
+    vr = rng if value_rng is None else value_rng
+    c = vq.QCircuit()
+    diag = [lambda q, a: vq.RzGate(q, a), lambda q, a: vq.PHASEGate(q, a), lambda q, a: vq.SGate(q),
+            lambda q, a: vq.TGate(q), lambda q, a: vq.ZGate(q)]
+    for layer in range(layers):
+        for q in range(1, n + 1):
+            c.append(vq.RyGate(q, float(vr.uniform(0, 6.3))))
+            c.append(diag[int(rng.integers(len(diag)))](q, float(vr.uniform(0, 6.3))))
+            c.append(vq.RzGate(q, float(vr.uniform(0, 6.3))))
+        for q in range(1, n):
+            c.append(vq.CNOTGate(q + 1, q) if (q + layer) % 2 else vq.CNOTGate(q, q + 1))
+    return c

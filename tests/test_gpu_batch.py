@@ -7,7 +7,7 @@ import pytest
 
 import oracle
 from brute import rand_unitary
-from circuits import chain_circuit
+from circuits import chain_circuit, structured_circuit
 import vqchip as vq
 from vqchip.batch import Batch
 from vqchip.circuit import Program
@@ -103,3 +103,15 @@ def test_same_qubit_runs_in_batch():
     b.update([c for c, _, _ in inst2])
     E2, g2 = b.run()
     check(inst2, E2, g2)
+
+
+def test_structure_hints_in_batch():
+    # batches take gate structure from the hints (their matrices arrive at launch)
+    ns = (5, 9, 12, 14, 15)
+    vr = np.random.default_rng(20)
+    inst = [(structured_circuit(n, np.random.default_rng(n), value_rng=vr), heisenberg(n, np.ones(n - 1)), n) for n in ns]
+    b = Batch(inst)
+    check(inst, *b.run())
+    inst2 = [(structured_circuit(n, np.random.default_rng(n), value_rng=vr), H, n) for _, H, n in inst]
+    b.update([c for c, _, _ in inst2])
+    check(inst2, *b.run())

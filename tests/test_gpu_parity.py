@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import oracle
-from circuits import chain_circuit
+from circuits import chain_circuit, structured_circuit
 from brute import full_op, rand_density, rand_matrix, rand_state, rand_unitary, unvec, vec
 import vqchip as vq
 from vqchip import _lib
@@ -269,6 +269,18 @@ def test_same_qubit_runs_energy_gradient(n):
     s = sv_from(psi0)
     vq.apply_(c, s)
     assert np.max(np.abs(s.storage() - psi_r)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [6, 12, 16])
+def test_structured_gates_energy_gradient(n):
+    rng = np.random.default_rng(90 + n)
+    c = structured_circuit(n, rng)
+    H = tfim(n, 0.8)
+    psi0 = rand_state(n, rng)
+    E, g, psi, Er, gr, psi_r = _sv_energy_grad(c, H, n, psi0, True)
+    assert np.max(np.abs(psi - psi_r)) < 1e-12
+    assert abs(E - Er) < 1e-10 * max(1, abs(Er))
+    assert np.max(np.abs(g - gr)) < 1e-10 * max(1, np.max(np.abs(gr)))
 
 
 @pytest.mark.parametrize("n,fused", [(3, True), (5, True), (7, True), (5, False)])

@@ -109,6 +109,39 @@ static MicroOp mop_init(int kind) {
     return m;
 }
 
+// Structure (MS_*) of a D x D pool matrix: from its values when the pool is on
+// the host, else from the op's hint bits (vqc_op_t::reserved): bit 0 U
+// diagonal, bit 1 generators diagonal, bit 2 / 3 U is a CNOT whose control is
+// the lower / higher position. Values decide whenever they are known, so a
+// hint can only matter for batches, whose matrices arrive at launch.
+static int host_mstruct(const double* M, int D) {
+    auto z = [&](int r, int c) { return M[2 * (r + c * D)] == 0.0 && M[2 * (r + c * D) + 1] == 0.0; };
+    auto one = [&](int r, int c) { return M[2 * (r + c * D)] == 1.0 && M[2 * (r + c * D) + 1] == 0.0; };
+    if (D == 2) return z(1, 0) && z(0, 1) ? MS_DIAG : MS_DENSE;
+    if (D != 4) return MS_DENSE;
+    bool lo = true, hi = true;
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) {
+            const int plo = c == 1 ? 3 : (c == 3 ? 1 : c), phi = c == 2 ? 3 : (c == 3 ? 2 : c);
+            lo = lo && (r == plo ? one(r, c) : z(r, c));
+            hi = hi && (r == phi ? one(r, c) : z(r, c));
+        }
+    return lo ? MS_CX_LO : (hi ? MS_CX_HI : MS_DENSE);
+}
+
+static int mstruct_of(const XPool& xp, int64_t off, int D, int hint_ms) {
+    if (off < 0) return MS_DENSE;
+    if (!xp.deferred) return host_mstruct(xp.h.data() + 2 * off, D);
+    return hint_ms;
+}
+static int hint_mat(const vqc_op_t& o) {
+    if (o.nq == 1 && (o.reserved & 1)) return MS_DIAG;
+    if (o.nq == 2 && (o.reserved & 4)) return MS_CX_LO;
+    if (o.nq == 2 && (o.reserved & 8)) return MS_CX_HI;
+    return MS_DENSE;
+}
+static int hint_gen(const vqc_op_t& o) { return o.nq == 1 && (o.reserved & 2) ? MS_DIAG : MS_DENSE; }
+
 // Group items into phases; emit(phase, item) appends micro-ops.
 //
 // Lookahead: scanning the unplaced items in order, an item joins the current
@@ -256,6 +289,7 @@ struct BStep {
     int tgt;
     int64_t mat, gen;
     int32_t acc;
+    int ms;  // MS_* of mat in bits 0..1, of gen in bits 2..3
 };
 bool touches(const MicroOp& m, int bit) {
     for (int j = 0; j < m.kb; ++j)
@@ -279,7 +313,7 @@ void form_blocks(std::vector<MicroOp>& out, size_t b, int phase_kind) {
     auto make_block = [&](size_t q) {
         if (isblk[q]) return;
         const MicroOp& P = seg[q];
-        steps[q].push_back({BS_BOTH, P.mat, rev && P.ng ? P.gen[0] : -1, rev && P.ng ? P.acc[0] : 0});
+        steps[q].push_back({BS_BOTH, P.mat, rev && P.ng ? P.gen[0] : -1, rev && P.ng ? P.acc[0] : 0, P.mstr & 15});
         isblk[q] = 1;
     };
     for (size_t j = 0; j < L; ++j) {
@@ -290,17 +324,17 @@ void form_blocks(std::vector<MicroOp>& out, size_t b, int phase_kind) {
                 int k = 0;
                 for (int s2 = 0; s2 < C.nchain; ++s2) {
                     const bool g = C.gen[s2] >= 0;
-                    cs.push_back({0, C.cmat[s2], g ? C.gen[s2] : -1, g ? C.acc[k] : 0});
+                    cs.push_back({0, C.cmat[s2], g ? C.gen[s2] : -1, g ? C.acc[k] : 0, (C.mstr >> (4 * s2)) & 15});
                     k += g;
                 }
             } else if (C.kind == MK_APPLY2 && C.kb == 1 && C.ng <= 1) {
-                cs.push_back({0, C.mat, C.ng ? C.gen[0] : -1, C.ng ? C.acc[0] : 0});
+                cs.push_back({0, C.mat, C.ng ? C.gen[0] : -1, C.ng ? C.acc[0] : 0, C.mstr & 15});
             } else {
                 continue;
             }
         } else {
             if (C.kind != MK_APPLY || C.kb != 1) continue;
-            cs.push_back({0, C.mat, -1, 0});
+            cs.push_back({0, C.mat, -1, 0, C.mstr & 3});
         }
         const int a = C.lpos[0];
         size_t q = L;
@@ -347,9 +381,11 @@ void form_blocks(std::vector<MicroOp>& out, size_t b, int phase_kind) {
         m.nchain = int(steps[k].size());
         m.ng = 0;
         m.smask = 0;
+        m.mstr = 0;
         for (int j = 0; j < m.nchain; ++j) {
             const BStep& st = steps[k][j];
             m.smask |= st.tgt << (2 * j);
+            m.mstr |= st.ms << (4 * j);
             m.cmat[j] = st.mat;
             m.gen[j] = st.gen;
             if (st.gen >= 0) m.acc[m.ng++] = st.acc;
@@ -386,6 +422,7 @@ void vqc_fuse_chains(PlanBuilder& pb, size_t phase_begin, size_t phase_end) {
                     for (int j = 0; j < run; ++j) srcs[j] = pb.mops[o + j].mat;
                     MicroOp f = m;
                     f.mat = xp.product(srcs, run, 2);
+                    f.mstr = xp.deferred ? MS_DENSE : host_mstruct(xp.h.data() + 2 * f.mat, 2);
                     out.push_back(f);
                     o += run;
                     continue;
@@ -405,11 +442,13 @@ void vqc_fuse_chains(PlanBuilder& pb, size_t phase_begin, size_t phase_end) {
                     f.lpos[0] = m.lpos[0];
                     f.nchain = run;
                     f.ng = 0;
+                    f.mstr = 0;
                     for (int j = 0; j < run; ++j) {
                         const MicroOp& x = pb.mops[o + j];
                         f.cmat[j] = x.mat;
                         f.gen[j] = x.ng ? x.gen[0] : -1;
                         if (x.ng) f.acc[f.ng++] = x.acc[0];
+                        f.mstr |= (x.mstr & 15) << (4 * j);
                     }
                     out.push_back(f);
                     o += run;
@@ -443,6 +482,7 @@ int vqc_plan_forward(PlanBuilder& pb, int nbits, int n, int is_dm, int nops, con
                 MicroOp m = mop_init(MK_APPLY);
                 set_lpos(m, p, b.data(), int(b.size()));
                 m.mat = o.mat;
+                if (o.kind != VQC_OP_CHANNEL) m.mstr = mstruct_of(xp, m.mat, 1 << o.nq, hint_mat(o));
                 pb.mops.push_back(m);
             } else {
                 int cols[5];
@@ -450,10 +490,12 @@ int vqc_plan_forward(PlanBuilder& pb, int nbits, int n, int is_dm, int nops, con
                 MicroOp m = mop_init(MK_APPLY);
                 set_lpos(m, p, o.pos, o.nq);
                 m.mat = o.mat;
+                m.mstr = mstruct_of(xp, m.mat, 1 << o.nq, hint_mat(o));
                 pb.mops.push_back(m);
                 m = mop_init(MK_APPLY);
                 set_lpos(m, p, cols, o.nq);
                 m.mat = xp.variant(o.mat, 1 << o.nq, MF_CONJ);
+                m.mstr = mstruct_of(xp, m.mat, 1 << o.nq, hint_mat(o));
                 pb.mops.push_back(m);
             }
         });
@@ -520,12 +562,14 @@ int vqc_plan_reverse(PlanBuilder& pb, int nbits, int n, int is_dm, int nops, con
                     MicroOp m = mop_init(MK_APPLY2);
                     set_lpos(m, p, bits, o.nq);
                     m.mat = m.mat2 = mat;
+                    m.mstr = mstruct_of(xp, mat, D, hint_mat(o));
                     if (with_grads) {
                         m.ng = na;
                         for (int j = 0; j < na; ++j) {
                             m.gen[j] = gens[j];
                             m.acc[j] = accs[j];
                         }
+                        if (na == 1) m.mstr |= mstruct_of(xp, gens[0], D, hint_gen(o)) << 2;
                     }
                     pb.mops.push_back(m);
                     return;

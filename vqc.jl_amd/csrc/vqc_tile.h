@@ -33,6 +33,7 @@
 
 enum { MK_APPLY = 0, MK_APPLY2 = 1, MK_APPLY2X = 2, MK_GRAD = 3, MK_HAM = 4, MK_CHAIN = 5, MK_BLOCK = 6 };
 enum { BS_BOTH = 0, BS_LO = 1, BS_HI = 2 };  // MK_BLOCK step targets
+enum { MS_DENSE = 0, MS_DIAG = 1, MS_CX_LO = 2, MS_CX_HI = 3 };  // matrix structure
 enum { MF_CONJ = 1, MF_TRANS = 2 };
 enum { PH_FWD = 0, PH_REV = 1, PH_HAM = 2 };
 // kind sets a kernel is built for (bit k = micro-op kind k)
@@ -59,6 +60,8 @@ struct MicroOp {
     int32_t ng;         // partial sums produced (MK_APPLY2: 0..3, MK_GRAD / MK_HAM: 1, MK_CHAIN: 0..4)
     int32_t acc[4];     // accumulator index of each partial
     int32_t smask;      // MK_BLOCK: target of step j in bits 2j..2j+1 (BS_BOTH / BS_LO / BS_HI)
+    int32_t mstr;       // structure (MS_*) of step j's matrix in bits 4j..4j+1, of its generator in 4j+2..4j+3
+                        // (step 0 = the single matrix of MK_APPLY / MK_APPLY2)
     int64_t mat;
     int64_t mat2;
     int64_t gen[4];     // generators; MK_CHAIN / MK_BLOCK: one per step, -1 when the step has none
@@ -167,12 +170,46 @@ __device__ __forceinline__ void acc_push(AccState& st, int n, const int32_t* e) 
 
 // Row loops are unrolled for up to 3 local bits; for 4 and 5 bits only the
 // inner product is unrolled (it indexes the register-held group).
+// Matrix structure of a micro-op's matrices, decided by the planner (from the
+// values when it has them, else from the caller's per-op hints) and stored in
+// MicroOp::mstr: diagonal 2x2 matrices (Rz, phases and their generators) take
+// two products instead of a 2x2 product, and the exact CNOT patterns (control
+// on the low or the high bit of the pair) become a register swap.
+__device__ __forceinline__ int ms_mat(const MicroOp& op, int j) { return (op.mstr >> (4 * j)) & 3; }
+__device__ __forceinline__ int ms_gen(const MicroOp& op, int j) { return (op.mstr >> (4 * j + 2)) & 3; }
+__device__ __forceinline__ void dswap(double2& x, double2& y) {
+    const double2 t = x;
+    x = y;
+    y = t;
+}
+
 template <int KB>
 __device__ __forceinline__ void mop_apply(double2* __restrict__ s, int t, const MicroOp& op, const double2* __restrict__ pool) {
     constexpr int D = 1 << KB;
     constexpr int RU = KB <= 3 ? D : 1;
     const double2* __restrict__ M = pool + op.mat;
     const uint32_t ng = 1u << (t - KB);
+    if constexpr (KB == 1 || KB == 2) {
+        const int ms = ms_mat(op, 0);
+        if (ms != MS_DENSE) {
+            const double2 d0 = M[0], d1 = M[3];
+            for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
+                const uint32_t base = ins32<KB>(g, op.lpos);
+                if (KB == 1) {
+                    const uint32_t o1 = 1u << op.lpos[0];
+                    s[base] = cmul(d0, s[base]);
+                    s[base + o1] = cmul(d1, s[base + o1]);
+                } else {
+                    const uint32_t o1 = off32<KB>(1, op.lpos), o2 = off32<KB>(2, op.lpos), o3 = off32<KB>(3, op.lpos);
+                    const uint32_t x = base + (ms == MS_CX_LO ? o1 : o2), y = base + o3;
+                    const double2 vx = s[x], vy = s[y];
+                    s[x] = vy;
+                    s[y] = vx;
+                }
+            }
+            return;
+        }
+    }
     for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
         const uint32_t base = ins32<KB>(g, op.lpos);
         double2 a[D];
@@ -277,16 +314,25 @@ __device__ __forceinline__ void mop_chain(double2* __restrict__ sa, double2* __r
         for (int j = 0; j < 4; ++j) {
             if (j >= nch) break;
             const double2* __restrict__ M = pool + op.cmat[j];
-            const double2 m00 = M[0], m10 = M[1], m01 = M[2], m11 = M[3];
-            double2 x0 = cfma(m01, a1, cmul(m00, a0)), x1 = cfma(m11, a1, cmul(m10, a0));
-            double2 y0 = cfma(m01, b1, cmul(m00, b0)), y1 = cfma(m11, b1, cmul(m10, b0));
-            a0 = x0;
-            a1 = x1;
-            b0 = y0;
-            b1 = y1;
+            if (ms_mat(op, j) == MS_DIAG) {
+                const double2 m00 = M[0], m11 = M[3];
+                a0 = cmul(m00, a0);
+                a1 = cmul(m11, a1);
+                b0 = cmul(m00, b0);
+                b1 = cmul(m11, b1);
+            } else {
+                const double2 m00 = M[0], m10 = M[1], m01 = M[2], m11 = M[3];
+                double2 x0 = cfma(m01, a1, cmul(m00, a0)), x1 = cfma(m11, a1, cmul(m10, a0));
+                double2 y0 = cfma(m01, b1, cmul(m00, b0)), y1 = cfma(m11, b1, cmul(m10, b0));
+                a0 = x0;
+                a1 = x1;
+                b0 = y0;
+                b1 = y1;
+            }
             if (op.gen[j] >= 0) {
                 const double2* __restrict__ G = pool + op.gen[j];
-                const double2 s0 = cfma(G[2], a1, cmul(G[0], a0)), s1 = cfma(G[3], a1, cmul(G[1], a0));
+                const double2 s0 = ms_gen(op, j) == MS_DIAG ? cmul(G[0], a0) : cfma(G[2], a1, cmul(G[0], a0));
+                const double2 s1 = ms_gen(op, j) == MS_DIAG ? cmul(G[3], a1) : cfma(G[3], a1, cmul(G[1], a0));
                 const double v = fma(b0.x, s0.x, fma(b0.y, s0.y, fma(b1.x, s1.x, b1.y * s1.y)));
                 if (j == 0) g0 += v;
                 else if (j == 1) g1 += v;
@@ -354,9 +400,32 @@ __device__ __forceinline__ void mop_block(double2* __restrict__ sa, double2* __r
             if (j >= nst) break;
             const double2* __restrict__ M = pool + op.cmat[j];
             const int tgt = (op.smask >> (2 * j)) & 3;
+            const int skj = ms_mat(op, j), gkj = ms_gen(op, j);
             if (tgt == BS_BOTH) {
-                blk_4x4(M, a);
-                if (two) blk_4x4(M, b);
+                if (skj == MS_CX_LO) {
+                    dswap(a[1], a[3]);
+                    if (two) dswap(b[1], b[3]);
+                } else if (skj == MS_CX_HI) {
+                    dswap(a[2], a[3]);
+                    if (two) dswap(b[2], b[3]);
+                } else {
+                    blk_4x4(M, a);
+                    if (two) blk_4x4(M, b);
+                }
+            } else if (skj == MS_DIAG) {
+                // d0 on the amplitudes with the step's bit clear, d1 where it is set
+                const double2 d0 = M[0], d1 = M[3];
+                const bool lo = tgt == BS_LO;
+                a[0] = cmul(d0, a[0]);
+                a[1] = cmul(lo ? d1 : d0, a[1]);
+                a[2] = cmul(lo ? d0 : d1, a[2]);
+                a[3] = cmul(d1, a[3]);
+                if (two) {
+                    b[0] = cmul(d0, b[0]);
+                    b[1] = cmul(lo ? d1 : d0, b[1]);
+                    b[2] = cmul(lo ? d0 : d1, b[2]);
+                    b[3] = cmul(d1, b[3]);
+                }
             } else if (tgt == BS_LO) {
                 blk_2x2(M, a[0], a[1]);
                 blk_2x2(M, a[2], a[3]);
@@ -383,6 +452,14 @@ __device__ __forceinline__ void mop_block(double2* __restrict__ sa, double2* __r
                         for (int c = 0; c < 4; ++c) sg = cfma(G[r + 4 * c], a[c], sg);
                         v = fma(b[r].x, sg.x, fma(b[r].y, sg.y, v));
                     }
+                } else if (gkj == MS_DIAG) {
+                    // Re sum conj(b) (G a) with G diagonal: G00 where the step's bit is clear
+                    const double2 e0 = G[0], e1 = G[3];
+                    const bool lo = tgt == BS_LO;
+                    const double2 s0 = cmul(e0, a[0]), s1 = cmul(lo ? e1 : e0, a[1]);
+                    const double2 s2 = cmul(lo ? e0 : e1, a[2]), s3 = cmul(e1, a[3]);
+                    v = fma(b[0].x, s0.x, fma(b[0].y, s0.y, fma(b[1].x, s1.x, b[1].y * s1.y)));
+                    v = fma(b[2].x, s2.x, fma(b[2].y, s2.y, fma(b[3].x, s3.x, fma(b[3].y, s3.y, v))));
                 } else if (tgt == BS_LO) {
                     v = blk_grad2(G, a[0], a[1], b[0], b[1]) + blk_grad2(G, a[2], a[3], b[2], b[3]);
                 } else {

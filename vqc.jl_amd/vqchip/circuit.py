@@ -35,6 +35,19 @@ class QCircuit(list):
         return QCircuit(g.adjoint() for g in reversed(self))
 
 
+def _structure_hint(g):
+    """vqc_op_t.reserved: bit 0 U diagonal, bit 1 generators diagonal, bit 2 / 3
+    U is a CNOT with control on the lower / higher position. Batches plan before
+    their angles arrive, so the planner takes a gate's structure from its type."""
+    from .gates import CNOTGate, PHASEGate, RzGate, SGate, TGate, ZGate
+    if isinstance(g, (RzGate, PHASEGate, ZGate, SGate, TGate)):
+        return 1 | 2
+    if isinstance(g, CNOTGate):
+        c, t = g.positions
+        return 4 if c < t else 8
+    return 0
+
+
 class Program:
     """A circuit packed as vqc_op_t records plus one complex matrix pool.
 
@@ -87,6 +100,8 @@ class Program:
                 for j in range(3):
                     op.slot[j] = -1
                     op.gen[j] = -1
+            else:
+                op.reserved = _structure_hint(g)
             self.ops.append(op)
         self.nparams = slot - slot_base
         self.slot_end = slot
