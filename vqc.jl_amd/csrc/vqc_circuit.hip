@@ -277,16 +277,20 @@ int64_t XPool::product(const int64_t* srcs, int n, int D) {
 }
 
 // Register blocks on a pair of bits (MK_BLOCK). Within one phase's micro-ops,
-// a one-bit op C on bit a joins a two-bit op P on {a, b} when no op between
-// them touches a, so C commutes with everything it is moved across:
-//   reverse: P is the last op before C touching a (an uncompute without
-//            gradients or with one, e.g. CNOT^dagger); C's steps run after P's;
-//   forward: P is the next op after C touching a (else the last one before);
-//            C's step runs before (after) P's.
-// A block holds at most 4 steps.
+// a one-bit op C on bit a moves to a block position q when no op between them
+// touches a, so C commutes with everything it is moved across:
+//   * q holds an op or block whose bits include a (reverse: the last op before C
+//     touching a; forward: the next op after C touching a, else the last before);
+//   * or, moving backwards, q is the nearest one-bit op or one-bit block on
+//     another bit with no op touching a in between; the block gets both bits
+//     (e.g. the row and column uncompute of a density-matrix gate).
+// Reverse one-bit ops are uncomputes with at most one gradient (MK_APPLY2,
+// MK_CHAIN) and gradient terms (MK_GRAD, a step without a matrix). A block
+// holds at most 4 steps.
 namespace {
 struct BStep {
-    int tgt;
+    int tgt;  // filled when the block's bits are final
+    int bit;
     int64_t mat, gen;
     int32_t acc;
     int ms;  // MS_* of mat in bits 0..1, of gen in bits 2..3
@@ -296,74 +300,105 @@ bool touches(const MicroOp& m, int bit) {
         if (m.lpos[j] == bit) return true;
     return false;
 }
+// the steps of a one-bit op (empty when it cannot be a step)
+std::vector<BStep> one_bit_steps(const MicroOp& C, bool rev) {
+    std::vector<BStep> cs;
+    if (C.kb != 1) return cs;
+    const int a = C.lpos[0];
+    if (rev) {
+        if (C.kind == MK_CHAIN) {
+            int k = 0;
+            for (int s2 = 0; s2 < C.nchain; ++s2) {
+                const bool g = C.gen[s2] >= 0;
+                cs.push_back({0, a, C.cmat[s2], g ? C.gen[s2] : -1, g ? C.acc[k] : 0, (C.mstr >> (4 * s2)) & 15});
+                k += g;
+            }
+        } else if (C.kind == MK_APPLY2 && C.ng <= 1) {
+            cs.push_back({0, a, C.mat, C.ng ? C.gen[0] : -1, C.ng ? C.acc[0] : 0, C.mstr & 15});
+        } else if (C.kind == MK_GRAD && C.ng == 1) {
+            cs.push_back({0, a, -1, C.gen[0], C.acc[0], C.mstr & 12});
+        }
+    } else if (C.kind == MK_APPLY) {
+        cs.push_back({0, a, C.mat, -1, 0, C.mstr & 3});
+    }
+    return cs;
+}
 void form_blocks(std::vector<MicroOp>& out, size_t b, int phase_kind) {
     const size_t L = out.size() - b;
     if (L < 2 || (phase_kind != PH_FWD && phase_kind != PH_REV)) return;
+    const bool rev = phase_kind == PH_REV;
     std::vector<MicroOp> seg(out.begin() + b, out.end());
     std::vector<std::vector<BStep>> steps(L);
+    std::vector<std::vector<int>> bits(L);
     std::vector<char> dead(L, 0), isblk(L, 0);
-    const bool rev = phase_kind == PH_REV;
-    auto anchor = [&](size_t q) -> bool {  // can seg[q] become or grow a block?
+    auto touches_k = [&](size_t k, int bit) {
+        if (isblk[k]) return std::find(bits[k].begin(), bits[k].end(), bit) != bits[k].end();
+        return touches(seg[k], bit);
+    };
+    // can seg[q] become (or grow into) a block holding bit a and `extra` more steps?
+    auto can_host = [&](size_t q, int a, size_t extra) -> bool {
+        if (dead[q]) return false;
+        if (isblk[q])  // blocks always hold two bits
+            return steps[q].size() + extra <= 4 && std::find(bits[q].begin(), bits[q].end(), a) != bits[q].end();
         const MicroOp& P = seg[q];
-        if (isblk[q]) return steps[q].size() < 4;
-        if (P.kb != 2) return false;
-        if (rev) return P.kind == MK_APPLY2 && P.ng <= 1;
-        return P.kind == MK_APPLY;
+        if (P.kb == 2) {
+            if (!(P.lpos[0] == a || P.lpos[1] == a)) return false;
+            if (rev ? !(P.kind == MK_APPLY2 && P.ng <= 1) : P.kind != MK_APPLY) return false;
+            return 1 + extra <= 4;
+        }
+        // a one-bit op hosts only an op on another bit (same-bit runs are chains already)
+        if (P.lpos[0] == a) return false;
+        const std::vector<BStep> ps = one_bit_steps(P, rev);
+        return !ps.empty() && ps.size() + extra <= 4;
     };
     auto make_block = [&](size_t q) {
         if (isblk[q]) return;
         const MicroOp& P = seg[q];
-        steps[q].push_back({BS_BOTH, P.mat, rev && P.ng ? P.gen[0] : -1, rev && P.ng ? P.acc[0] : 0, P.mstr & 15});
+        if (P.kb == 2) {
+            steps[q].push_back({BS_BOTH, -1, P.mat, rev && P.ng ? P.gen[0] : -1, rev && P.ng ? P.acc[0] : 0, P.mstr & 15});
+            bits[q] = {P.lpos[0], P.lpos[1]};
+        } else {
+            steps[q] = one_bit_steps(P, rev);
+            bits[q] = {P.lpos[0]};
+        }
         isblk[q] = 1;
     };
     for (size_t j = 0; j < L; ++j) {
         const MicroOp& C = seg[j];
-        std::vector<BStep> cs;
-        if (rev) {
-            if (C.kind == MK_CHAIN) {
-                int k = 0;
-                for (int s2 = 0; s2 < C.nchain; ++s2) {
-                    const bool g = C.gen[s2] >= 0;
-                    cs.push_back({0, C.cmat[s2], g ? C.gen[s2] : -1, g ? C.acc[k] : 0, (C.mstr >> (4 * s2)) & 15});
-                    k += g;
-                }
-            } else if (C.kind == MK_APPLY2 && C.kb == 1 && C.ng <= 1) {
-                cs.push_back({0, C.mat, C.ng ? C.gen[0] : -1, C.ng ? C.acc[0] : 0, C.mstr & 15});
-            } else {
-                continue;
-            }
-        } else {
-            if (C.kind != MK_APPLY || C.kb != 1) continue;
-            cs.push_back({0, C.mat, -1, 0, C.mstr & 3});
-        }
+        std::vector<BStep> cs = one_bit_steps(C, rev);
+        if (cs.empty() || (!rev && seg[j].kind != MK_APPLY)) continue;
         const int a = C.lpos[0];
         size_t q = L;
-        bool after = true;  // the block runs at q, C's steps after P's
+        bool before = false;  // C's steps run before the host's (forward, next op)
         if (!rev) {
             for (size_t k = j + 1; k < L; ++k)
-                if (!dead[k] && touches(seg[k], a)) {
+                if (!dead[k] && touches_k(k, a)) {
+                    if (seg[k].buf == C.buf && (isblk[k] || seg[k].kb == 2) && can_host(k, a, cs.size())) {
+                        q = k;
+                        before = true;
+                    }
+                    break;
+                }
+        }
+        if (q == L) {
+            // backwards: the last op touching a, else the nearest one-bit host
+            for (size_t k = j; k-- > 0;) {
+                if (dead[k]) continue;
+                if (touches_k(k, a)) {
+                    if (seg[k].buf == C.buf && can_host(k, a, cs.size())) q = k;
+                    break;
+                }
+                if (!isblk[k] && seg[k].kb == 1 && seg[k].buf == C.buf && can_host(k, a, cs.size())) {
                     q = k;
                     break;
                 }
-            if (q < L && anchor(q) && steps[q].size() + cs.size() + (isblk[q] ? 0 : 1) <= 4 && seg[q].buf == C.buf) {
-                after = false;
-            } else {
-                q = L;
             }
         }
-        if (q == L)
-            for (size_t k = j; k-- > 0;)
-                if (!dead[k] && touches(seg[k], a)) {
-                    q = k;
-                    break;
-                }
-        if (q == L || !anchor(q) || seg[q].buf != C.buf) continue;
-        if (steps[q].size() + cs.size() + (isblk[q] ? 0 : 1) > 4) continue;
+        if (q == L) continue;
         make_block(q);
-        const int tgt = seg[q].lpos[0] == a ? BS_LO : BS_HI;
-        for (BStep& st : cs) st.tgt = tgt;
-        if (after) steps[q].insert(steps[q].end(), cs.begin(), cs.end());
-        else steps[q].insert(steps[q].begin(), cs.begin(), cs.end());
+        if (std::find(bits[q].begin(), bits[q].end(), a) == bits[q].end()) bits[q].push_back(a);
+        if (before) steps[q].insert(steps[q].begin(), cs.begin(), cs.end());
+        else steps[q].insert(steps[q].end(), cs.begin(), cs.end());
         dead[j] = 1;
     }
     out.resize(b);
@@ -373,18 +408,20 @@ void form_blocks(std::vector<MicroOp>& out, size_t b, int phase_kind) {
             out.push_back(seg[k]);
             continue;
         }
+        const int lo = std::min(bits[k][0], bits[k][1]), hi = std::max(bits[k][0], bits[k][1]);
         MicroOp m = mop_init(MK_BLOCK);
         m.kb = 2;
         m.buf = seg[k].buf;
-        m.lpos[0] = seg[k].lpos[0];
-        m.lpos[1] = seg[k].lpos[1];
+        m.lpos[0] = lo;
+        m.lpos[1] = hi;
         m.nchain = int(steps[k].size());
         m.ng = 0;
         m.smask = 0;
         m.mstr = 0;
         for (int j = 0; j < m.nchain; ++j) {
             const BStep& st = steps[k][j];
-            m.smask |= st.tgt << (2 * j);
+            const int tgt = st.bit < 0 ? BS_BOTH : (st.bit == lo ? BS_LO : BS_HI);
+            m.smask |= tgt << (2 * j);
             m.mstr |= st.ms << (4 * j);
             m.cmat[j] = st.mat;
             m.gen[j] = st.gen;
@@ -1130,8 +1167,10 @@ extern "C" int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_
     const size_t nfwd = pb.phases.size();
     pb.nrows = nterms;  // rows 0..nterms-1: energies (DM energies come from the trace kernels)
     for (int t = 0; t < nterms; ++t) pb.row_slot.push_back(-1 - t);
-    if (!plain) VQC_TRY(vqc_plan_reverse(pb, nb, n, 1, nops, ops, tt.t_rev, tt.mlow));
-    else {
+    if (!plain) {
+        VQC_TRY(vqc_plan_reverse(pb, nb, n, 1, nops, ops, tt.t_rev, tt.mlow));
+        vqc_fuse_chains(pb, nfwd, pb.phases.size());
+    } else {
         for (int i = nops - 1; i >= 0; --i)
             for (int j = 0; j < ops[i].nparams; ++j)
                 if (ops[i].slot[j] >= 0) {

@@ -20,9 +20,11 @@
 //   MK_CHAIN   up to 4 one-bit MK_APPLY2 steps on the same bit, done in registers
 //              (one LDS round trip for e.g. the Rz, Ry, Rx of one qubit)
 //   MK_BLOCK   up to 4 steps on one pair of bits, done in registers: each step a
-//              4x4 on both bits or a 2x2 on either bit, on A alone (forward
-//              phases) or on A and B with an optional generator per step
-//              (reverse phases), e.g. a CNOT and the rotations of one of its qubits
+//              4x4 on both bits or a 2x2 on either bit (or no matrix), on A alone
+//              (forward phases) or on A and B with an optional generator per step
+//              (reverse phases), e.g. a CNOT and the rotations of one of its qubits,
+//              or the row and column uncompute of a density-matrix gate with its
+//              two gradient terms
 // Matrices are read from a device pool in which the planner has already stored
 // every variant a micro-op needs (U, U^dagger, conj(U), U^T, M + M^dagger), so
 // the inner loops are plain complex matrix-vector products.
@@ -144,14 +146,17 @@ __device__ __forceinline__ void wave_partial(double2* red, int buf, int q, doubl
     if ((threadIdx.x & 63) == 0) red[(buf * kRedSlots + q) * kTileWaves + (threadIdx.x >> 6)] = make_double2(re, im);
 }
 
-// fold pending partials into gacc (wave 0 only; callers order it with barriers)
+// fold pending partials into gacc (callers order it with barriers). One thread
+// folds them in turn: two partials of one micro-op may belong to the same row
+// (the row and column terms of a density-matrix gradient).
 __device__ __forceinline__ void acc_fold(AccState& st, const double2* red, double2* gacc) {
-    if (st.pend_n && threadIdx.x < uint32_t(st.pend_n)) {
-        const int q = threadIdx.x;
-        const int e = q == 0 ? st.e0 : (q == 1 ? st.e1 : (q == 2 ? st.e2 : st.e3));
-        double2 s = gacc[e];
-        for (int w = 0; w < kTileWaves; ++w) s = cadd(s, red[(st.pend_buf * kRedSlots + q) * kTileWaves + w]);
-        gacc[e] = s;
+    if (st.pend_n && threadIdx.x == 0) {
+        for (int q = 0; q < st.pend_n; ++q) {
+            const int e = q == 0 ? st.e0 : (q == 1 ? st.e1 : (q == 2 ? st.e2 : st.e3));
+            double2 s = gacc[e];
+            for (int w = 0; w < kTileWaves; ++w) s = cadd(s, red[(st.pend_buf * kRedSlots + q) * kTileWaves + w]);
+            gacc[e] = s;
+        }
     }
     st.pend_n = 0;
 }
@@ -401,7 +406,9 @@ __device__ __forceinline__ void mop_block(double2* __restrict__ sa, double2* __r
             const double2* __restrict__ M = pool + op.cmat[j];
             const int tgt = (op.smask >> (2 * j)) & 3;
             const int skj = ms_mat(op, j), gkj = ms_gen(op, j);
-            if (tgt == BS_BOTH) {
+            if (op.cmat[j] < 0) {
+                // a gradient-only step: the generator on the current state
+            } else if (tgt == BS_BOTH) {
                 if (skj == MS_CX_LO) {
                     dswap(a[1], a[3]);
                     if (two) dswap(b[1], b[3]);
