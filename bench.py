@@ -381,7 +381,13 @@ def main():
     msf = timer.stop_ms() / args.steps
     extra["fused_ms_per_step"] = round(msf, 3)
     extra["fused_effective_gate_GBps"] = round(bytes_per_gate * G / (msf * 1e-3) / 1e9, 1)
-    log(f"fused: {msf:.1f} ms/step, effective {extra['fused_effective_gate_GBps']} GB/s")
+    info = _lib.PlanInfo()
+    _lib.call("vqc_plan_info", ctypes.byref(info))
+    # physical HBM bandwidth of the fused path: each forward tile pass reads and writes the state once
+    extra["fused_tile_passes"] = info.phases_fwd
+    extra["fused_physical_GBps"] = round(bytes_per_gate * info.phases_fwd / (msf * 1e-3) / 1e9, 1)
+    log(f"fused: {msf:.1f} ms/step, effective {extra['fused_effective_gate_GBps']} GB/s, "
+        f"{info.phases_fwd} tile passes = {extra['fused_physical_GBps']} GB/s physical")
 
     if not args.skip_extra:
         Hp = random_pauli_operator(n, 200)

@@ -212,6 +212,16 @@ int vqc_apply_circuit(vqc_state* s, int nops, const vqc_op_t* ops, const double*
 
 #define VQC_FLAG_UNFUSED 1    /* one kernel per gate and per reduction (the plain path) */
 
+/* Shape of the most recent fused plan run by the calling thread (no reference
+ * counterpart): LDS tile passes by kind (each forward pass reads and writes the
+ * state once, each Hamiltonian / reverse pass two states) and micro-ops by kind
+ * (apply, apply2, apply2x, grad, ham, chain, block). */
+typedef struct vqc_plan_info {
+    int32_t phases_fwd, phases_ham, phases_rev;
+    int32_t mops[8];
+} vqc_plan_info_t;
+int vqc_plan_info(vqc_plan_info_t* out);
+
 /* Energy E = sum_keys <psi|M|psi> with psi = circuit * psi0 and the adjoint
  * gradient dRe(E)/dtheta for every active parameter. Replaces Zygote.gradient(
  * c -> real(expectation(H, c * psi0)), circuit) through @adjoint *(circuit, x)

@@ -841,7 +841,29 @@ static int run_forward(const PlanBuilder& pb, size_t ph_begin, size_t ph_end, co
     return VQC_OK;
 }
 
+// shape of the most recent plan uploaded by this thread (vqc_plan_info)
+static thread_local vqc_plan_info_t g_last_plan{};
+
+static void record_plan(const PlanBuilder& pb) {
+    vqc_plan_info_t& r = g_last_plan;
+    memset(&r, 0, sizeof r);
+    for (const PhaseDesc& p : pb.phases) {
+        if (p.kind == PH_FWD) ++r.phases_fwd;
+        else if (p.kind == PH_HAM) ++r.phases_ham;
+        else ++r.phases_rev;
+    }
+    for (const MicroOp& m : pb.mops)
+        if (m.kind >= 0 && m.kind < 8) ++r.mops[m.kind];
+}
+
+extern "C" int vqc_plan_info(vqc_plan_info_t* out) {
+    if (!out) return vqc_fail(VQC_EINVAL, "null plan info");
+    *out = g_last_plan;
+    return VQC_OK;
+}
+
 static int upload_program(const PlanBuilder& pb, hipStream_t st) {
+    record_plan(pb);
     const std::vector<double>& h = pb.xp->h;
     VQC_TRY(ensure(g_wpool, h.size() * 8 + 16));
     VQC_TRY(ensure(g_wmops, std::max<size_t>(pb.mops.size(), 1) * sizeof(MicroOp)));

@@ -89,6 +89,11 @@ class Tuning(ctypes.Structure):
     ]
 
 
+class PlanInfo(ctypes.Structure):
+    _fields_ = [("phases_fwd", ctypes.c_int32), ("phases_ham", ctypes.c_int32), ("phases_rev", ctypes.c_int32),
+                ("mops", ctypes.c_int32 * 8)]
+
+
 _P = ctypes.c_void_p
 _I = ctypes.c_int
 _D = ctypes.POINTER(ctypes.c_double)
@@ -133,6 +138,7 @@ SIGNATURES = {
     "vqc_partial_trace": [_P, _I, _I32, _P],
     "vqc_dm_sandwich": [_P, _P, _D],
     "vqc_apply_circuit": [_P, _I, ctypes.POINTER(Op), _D],
+    "vqc_plan_info": [ctypes.POINTER(PlanInfo)],
     "vqc_sv_energy_grad": [_P, _I, ctypes.POINTER(Op), _D, _I, ctypes.POINTER(Term), _D, _I, _D, _D, _P, _I],
     "vqc_dm_energy_grad": [_P, _I, ctypes.POINTER(Op), _D, _I, ctypes.POINTER(Term), _D, _I, _D, _D, _P, _I],
     "vqc_batch_create": [_I, ctypes.POINTER(Instance), ctypes.POINTER(Op), ctypes.c_int64, ctypes.POINTER(Term),
