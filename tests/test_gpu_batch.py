@@ -115,3 +115,18 @@ def test_structure_hints_in_batch():
     inst2 = [(structured_circuit(n, np.random.default_rng(n), value_rng=vr), H, n) for _, H, n in inst]
     b.update([c for c, _, _ in inst2])
     check(inst2, *b.run())
+
+
+def test_batch_beyond_memory_limit_runs_in_waves():
+    ns = [13, 15, 9, 14, 16, 12, 15, 13]
+    inst = make_instances(7, ns)
+    one = Batch(inst)
+    E1, g1 = one.run()
+    limit = 3 << 19  # 1.5 MiB: a 16-qubit instance (2 MiB) goes alone, the rest in small groups
+    b = Batch(inst, max_bytes=limit)
+    assert b.nwaves() > 1 and b.arena_bytes() <= max(limit, 32 << 16)
+    E2, g2 = b.run()
+    assert np.max(np.abs(E2 - E1)) < 1e-12
+    for x, y in zip(g1, g2):
+        assert x.shape == y.shape and (x.size == 0 or np.max(np.abs(x - y)) < 1e-12)
+    check(inst, E2, g2)

@@ -191,3 +191,17 @@ def test_krylov_rejects_non_hermitian_before_any_device_call():
     H = QubitsOperator([QubitsTerm({1: "+", 2: "-"}, coeff=1.0)])  # s+ s- alone is not Hermitian
     with pytest.raises(ValueError, match="hermitian"):
         krylov.ground_state(H, 2)
+
+
+def test_batch_wave_grouping():
+    from vqchip.batch import instance_bytes, waves
+    assert instance_bytes(12, t_small=12) == 0 and instance_bytes(13, t_small=12) == 32 << 13
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("libvqc_hip.so not built")
+    inst = [(None, None, n) for n in (13, 15, 9, 14, 16, 12, 15, 13)]
+    w = waves(inst, 3 << 19)
+    assert [x for g in w for x in g] == inst  # order kept, nothing lost
+    for g in w:
+        tot = sum(instance_bytes(n) for _, _, n in g)
+        assert tot <= 3 << 19 or len(g) == 1
+    assert [len(g) for g in w][0] >= 2

@@ -4,6 +4,15 @@ counterpart in the reference, which evaluates one circuit per call).
     b = Batch([(circuit_1, H_1, n_1), (circuit_2, H_2, n_2), ...])
     energies, grads = b.run()            # one persistent-kernel graph launch
     b.update([circuit_1', ...])          # same structure, new angles
+
+    b = Batch(instances, max_bytes=64 << 30)   # more state than that: waves
+
+The adjoint keeps no forward states (the reverse sweep uncomputes), so an
+instance above one tile holds exactly two states (psi and lambda, 32 * 2^n
+bytes) for the length of the launch. A batch whose states exceed max_bytes runs
+as consecutive waves that each fit; a wave is planned, allocated, run and freed
+in turn, so device memory stays within the largest wave (its planning is redone
+on every run).
 """
 import ctypes
 
@@ -14,7 +23,84 @@ from .circuit import Program
 from .operators import pack_terms
 
 
+def instance_bytes(n, t_small=None):
+    """Device bytes of one instance's states in a batch: psi and lambda above one
+    tile, none when the whole evaluation stays in LDS."""
+    if t_small is None:
+        from .autotune import get_tuning
+        t_small = get_tuning()["tile_small_bits"]
+    return 0 if n <= t_small else 32 << n
+
+
+def waves(instances, max_bytes):
+    """Consecutive groups of instances whose states fit max_bytes (an instance
+    larger than that forms a wave of its own)."""
+    out, cur, used = [], [], 0
+    for inst in instances:
+        b = instance_bytes(inst[2])
+        if cur and used + b > max_bytes:
+            out.append(cur)
+            cur, used = [], 0
+        cur.append(inst)
+        used += b
+    if cur:
+        out.append(cur)
+    return out
+
+
 class Batch:
+    """One persistent-kernel batch, or consecutive waves of them (max_bytes)."""
+
+    def __init__(self, instances, max_bytes=None):
+        instances = list(instances)
+        if max_bytes is not None and sum(instance_bytes(n) for _, _, n in instances) > max_bytes:
+            self._waves = waves(instances, max_bytes)
+            self._part = None
+        else:
+            self._waves = None
+            self._part = _Part(instances)
+        self.n_inst = len(instances)
+
+    def update(self, circuits):
+        if self._part is not None:
+            return self._part.update(circuits)
+        circuits = list(circuits)
+        k = 0
+        for w in self._waves:
+            for i, (_, H, n) in enumerate(w):
+                w[i] = (circuits[k], H, n)
+                k += 1
+
+    def run(self):
+        if self._part is not None:
+            return self._part.run()
+        E, grads = [], []
+        for w in self._waves:
+            part = _Part(w)
+            e, g = part.run()
+            part.close()
+            E.append(e)
+            grads += g
+        return np.concatenate(E), grads
+
+    def launch(self):
+        if self._part is None:
+            raise RuntimeError("a batch in waves runs with run()")
+        self._part.launch()
+
+    def wait(self):
+        return self._part.wait()
+
+    def arena_bytes(self):
+        if self._part is not None:
+            return self._part.arena_bytes()
+        return max(sum(instance_bytes(n) for _, _, n in w) for w in self._waves)
+
+    def nwaves(self):
+        return 1 if self._part is not None else len(self._waves)
+
+
+class _Part:
     def __init__(self, instances):
         self.n_inst = len(instances)
         if not self.n_inst:
@@ -89,11 +175,14 @@ class Batch:
         _lib.call("vqc_batch_arena_bytes", self._h, ctypes.byref(v))
         return v.value
 
-    def __del__(self):
+    def close(self):
         h = getattr(self, "_h", None)
         if h:
-            try:
-                _lib.call("vqc_batch_destroy", h)
-            except Exception:
-                pass
+            _lib.call("vqc_batch_destroy", h)
             self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
