@@ -119,6 +119,10 @@ int vqc_get_tuning(vqc_tuning_t* t);
  * src/statevector.jl:1-21 (StateVector{T}(n) = |0..0>), src/densitymatrix.jl:38-42. */
 typedef struct vqc_state vqc_state;
 
+/* nqubits 0..34. A 0-qubit state vector holds one amplitude: what measure /
+ * post_select leave after the last qubit (test/algs.jl:59-74). It supports the
+ * whole-state calls (upload, download, copy, scale, axpy, dot, reset); calls that
+ * act on qubits reject it. */
 int vqc_sv_create(int nqubits, vqc_state** out);             /* |0...0> */
 int vqc_dm_create(int nqubits, vqc_state** out);             /* |0...0><0...0| */
 /* Borrow caller-owned device memory of 2^nbits complex doubles (not freed). */

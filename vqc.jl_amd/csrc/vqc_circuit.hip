@@ -877,7 +877,7 @@ static int upload_program(const PlanBuilder& pb, hipStream_t st) {
 }
 
 extern "C" int vqc_apply_circuit(vqc_state* s, int nops, const vqc_op_t* ops, const double* pool) {
-    VQC_TRY(vqc_check_state(s, "vqc_apply_circuit"));
+    VQC_TRY(vqc_check_qubits(s, "vqc_apply_circuit"));
     int64_t plen = 0;
     VQC_TRY(check_program(s->nqubits, s->is_dm, nops, ops, &plen, 1 << 30));
     if (nops == 0) return VQC_OK;
@@ -1003,7 +1003,7 @@ static int max_small_rows(int nbits) {
 extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_t* ops, const double* pool, int nterms,
                                   const vqc_term_t* terms, const double* hpool, int nparams, double* energy2, double* grad,
                                   vqc_state* psi_out, int flags) {
-    VQC_TRY(vqc_check_state(psi0, "vqc_sv_energy_grad"));
+    VQC_TRY(vqc_check_qubits(psi0, "vqc_sv_energy_grad"));
     if (psi0->is_dm) return vqc_fail(VQC_EINVAL, "vqc_sv_energy_grad: state is a density matrix");
     if (nparams < 0 || (nparams && !grad)) return vqc_fail(VQC_EINVAL, "bad gradient output");
     if (psi_out && (psi_out->nbits != psi0->nbits || psi_out->is_dm)) return vqc_fail(VQC_EINVAL, "psi_out has the wrong shape");

@@ -141,6 +141,12 @@ int vqc_check_state(const vqc_state* s, const char* what) {
     return VQC_OK;
 }
 
+int vqc_check_qubits(const vqc_state* s, const char* what) {
+    VQC_TRY(vqc_check_state(s, what));
+    if (s->nbits < 1) return vqc_fail(VQC_EINVAL, "%s: the state has no qubits", what);
+    return VQC_OK;
+}
+
 // ---------------------------------------------------------------------------
 // elementwise kernels
 // ---------------------------------------------------------------------------
@@ -206,9 +212,11 @@ static unsigned stream_grid(uint64_t len) {
 // ---------------------------------------------------------------------------
 // state handles
 // ---------------------------------------------------------------------------
+// A state vector may have 0 qubits (one amplitude): what measure / post_select
+// leave after the last qubit is taken out (test/algs.jl:59-74).
 static int state_alloc(int nbits, int nq, int is_dm, vqc_state** out) {
     if (!out) return vqc_fail(VQC_EINVAL, "null output handle");
-    if (nbits < 1 || nbits > 34) return vqc_fail(VQC_EINVAL, "unsupported state size: %d index bits", nbits);
+    if (nbits < (is_dm ? 2 : 0) || nbits > 34) return vqc_fail(VQC_EINVAL, "unsupported state size: %d index bits", nbits);
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));
     vqc_state* s = new vqc_state;
@@ -234,7 +242,7 @@ extern "C" int vqc_dm_create(int nqubits, vqc_state** out) {
 
 extern "C" int vqc_state_wrap(int nbits, int is_dm, void* device_ptr, vqc_state** out) {
     if (!out || !device_ptr) return vqc_fail(VQC_EINVAL, "null pointer");
-    if (nbits < 1 || nbits > 34 || (is_dm && (nbits & 1))) return vqc_fail(VQC_EINVAL, "bad state size %d", nbits);
+    if (nbits < (is_dm ? 2 : 0) || nbits > 34 || (is_dm && (nbits & 1))) return vqc_fail(VQC_EINVAL, "bad state size %d", nbits);
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));
     vqc_state* s = new vqc_state;

@@ -131,7 +131,7 @@ int vqc_expec_sweep(const double2* ket, const double2* bra, int nbits, int k, co
 }
 
 extern "C" int vqc_sv_accum_dense(const vqc_state* in, vqc_state* out, int k, const int32_t* pos, const double* M) {
-    VQC_TRY(vqc_check_state(in, "vqc_sv_accum_dense"));
+    VQC_TRY(vqc_check_qubits(in, "vqc_sv_accum_dense"));
     VQC_TRY(vqc_check_state(out, "vqc_sv_accum_dense"));
     if (in->nbits != out->nbits) return vqc_fail(VQC_EINVAL, "accumulate between states of different size");
     if (in->d == out->d) return vqc_fail(VQC_EINVAL, "accumulate needs distinct input and output states");
@@ -146,7 +146,7 @@ extern "C" int vqc_sv_accum_dense(const vqc_state* in, vqc_state* out, int k, co
 
 extern "C" int vqc_sv_expec_dense(const vqc_state* ket, const vqc_state* bra, int k, const int32_t* pos, const double* M,
                                   double* out2) {
-    VQC_TRY(vqc_check_state(ket, "vqc_sv_expec_dense"));
+    VQC_TRY(vqc_check_qubits(ket, "vqc_sv_expec_dense"));
     if (bra && bra->nbits != ket->nbits) return vqc_fail(VQC_EINVAL, "bra and ket differ in size");
     VQC_TRY(vqc_check_positions(ket->nbits, k, pos, 5));
     if (!M || !out2) return vqc_fail(VQC_EINVAL, "null matrix or output");
@@ -324,7 +324,7 @@ static int check_paulis(int nbits, int nterms, const vqc_pauli_t* terms) {
 
 extern "C" int vqc_sv_expec_pauli(const vqc_state* ket, const vqc_state* bra, int nterms, const vqc_pauli_t* terms,
                                   double* out, double* sum2) {
-    VQC_TRY(vqc_check_state(ket, "vqc_sv_expec_pauli"));
+    VQC_TRY(vqc_check_qubits(ket, "vqc_sv_expec_pauli"));
     if (bra && bra->nbits != ket->nbits) return vqc_fail(VQC_EINVAL, "bra and ket differ in size");
     VQC_TRY(check_paulis(ket->nbits, nterms, terms));
     VqcCtx* c;
@@ -380,7 +380,7 @@ void vqc_zero(double2* v, uint64_t len, hipStream_t st) {
 }
 
 extern "C" int vqc_sv_apply_pauli_sum(const vqc_state* in, vqc_state* out, int nterms, const vqc_pauli_t* terms) {
-    VQC_TRY(vqc_check_state(in, "vqc_sv_apply_pauli_sum"));
+    VQC_TRY(vqc_check_qubits(in, "vqc_sv_apply_pauli_sum"));
     VQC_TRY(vqc_check_state(out, "vqc_sv_apply_pauli_sum"));
     if (in->nbits != out->nbits) return vqc_fail(VQC_EINVAL, "states differ in size");
     if (in->d == out->d) return vqc_fail(VQC_EINVAL, "H*psi needs distinct input and output states");
@@ -412,7 +412,7 @@ static size_t g_hpool_cap = 0;
 
 extern "C" int vqc_sv_apply_terms(const vqc_state* in, vqc_state* out, int nterms, const vqc_term_t* terms, const double* pool,
                                   double* energy2) {
-    VQC_TRY(vqc_check_state(in, "vqc_sv_apply_terms"));
+    VQC_TRY(vqc_check_qubits(in, "vqc_sv_apply_terms"));
     VQC_TRY(vqc_check_state(out, "vqc_sv_apply_terms"));
     if (in->nbits != out->nbits || in->d == out->d) return vqc_fail(VQC_EINVAL, "need distinct states of equal size");
     if (nterms < 0 || (nterms && (!terms || !pool))) return vqc_fail(VQC_EINVAL, "bad term array");

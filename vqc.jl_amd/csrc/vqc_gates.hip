@@ -160,7 +160,7 @@ int vqc_dense_sweep(double2* v, int nbits, int k, const int32_t* pos, const doub
 }
 
 extern "C" int vqc_sv_apply_dense(vqc_state* s, int k, const int32_t* pos, const double* U) {
-    VQC_TRY(vqc_check_state(s, "vqc_sv_apply_dense"));
+    VQC_TRY(vqc_check_qubits(s, "vqc_sv_apply_dense"));
     VQC_TRY(vqc_check_positions(s->nbits, k, pos, 5));
     if (!U) return vqc_fail(VQC_EINVAL, "null gate matrix");
     VqcCtx* c;
@@ -229,7 +229,7 @@ static int launch_named(double2* v, int nbits, const int32_t* pos, double2 ph, h
 }
 
 extern "C" int vqc_sv_apply_named(vqc_state* s, int kind, const int32_t* pos, double phi) {
-    VQC_TRY(vqc_check_state(s, "vqc_sv_apply_named"));
+    VQC_TRY(vqc_check_qubits(s, "vqc_sv_apply_named"));
     if (!pos) return vqc_fail(VQC_EINVAL, "null position array");
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));

@@ -146,7 +146,7 @@ extern "C" int vqc_sv_select(const vqc_state* s, int pos, int value, vqc_state* 
     VQC_TRY(vqc_check_state(out, "vqc_sv_select"));
     if (s->is_dm || out->is_dm) return vqc_fail(VQC_EINVAL, "vqc_sv_select: state vectors only");
     if (value != 0 && value != 1) return vqc_fail(VQC_EINVAL, "vqc_sv_select: value must be 0 or 1");
-    if (s->nqubits < 2 || out->nqubits != s->nqubits - 1)
+    if (s->nqubits < 1 || out->nqubits != s->nqubits - 1)
         return vqc_fail(VQC_EINVAL, "vqc_sv_select: out must have one qubit less than the input");
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));

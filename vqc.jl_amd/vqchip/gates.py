@@ -31,6 +31,7 @@ SIGMA_PLUS = np.array([[0, 1], [0, 0]], dtype=np.complex128)
 SIGMA_MINUS = np.array([[0, 0], [1, 0]], dtype=np.complex128)
 ZERO = np.array([1, 0], dtype=np.complex128)
 ONE = np.array([0, 1], dtype=np.complex128)
+SWAP = np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]], dtype=np.complex128)
 
 
 def kron_le(*ops):
@@ -61,6 +62,23 @@ def reorder(mat, order):
                 g |= 1 << where[b]
         perm[j] = g
     return np.ascontiguousarray(mat[np.ix_(perm, perm)], dtype=np.complex128)
+
+
+def R(k):
+    """diag(1, exp(2 pi i / 2^k)): the phase step of the QFT (src/utility/qft.jl:11).
+    Its sign is fixed by the phase-estimation known answers (test/algs.jl:9-24),
+    whose inverse QFT must read back exp(+2 pi i phi)."""
+    return np.diag([1, np.exp(2j * np.pi / (1 << k))]).astype(np.complex128)
+
+
+def CONTROL(U):
+    """Controlled-U as a matrix over (control, targets...): the control is the
+    first position, as for CNOT (specific/czgate.jl:60-64), so CONTROL(X) is
+    CNOT's matrix. The phase-estimation circuit of test/algs.jl:17-20 puts its
+    controls first in the same way."""
+    U = np.asarray(U, dtype=np.complex128)
+    eye = np.eye(U.shape[0], dtype=np.complex128)
+    return kron_le(P0, eye) + kron_le(P1, U)
 
 
 class Gate:
@@ -236,7 +254,7 @@ class SWAPGate(Gate):
     name, named_kind = "SWAP", _lib.GATE_SWAP
 
     def __init__(self, a, b):
-        super().__init__((a, b), np.array([[1, 0, 0, 0], [0, 0, 1, 0], [0, 1, 0, 0], [0, 0, 0, 1]]))
+        super().__init__((a, b), SWAP)
 
 
 class iSWAPGate(Gate):

@@ -1,10 +1,11 @@
 """VQC's own workload builders (src/utility/spin_hamiltonians.jl,
-src/utility/variationalcircuit.jl), so the device path is driven by the same
-Hamiltonians and ansatz circuits a VQC.jl user builds. Positions are 1-based."""
+src/utility/variationalcircuit.jl, src/utility/qft.jl), so the device path is
+driven by the same Hamiltonians and circuits a VQC.jl user builds. Positions
+are 1-based."""
 import numpy as np
 
 from .circuit import QCircuit
-from .gates import CNOTGate, RyGate, RzGate
+from .gates import CONTROL, SWAP, CNOTGate, HGate, QuantumGate, R, RyGate, RzGate
 from .operators import QubitsOperator, QubitsTerm
 
 
@@ -117,3 +118,26 @@ def real_variational_circuit_1d(L, depth, thetas=None, rng=None):
         for j in range(1, L + 1):
             c.append(RyGate(j, next(it)))
     return c
+
+
+def _qft_block(L):
+    """H on qubit 1, then CONTROL(R(m)) * SWAP on (m, m-1) for m = 2..L: qubit 1's
+    content is rotated by every later qubit while it moves up to qubit L
+    (_nnqft_one_block, qft.jl:3-14)."""
+    if L <= 0:
+        raise ValueError("input size can not be 0")
+    r = QCircuit([HGate(1)])
+    for m in range(2, L + 1):
+        r.append(QuantumGate((m, m - 1), CONTROL(R(m)) @ SWAP))
+    return r
+
+
+def QFT(L):
+    """The quantum Fourier transform on qubits 1..L from nearest-neighbour gates
+    only: blocks L, L-1, ..., 1 (qft.jl:19-25). With the bit reversal its swaps
+    perform, it maps basis state j to sum_k exp(2 pi i j k / 2^L) |k> / 2^(L/2),
+    where j and k are read with qubit 1 as the MOST significant bit."""
+    r = QCircuit()
+    for i in range(L, 0, -1):
+        r.extend(_qft_block(i))
+    return r
