@@ -115,6 +115,29 @@ __device__ __forceinline__ double wave_sum(double v) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     return v;
 }
+// Lane permutes of a double through DPP (in the VALU, no LDS traffic): CTRL is
+// a DPP control word (quad_perm 0x00-0xff, row_mirror 0x140, row_half_mirror
+// 0x141). Every source lane must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int lane) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), lane),
+                            __builtin_amdgcn_readlane(__double2loint(v), lane));
+}
+// Sum over the 64 lanes of a wave with every lane active (call it outside
+// divergent code): pairs, quads, half rows and rows of 16 by DPP, then the four
+// row sums read as scalars. Same result in every lane, in a fixed order.
+__device__ __forceinline__ double wave_sum_full(double v) {
+    v += dpp_f64<0xB1>(v);   // quad_perm [1,0,3,2]
+    v += dpp_f64<0x4E>(v);   // quad_perm [2,3,0,1]
+    v += dpp_f64<0x141>(v);  // row_half_mirror
+    v += dpp_f64<0x140>(v);  // row_mirror
+    return (readlane_f64(v, 0) + readlane_f64(v, 16)) + (readlane_f64(v, 32) + readlane_f64(v, 48));
+}
 // Sum over a block of blockDim.x threads (multiple of 64); every thread gets
 // the result. `red` holds at least blockDim.x/64 doubles of LDS.
 __device__ __forceinline__ double block_sum(double v, double* red) {
@@ -137,6 +160,8 @@ inline unsigned grid_for(uint64_t work, unsigned per_block) {
 }
 int vqc_check_positions(int nbits, int k, const int32_t* pos, int kmax);
 int vqc_check_state(const vqc_state* s, const char* what);
+// vqc_check_state, and at least one qubit (a 0-qubit state has one amplitude)
+int vqc_check_qubits(const vqc_state* s, const char* what);
 
 // Reduce `nparts` per-block complex partials for each of `nterms` terms
 // (layout [term][part]) into `out` (device, nterms complex), in a fixed order.

@@ -140,10 +140,16 @@ __device__ __forceinline__ void acc_init(AccState& st) {
     st.e0 = st.e1 = st.e2 = st.e3 = 0;
 }
 
+// (every lane of the wave must be active: called after the group loops)
 __device__ __forceinline__ void wave_partial(double2* red, int buf, int q, double re, double im) {
-    re = wave_sum(re);
-    im = wave_sum(im);
+    re = wave_sum_full(re);
+    im = wave_sum_full(im);
     if ((threadIdx.x & 63) == 0) red[(buf * kRedSlots + q) * kTileWaves + (threadIdx.x >> 6)] = make_double2(re, im);
+}
+// a real partial (gradient terms)
+__device__ __forceinline__ void wave_partial_re(double2* red, int buf, int q, double re) {
+    re = wave_sum_full(re);
+    if ((threadIdx.x & 63) == 0) red[(buf * kRedSlots + q) * kTileWaves + (threadIdx.x >> 6)] = make_double2(re, 0.0);
 }
 
 // fold pending partials into gacc (callers order it with barriers). One thread
@@ -298,9 +304,9 @@ __device__ __forceinline__ void mop_pair(double2* __restrict__ sa, double2* __re
             else gs2 += acc;
         }
     }
-    if (ngen > 0) wave_partial(red, buf, 0, gs0, 0.0);
-    if (ngen > 1) wave_partial(red, buf, 1, gs1, 0.0);
-    if (ngen > 2) wave_partial(red, buf, 2, gs2, 0.0);
+    if (ngen > 0) wave_partial_re(red, buf, 0, gs0);
+    if (ngen > 1) wave_partial_re(red, buf, 1, gs1);
+    if (ngen > 2) wave_partial_re(red, buf, 2, gs2);
 }
 
 // MK_CHAIN: for each step j, (A, B) <- U_j^dagger (A, B) on one bit and, when the
@@ -351,10 +357,10 @@ __device__ __forceinline__ void mop_chain(double2* __restrict__ sa, double2* __r
         sb[base + o1] = b1;
     }
     int q = 0;
-    if (nch > 0 && op.gen[0] >= 0) wave_partial(red, buf, q++, g0, 0.0);
-    if (nch > 1 && op.gen[1] >= 0) wave_partial(red, buf, q++, g1, 0.0);
-    if (nch > 2 && op.gen[2] >= 0) wave_partial(red, buf, q++, g2, 0.0);
-    if (nch > 3 && op.gen[3] >= 0) wave_partial(red, buf, q++, g3, 0.0);
+    if (nch > 0 && op.gen[0] >= 0) wave_partial_re(red, buf, q++, g0);
+    if (nch > 1 && op.gen[1] >= 0) wave_partial_re(red, buf, q++, g1);
+    if (nch > 2 && op.gen[2] >= 0) wave_partial_re(red, buf, q++, g2);
+    if (nch > 3 && op.gen[3] >= 0) wave_partial_re(red, buf, q++, g3);
 }
 
 // MK_BLOCK: the amplitudes of one pair of bits (lo, hi) in registers, index
@@ -491,10 +497,10 @@ __device__ __forceinline__ void mop_block(double2* __restrict__ sa, double2* __r
     }
     if (two) {
         int q = 0;
-        if (nst > 0 && op.gen[0] >= 0) wave_partial(red, buf, q++, g0, 0.0);
-        if (nst > 1 && op.gen[1] >= 0) wave_partial(red, buf, q++, g1, 0.0);
-        if (nst > 2 && op.gen[2] >= 0) wave_partial(red, buf, q++, g2, 0.0);
-        if (nst > 3 && op.gen[3] >= 0) wave_partial(red, buf, q++, g3, 0.0);
+        if (nst > 0 && op.gen[0] >= 0) wave_partial_re(red, buf, q++, g0);
+        if (nst > 1 && op.gen[1] >= 0) wave_partial_re(red, buf, q++, g1);
+        if (nst > 2 && op.gen[2] >= 0) wave_partial_re(red, buf, q++, g2);
+        if (nst > 3 && op.gen[3] >= 0) wave_partial_re(red, buf, q++, g3);
     }
 }
 
