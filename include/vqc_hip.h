@@ -66,7 +66,10 @@ typedef struct vqc_op {
     int32_t slot[3];     /* gradient index of each parameter (-1: inactive) */
     int32_t reserved;    /* structure hints, optional (batches use them; other paths read the values):
                             bit 0 U diagonal, bit 1 generators diagonal, bit 2 / 3 U is a CNOT
-                            with control on pos[0] / pos[1]; 0 when unknown */
+                            with control on pos[0] / pos[1], bit 4 / 5 U real / X-like, bit 6 / 7
+                            generators real / X-like (X-like: real diagonal, imaginary
+                            off-diagonal, as Rx); 0 when unknown. A wrong hint gives wrong
+                            results. */
     int64_t mat;         /* U (2^nq square) or superoperator S (4^nq square) */
     int64_t aux;         /* channels: S^-1 for the reverse sweep; unitaries: -1 */
     int64_t gen[3];      /* G_j = U^dagger dU/dtheta_j (2^nq square), one per parameter */

@@ -112,12 +112,22 @@ static MicroOp mop_init(int kind) {
 // Structure (MS_*) of a D x D pool matrix: from its values when the pool is on
 // the host, else from the op's hint bits (vqc_op_t::reserved): bit 0 U
 // diagonal, bit 1 generators diagonal, bit 2 / 3 U is a CNOT whose control is
-// the lower / higher position. Values decide whenever they are known, so a
-// hint can only matter for batches, whose matrices arrive at launch.
+// the lower / higher position, bit 4 / 5 U real / X-like, bit 6 / 7 generators
+// real / X-like (X-like: real diagonal, imaginary off-diagonal). Values decide
+// whenever they are known, so a hint can only matter for batches, whose
+// matrices arrive at launch. Every class is closed under the conjugate,
+// transpose and adjoint variants the planner derives.
 static int host_mstruct(const double* M, int D) {
     auto z = [&](int r, int c) { return M[2 * (r + c * D)] == 0.0 && M[2 * (r + c * D) + 1] == 0.0; };
     auto one = [&](int r, int c) { return M[2 * (r + c * D)] == 1.0 && M[2 * (r + c * D) + 1] == 0.0; };
-    if (D == 2) return z(1, 0) && z(0, 1) ? MS_DIAG : MS_DENSE;
+    auto re0 = [&](int r, int c) { return M[2 * (r + c * D)] == 0.0; };
+    auto im0 = [&](int r, int c) { return M[2 * (r + c * D) + 1] == 0.0; };
+    if (D == 2) {
+        if (z(1, 0) && z(0, 1)) return MS_DIAG;
+        if (im0(0, 0) && im0(1, 0) && im0(0, 1) && im0(1, 1)) return MS_REAL;
+        if (im0(0, 0) && im0(1, 1) && re0(1, 0) && re0(0, 1)) return MS_XLIKE;
+        return MS_DENSE;
+    }
     if (D != 4) return MS_DENSE;
     bool lo = true, hi = true;
     for (int c = 0; c < 4; ++c)
@@ -136,11 +146,19 @@ static int mstruct_of(const XPool& xp, int64_t off, int D, int hint_ms) {
 }
 static int hint_mat(const vqc_op_t& o) {
     if (o.nq == 1 && (o.reserved & 1)) return MS_DIAG;
+    if (o.nq == 1 && (o.reserved & 16)) return MS_REAL;
+    if (o.nq == 1 && (o.reserved & 32)) return MS_XLIKE;
     if (o.nq == 2 && (o.reserved & 4)) return MS_CX_LO;
     if (o.nq == 2 && (o.reserved & 8)) return MS_CX_HI;
     return MS_DENSE;
 }
-static int hint_gen(const vqc_op_t& o) { return o.nq == 1 && (o.reserved & 2) ? MS_DIAG : MS_DENSE; }
+static int hint_gen(const vqc_op_t& o) {
+    if (o.nq != 1) return MS_DENSE;
+    if (o.reserved & 2) return MS_DIAG;
+    if (o.reserved & 64) return MS_REAL;
+    if (o.reserved & 128) return MS_XLIKE;
+    return MS_DENSE;
+}
 
 // Group items into phases; emit(phase, item) appends micro-ops.
 //

@@ -35,7 +35,10 @@
 
 enum { MK_APPLY = 0, MK_APPLY2 = 1, MK_APPLY2X = 2, MK_GRAD = 3, MK_HAM = 4, MK_CHAIN = 5, MK_BLOCK = 6 };
 enum { BS_BOTH = 0, BS_LO = 1, BS_HI = 2 };  // MK_BLOCK step targets
-enum { MS_DENSE = 0, MS_DIAG = 1, MS_CX_LO = 2, MS_CX_HI = 3 };  // matrix structure
+// matrix structure; codes 2 and 3 name CNOT patterns for 4x4 matrices and
+// real / "X-like" (real diagonal, imaginary off-diagonal: Rx and its generator)
+// forms for 2x2 ones
+enum { MS_DENSE = 0, MS_DIAG = 1, MS_CX_LO = 2, MS_CX_HI = 3, MS_REAL = 2, MS_XLIKE = 3 };
 enum { MF_CONJ = 1, MF_TRANS = 2 };
 enum { PH_FWD = 0, PH_REV = 1, PH_HAM = 2 };
 // kind sets a kernel is built for (bit k = micro-op kind k)
@@ -194,29 +197,86 @@ __device__ __forceinline__ void dswap(double2& x, double2& y) {
     y = t;
 }
 
+// 2x2 matrices by structure. The four entries are read once per step and the
+// structure is dispatched once per step into straight-line code over every pair
+// the step updates. A real matrix or an X-like one takes half the multiplies of
+// a complex one; entries that the structure says are zero are exactly zero, so
+// results equal the dense ones. Pairs are passed and returned by value: every
+// amplitude stays in a register whatever branch the target bit takes.
+struct Mat2 {
+    double2 m00, m10, m01, m11;
+};
+struct Pair2 {
+    double2 x0, x1;
+};
+__device__ __forceinline__ Mat2 load_mat2(const double2* __restrict__ M) { return Mat2{M[0], M[1], M[2], M[3]}; }
+// M (x0, x1)
+template <int MS>
+__device__ __forceinline__ Pair2 m2(const Mat2& m, double2 x0, double2 x1) {
+    if constexpr (MS == MS_DIAG) {
+        return Pair2{cmul(m.m00, x0), cmul(m.m11, x1)};
+    } else if constexpr (MS == MS_REAL) {
+        return Pair2{make_double2(fma(m.m01.x, x1.x, m.m00.x * x0.x), fma(m.m01.x, x1.y, m.m00.x * x0.y)),
+                     make_double2(fma(m.m11.x, x1.x, m.m10.x * x0.x), fma(m.m11.x, x1.y, m.m10.x * x0.y))};
+    } else if constexpr (MS == MS_XLIKE) {
+        // M = [[d0, i u], [i l, d1]] with d0, d1, u, l real
+        const double d0 = m.m00.x, l = m.m10.y, u = m.m01.y, d1 = m.m11.x;
+        return Pair2{make_double2(fma(-u, x1.y, d0 * x0.x), fma(u, x1.x, d0 * x0.y)),
+                     make_double2(fma(-l, x0.y, d1 * x1.x), fma(l, x0.x, d1 * x1.y))};
+    } else {
+        return Pair2{cfma(m.m01, x1, cmul(m.m00, x0)), cfma(m.m11, x1, cmul(m.m10, x0))};
+    }
+}
+// Re (conj(b0) s0 + conj(b1) s1) with s = M a
+template <int MS>
+__device__ __forceinline__ double gdot2(const Mat2& m, double2 a0, double2 a1, double2 b0, double2 b1) {
+    const Pair2 s = m2<MS>(m, a0, a1);
+    return fma(b0.x, s.x0.x, fma(b0.y, s.x0.y, fma(b1.x, s.x1.x, b1.y * s.x1.y)));
+}
+// in-place update of one pair
+template <int MS>
+__device__ __forceinline__ void m2_ip(const Mat2& m, double2& x0, double2& x1) {
+    const Pair2 r = m2<MS>(m, x0, x1);
+    x0 = r.x0;
+    x1 = r.x1;
+}
+
+// MK_APPLY on one bit with a 2x2 of structure MS
+template <int MS>
+__device__ __forceinline__ void apply1(double2* __restrict__ s, uint32_t ng, const MicroOp& op, const Mat2 m) {
+    const uint32_t o1 = 1u << op.lpos[0];
+    for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
+        const uint32_t base = ins32<1>(g, op.lpos);
+        const Pair2 r = m2<MS>(m, s[base], s[base + o1]);
+        s[base] = r.x0;
+        s[base + o1] = r.x1;
+    }
+}
+
 template <int KB>
 __device__ __forceinline__ void mop_apply(double2* __restrict__ s, int t, const MicroOp& op, const double2* __restrict__ pool) {
     constexpr int D = 1 << KB;
     constexpr int RU = KB <= 3 ? D : 1;
     const double2* __restrict__ M = pool + op.mat;
     const uint32_t ng = 1u << (t - KB);
-    if constexpr (KB == 1 || KB == 2) {
+    if constexpr (KB == 1) {
+        // forward one-bit ops are mostly fused products (dense); diagonal ones
+        // (lone Rz / phases) take two products per pair
+        if (ms_mat(op, 0) == MS_DIAG) {
+            apply1<MS_DIAG>(s, ng, op, load_mat2(M));
+            return;
+        }
+    }
+    if constexpr (KB == 2) {
         const int ms = ms_mat(op, 0);
-        if (ms != MS_DENSE) {
-            const double2 d0 = M[0], d1 = M[3];
+        if (ms == MS_CX_LO || ms == MS_CX_HI) {
             for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
                 const uint32_t base = ins32<KB>(g, op.lpos);
-                if (KB == 1) {
-                    const uint32_t o1 = 1u << op.lpos[0];
-                    s[base] = cmul(d0, s[base]);
-                    s[base + o1] = cmul(d1, s[base + o1]);
-                } else {
-                    const uint32_t o1 = off32<KB>(1, op.lpos), o2 = off32<KB>(2, op.lpos), o3 = off32<KB>(3, op.lpos);
-                    const uint32_t x = base + (ms == MS_CX_LO ? o1 : o2), y = base + o3;
-                    const double2 vx = s[x], vy = s[y];
-                    s[x] = vy;
-                    s[y] = vx;
-                }
+                const uint32_t o1 = off32<KB>(1, op.lpos), o2 = off32<KB>(2, op.lpos), o3 = off32<KB>(3, op.lpos);
+                const uint32_t x = base + (ms == MS_CX_LO ? o1 : o2), y = base + o3;
+                const double2 vx = s[x], vy = s[y];
+                s[x] = vy;
+                s[y] = vx;
             }
             return;
         }
@@ -324,27 +384,28 @@ __device__ __forceinline__ void mop_chain(double2* __restrict__ sa, double2* __r
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             if (j >= nch) break;
-            const double2* __restrict__ M = pool + op.cmat[j];
-            if (ms_mat(op, j) == MS_DIAG) {
-                const double2 m00 = M[0], m11 = M[3];
-                a0 = cmul(m00, a0);
-                a1 = cmul(m11, a1);
-                b0 = cmul(m00, b0);
-                b1 = cmul(m11, b1);
+            const Mat2 mj = load_mat2(pool + op.cmat[j]);
+            const int ms = ms_mat(op, j);
+            if (ms == MS_DIAG) {
+                m2_ip<MS_DIAG>(mj, a0, a1);
+                m2_ip<MS_DIAG>(mj, b0, b1);
+            } else if (ms == MS_REAL) {
+                m2_ip<MS_REAL>(mj, a0, a1);
+                m2_ip<MS_REAL>(mj, b0, b1);
+            } else if (ms == MS_XLIKE) {
+                m2_ip<MS_XLIKE>(mj, a0, a1);
+                m2_ip<MS_XLIKE>(mj, b0, b1);
             } else {
-                const double2 m00 = M[0], m10 = M[1], m01 = M[2], m11 = M[3];
-                double2 x0 = cfma(m01, a1, cmul(m00, a0)), x1 = cfma(m11, a1, cmul(m10, a0));
-                double2 y0 = cfma(m01, b1, cmul(m00, b0)), y1 = cfma(m11, b1, cmul(m10, b0));
-                a0 = x0;
-                a1 = x1;
-                b0 = y0;
-                b1 = y1;
+                m2_ip<MS_DENSE>(mj, a0, a1);
+                m2_ip<MS_DENSE>(mj, b0, b1);
             }
             if (op.gen[j] >= 0) {
-                const double2* __restrict__ G = pool + op.gen[j];
-                const double2 s0 = ms_gen(op, j) == MS_DIAG ? cmul(G[0], a0) : cfma(G[2], a1, cmul(G[0], a0));
-                const double2 s1 = ms_gen(op, j) == MS_DIAG ? cmul(G[3], a1) : cfma(G[3], a1, cmul(G[1], a0));
-                const double v = fma(b0.x, s0.x, fma(b0.y, s0.y, fma(b1.x, s1.x, b1.y * s1.y)));
+                const Mat2 gj = load_mat2(pool + op.gen[j]);
+                const int gs = ms_gen(op, j);
+                const double v = gs == MS_DIAG    ? gdot2<MS_DIAG>(gj, a0, a1, b0, b1)
+                                 : gs == MS_REAL  ? gdot2<MS_REAL>(gj, a0, a1, b0, b1)
+                                 : gs == MS_XLIKE ? gdot2<MS_XLIKE>(gj, a0, a1, b0, b1)
+                                                  : gdot2<MS_DENSE>(gj, a0, a1, b0, b1);
                 if (j == 0) g0 += v;
                 else if (j == 1) g1 += v;
                 else if (j == 2) g2 += v;
@@ -363,143 +424,137 @@ __device__ __forceinline__ void mop_chain(double2* __restrict__ sa, double2* __r
     if (nch > 3 && op.gen[3] >= 0) wave_partial_re(red, buf, q++, g3);
 }
 
-// MK_BLOCK: the amplitudes of one pair of bits (lo, hi) in registers, index
-// j = bit_lo + 2 bit_hi; each step updates them in place, and a reverse step
-// with a generator adds Re sum conj(b) (G a) over the pairs it touched.
-__device__ __forceinline__ void blk_2x2(const double2* __restrict__ M, double2& x0, double2& x1) {
-    const double2 m00 = M[0], m10 = M[1], m01 = M[2], m11 = M[3];
-    const double2 y0 = cfma(m01, x1, cmul(m00, x0)), y1 = cfma(m11, x1, cmul(m10, x0));
-    x0 = y0;
-    x1 = y1;
-}
-__device__ __forceinline__ void blk_4x4(const double2* __restrict__ M, double2* x) {
+// MK_BLOCK: the amplitudes of one pair of bits (lo, hi) in registers, x0..x3
+// with index bit_lo + 2 bit_hi; each step updates them in place, and a reverse
+// step with a generator adds Re sum conj(b) (G a) over the pairs it touched.
+__device__ __forceinline__ void blk_4x4(const double2* __restrict__ M, double2& x0, double2& x1, double2& x2, double2& x3) {
     double2 y[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        double2 acc = make_double2(0.0, 0.0);
+    for (int r = 0; r < 4; ++r) y[r] = cfma(M[r + 12], x3, cfma(M[r + 8], x2, cfma(M[r + 4], x1, cmul(M[r], x0))));
+    x0 = y[0];
+    x1 = y[1];
+    x2 = y[2];
+    x3 = y[3];
+}
+__device__ __forceinline__ double blk_grad4(const double2* __restrict__ G, double2 a0, double2 a1, double2 a2, double2 a3,
+                                            double2 b0, double2 b1, double2 b2, double2 b3) {
+    double2 s[4];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) acc = cfma(M[r + 4 * c], x[c], acc);
-        y[r] = acc;
+    for (int r = 0; r < 4; ++r) s[r] = cfma(G[r + 12], a3, cfma(G[r + 8], a2, cfma(G[r + 4], a1, cmul(G[r], a0))));
+    double v = fma(b0.x, s[0].x, b0.y * s[0].y);
+    v = fma(b1.x, s[1].x, fma(b1.y, s[1].y, v));
+    v = fma(b2.x, s[2].x, fma(b2.y, s[2].y, v));
+    return fma(b3.x, s[3].x, fma(b3.y, s[3].y, v));
+}
+// A 2x2 step of structure MS on the first bit of the register quad: pairs
+// (x0, x1) and (x2, x3) of A and, when `two`, of B.
+template <int MS>
+__device__ __forceinline__ void blk_step2(const Mat2& m, bool two, double2& a0, double2& a1, double2& a2, double2& a3, double2& b0,
+                                          double2& b1, double2& b2, double2& b3) {
+    m2_ip<MS>(m, a0, a1);
+    m2_ip<MS>(m, a2, a3);
+    if (two) {
+        m2_ip<MS>(m, b0, b1);
+        m2_ip<MS>(m, b2, b3);
     }
-#pragma unroll
-    for (int r = 0; r < 4; ++r) x[r] = y[r];
 }
-__device__ __forceinline__ double blk_grad2(const double2* __restrict__ G, double2 a0, double2 a1, double2 b0, double2 b1) {
-    const double2 s0 = cfma(G[2], a1, cmul(G[0], a0)), s1 = cfma(G[3], a1, cmul(G[1], a0));
-    return fma(b0.x, s0.x, fma(b0.y, s0.y, fma(b1.x, s1.x, b1.y * s1.y)));
+template <int MS>
+__device__ __forceinline__ double blk_grad2(const Mat2& m, double2 a0, double2 a1, double2 a2, double2 a3, double2 b0, double2 b1,
+                                            double2 b2, double2 b3) {
+    return gdot2<MS>(m, a0, a1, b0, b1) + gdot2<MS>(m, a2, a3, b2, b3);
 }
+// The quad is held as x0..x3 = (00, 01, 10, 11) over (first bit, second bit),
+// where the first bit is lo while `sw` is false and hi while it is true: a
+// one-bit step on the other bit first exchanges x1 and x2 (a uniform branch
+// with fixed registers), so every one-bit step runs the same code on the
+// pairs (x0, x1) and (x2, x3). Two-bit steps and the final stores use sw = false.
 __device__ __forceinline__ void mop_block(double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
                                           const double2* __restrict__ pool, double2* red, int buf, bool two) {
     const int nst = op.nchain;
-    double g0 = 0.0, g1 = 0.0, g2 = 0.0, g3 = 0.0;
+    double g0 = 0.0, g1 = 0.0, g2s = 0.0, g3 = 0.0;
     const uint32_t ng = 1u << (t - 2);
     const uint32_t olo = 1u << op.lpos[0], ohi = 1u << op.lpos[1];
+    const double2 zero = make_double2(0.0, 0.0);
     for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
         const uint32_t base = ins32<2>(g, op.lpos);
-        double2 a[4], b[4];
-        a[0] = sa[base];
-        a[1] = sa[base + olo];
-        a[2] = sa[base + ohi];
-        a[3] = sa[base + olo + ohi];
+        double2 a0 = sa[base], a1 = sa[base + olo], a2 = sa[base + ohi], a3 = sa[base + olo + ohi];
+        double2 b0 = zero, b1 = zero, b2 = zero, b3 = zero;
         if (two) {
-            b[0] = sb[base];
-            b[1] = sb[base + olo];
-            b[2] = sb[base + ohi];
-            b[3] = sb[base + olo + ohi];
+            b0 = sb[base];
+            b1 = sb[base + olo];
+            b2 = sb[base + ohi];
+            b3 = sb[base + olo + ohi];
         }
+        bool sw = false;
+#pragma unroll
         for (int j = 0; j < 4; ++j) {
             if (j >= nst) break;
-            const double2* __restrict__ M = pool + op.cmat[j];
             const int tgt = (op.smask >> (2 * j)) & 3;
             const int skj = ms_mat(op, j), gkj = ms_gen(op, j);
+            if ((tgt == BS_BOTH && sw) || (tgt == BS_LO && sw) || (tgt == BS_HI && !sw)) {
+                dswap(a1, a2);
+                if (two) dswap(b1, b2);
+                sw = !sw;
+            }
             if (op.cmat[j] < 0) {
                 // a gradient-only step: the generator on the current state
             } else if (tgt == BS_BOTH) {
                 if (skj == MS_CX_LO) {
-                    dswap(a[1], a[3]);
-                    if (two) dswap(b[1], b[3]);
+                    dswap(a1, a3);
+                    if (two) dswap(b1, b3);
                 } else if (skj == MS_CX_HI) {
-                    dswap(a[2], a[3]);
-                    if (two) dswap(b[2], b[3]);
+                    dswap(a2, a3);
+                    if (two) dswap(b2, b3);
                 } else {
-                    blk_4x4(M, a);
-                    if (two) blk_4x4(M, b);
-                }
-            } else if (skj == MS_DIAG) {
-                // d0 on the amplitudes with the step's bit clear, d1 where it is set
-                const double2 d0 = M[0], d1 = M[3];
-                const bool lo = tgt == BS_LO;
-                a[0] = cmul(d0, a[0]);
-                a[1] = cmul(lo ? d1 : d0, a[1]);
-                a[2] = cmul(lo ? d0 : d1, a[2]);
-                a[3] = cmul(d1, a[3]);
-                if (two) {
-                    b[0] = cmul(d0, b[0]);
-                    b[1] = cmul(lo ? d1 : d0, b[1]);
-                    b[2] = cmul(lo ? d0 : d1, b[2]);
-                    b[3] = cmul(d1, b[3]);
-                }
-            } else if (tgt == BS_LO) {
-                blk_2x2(M, a[0], a[1]);
-                blk_2x2(M, a[2], a[3]);
-                if (two) {
-                    blk_2x2(M, b[0], b[1]);
-                    blk_2x2(M, b[2], b[3]);
+                    const double2* __restrict__ M = pool + op.cmat[j];
+                    blk_4x4(M, a0, a1, a2, a3);
+                    if (two) blk_4x4(M, b0, b1, b2, b3);
                 }
             } else {
-                blk_2x2(M, a[0], a[2]);
-                blk_2x2(M, a[1], a[3]);
-                if (two) {
-                    blk_2x2(M, b[0], b[2]);
-                    blk_2x2(M, b[1], b[3]);
-                }
+                const Mat2 m = load_mat2(pool + op.cmat[j]);
+                if (skj == MS_DIAG) blk_step2<MS_DIAG>(m, two, a0, a1, a2, a3, b0, b1, b2, b3);
+                else if (skj == MS_REAL) blk_step2<MS_REAL>(m, two, a0, a1, a2, a3, b0, b1, b2, b3);
+                else if (skj == MS_XLIKE) blk_step2<MS_XLIKE>(m, two, a0, a1, a2, a3, b0, b1, b2, b3);
+                else blk_step2<MS_DENSE>(m, two, a0, a1, a2, a3, b0, b1, b2, b3);
             }
             if (two && op.gen[j] >= 0) {
                 const double2* __restrict__ G = pool + op.gen[j];
-                double v = 0.0;
+                double v;
                 if (tgt == BS_BOTH) {
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) {
-                        double2 sg = make_double2(0.0, 0.0);
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) sg = cfma(G[r + 4 * c], a[c], sg);
-                        v = fma(b[r].x, sg.x, fma(b[r].y, sg.y, v));
-                    }
-                } else if (gkj == MS_DIAG) {
-                    // Re sum conj(b) (G a) with G diagonal: G00 where the step's bit is clear
-                    const double2 e0 = G[0], e1 = G[3];
-                    const bool lo = tgt == BS_LO;
-                    const double2 s0 = cmul(e0, a[0]), s1 = cmul(lo ? e1 : e0, a[1]);
-                    const double2 s2 = cmul(lo ? e0 : e1, a[2]), s3 = cmul(e1, a[3]);
-                    v = fma(b[0].x, s0.x, fma(b[0].y, s0.y, fma(b[1].x, s1.x, b[1].y * s1.y)));
-                    v = fma(b[2].x, s2.x, fma(b[2].y, s2.y, fma(b[3].x, s3.x, fma(b[3].y, s3.y, v))));
-                } else if (tgt == BS_LO) {
-                    v = blk_grad2(G, a[0], a[1], b[0], b[1]) + blk_grad2(G, a[2], a[3], b[2], b[3]);
+                    v = blk_grad4(G, a0, a1, a2, a3, b0, b1, b2, b3);
                 } else {
-                    v = blk_grad2(G, a[0], a[2], b[0], b[2]) + blk_grad2(G, a[1], a[3], b[1], b[3]);
+                    const Mat2 gm = load_mat2(G);
+                    v = gkj == MS_DIAG    ? blk_grad2<MS_DIAG>(gm, a0, a1, a2, a3, b0, b1, b2, b3)
+                        : gkj == MS_REAL  ? blk_grad2<MS_REAL>(gm, a0, a1, a2, a3, b0, b1, b2, b3)
+                        : gkj == MS_XLIKE ? blk_grad2<MS_XLIKE>(gm, a0, a1, a2, a3, b0, b1, b2, b3)
+                                          : blk_grad2<MS_DENSE>(gm, a0, a1, a2, a3, b0, b1, b2, b3);
                 }
                 if (j == 0) g0 += v;
                 else if (j == 1) g1 += v;
-                else if (j == 2) g2 += v;
+                else if (j == 2) g2s += v;
                 else g3 += v;
             }
         }
-        sa[base] = a[0];
-        sa[base + olo] = a[1];
-        sa[base + ohi] = a[2];
-        sa[base + olo + ohi] = a[3];
+        if (sw) {
+            dswap(a1, a2);
+            if (two) dswap(b1, b2);
+        }
+        sa[base] = a0;
+        sa[base + olo] = a1;
+        sa[base + ohi] = a2;
+        sa[base + olo + ohi] = a3;
         if (two) {
-            sb[base] = b[0];
-            sb[base + olo] = b[1];
-            sb[base + ohi] = b[2];
-            sb[base + olo + ohi] = b[3];
+            sb[base] = b0;
+            sb[base + olo] = b1;
+            sb[base + ohi] = b2;
+            sb[base + olo + ohi] = b3;
         }
     }
     if (two) {
         int q = 0;
         if (nst > 0 && op.gen[0] >= 0) wave_partial_re(red, buf, q++, g0);
         if (nst > 1 && op.gen[1] >= 0) wave_partial_re(red, buf, q++, g1);
-        if (nst > 2 && op.gen[2] >= 0) wave_partial_re(red, buf, q++, g2);
+        if (nst > 2 && op.gen[2] >= 0) wave_partial_re(red, buf, q++, g2s);
         if (nst > 3 && op.gen[3] >= 0) wave_partial_re(red, buf, q++, g3);
     }
 }

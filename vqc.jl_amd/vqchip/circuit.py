@@ -37,11 +37,18 @@ class QCircuit(list):
 
 def _structure_hint(g):
     """vqc_op_t.reserved: bit 0 U diagonal, bit 1 generators diagonal, bit 2 / 3
-    U is a CNOT with control on the lower / higher position. Batches plan before
-    their angles arrive, so the planner takes a gate's structure from its type."""
-    from .gates import CNOTGate, PHASEGate, RzGate, SGate, TGate, ZGate
+    U is a CNOT with control on the lower / higher position, bit 4 / 5 U real /
+    X-like, bit 6 / 7 generators real / X-like (real diagonal, imaginary
+    off-diagonal). Batches plan before their angles arrive, so the planner takes
+    a gate's structure from its type; every angle keeps it (Ry(t) is real and
+    its generator -iY/2 too; Rx(t) and -iX/2 are X-like)."""
+    from .gates import CNOTGate, PHASEGate, RxGate, RyGate, RzGate, SGate, TGate, ZGate
     if isinstance(g, (RzGate, PHASEGate, ZGate, SGate, TGate)):
         return 1 | 2
+    if isinstance(g, RyGate):
+        return 16 | 64
+    if isinstance(g, RxGate):
+        return 32 | 128
     if isinstance(g, CNOTGate):
         c, t = g.positions
         return 4 if c < t else 8
