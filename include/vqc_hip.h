@@ -111,7 +111,7 @@ typedef struct vqc_tuning {
     int32_t tile_fwd_bits;          /* tile bits of forward phases, 6..13 (default 12) */
     int32_t tile_rev_bits;          /* tile bits of Hamiltonian / reverse phases, 6..12 (11) */
     int32_t tile_small_bits;        /* whole evaluation in one workgroup up to this many bits, 1..12 (12) */
-    int32_t tile_low_bits;          /* contiguous low index bits in every tile, 0..8 (6) */
+    int32_t tile_low_bits;          /* contiguous low index bits in every tile, 0..8 (4) */
     int32_t dense_groups[3];        /* amplitude groups per thread, 1-, 2-, 3-qubit sweeps: 1, 2, 4, 8 (1) */
     int32_t dense_nontemporal[3];   /* non-temporal loads/stores, 1-, 2-, 3-qubit sweeps: 0/1 (1, 1, 0) */
 } vqc_tuning_t;

@@ -378,7 +378,7 @@ void tune_init_locked() {
     g_tune.tile_fwd_bits = env_or("VQC_TILE_FWD", 12);
     g_tune.tile_rev_bits = env_or("VQC_TILE_REV", 11);
     g_tune.tile_small_bits = env_or("VQC_TILE_SMALL", 12);
-    g_tune.tile_low_bits = env_or("VQC_TILE_MLOW", 6);
+    g_tune.tile_low_bits = env_or("VQC_TILE_MLOW", 4);
     const int r = env_or("VQC_DENSE_R", 1), nt = env_or("VQC_DENSE_NT", -1);
     for (int k = 0; k < 3; ++k) {
         g_tune.dense_groups[k] = r;

@@ -761,7 +761,7 @@ struct TileTuning {
     int t_fwd = 12;     // tile bits of forward phases
     int t_rev = 11;     // tile bits of reverse / Hamiltonian phases (two buffers)
     int t_small = 12;   // whole state in one tile up to this many bits (both buffers)
-    int mlow = 6;       // contiguous low bits kept in every tile
+    int mlow = 4;       // contiguous low bits kept in every tile
 };
 TileTuning vqc_tile_tuning();
 
