@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(kEB) k_accum(const double2* __restrict__ in, d
     __shared__ double2 Us[D * D];
     for (int i = threadIdx.x; i < D * D; i += kEB) Us[i] = Ug[i];
     __syncthreads();
-    for (uint64_t g = uint64_t(blockIdx.x) * kEB + threadIdx.x; g < ngroups; g += uint64_t(gridDim.x) * kEB) {
+    for (uint64_t g = uint64_t(xcd_block()) * kEB + threadIdx.x; g < ngroups; g += uint64_t(gridDim.x) * kEB) {
         const uint64_t base = insert_zeros<K>(g, gp.p);
         double2 a[D];
 #pragma unroll
@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(kEB) k_expec(const double2* __restrict__ ket, 
     for (int i = threadIdx.x; i < D * D; i += kEB) Us[i] = Ug[i];
     __syncthreads();
     double2 acc = make_double2(0.0, 0.0);
-    for (uint64_t g = uint64_t(blockIdx.x) * kEB + threadIdx.x; g < ngroups; g += uint64_t(gridDim.x) * kEB) {
+    for (uint64_t g = uint64_t(xcd_block()) * kEB + threadIdx.x; g < ngroups; g += uint64_t(gridDim.x) * kEB) {
         const uint64_t base = insert_zeros<K>(g, gp.p);
         double2 a[D];
 #pragma unroll
@@ -192,7 +192,7 @@ __global__ void __launch_bounds__(kEB) k_pauli_expec(const double2* __restrict__
     for (int t = 0; t < kPauliChunk; ++t) acc[t] = make_double2(0.0, 0.0);
     const uint64_t L = uint64_t(1) << nbits;
     if (pc.x == 0) {
-        for (uint64_t b = uint64_t(blockIdx.x) * kEB + threadIdx.x; b < L; b += uint64_t(gridDim.x) * kEB) {
+        for (uint64_t b = uint64_t(xcd_block()) * kEB + threadIdx.x; b < L; b += uint64_t(gridDim.x) * kEB) {
             const double2 k = ket[b];
             const double2 q = BRA ? bra[b] : k;
             double2 w = make_double2(0.0, 0.0);
@@ -207,7 +207,7 @@ __global__ void __launch_bounds__(kEB) k_pauli_expec(const double2* __restrict__
         }
     } else {
         const uint64_t half = L >> 1;
-        for (uint64_t g = uint64_t(blockIdx.x) * kEB + threadIdx.x; g < half; g += uint64_t(gridDim.x) * kEB) {
+        for (uint64_t g = uint64_t(xcd_block()) * kEB + threadIdx.x; g < half; g += uint64_t(gridDim.x) * kEB) {
             const uint64_t lo = g & ((uint64_t(1) << pc.hbit) - 1);
             const uint64_t b = ((g - lo) << 1) | lo;
             const uint64_t a = b ^ pc.x;
@@ -240,7 +240,7 @@ __global__ void __launch_bounds__(kEB) k_pauli_apply(const double2* __restrict__
                                                      PauliChunk pc, int accumulate) {
     const uint64_t L = uint64_t(1) << nbits;
     if (pc.x == 0) {
-        for (uint64_t b = uint64_t(blockIdx.x) * kEB + threadIdx.x; b < L; b += uint64_t(gridDim.x) * kEB) {
+        for (uint64_t b = uint64_t(xcd_block()) * kEB + threadIdx.x; b < L; b += uint64_t(gridDim.x) * kEB) {
             double2 f = make_double2(0.0, 0.0);
             for (int t = 0; t < pc.nt; ++t) {
                 const double s = sgn(b, pc.z[t]);
@@ -254,7 +254,7 @@ __global__ void __launch_bounds__(kEB) k_pauli_apply(const double2* __restrict__
         return;
     }
     const uint64_t half = L >> 1;
-    for (uint64_t g = uint64_t(blockIdx.x) * kEB + threadIdx.x; g < half; g += uint64_t(gridDim.x) * kEB) {
+    for (uint64_t g = uint64_t(xcd_block()) * kEB + threadIdx.x; g < half; g += uint64_t(gridDim.x) * kEB) {
         const uint64_t lo = g & ((uint64_t(1) << pc.hbit) - 1);
         const uint64_t b = ((g - lo) << 1) | lo;
         const uint64_t a = b ^ pc.x;
@@ -445,7 +445,7 @@ __global__ void __launch_bounds__(kEB) k_dm_expec(const double2* __restrict__ rh
     for (int i = threadIdx.x; i < D * D; i += kEB) Us[i] = Ug[i];
     __syncthreads();
     double2 acc = make_double2(0.0, 0.0);
-    for (uint64_t g = uint64_t(blockIdx.x) * kEB + threadIdx.x; g < ngroups; g += uint64_t(gridDim.x) * kEB) {
+    for (uint64_t g = uint64_t(xcd_block()) * kEB + threadIdx.x; g < ngroups; g += uint64_t(gridDim.x) * kEB) {
         const uint64_t base = insert_zeros<K>(g, gp.p);
 #pragma unroll
         for (int a = 0; a < D; ++a) {

@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(kBlock) k_dense_small(double2* __restrict__ v,
     uint64_t off[D];
 #pragma unroll
     for (int j = 0; j < D; ++j) off[j] = sub_offset<K>(j, gp.p);
-    const uint64_t g0 = uint64_t(blockIdx.x) * (R * kBlock) + threadIdx.x;
+    const uint64_t g0 = uint64_t(xcd_block()) * (R * kBlock) + threadIdx.x;
     double2 a[R][D];
     uint64_t base[R];
 #pragma unroll
@@ -92,7 +92,7 @@ __global__ void __launch_bounds__(kBlock) k_dense_big(double2* __restrict__ v, u
     __shared__ double2 Us[D * D];
     for (int i = threadIdx.x; i < D * D; i += kBlock) Us[i] = Ug[i];
     __syncthreads();
-    const uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const uint64_t g = uint64_t(xcd_block()) * kBlock + threadIdx.x;
     if (g >= ngroups) return;
     const uint64_t base = insert_zeros<K>(g, gp.p);
     double2 a[D];
@@ -175,7 +175,7 @@ extern "C" int vqc_sv_apply_dense(vqc_state* s, int k, const int32_t* pos, const
 template <int KIND, int K>
 __global__ void __launch_bounds__(kBlock) k_named(double2* __restrict__ v, uint64_t ngroups, GPos<K> sorted,
                                                   GPos<K> given, double2 ph) {
-    const uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const uint64_t g = uint64_t(xcd_block()) * kBlock + threadIdx.x;
     if (g >= ngroups) return;
     const uint64_t l = insert_zeros<K>(g, sorted.p);
     const uint64_t p0 = uint64_t(1) << given.p[0];
@@ -259,7 +259,7 @@ extern "C" int vqc_sv_apply_named(vqc_state* s, int kind, const int32_t* pos, do
 template <int K>
 __global__ void __launch_bounds__(kBlock) k_dm_gate(double2* __restrict__ v, uint64_t ngroups, GPos<2 * K> gp, GMat<K> U) {
     constexpr int D = 1 << K;
-    const uint64_t g = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+    const uint64_t g = uint64_t(xcd_block()) * kBlock + threadIdx.x;
     if (g >= ngroups) return;
     const uint64_t base = insert_zeros<2 * K>(g, gp.p);
     double2 x[D][D];  // [row][col]
