@@ -217,7 +217,7 @@ def bench_config3(vq, _lib, oracle, timer):
 # rocprofv3 --pmc passes (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a
 # copy of known size reads; profiles/hbm_traffic_dense_28q.md). Algorithmic:
 # 2 * 16 * 2^28 = 8.590e9.
-TRAFFIC_1Q_BYTES = 8.628e9
+TRAFFIC_1Q_BYTES = 8.5913e9
 
 
 def bench_config4(vq, _lib, count, seed=20261203):
@@ -342,8 +342,9 @@ def main():
     log(f"unfused: {ms / args.steps:.1f} ms/step, {gbs:.0f} GB/s (wall {wall:.2f}s)")
     norm = state.norm()
 
-    # dominant kernel: the one-qubit dense sweep, timed alone
-    one_q = [m for m, g in zip(mats, circuit) if len(g.positions) == 1]
+    # dominant kernel: the one-qubit dense sweep k_dense_small<1, 1, true>, timed
+    # alone; one-qubit gates on bits 0..2 run the temporal variant and are left out
+    one_q = [m for m, g in zip(mats, circuit) if len(g.positions) == 1 and min(g.positions) - 1 >= 3]
     timer.start()
     for k, p, U, _keep in one_q:
         _lib.call("vqc_sv_apply_dense", state.handle, k, p, U)

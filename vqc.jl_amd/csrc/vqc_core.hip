@@ -382,8 +382,8 @@ void tune_init_locked() {
     const int r = env_or("VQC_DENSE_R", 1), nt = env_or("VQC_DENSE_NT", -1);
     for (int k = 0; k < 3; ++k) {
         g_tune.dense_groups[k] = r;
-        // measured at 28 qubits: non-temporal helps the 2- and 4-stream sweeps, hurts the 8-stream one
-        g_tune.dense_nontemporal[k] = nt >= 0 ? (nt != 0) : (k < 2);
+        // non-temporal accesses, taken by the sweeps whose targets are all at bit 3 or above
+        g_tune.dense_nontemporal[k] = nt >= 0 ? (nt != 0) : 1;
     }
     g_tune_init = true;
 }

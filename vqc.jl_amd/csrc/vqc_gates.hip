@@ -122,12 +122,14 @@ static int launch_dense(double2* v, int nbits, const int32_t* pos, const double*
         GMat<K> U;
         for (int i = 0; i < D * D; ++i) U.m[i] = make_double2(Uh[2 * i], Uh[2 * i + 1]);
         // Groups per thread and non-temporal hints (vqc_set_tuning). Measured on
-        // MI355X at 28 qubits: one group per thread is best for every K;
-        // non-temporal accesses help the 2- and 4-stream sweeps (K = 1, 2:
-        // 5.9 / 5.6 TB/s) and hurt the 8-stream K = 3 sweep.
+        // MI355X at 28 qubits: one group per thread is best for every K.
+        // Non-temporal accesses pay when every target is at bit 3 or above (a
+        // 3-qubit gate on bits 20-22: 5.1 -> 5.9 TB/s) and cost when one is
+        // below (bits 0, 13, 27: 5.3 -> 3.5 TB/s), where each lane's accesses
+        // fall in a few 128-byte lines; those sweeps stay temporal.
         const vqc_tuning_t tu = vqc_tuning_now();
         const int R = tu.dense_groups[K - 1];
-        const bool NT = tu.dense_nontemporal[K - 1] != 0;
+        const bool NT = tu.dense_nontemporal[K - 1] != 0 && gp.p[0] >= 3;
         const dim3 blk(kBlock);
 #define VQC_DENSE_LAUNCH(RR)                                                                                             \
     if (R == RR) {                                                                                                      \
