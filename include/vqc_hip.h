@@ -19,8 +19,12 @@
  *     reference's Julia exceptions (@assert / throw / error).
  *   - Calls are ordered on one HIP stream per device. Calls that return data
  *     to the host (downloads, expectations, gradients) synchronise; the rest
- *     return once the work is enqueued. A state may not be used by two host
- *     threads at once (the reference's apply! is not re-entrant either).
+ *     return once the work is enqueued. The single-state calls may come from
+ *     any host thread: the library serialises them with one lock, since they
+ *     share that stream and its staging buffers. A state used by two threads
+ *     sees their calls in an unspecified order (the reference's apply! is not
+ *     re-entrant either). Batches and servers own their streams and buffers
+ *     and run beside the single-state calls.
  */
 #ifndef VQC_HIP_H
 #define VQC_HIP_H

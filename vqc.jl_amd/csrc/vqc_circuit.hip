@@ -897,6 +897,7 @@ static int upload_program(const PlanBuilder& pb, hipStream_t st) {
 }
 
 extern "C" int vqc_apply_circuit(vqc_state* s, int nops, const vqc_op_t* ops, const double* pool) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_qubits(s, "vqc_apply_circuit"));
     int64_t plen = 0;
     VQC_TRY(check_program(s->nqubits, s->is_dm, nops, ops, &plen, 1 << 30));
@@ -1023,6 +1024,7 @@ static int max_small_rows(int nbits) {
 extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_t* ops, const double* pool, int nterms,
                                   const vqc_term_t* terms, const double* hpool, int nparams, double* energy2, double* grad,
                                   vqc_state* psi_out, int flags) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_qubits(psi0, "vqc_sv_energy_grad"));
     if (psi0->is_dm) return vqc_fail(VQC_EINVAL, "vqc_sv_energy_grad: state is a density matrix");
     if (nparams < 0 || (nparams && !grad)) return vqc_fail(VQC_EINVAL, "bad gradient output");
@@ -1172,6 +1174,7 @@ int vqc_dm_gate_sweep(double2* v, int n, int k, const int32_t* pos, const double
 extern "C" int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_t* ops, const double* pool, int nterms,
                                   const vqc_term_t* terms, const double* hpool, int nparams, double* energy2, double* grad,
                                   vqc_state* rho_out, int flags) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_state(rho0, "vqc_dm_energy_grad"));
     if (!rho0->is_dm) return vqc_fail(VQC_EINVAL, "vqc_dm_energy_grad: state is not a density matrix");
     if (nparams < 0 || (nparams && !grad)) return vqc_fail(VQC_EINVAL, "bad gradient output");

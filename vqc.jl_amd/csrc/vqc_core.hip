@@ -23,6 +23,11 @@ extern "C" int vqc_version(void) { return 10000; }
 static VqcCtx g_ctx;
 static std::mutex g_ctx_mu;
 
+std::recursive_mutex& vqc_api_mutex() {
+    static std::recursive_mutex mu;
+    return mu;
+}
+
 int vqc_ctx(VqcCtx** out) {
     std::lock_guard<std::mutex> lk(g_ctx_mu);
     if (g_ctx.device < 0) {
@@ -110,6 +115,7 @@ extern "C" int vqc_device_info(int* cu_count, int* lds_bytes_per_cu, uint64_t* h
 }
 
 extern "C" int vqc_sync(void) {
+    VQC_API_LOCK;
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));
     VQC_HIP(hipStreamSynchronize(c->stream));
@@ -117,6 +123,7 @@ extern "C" int vqc_sync(void) {
 }
 
 extern "C" int vqc_get_stream(void** stream) {
+    VQC_API_LOCK;
     if (!stream) return vqc_fail(VQC_EINVAL, "null output");
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));
@@ -234,13 +241,18 @@ static int state_alloc(int nbits, int nq, int is_dm, vqc_state** out) {
     return vqc_state_reset(s);
 }
 
-extern "C" int vqc_sv_create(int nqubits, vqc_state** out) { return state_alloc(nqubits, nqubits, 0, out); }
+extern "C" int vqc_sv_create(int nqubits, vqc_state** out) {
+    VQC_API_LOCK;
+    return state_alloc(nqubits, nqubits, 0, out);
+}
 extern "C" int vqc_dm_create(int nqubits, vqc_state** out) {
+    VQC_API_LOCK;
     if (nqubits < 1 || nqubits > 17) return vqc_fail(VQC_EINVAL, "unsupported density-matrix size: %d qubits", nqubits);
     return state_alloc(2 * nqubits, nqubits, 1, out);
 }
 
 extern "C" int vqc_state_wrap(int nbits, int is_dm, void* device_ptr, vqc_state** out) {
+    VQC_API_LOCK;
     if (!out || !device_ptr) return vqc_fail(VQC_EINVAL, "null pointer");
     if (nbits < (is_dm ? 2 : 0) || nbits > 34 || (is_dm && (nbits & 1))) return vqc_fail(VQC_EINVAL, "bad state size %d", nbits);
     VqcCtx* c;
@@ -256,6 +268,7 @@ extern "C" int vqc_state_wrap(int nbits, int is_dm, void* device_ptr, vqc_state*
 }
 
 extern "C" int vqc_state_destroy(vqc_state* s) {
+    VQC_API_LOCK;
     if (!s) return VQC_OK;
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));
@@ -266,18 +279,21 @@ extern "C" int vqc_state_destroy(vqc_state* s) {
 }
 
 extern "C" int vqc_state_nbits(const vqc_state* s, int* nbits) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_state(s, "vqc_state_nbits"));
     *nbits = s->nbits;
     return VQC_OK;
 }
 
 extern "C" int vqc_state_device_ptr(vqc_state* s, void** out) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_state(s, "vqc_state_device_ptr"));
     *out = s->d;
     return VQC_OK;
 }
 
 extern "C" int vqc_state_upload(vqc_state* s, const double* host, uint64_t n) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_state(s, "vqc_state_upload"));
     if (!host || n != s->len()) return vqc_fail(VQC_EINVAL, "upload of %llu amplitudes into a state of %llu",
                                                 (unsigned long long)n, (unsigned long long)s->len());
@@ -289,6 +305,7 @@ extern "C" int vqc_state_upload(vqc_state* s, const double* host, uint64_t n) {
 }
 
 extern "C" int vqc_state_download(const vqc_state* s, double* host, uint64_t n) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_state(s, "vqc_state_download"));
     if (!host || n != s->len()) return vqc_fail(VQC_EINVAL, "download of %llu amplitudes from a state of %llu",
                                                 (unsigned long long)n, (unsigned long long)s->len());
@@ -300,6 +317,7 @@ extern "C" int vqc_state_download(const vqc_state* s, double* host, uint64_t n) 
 }
 
 extern "C" int vqc_state_reset(vqc_state* s) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_state(s, "vqc_state_reset"));
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));
@@ -309,6 +327,7 @@ extern "C" int vqc_state_reset(vqc_state* s) {
 }
 
 extern "C" int vqc_state_copy(vqc_state* dst, const vqc_state* src) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_state(dst, "vqc_state_copy"));
     VQC_TRY(vqc_check_state(src, "vqc_state_copy"));
     if (dst->nbits != src->nbits) return vqc_fail(VQC_EINVAL, "copy between states of different size");
@@ -324,6 +343,7 @@ __global__ void __launch_bounds__(256) k_axpy(double2* __restrict__ y, const dou
 }
 
 extern "C" int vqc_state_axpy(vqc_state* y, double re, double im, const vqc_state* x) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_state(y, "vqc_state_axpy"));
     VQC_TRY(vqc_check_state(x, "vqc_state_axpy"));
     if (x->nbits != y->nbits) return vqc_fail(VQC_EINVAL, "vqc_state_axpy: states differ in size");
@@ -335,6 +355,7 @@ extern "C" int vqc_state_axpy(vqc_state* y, double re, double im, const vqc_stat
 }
 
 extern "C" int vqc_state_scale(vqc_state* s, double re, double im) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_state(s, "vqc_state_scale"));
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));
@@ -344,6 +365,7 @@ extern "C" int vqc_state_scale(vqc_state* s, double re, double im) {
 }
 
 extern "C" int vqc_state_dot(const vqc_state* a, const vqc_state* b, double* out2) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_state(a, "vqc_state_dot"));
     VQC_TRY(vqc_check_state(b, "vqc_state_dot"));
     if (a->nbits != b->nbits) return vqc_fail(VQC_EINVAL, "dot of states of different size");

@@ -131,6 +131,7 @@ int vqc_expec_sweep(const double2* ket, const double2* bra, int nbits, int k, co
 }
 
 extern "C" int vqc_sv_accum_dense(const vqc_state* in, vqc_state* out, int k, const int32_t* pos, const double* M) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_qubits(in, "vqc_sv_accum_dense"));
     VQC_TRY(vqc_check_state(out, "vqc_sv_accum_dense"));
     if (in->nbits != out->nbits) return vqc_fail(VQC_EINVAL, "accumulate between states of different size");
@@ -146,6 +147,7 @@ extern "C" int vqc_sv_accum_dense(const vqc_state* in, vqc_state* out, int k, co
 
 extern "C" int vqc_sv_expec_dense(const vqc_state* ket, const vqc_state* bra, int k, const int32_t* pos, const double* M,
                                   double* out2) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_qubits(ket, "vqc_sv_expec_dense"));
     if (bra && bra->nbits != ket->nbits) return vqc_fail(VQC_EINVAL, "bra and ket differ in size");
     VQC_TRY(vqc_check_positions(ket->nbits, k, pos, 5));
@@ -324,6 +326,7 @@ static int check_paulis(int nbits, int nterms, const vqc_pauli_t* terms) {
 
 extern "C" int vqc_sv_expec_pauli(const vqc_state* ket, const vqc_state* bra, int nterms, const vqc_pauli_t* terms,
                                   double* out, double* sum2) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_qubits(ket, "vqc_sv_expec_pauli"));
     if (bra && bra->nbits != ket->nbits) return vqc_fail(VQC_EINVAL, "bra and ket differ in size");
     VQC_TRY(check_paulis(ket->nbits, nterms, terms));
@@ -380,6 +383,7 @@ void vqc_zero(double2* v, uint64_t len, hipStream_t st) {
 }
 
 extern "C" int vqc_sv_apply_pauli_sum(const vqc_state* in, vqc_state* out, int nterms, const vqc_pauli_t* terms) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_qubits(in, "vqc_sv_apply_pauli_sum"));
     VQC_TRY(vqc_check_state(out, "vqc_sv_apply_pauli_sum"));
     if (in->nbits != out->nbits) return vqc_fail(VQC_EINVAL, "states differ in size");
@@ -412,6 +416,7 @@ static size_t g_hpool_cap = 0;
 
 extern "C" int vqc_sv_apply_terms(const vqc_state* in, vqc_state* out, int nterms, const vqc_term_t* terms, const double* pool,
                                   double* energy2) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_qubits(in, "vqc_sv_apply_terms"));
     VQC_TRY(vqc_check_state(out, "vqc_sv_apply_terms"));
     if (in->nbits != out->nbits || in->d == out->d) return vqc_fail(VQC_EINVAL, "need distinct states of equal size");
@@ -484,6 +489,7 @@ int vqc_dm_expec_sweep(const double2* rho, int n, int k, const int32_t* pos, con
 }
 
 extern "C" int vqc_dm_expec_dense(const vqc_state* rho, int k, const int32_t* pos, const double* M, double* out2) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_state(rho, "vqc_dm_expec_dense"));
     if (!rho->is_dm) return vqc_fail(VQC_EINVAL, "vqc_dm_expec_dense: state is not a density matrix");
     VQC_TRY(vqc_check_positions(rho->nqubits, k, pos, 5));

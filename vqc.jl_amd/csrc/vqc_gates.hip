@@ -162,6 +162,7 @@ int vqc_dense_sweep(double2* v, int nbits, int k, const int32_t* pos, const doub
 }
 
 extern "C" int vqc_sv_apply_dense(vqc_state* s, int k, const int32_t* pos, const double* U) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_qubits(s, "vqc_sv_apply_dense"));
     VQC_TRY(vqc_check_positions(s->nbits, k, pos, 5));
     if (!U) return vqc_fail(VQC_EINVAL, "null gate matrix");
@@ -231,6 +232,7 @@ static int launch_named(double2* v, int nbits, const int32_t* pos, double2 ph, h
 }
 
 extern "C" int vqc_sv_apply_named(vqc_state* s, int kind, const int32_t* pos, double phi) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_qubits(s, "vqc_sv_apply_named"));
     if (!pos) return vqc_fail(VQC_EINVAL, "null position array");
     VqcCtx* c;
@@ -332,6 +334,7 @@ int vqc_dm_gate_sweep(double2* v, int n, int k, const int32_t* pos, const double
 }
 
 extern "C" int vqc_dm_apply_gate(vqc_state* rho, int k, const int32_t* pos, const double* U) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_state(rho, "vqc_dm_apply_gate"));
     if (!rho->is_dm) return vqc_fail(VQC_EINVAL, "vqc_dm_apply_gate: state is not a density matrix");
     VQC_TRY(vqc_check_positions(rho->nqubits, k, pos, 5));
@@ -342,6 +345,7 @@ extern "C" int vqc_dm_apply_gate(vqc_state* rho, int k, const int32_t* pos, cons
 }
 
 extern "C" int vqc_dm_apply_superop(vqc_state* rho, int N, const int32_t* pos, const double* S, int mode) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_state(rho, "vqc_dm_apply_superop"));
     if (!rho->is_dm) return vqc_fail(VQC_EINVAL, "vqc_dm_apply_superop: state is not a density matrix");
     VQC_TRY(vqc_check_positions(rho->nqubits, N, pos, 2));

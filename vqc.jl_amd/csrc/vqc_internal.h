@@ -7,6 +7,7 @@
 #include <cstdint>
 #include <cstring>
 #include <cstdio>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -26,6 +27,13 @@ int vqc_fail(int code, const char* fmt, ...);
         int r_ = (call);            \
         if (r_ != VQC_OK) return r_; \
     } while (0)
+
+// One lock for the single-state entry points. They share the context's stream
+// and its scratch, pinned and staging buffers, so calls from several host
+// threads are serialised here (recursive: an entry point may call another).
+// Batches and the server own their buffers and streams and do not take it.
+std::recursive_mutex& vqc_api_mutex();
+#define VQC_API_LOCK std::lock_guard<std::recursive_mutex> vqc_api_guard_(vqc_api_mutex())
 
 // ---------------------------------------------------------------------------
 // library context: one stream per process on the current device, a growable

@@ -96,6 +96,7 @@ int apply_1q(vqc_state* s, int pos, const double* M, hipStream_t st) {
 }  // namespace
 
 extern "C" int vqc_prob(const vqc_state* s, int pos, int value, double* p) {
+    VQC_API_LOCK;
     VQC_TRY(check_qubit(s, pos, "vqc_prob"));
     if ((value != 0 && value != 1) || !p) return vqc_fail(VQC_EINVAL, "vqc_prob: value must be 0 or 1 and p non-null");
     VqcCtx* c;
@@ -104,6 +105,7 @@ extern "C" int vqc_prob(const vqc_state* s, int pos, int value, double* p) {
 }
 
 extern "C" int vqc_measure(vqc_state* s, int pos, double u, int auto_reset, int* outcome, double* prob) {
+    VQC_API_LOCK;
     VQC_TRY(check_qubit(s, pos, "vqc_measure"));
     if (!(u >= 0.0 && u < 1.0)) return vqc_fail(VQC_EINVAL, "vqc_measure: u must lie in [0, 1)");
     VqcCtx* c;
@@ -125,6 +127,7 @@ extern "C" int vqc_measure(vqc_state* s, int pos, double u, int auto_reset, int*
 }
 
 extern "C" int vqc_post_select(vqc_state* s, int pos, int value, double* norm) {
+    VQC_API_LOCK;
     VQC_TRY(check_qubit(s, pos, "vqc_post_select"));
     if (s->is_dm) return vqc_fail(VQC_EINVAL, "vqc_post_select: state vectors only (postselect.jl:3-8)");
     if (value != 0 && value != 1) return vqc_fail(VQC_EINVAL, "vqc_post_select: value must be 0 or 1");
@@ -142,6 +145,7 @@ extern "C" int vqc_post_select(vqc_state* s, int pos, int value, double* norm) {
 }
 
 extern "C" int vqc_sv_select(const vqc_state* s, int pos, int value, vqc_state* out, double* norm) {
+    VQC_API_LOCK;
     VQC_TRY(check_qubit(s, pos, "vqc_sv_select"));
     VQC_TRY(vqc_check_state(out, "vqc_sv_select"));
     if (s->is_dm || out->is_dm) return vqc_fail(VQC_EINVAL, "vqc_sv_select: state vectors only");

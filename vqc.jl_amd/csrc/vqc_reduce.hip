@@ -97,6 +97,7 @@ __global__ void __launch_bounds__(kRB) k_sandwich(const double2* __restrict__ rh
 }  // namespace
 
 extern "C" int vqc_partial_trace(const vqc_state* s, int nsites, const int32_t* sites, vqc_state* out) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_state(s, "vqc_partial_trace"));
     VQC_TRY(vqc_check_state(out, "vqc_partial_trace"));
     const int n = s->nqubits;
@@ -131,6 +132,7 @@ extern "C" int vqc_partial_trace(const vqc_state* s, int nsites, const int32_t* 
 }
 
 extern "C" int vqc_dm_sandwich(const vqc_state* rho, const vqc_state* psi, double* out2) {
+    VQC_API_LOCK;
     VQC_TRY(vqc_check_state(rho, "vqc_dm_sandwich"));
     VQC_TRY(vqc_check_state(psi, "vqc_dm_sandwich"));
     if (!rho->is_dm || psi->is_dm || psi->nqubits != rho->nqubits || !out2)
