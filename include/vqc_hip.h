@@ -249,6 +249,17 @@ int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_t* ops, con
 int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_t* ops, const double* pool,
                        int nterms, const vqc_term_t* terms, const double* hpool,
                        int nparams, double* energy2, double* grad, vqc_state* rho_out, int flags);
+/* The pullback of y = circuit * x for a cotangent delta_y on y: replaces
+ * @adjoint *(circuit, x) with back_propagate (circuitdiff.jl:2-8, 23-78) for
+ * any loss. grad[slot] (nparams entries, zeroed first) receives
+ * sum Re <Delta|G|y> over the active parameters, and delta_x (may be NULL) the
+ * cotangent on x, circuit^dagger delta_y (channels undone with S^dagger). x may
+ * be a state vector or a density matrix; delta_y and delta_x have its shape.
+ * y_out (may be NULL) receives circuit * x and must not alias delta_y;
+ * delta_x may alias delta_y or x. Runs on the fused path: flags must be 0 for
+ * state vectors (density matrices also take VQC_FLAG_UNFUSED). */
+int vqc_circuit_vjp(const vqc_state* x, int nops, const vqc_op_t* ops, const double* pool, int nparams,
+                    const vqc_state* delta_y, double* grad, vqc_state* delta_x, vqc_state* y_out, int flags);
 
 /* ---- batched evaluation (no counterpart in the reference) ----------------
  * Many independent state-vector instances of uneven size, each |0..0> pushed

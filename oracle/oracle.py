@@ -167,6 +167,19 @@ def apply_circuit(v, n, is_dm, ops, nops, pool):
     return v
 
 
+def circuit_vjp(x, n, is_dm, ops, nops, pool, nparams, delta_y):
+    """(grad, delta_x, y) of the pullback of y = circuit * x for the cotangent
+    delta_y (@adjoint *(circuit, x) with back_propagate, circuitdiff.jl:2-8, 23-78)."""
+    x = np.ascontiguousarray(x, dtype=np.complex128)
+    d = np.ascontiguousarray(delta_y, dtype=np.complex128)
+    g = np.zeros(max(nparams, 1))
+    dx = np.zeros_like(x)
+    y = np.zeros_like(x)
+    _rc(lib().orc_circuit_vjp(n, int(is_dm), _c(x), nops, ops, _c(pool), nparams, _c(d), g.ctypes.data_as(_D), _c(dx),
+                              _c(y)))
+    return g[:nparams], dx, y
+
+
 def energy_grad(state0, n, is_dm, ops, nops, pool, terms, nterms, hpool, nparams):
     """(E, grad, final state) of the restated adjoint (circuitdiff.jl / expecdiff.jl)."""
     e2 = np.zeros(2)

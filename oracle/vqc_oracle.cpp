@@ -430,6 +430,32 @@ extern "C" int orc_apply_circuit(int n, int is_dm, cd* v, int nops, const vqc_op
     return 0;
 }
 
+// back_propagate(Delta, circuit, y) for a state vector (circuitdiff.jl:23-32,
+// 61-78): per op in reverse, Delta <- m' Delta, y <- m' y, then
+// grad = real(expectation(y, item, Delta)) for item in differentiate(m). The op
+// carries G = m' dm/dtheta; item = G' so that real(<y|G'|Delta>) =
+// real(<Delta|G|y>), which is what is evaluated here. y and delta are updated
+// in place (they end as the circuit's input and the cotangent on it).
+static void sv_back_propagate(int n, int nops, const vqc_op_t* ops, const cd* pool, int nparams, cd* y, cd* delta,
+                              double* grad) {
+    for (int p = 0; p < nparams; ++p) grad[p] = 0;
+    std::vector<cd> ud;
+    for (int i = nops - 1; i >= 0; --i) {
+        const vqc_op_t& o = ops[i];
+        const int K = 1 << o.nq;
+        ud.assign(K * K, 0);
+        dagger(pool + o.mat, K, ud.data());
+        orc_apply_dense(n, delta, o.nq, o.pos, ud.data());
+        orc_apply_dense(n, y, o.nq, o.pos, ud.data());
+        for (int j = 0; j < o.nparams; ++j) {
+            if (o.slot[j] < 0) continue;
+            double g2[2];
+            orc_expec_dense(n, y, delta, o.nq, o.pos, pool + o.gen[j], g2);
+            grad[o.slot[j]] += g2[0];
+        }
+    }
+}
+
 // Zygote.gradient(c -> real(expectation(H, c * psi0)), c) for a state vector:
 //  forward   y = circuit * psi0                       (circuitdiff.jl:2-4)
 //  energy    E = sum_keys expectation_value(key, M, y) (expecs.jl:22-28)
@@ -462,22 +488,7 @@ extern "C" int orc_sv_energy_grad(int n, const cd* psi0, int nops, const vqc_op_
     }
     energy2[0] = E.real();
     energy2[1] = E.imag();
-    for (int p = 0; p < nparams; ++p) grad[p] = 0;
-    std::vector<cd> ud;
-    for (int i = nops - 1; i >= 0; --i) {
-        const vqc_op_t& o = ops[i];
-        const int K = 1 << o.nq;
-        ud.assign(K * K, 0);
-        dagger(pool + o.mat, K, ud.data());
-        orc_apply_dense(n, delta.data(), o.nq, o.pos, ud.data());
-        orc_apply_dense(n, y.data(), o.nq, o.pos, ud.data());
-        for (int j = 0; j < o.nparams; ++j) {
-            if (o.slot[j] < 0) continue;
-            double g2[2];
-            orc_expec_dense(n, y.data(), delta.data(), o.nq, o.pos, pool + o.gen[j], g2);
-            grad[o.slot[j]] += g2[0];
-        }
-    }
+    sv_back_propagate(n, nops, ops, pool, nparams, y.data(), delta.data(), grad);
     return 0;
 }
 
@@ -493,6 +504,9 @@ static cd trace_prod(int n, const cd* x, const cd* y) {
     });
 }
 
+static void dm_back_propagate(int n, int nops, const vqc_op_t* ops, const cd* pool, int nparams, cd* yp, cd* deltap,
+                              double* grad);
+
 // Density-matrix version (circuitdiff.jl:36-59, expecdiff.jl:64-68):
 //  energy  E = sum_keys tr(M rho)           (expecs_dm.jl:17-27)
 //  seed    Delta = z H' applied to the identity, z = 1
@@ -505,7 +519,7 @@ extern "C" int orc_dm_energy_grad(int n, const cd* rho0, int nops, const vqc_op_
     if (check_ops(n, 1, nops, ops)) return -1;
     const uint64_t N = uint64_t(1) << n;
     const int64_t L = int64_t(N * N);
-    std::vector<cd> y(rho0, rho0 + L), delta(L, cd(0)), ident(L, cd(0)), tmp(L);
+    std::vector<cd> y(rho0, rho0 + L), delta(L, cd(0)), ident(L, cd(0));
     orc_apply_circuit(n, 1, y.data(), nops, ops, pool);
     if (rho_out) std::copy(y.begin(), y.end(), rho_out);
     for (uint64_t i = 0; i < N; ++i) ident[i + i * N] = 1;
@@ -523,36 +537,66 @@ extern "C" int orc_dm_energy_grad(int n, const cd* rho0, int nops, const vqc_op_
     }
     energy2[0] = E.real();
     energy2[1] = E.imag();
+    dm_back_propagate(n, nops, ops, pool, nparams, y.data(), delta.data(), grad);
+    return 0;
+}
+
+// back_propagate for a density matrix (circuitdiff.jl:42-59): per gate
+// Delta <- m' Delta m, y <- m' y m, grad = real(tr(y item Delta) + tr(Delta item' y)),
+// item = G'; per channel Delta <- S' Delta, y <- S^-1 y. In place, like the
+// state-vector version.
+static void dm_back_propagate(int n, int nops, const vqc_op_t* ops, const cd* pool, int nparams, cd* yp, cd* deltap,
+                              double* grad) {
+    const uint64_t N = uint64_t(1) << n;
+    const int64_t L = int64_t(N * N);
+    std::vector<cd> tmp(L);
+    cd* y = yp;
+    cd* delta = deltap;
     for (int p = 0; p < nparams; ++p) grad[p] = 0;
     std::vector<cd> ud, gd;
     for (int i = nops - 1; i >= 0; --i) {
         const vqc_op_t& o = ops[i];
         if (o.kind == VQC_OP_CHANNEL) {
-            orc_dm_apply_superop(n, delta.data(), o.nq, o.pos, pool + o.mat, 1);
-            orc_dm_apply_superop(n, y.data(), o.nq, o.pos, pool + o.aux, 0);
+            orc_dm_apply_superop(n, delta, o.nq, o.pos, pool + o.mat, 1);
+            orc_dm_apply_superop(n, y, o.nq, o.pos, pool + o.aux, 0);
             continue;
         }
         const int K = 1 << o.nq;
         ud.assign(K * K, 0);
         dagger(pool + o.mat, K, ud.data());
-        orc_dm_apply_gate(n, delta.data(), o.nq, o.pos, ud.data());
-        orc_dm_apply_gate(n, y.data(), o.nq, o.pos, ud.data());
+        orc_dm_apply_gate(n, delta, o.nq, o.pos, ud.data());
+        orc_dm_apply_gate(n, y, o.nq, o.pos, ud.data());
         for (int j = 0; j < o.nparams; ++j) {
             if (o.slot[j] < 0) continue;
             const cd* G = pool + o.gen[j];
             gd.assign(K * K, 0);
             dagger(G, K, gd.data());  // item = G'
             // tr(y * (item on rows of Delta))
-            std::copy(delta.begin(), delta.end(), tmp.begin());
+            std::copy(delta, delta + L, tmp.begin());
             orc_apply_dense(2 * n, tmp.data(), o.nq, o.pos, gd.data());
-            cd a = trace_prod(n, y.data(), tmp.data());
+            cd a = trace_prod(n, y, tmp.data());
             // tr(Delta * (item' on rows of y)), item' = G
-            std::copy(y.begin(), y.end(), tmp.begin());
+            std::copy(y, y + L, tmp.begin());
             orc_apply_dense(2 * n, tmp.data(), o.nq, o.pos, G);
-            cd b = trace_prod(n, delta.data(), tmp.data());
+            cd b = trace_prod(n, delta, tmp.data());
             grad[o.slot[j]] += (a + b).real();
         }
     }
+}
+
+// The pullback of y = circuit * x (@adjoint *(circuit, x), circuitdiff.jl:2-8)
+// for a cotangent delta_y on y: grad (the circuit's active parameters, forward
+// order) and delta_x, the cotangent on x. y_out (may be null) receives y.
+extern "C" int orc_circuit_vjp(int n, int is_dm, const cd* x, int nops, const vqc_op_t* ops, const cd* pool, int nparams,
+                               const cd* delta_y, double* grad, cd* delta_x, cd* y_out) {
+    if (check_ops(n, is_dm, nops, ops)) return -1;
+    const int64_t L = int64_t(1) << (is_dm ? 2 * n : n);
+    std::vector<cd> y(x, x + L), delta(delta_y, delta_y + L);
+    orc_apply_circuit(n, is_dm, y.data(), nops, ops, pool);
+    if (y_out) std::copy(y.begin(), y.end(), y_out);
+    if (is_dm) dm_back_propagate(n, nops, ops, pool, nparams, y.data(), delta.data(), grad);
+    else sv_back_propagate(n, nops, ops, pool, nparams, y.data(), delta.data(), grad);
+    if (delta_x) std::copy(delta.begin(), delta.end(), delta_x);
     return 0;
 }
 

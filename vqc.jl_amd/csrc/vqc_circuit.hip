@@ -1021,10 +1021,13 @@ static int max_small_rows(int nbits) {
     return used >= 160 * 1024 ? 0 : int((160 * 1024 - used) / 16);
 }
 
-extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_t* ops, const double* pool, int nterms,
-                                  const vqc_term_t* terms, const double* hpool, int nparams, double* energy2, double* grad,
-                                  vqc_state* psi_out, int flags) {
-    VQC_API_LOCK;
+// The state-vector adjoint sweep. The reverse buffer B starts as
+// (H + H^dagger) psi (the energy's cotangent) or, when `seed` is given (no
+// terms then), as a copy of it; `seed_out` (may be null) receives B after the
+// reverse sweep: the cotangent on psi0.
+static int sv_adjoint(const vqc_state* psi0, int nops, const vqc_op_t* ops, const double* pool, int nterms,
+                      const vqc_term_t* terms, const double* hpool, int nparams, double* energy2, double* grad,
+                      vqc_state* psi_out, int flags, const vqc_state* seed, vqc_state* seed_out) {
     VQC_TRY(vqc_check_qubits(psi0, "vqc_sv_energy_grad"));
     if (psi0->is_dm) return vqc_fail(VQC_EINVAL, "vqc_sv_energy_grad: state is a density matrix");
     if (nparams < 0 || (nparams && !grad)) return vqc_fail(VQC_EINVAL, "bad gradient output");
@@ -1044,6 +1047,7 @@ extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_
     double2* B = (double2*)g_wB.buf;
     double2* fwd_out = psi_out ? psi_out->d : nullptr;
     if (flags & VQC_FLAG_UNFUSED) {
+        if (seed) return vqc_fail(VQC_EUNSUPPORTED, "vqc_circuit_vjp runs on the fused path only (flags must be 0)");
         VQC_TRY(sv_energy_grad_plain(psi0, nops, ops, pool, plen, nterms, terms, hpool, nparams, energy2, grad, A, B, fwd_out, st));
     } else {
         const TileTuning tt = vqc_tile_tuning();
@@ -1052,7 +1056,7 @@ extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_
         xp.h.insert(xp.h.end(), hpool, hpool + 2 * hlen);
         PlanBuilder pb;
         pb.xp = &xp;
-        const bool small = nb <= tt.t_small;
+        const bool small = nb <= tt.t_small && !seed;
         if (small) {
             VQC_TRY(vqc_plan_forward(pb, nb, nb, 0, nops, ops, nb, 0));
             const size_t nf = pb.phases.size();
@@ -1112,11 +1116,13 @@ extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_
             if (nfwd == 0) VQC_HIP(hipMemcpyAsync(A, psi0->d, L * 16, hipMemcpyDeviceToDevice, st));
             else VQC_TRY(run_forward(pb, 0, nfwd, psi0->d, A, dm, dp, (const PhaseDesc*)g_wphases.buf, st));
             if (fwd_out) VQC_HIP(hipMemcpyAsync(fwd_out, A, L * 16, hipMemcpyDeviceToDevice, st));
-            if (nham_end == nfwd) vqc_zero(B, L, st);
+            if (seed) VQC_HIP(hipMemcpyAsync(B, seed->d, L * 16, hipMemcpyDeviceToDevice, st));
+            else if (nham_end == nfwd) vqc_zero(B, L, st);
             for (size_t i = nfwd; i < pb.phases.size(); ++i) {
                 PhaseBufs b{A, A, B, dm, dp, parts, stride};
                 VQC_TRY(launch_phase(pb, i, (const PhaseDesc*)g_wphases.buf, b, phase_grid(pb.phases[i], stride), st));
             }
+            if (seed_out) VQC_HIP(hipMemcpyAsync(seed_out->d, B, L * 16, hipMemcpyDeviceToDevice, st));
             if (pb.nrows) vqc_reduce_partials(parts, pb.nrows, stride, rowsum, st);
             VQC_HIP(hipGetLastError());
             std::vector<double> rows(2 * size_t(nrows));
@@ -1126,6 +1132,13 @@ extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_
         }
     }
     return VQC_OK;
+}
+
+extern "C" int vqc_sv_energy_grad(const vqc_state* psi0, int nops, const vqc_op_t* ops, const double* pool, int nterms,
+                                  const vqc_term_t* terms, const double* hpool, int nparams, double* energy2, double* grad,
+                                  vqc_state* psi_out, int flags) {
+    VQC_API_LOCK;
+    return sv_adjoint(psi0, nops, ops, pool, nterms, terms, hpool, nparams, energy2, grad, psi_out, flags, nullptr, nullptr);
 }
 
 // ------------------------------- density matrices ---------------------------
@@ -1171,10 +1184,11 @@ static int dm_seed(double2* Dl, int n, const vqc_term_t& t, const double2* M, hi
 
 int vqc_dm_gate_sweep(double2* v, int n, int k, const int32_t* pos, const double* U, hipStream_t st);
 
-extern "C" int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_t* ops, const double* pool, int nterms,
-                                  const vqc_term_t* terms, const double* hpool, int nparams, double* energy2, double* grad,
-                                  vqc_state* rho_out, int flags) {
-    VQC_API_LOCK;
+// The density-matrix adjoint sweep; `seed` / `seed_out` as for sv_adjoint
+// (the seed replaces Delta = H^dagger).
+static int dm_adjoint(const vqc_state* rho0, int nops, const vqc_op_t* ops, const double* pool, int nterms,
+                      const vqc_term_t* terms, const double* hpool, int nparams, double* energy2, double* grad,
+                      vqc_state* rho_out, int flags, const vqc_state* seed, vqc_state* seed_out) {
     VQC_TRY(vqc_check_state(rho0, "vqc_dm_energy_grad"));
     if (!rho0->is_dm) return vqc_fail(VQC_EINVAL, "vqc_dm_energy_grad: state is not a density matrix");
     if (nparams < 0 || (nparams && !grad)) return vqc_fail(VQC_EINVAL, "bad gradient output");
@@ -1258,8 +1272,12 @@ extern "C" int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_
     unsigned grid;
     for (int t = 0; t < nterms; ++t)
         VQC_TRY(vqc_dm_expec_sweep(A, n, terms[t].nq, terms[t].pos, dp + plen + terms[t].mat, parts + size_t(t) * stride, &grid, st));
-    vqc_zero(B, L, st);
-    for (int t = 0; t < nterms; ++t) VQC_TRY(dm_seed(B, n, terms[t], dp + plen + terms[t].mat, st));
+    if (seed) {
+        VQC_HIP(hipMemcpyAsync(B, seed->d, L * 16, hipMemcpyDeviceToDevice, st));
+    } else {
+        vqc_zero(B, L, st);
+        for (int t = 0; t < nterms; ++t) VQC_TRY(dm_seed(B, n, terms[t], dp + plen + terms[t].mat, st));
+    }
     // reverse
     if (plain) {
         int row = nterms;
@@ -1302,6 +1320,7 @@ extern "C" int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_
             VQC_TRY(launch_phase(pb, i, (const PhaseDesc*)g_wphases.buf, b, phase_grid(pb.phases[i], stride / 2), st));
         }
     }
+    if (seed_out) VQC_HIP(hipMemcpyAsync(seed_out->d, B, L * 16, hipMemcpyDeviceToDevice, st));
     if (pb.nrows) vqc_reduce_partials(parts, pb.nrows, stride, rowsum, st);
     VQC_HIP(hipGetLastError());
     std::vector<double> rows(2 * size_t(nrows));
@@ -1309,6 +1328,25 @@ extern "C" int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_
     else VQC_HIP(hipStreamSynchronize(st));
     finish_rows(rows, pb.row_slot, nparams, energy2, grad);
     return VQC_OK;
+}
+
+extern "C" int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_t* ops, const double* pool, int nterms,
+                                  const vqc_term_t* terms, const double* hpool, int nparams, double* energy2, double* grad,
+                                  vqc_state* rho_out, int flags) {
+    VQC_API_LOCK;
+    return dm_adjoint(rho0, nops, ops, pool, nterms, terms, hpool, nparams, energy2, grad, rho_out, flags, nullptr, nullptr);
+}
+
+extern "C" int vqc_circuit_vjp(const vqc_state* x, int nops, const vqc_op_t* ops, const double* pool, int nparams,
+                               const vqc_state* delta_y, double* grad, vqc_state* delta_x, vqc_state* y_out, int flags) {
+    VQC_API_LOCK;
+    VQC_TRY(vqc_check_state(x, "vqc_circuit_vjp"));
+    VQC_TRY(vqc_check_state(delta_y, "vqc_circuit_vjp"));
+    if (delta_y->nbits != x->nbits || delta_y->is_dm != x->is_dm || (delta_x && (delta_x->nbits != x->nbits || delta_x->is_dm != x->is_dm)))
+        return vqc_fail(VQC_EINVAL, "vqc_circuit_vjp: delta_y and delta_x must have the shape of x");
+    double e2[2];
+    if (x->is_dm) return dm_adjoint(x, nops, ops, pool, 0, nullptr, nullptr, nparams, e2, grad, y_out, flags, delta_y, delta_x);
+    return sv_adjoint(x, nops, ops, pool, 0, nullptr, nullptr, nparams, e2, grad, y_out, flags, delta_y, delta_x);
 }
 
 // ------------------------- one evaluation on a caller's stream ---------------

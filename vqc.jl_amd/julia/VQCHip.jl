@@ -255,6 +255,30 @@ function ChainRulesCore.rrule(::typeof(energy), h::QubitsOperator, circuit::QCir
     return real(E), pullback
 end
 
+# The pullback of y = circuit * x for any loss: replaces @adjoint *(circuit, x)
+# with back_propagate (circuitdiff.jl:2-8, 23-78). Returns the circuit's flat
+# gradient (active parameters in circuit order, as the reference's pullback does)
+# and the cotangent on x, circuit' * Δy, as a GPUState.
+function circuit_vjp(circuit::QCircuit, x::GPUState, Δy::GPUState)
+    (Δy.nqubits == x.nqubits && Δy.isdm == x.isdm) || throw(DimensionMismatch("Δy must have the shape of x"))
+    ops, pool, np = pack(circuit, x.isdm)
+    g = zeros(Float64, max(np, 1))
+    Δx = GPUState(x.nqubits, x.isdm)
+    check(ccall((:vqc_circuit_vjp, libvqc), Cint,
+                (Ptr{Cvoid}, Cint, Ptr{COp}, Ptr{ComplexF64}, Cint, Ptr{Cvoid}, Ptr{Float64}, Ptr{Cvoid}, Ptr{Cvoid}, Cint),
+                x.h, length(ops), ops, pool, np, Δy.h, g, Δx.h, C_NULL, 0))
+    return g[1:np], Δx
+end
+
+function ChainRulesCore.rrule(::typeof(*), circuit::QCircuit, x::GPUState)
+    y = circuit * x
+    pullback(Δy::GPUState) = begin
+        g, Δx = circuit_vjp(circuit, x, Δy)
+        (NoTangent(), g, Δx)
+    end
+    return y, pullback
+end
+
 # ---- batches and serving (no counterpart in the reference) ---------------------
 struct CInstance
     nqubits::Int32

@@ -140,6 +140,7 @@ SIGNATURES = {
     "vqc_apply_circuit": [_P, _I, ctypes.POINTER(Op), _D],
     "vqc_plan_info": [ctypes.POINTER(PlanInfo)],
     "vqc_sv_energy_grad": [_P, _I, ctypes.POINTER(Op), _D, _I, ctypes.POINTER(Term), _D, _I, _D, _D, _P, _I],
+    "vqc_circuit_vjp": [_P, _I, ctypes.POINTER(Op), _D, _I, _P, _D, _P, _P, _I],
     "vqc_dm_energy_grad": [_P, _I, ctypes.POINTER(Op), _D, _I, ctypes.POINTER(Term), _D, _I, _D, _D, _P, _I],
     "vqc_batch_create": [_I, ctypes.POINTER(Instance), ctypes.POINTER(Op), ctypes.c_int64, ctypes.POINTER(Term),
                          ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _I, ctypes.POINTER(_P)],

@@ -142,6 +142,27 @@ QubitsOperator.__mul__ = _qop_mul
 QubitsTerm.__mul__ = _qop_mul
 
 
+def circuit_vjp(circuit, x, delta_y, return_state=False):
+    """The pullback of y = circuit * x (@adjoint *(circuit, x), circuitdiff.jl:2-8,
+    with back_propagate, :23-78) for the cotangent delta_y on y (a state of x's
+    kind): (grad, delta_x[, y]). grad holds d loss / d theta for the circuit's
+    active parameters in forward order; delta_x is circuit^dagger delta_y, the
+    cotangent on x."""
+    dm = bool(x.is_dm)
+    if bool(delta_y.is_dm) != dm or delta_y.nqubits() != x.nqubits():
+        raise ValueError("delta_y must have the shape of x")
+    prog = Program(circuit, n=x.nqubits(), dm=dm)
+    pool = prog.pool()
+    grad = np.zeros(max(prog.nparams, 1))
+    new = (lambda: DensityMatrix(x.nqubits())) if dm else (lambda: StateVector(x.nqubits()))
+    dx = new()
+    y = new() if return_state else None
+    _lib.call("vqc_circuit_vjp", x.handle, len(prog.ops), prog.op_array(), _lib.dptr(pool), prog.nparams, delta_y.handle,
+              _lib.dptr(grad), dx.handle, y.handle if y else None, 0)
+    res = (grad[: prog.nparams], dx)
+    return res + (y,) if return_state else res
+
+
 def energy_and_gradient(H, circuit, state0, fused=True, return_state=False):
     """E = expectation(H, circuit * state0) and dRe(E)/dtheta for every active
     parameter of the circuit, in forward order (circuitdiff.jl:61-75)."""
