@@ -61,3 +61,63 @@ def test_vjp_rejects_mismatched_shapes():
     c = sv_circuit(5, np.random.default_rng(1))
     with pytest.raises(ValueError):
         vq.circuit_vjp(c, vq.StateVector(5), vq.StateVector(4))
+
+
+def _dense(op, n):
+    from brute import full_op
+    M = np.zeros((1 << n, 1 << n), dtype=complex)
+    for t in op.terms:
+        M += full_op(n, [p - 1 for p in t.positions], t.matrix())
+    return M
+
+
+def _ops(L):
+    return [vq.QubitsOperator([vq.QubitsTerm({1: "+", L - 1: "Y"}, coeff=0.36)]),
+            vq.QubitsOperator([vq.QubitsTerm({1: "+", L - 1: "-"}, coeff=0.77), vq.QubitsTerm({1: "-", L - 1: "+"}, coeff=0.77),
+                               vq.QubitsTerm({1: "X", L: "Y"}, coeff=0.36)])]
+
+
+@pytest.mark.parametrize("L", [3, 4, 6, 7])
+def test_expectation_vjp_matches_finite_differences(L):
+    # loss(x) = abs(expectation(m, x)) as in test/check_ham_expec_diff.jl
+    rng = np.random.default_rng(600 + L)
+    ops = _ops(L)
+    if L == 7:  # a 7-site Pauli string takes the Pauli route (more than 5 sites)
+        ops.append(vq.QubitsOperator([vq.QubitsTerm({i: "X" for i in range(1, 8)}, coeff=0.77)]))
+    for op in ops:
+        x = rand_state(L, rng)
+        s = vq.StateVector(x)
+        E = vq.expectation(op, s)
+        dx = vq.expectation_vjp(op, s, E / abs(E)).storage()
+        M = _dense(op, L)
+        loss = lambda v: abs(np.vdot(v, M @ v))
+        fd = np.zeros(1 << L, dtype=complex)
+        for k in range(1 << L):
+            e = np.zeros(1 << L)
+            e[k] = 1e-6
+            fd[k] = complex((loss(x + e) - loss(x - e)) / 2e-6, (loss(x + 1j * e) - loss(x - 1j * e)) / 2e-6)
+        assert abs(E - np.vdot(x, M @ x)) < 1e-12
+        assert np.max(np.abs(dx - fd)) < 1e-7
+
+
+def test_expectation_vjp_density_matrix():
+    n = 3
+    op = _ops(n)[1]
+    rho = vq.DensityMatrix(vec(rand_density(n, np.random.default_rng(7))))
+    z = 0.3 - 0.8j
+    d = vq.expectation_vjp(op, rho, z).storage()
+    assert np.max(np.abs(d - z * _dense(op, n).conj().T)) < 1e-13
+
+
+def test_loss_chain_through_expectation_and_circuit():
+    # Zygote's chain for real(expectation(H, circuit * x)): the expectation pullback
+    # with z = 1 feeds the circuit pullback; equals the fused energy gradient
+    n = 9
+    rng = np.random.default_rng(700)
+    c = sv_circuit(n, rng)
+    H = tfim(n, 0.6)
+    s = vq.StateVector(rand_state(n, rng))
+    E, g, y = vq.energy_and_gradient(H, c, s, return_state=True)
+    dy = vq.expectation_vjp(H, y, 1.0)
+    gv, dx = vq.circuit_vjp(c, s, dy)
+    assert np.max(np.abs(gv - g)) < 1e-10 * max(1.0, np.max(np.abs(g)))

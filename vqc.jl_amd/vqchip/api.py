@@ -142,6 +142,26 @@ QubitsOperator.__mul__ = _qop_mul
 QubitsTerm.__mul__ = _qop_mul
 
 
+def expectation_vjp(H, state, z=1.0):
+    """The pullback of expectation(H, state) for the cotangent z of its value
+    (@adjoint expectation, expecdiff.jl:4-68): (conj(z) H + z H^dagger) state for a
+    state vector; z H^dagger as vec, i.e. H^dagger applied to the identity on the
+    row bits, for a density matrix. A new state of the same kind."""
+    H = _as_operator(H)
+    z = complex(z)
+    n = state.nqubits()
+    if state.is_dm:
+        ident = DensityMatrix(np.eye(1 << n, dtype=np.complex128))
+        out = apply_operator(H.adjoint(), ident)
+        out.scale_(z)
+        return out
+    out = apply_operator(H, state)
+    out.scale_(np.conj(z))
+    hd = apply_operator(H.adjoint(), state)
+    _lib.call("vqc_state_axpy", out.handle, z.real, z.imag, hd.handle)
+    return out
+
+
 def circuit_vjp(circuit, x, delta_y, return_state=False):
     """The pullback of y = circuit * x (@adjoint *(circuit, x), circuitdiff.jl:2-8,
     with back_propagate, :23-78) for the cotangent delta_y on y (a state of x's

@@ -50,6 +50,8 @@ class QubitsTerm:
         return all(l in ("X", "Y", "Z", "I") for l in self.labels)
 
     def adjoint(self):
+        if self.is_pauli():  # Pauli strings are Hermitian: keep the labels (the Pauli-string route needs them)
+            return QubitsTerm(dict(zip(self.positions, self.labels)), np.conj(self.coeff))
         return QubitsTerm(list(zip(self.positions, [o.conj().T for o in self.oplist])), np.conj(self.coeff))
 
 
