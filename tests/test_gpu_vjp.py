@@ -121,3 +121,47 @@ def test_loss_chain_through_expectation_and_circuit():
     dy = vq.expectation_vjp(H, y, 1.0)
     gv, dx = vq.circuit_vjp(c, s, dy)
     assert np.max(np.abs(gv - g)) < 1e-10 * max(1.0, np.max(np.abs(g)))
+
+
+@pytest.mark.parametrize("loss", ["real", "imag", "abs", "abs2", "distance"])
+def test_circuit_gradients_of_the_reference_losses(loss):
+    # test/circuitgrad.jl and crxgategrad.jl: loss(circuit * |0..0>) for a
+    # variational circuit with FSIM and CRx layers; the loss's cotangent is formed
+    # from the device output, the gradient comes from the device pullback and is
+    # checked against 5-point differences of the brute-force loss
+    from test_vjp import FD_H, FD_W, circuit_matrix
+    from vqchip.utility import variational_circuit_1d
+    L, depth = 5, 2
+    rng = np.random.default_rng(800)
+    c = variational_circuit_1d(L, depth, rng=rng)
+    c.insert(4, vq.FSIMGate(2, 4, 0.4, 0.9))
+    c.append(vq.CRxGate(1, L, 1.1))
+    t = rand_state(L, rng)
+    x0 = np.zeros(1 << L, complex)
+    x0[0] = 1
+
+    def value(y):
+        z = np.vdot(t, y)
+        if loss == "distance":
+            return np.sqrt(abs(np.vdot(t, t).real + np.vdot(y, y).real - 2 * z.real))
+        return {"real": z.real, "imag": z.imag, "abs": abs(z), "abs2": abs(z) ** 2}[loss]
+
+    x = vq.StateVector(x0)
+    ydev = vq.apply_(c, vq.StateVector(x0))
+    yv = ydev.storage()
+    z = np.vdot(t, yv)
+    delta = {"real": t, "imag": 1j * t, "abs": z / abs(z) * t, "abs2": 2 * z * t,
+             "distance": (yv - t) / value(yv)}[loss]
+    g, _ = vq.circuit_vjp(c, x, vq.StateVector(delta))
+    theta = np.array(c.active_parameters())
+    assert len(g) == len(theta)
+    for k in range(len(theta)):
+        fd = 0.0
+        for w, s in zip(FD_W, (-2, -1, 0, 1, 2)):
+            if w:
+                th = theta.copy()
+                th[k] += s * FD_H
+                c.reset_parameters(th)
+                fd += w * value(circuit_matrix(c, L) @ x0) / FD_H
+        c.reset_parameters(theta)
+        assert abs(fd - g[k]) < 1e-8, (loss, k, fd, g[k])
