@@ -165,3 +165,20 @@ def test_circuit_gradients_of_the_reference_losses(loss):
                 fd += w * value(circuit_matrix(c, L) @ x0) / FD_H
         c.reset_parameters(theta)
         assert abs(fd - g[k]) < 1e-8, (loss, k, fd, g[k])
+
+
+def test_qubit_encoding_gradient():
+    # test/check_ham_expec_diff.jl check_mps_ham_expec_grad:
+    # loss(v) = real(expectation(Z_1 Z_3, qubit_encoding(v)))
+    from vqchip.state import qubit_vector
+    L = 5
+    v = np.random.default_rng(900).standard_normal(L)
+    obs = vq.QubitsOperator([vq.QubitsTerm({1: "Z", 3: "Z"})])
+    s = vq.qubit_encoding(v)
+    g = vq.qubit_encoding_vjp(v, vq.expectation_vjp(obs, s, 1.0))
+    M = _dense(obs, L)
+    loss = lambda w: np.vdot(qubit_vector(w), M @ qubit_vector(w)).real
+    for k in range(L):
+        e = np.zeros(L)
+        e[k] = 1e-6
+        assert abs((loss(v + e) - loss(v - e)) / 2e-6 - g[k]) < 1e-8

@@ -2,7 +2,7 @@
 libvqc_hip.so, the MI355X (gfx950) kernels behind include/vqc_hip.h."""
 from . import _lib
 from ._lib import VQCError, load
-from .api import apply_, apply_operator, circuit_vjp, energy_and_gradient, expectation, expectation_vjp
+from .api import apply_, apply_operator, circuit_vjp, energy_and_gradient, expectation, expectation_vjp, qubit_encoding_vjp
 from .measure import measure, measure_, post_select, post_select_, probability_zero
 from .circuit import Program, QCircuit
 from .gates import (CONTROL, R, AmplitudeDamping, CCPHASEGate, CNOTGate, CPHASEGate, CRxGate, CRyGate, CRzGate, CZGate,

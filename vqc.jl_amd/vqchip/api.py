@@ -162,6 +162,18 @@ def expectation_vjp(H, state, z=1.0):
     return out
 
 
+def qubit_encoding_vjp(thetas, delta):
+    """The pullback of qubit_encoding(thetas) for a cotangent delta on the encoded
+    state (@adjoint qubit_encoding, circuitdiff.jl:10-17): the encoding is
+    Ry(pi theta_i) on every qubit of |0..0>, so the gradient is pi times that
+    circuit's parameter gradient."""
+    from .gates import RyGate
+    thetas = [float(t) for t in thetas]
+    c = QCircuit(RyGate(i + 1, np.pi * t) for i, t in enumerate(thetas))
+    g, _ = circuit_vjp(c, StateVector(len(thetas)), delta)
+    return g * np.pi
+
+
 def circuit_vjp(circuit, x, delta_y, return_state=False):
     """The pullback of y = circuit * x (@adjoint *(circuit, x), circuitdiff.jl:2-8,
     with back_propagate, :23-78) for the cotangent delta_y on y (a state of x's
