@@ -322,3 +322,27 @@ def test_errors_are_reported_not_crashes():
         _lib.call("vqc_dm_apply_superop", d.handle, 1, _lib.i32ptr([1]), _lib.dptr(S0), 3)
     with pytest.raises(vq.VQCError, match="density matrix"):
         _lib.call("vqc_dm_apply_superop", s.handle, 1, _lib.i32ptr([1]), _lib.dptr(S0), 0)
+
+
+@pytest.mark.parametrize("n,reverse_ladder", [(12, False), (12, True), (16, False), (16, True)])
+def test_ansatz_block_shapes_energy_gradient(n, reverse_ladder):
+    # Rx Ry Rz on every qubit, then a CNOT ladder: the reverse sweep runs the
+    # straight-line block for CNOT^dagger + Rz/Ry/Rx^dagger on the CNOT's target.
+    # CNOT(i, i+1) ascending puts the target on the pair's high bit; CNOT(i+1, i)
+    # descending puts it on the low bit (the mirrored shape).
+    rng = np.random.default_rng(1200 + n + reverse_ladder)
+    c = vq.QCircuit()
+    for _ in range(3):
+        for q in range(1, n + 1):
+            c.append(vq.RxGate(q, float(rng.uniform(0, 6.3))))
+            c.append(vq.RyGate(q, float(rng.uniform(0, 6.3))))
+            c.append(vq.RzGate(q, float(rng.uniform(0, 6.3))))
+        ladder = [(q + 1, q) for q in range(n - 1, 0, -1)] if reverse_ladder else [(q, q + 1) for q in range(1, n)]
+        for a, b in ladder:
+            c.append(vq.CNOTGate(a, b))
+    H = heisenberg(n, rng.standard_normal(n - 1))
+    psi0 = rand_state(n, rng)
+    E, g, psi, Er, gr, psi_r = _sv_energy_grad(c, H, n, psi0, True)
+    assert np.max(np.abs(psi - psi_r)) < 1e-12
+    assert abs(E - Er) < 1e-10 * max(1, abs(Er))
+    assert np.max(np.abs(g - gr)) < 1e-10 * max(1, np.max(np.abs(gr)))

@@ -682,7 +682,7 @@ struct PhaseBufs {
 };
 
 template <int MAXKB, int KS>
-__global__ void __launch_bounds__(kTileThreads) k_phase(const PhaseDesc* __restrict__ pd, PhaseBufs b) {
+__global__ void __launch_bounds__(kTileThreads, MAXKB <= 3 ? 4 : 1) k_phase(const PhaseDesc* __restrict__ pd, PhaseBufs b) {
     extern __shared__ double2 smem[];
     const PhaseDesc& p = *pd;
     const uint32_t T = 1u << p.t;

@@ -559,6 +559,91 @@ __device__ __forceinline__ void mop_block(double2* __restrict__ sa, double2* __r
     }
 }
 
+// The reverse block shape that hardware-efficient ansatz layers (Rx Ry Rz on
+// each qubit, then a CNOT ladder) produce almost exclusively (440 of 500 reverse
+// micro-ops at 24 qubits), as straight-line code: CNOT^dagger, then Rz^dagger,
+// Ry^dagger, Rx^dagger on the CNOT's target, each with its gradient (diagonal,
+// real, X-like classes). CXLO: the control is the pair's low bit. The forward
+// counterpart (CNOT, then a fused 2x2) gained 3% on such circuits and cost 3% on
+// others, so forward blocks keep the general code.
+constexpr int kHeaRevSmask = (BS_HI << 2) | (BS_HI << 4) | (BS_HI << 6);  // BOTH, then the target three times
+constexpr int kHeaRevSmaskHi = (BS_LO << 2) | (BS_LO << 4) | (BS_LO << 6);
+constexpr int kHeaRevMstr = MS_CX_LO | ((MS_DIAG | MS_DIAG << 2) << 4) | ((MS_REAL | MS_REAL << 2) << 8) |
+                            ((MS_XLIKE | MS_XLIKE << 2) << 12);
+constexpr int kHeaRevMstrHi = MS_CX_HI | (kHeaRevMstr & ~3);
+
+__device__ __forceinline__ int hea_rev_shape(const MicroOp& op) {
+    if (op.nchain != 4 || op.cmat[0] < 0 || op.gen[0] >= 0 || op.gen[1] < 0 || op.gen[2] < 0 || op.gen[3] < 0) return 0;
+    if (op.smask == kHeaRevSmask && op.mstr == kHeaRevMstr) return 1;
+    if (op.smask == kHeaRevSmaskHi && op.mstr == kHeaRevMstrHi) return 2;
+    return 0;
+}
+
+// one rotation step of the reverse shape on the pairs (p0, p1), (q0, q1) of A
+// and B: the matrix, then Re sum conj(b) (G a)
+template <int MS>
+__device__ __forceinline__ double hea_step(const double2* __restrict__ pool, int64_t mat, int64_t gen, double2& p0, double2& p1,
+                                           double2& q0, double2& q1, double2& r0, double2& r1, double2& s0, double2& s1) {
+    const Mat2 m = load_mat2(pool + mat);
+    m2_ip<MS>(m, p0, p1);
+    m2_ip<MS>(m, q0, q1);
+    m2_ip<MS>(m, r0, r1);
+    m2_ip<MS>(m, s0, s1);
+    const Mat2 g = load_mat2(pool + gen);
+    return gdot2<MS>(g, p0, p1, r0, r1) + gdot2<MS>(g, q0, q1, s0, s1);
+}
+
+template <bool CXLO>
+__device__ __forceinline__ void mop_hea_rev(double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
+                                            const double2* __restrict__ pool, double2* red, int buf) {
+    double g1 = 0.0, g2 = 0.0, g3 = 0.0;
+    const uint32_t ng = 1u << (t - 2);
+    const uint32_t olo = 1u << op.lpos[0], ohi = 1u << op.lpos[1];
+    const int64_t m1 = op.cmat[1], m2 = op.cmat[2], m3 = op.cmat[3], e1 = op.gen[1], e2 = op.gen[2], e3 = op.gen[3];
+    for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
+        const uint32_t base = ins32<2>(g, op.lpos);
+        // x00, x01, x10, x11 over (lo, hi) after the CNOT: the control's set half
+        // has its target flipped, i.e. two registers trade names
+        const uint32_t i00 = base, i10 = base + olo, i01 = base + ohi, i11 = base + olo + ohi;
+        double2 a00 = sa[i00], a10, a01, a11, b00 = sb[i00], b10, b01, b11;
+        if (CXLO) {  // control lo, target hi: (lo=1, hi=0) <-> (lo=1, hi=1)
+            a10 = sa[i11];
+            a11 = sa[i10];
+            a01 = sa[i01];
+            b10 = sb[i11];
+            b11 = sb[i10];
+            b01 = sb[i01];
+        } else {  // control hi, target lo: (lo=0, hi=1) <-> (lo=1, hi=1)
+            a01 = sa[i11];
+            a11 = sa[i01];
+            a10 = sa[i10];
+            b01 = sb[i11];
+            b11 = sb[i01];
+            b10 = sb[i10];
+        }
+        if (CXLO) {  // rotations on the hi bit: pairs (00, 01), (10, 11)
+            g1 += hea_step<MS_DIAG>(pool, m1, e1, a00, a01, a10, a11, b00, b01, b10, b11);
+            g2 += hea_step<MS_REAL>(pool, m2, e2, a00, a01, a10, a11, b00, b01, b10, b11);
+            g3 += hea_step<MS_XLIKE>(pool, m3, e3, a00, a01, a10, a11, b00, b01, b10, b11);
+        } else {  // rotations on the lo bit: pairs (00, 10), (01, 11)
+            g1 += hea_step<MS_DIAG>(pool, m1, e1, a00, a10, a01, a11, b00, b10, b01, b11);
+            g2 += hea_step<MS_REAL>(pool, m2, e2, a00, a10, a01, a11, b00, b10, b01, b11);
+            g3 += hea_step<MS_XLIKE>(pool, m3, e3, a00, a10, a01, a11, b00, b10, b01, b11);
+        }
+        sa[i00] = a00;
+        sa[i10] = a10;
+        sa[i01] = a01;
+        sa[i11] = a11;
+        sb[i00] = b00;
+        sb[i10] = b10;
+        sb[i01] = b01;
+        sb[i11] = b11;
+    }
+    wave_partial_re(red, buf, 0, g1);
+    wave_partial_re(red, buf, 1, g2);
+    wave_partial_re(red, buf, 2, g3);
+}
+
 // MK_HAM: B += Dm A with Dm = M + M^dagger (op.mat2); acc += sum conj(A) (M A)
 template <int KB>
 __device__ __forceinline__ void mop_ham(const double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
@@ -644,7 +729,14 @@ __device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const M
             if constexpr ((KS & (1 << MK_BLOCK)) != 0) {
                 // forward micro-ops act on A alone; reverse ones on A and B
                 constexpr bool two = KS != KS_FWD;
-                mop_block(sa, sb, t, op, pool, red, buf, two);
+                if constexpr (KS == KS_REV) {  // state-vector reverse: the ansatz shape first
+                    const int shape = hea_rev_shape(op);
+                    if (shape == 1) mop_hea_rev<true>(sa, sb, t, op, pool, red, buf);
+                    else if (shape == 2) mop_hea_rev<false>(sa, sb, t, op, pool, red, buf);
+                    else mop_block(sa, sb, t, op, pool, red, buf, two);
+                } else {
+                    mop_block(sa, sb, t, op, pool, red, buf, two);
+                }
                 nred = two ? op.ng : 0;
             }
             break;
