@@ -802,8 +802,8 @@ int vqc_max_kb(const std::vector<MicroOp>& mops, int begin, int end) {
 }
 
 #define VQC_FOR_KERNELS(X)                                                                                     \
-    X(2, KS_FWD) X(2, KS_HAM) X(2, KS_REV) X(2, KS_REVX) X(3, KS_FWD) X(3, KS_HAM) X(3, KS_REV) X(3, KS_REVX) \
-    X(5, KS_FWD) X(5, KS_HAM) X(5, KS_REV) X(5, KS_REVX)
+    X(2, KS_FWD) X(2, KS_FWDH) X(2, KS_HAM) X(2, KS_REV) X(2, KS_REVX) X(3, KS_FWD) X(3, KS_FWDH) X(3, KS_HAM)     \
+    X(3, KS_REV) X(3, KS_REVX) X(5, KS_FWD) X(5, KS_HAM) X(5, KS_REV) X(5, KS_REVX)
 
 static int set_lds_attrs() {
     static bool done = false;
@@ -822,7 +822,18 @@ static int set_lds_attrs() {
 // kind set of the micro-ops of phase i
 int vqc_phase_kindset(const PlanBuilder& pb, size_t i) {
     const PhaseDesc& p = pb.phases[i];
-    if (p.kind == PH_FWD) return KS_FWD;
+    if (p.kind == PH_FWD) {
+        // the ansatz-shape kernel when a block of that shape is present (and the
+        // phase needs no 4- or 5-bit micro-op kernel)
+        if (vqc_max_kb(pb.mops, p.op_begin, p.op_end) > 3) return KS_FWD;
+        for (int o = p.op_begin; o < p.op_end; ++o) {
+            const MicroOp& m = pb.mops[o];
+            if (m.kind == MK_BLOCK && m.nchain == 2 && m.cmat[0] >= 0 && m.cmat[1] >= 0 &&
+                ((m.smask == kHeaFwdSmask && m.mstr == kHeaFwdMstr) || (m.smask == kHeaFwdSmaskHi && m.mstr == kHeaFwdMstrHi)))
+                return KS_FWDH;
+        }
+        return KS_FWD;
+    }
     if (p.kind == PH_HAM) return KS_HAM;
     for (int o = p.op_begin; o < p.op_end; ++o)
         if (!vqc_rev_simple(pb.mops[o])) return KS_REVX;
@@ -837,10 +848,15 @@ static int launch_phase(const PlanBuilder& pb, size_t i, const PhaseDesc* dpd, c
     const int mkb = vqc_max_kb(pb.mops, p.op_begin, p.op_end);
     const int mk = mkb <= 2 ? 2 : (mkb <= 3 ? 3 : 5);
     const int ks = vqc_phase_kindset(pb, i);
-#define VQC_LAUNCH(M, K) \
-    if (mk == M && ks == K) hipLaunchKernelGGL((k_phase<M, K>), dim3(grid), dim3(kTileThreads), lds, st, dpd + i, b);
+    bool launched = false;
+#define VQC_LAUNCH(M, K)                                                                           \
+    if (mk == M && ks == K) {                                                                      \
+        hipLaunchKernelGGL((k_phase<M, K>), dim3(grid), dim3(kTileThreads), lds, st, dpd + i, b); \
+        launched = true;                                                                           \
+    }
     VQC_FOR_KERNELS(VQC_LAUNCH)
 #undef VQC_LAUNCH
+    if (!launched) return vqc_fail(VQC_EINVAL, "no tile kernel for micro-op width %d and kind set %d", mk, ks);
     VQC_HIP(hipGetLastError());
     return VQC_OK;
 }

@@ -47,6 +47,9 @@ enum {
     KS_HAM = 1 << MK_HAM,
     KS_REV = (1 << MK_APPLY2) | (1 << MK_CHAIN) | (1 << MK_BLOCK),  // state-vector reverse
     KS_REVX = (1 << MK_APPLY) | (1 << MK_APPLY2) | (1 << MK_APPLY2X) | (1 << MK_GRAD) | (1 << MK_CHAIN) | (1 << MK_BLOCK),
+    // forward, with the straight-line block for the ansatz shape (chosen per
+    // phase by the host, so other forward phases keep the plain forward kernel)
+    KS_FWDH = KS_FWD | (1 << 7),
 };
 
 constexpr int kTileThreads = 512;
@@ -563,19 +566,29 @@ __device__ __forceinline__ void mop_block(double2* __restrict__ sa, double2* __r
 // each qubit, then a CNOT ladder) produce almost exclusively (440 of 500 reverse
 // micro-ops at 24 qubits), as straight-line code: CNOT^dagger, then Rz^dagger,
 // Ry^dagger, Rx^dagger on the CNOT's target, each with its gradient (diagonal,
-// real, X-like classes). CXLO: the control is the pair's low bit. The forward
-// counterpart (CNOT, then a fused 2x2) gained 3% on such circuits and cost 3% on
-// others, so forward blocks keep the general code.
+// real, X-like classes), and its forward counterpart, a CNOT followed by a fused
+// 2x2 on its control (KS_FWDH kernels only). CXLO: the control is the pair's
+// low bit.
 constexpr int kHeaRevSmask = (BS_HI << 2) | (BS_HI << 4) | (BS_HI << 6);  // BOTH, then the target three times
 constexpr int kHeaRevSmaskHi = (BS_LO << 2) | (BS_LO << 4) | (BS_LO << 6);
 constexpr int kHeaRevMstr = MS_CX_LO | ((MS_DIAG | MS_DIAG << 2) << 4) | ((MS_REAL | MS_REAL << 2) << 8) |
                             ((MS_XLIKE | MS_XLIKE << 2) << 12);
 constexpr int kHeaRevMstrHi = MS_CX_HI | (kHeaRevMstr & ~3);
+// forward: CNOT, then one dense 2x2 (a fused rotation product) on its control
+constexpr int kHeaFwdSmask = BS_LO << 2, kHeaFwdSmaskHi = BS_HI << 2;
+constexpr int kHeaFwdMstr = MS_CX_LO, kHeaFwdMstrHi = MS_CX_HI;
 
 __device__ __forceinline__ int hea_rev_shape(const MicroOp& op) {
     if (op.nchain != 4 || op.cmat[0] < 0 || op.gen[0] >= 0 || op.gen[1] < 0 || op.gen[2] < 0 || op.gen[3] < 0) return 0;
     if (op.smask == kHeaRevSmask && op.mstr == kHeaRevMstr) return 1;
     if (op.smask == kHeaRevSmaskHi && op.mstr == kHeaRevMstrHi) return 2;
+    return 0;
+}
+
+__device__ __forceinline__ int hea_fwd_shape(const MicroOp& op) {
+    if (op.nchain != 2 || op.cmat[0] < 0 || op.cmat[1] < 0) return 0;
+    if (op.smask == kHeaFwdSmask && op.mstr == kHeaFwdMstr) return 1;
+    if (op.smask == kHeaFwdSmaskHi && op.mstr == kHeaFwdMstrHi) return 2;
     return 0;
 }
 
@@ -642,6 +655,35 @@ __device__ __forceinline__ void mop_hea_rev(double2* __restrict__ sa, double2* _
     wave_partial_re(red, buf, 0, g1);
     wave_partial_re(red, buf, 1, g2);
     wave_partial_re(red, buf, 2, g3);
+}
+
+template <bool CXLO>
+__device__ __forceinline__ void mop_hea_fwd(double2* __restrict__ sa, int t, const MicroOp& op, const double2* __restrict__ pool) {
+    const uint32_t ng = 1u << (t - 2);
+    const uint32_t olo = 1u << op.lpos[0], ohi = 1u << op.lpos[1];
+    const Mat2 m = load_mat2(pool + op.cmat[1]);
+    for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
+        const uint32_t base = ins32<2>(g, op.lpos);
+        const uint32_t i00 = base, i10 = base + olo, i01 = base + ohi, i11 = base + olo + ohi;
+        double2 a00 = sa[i00], a10, a01, a11;
+        if (CXLO) {  // CNOT with control lo, then the 2x2 on lo: pairs (00, 10), (01, 11)
+            a10 = sa[i11];
+            a11 = sa[i10];
+            a01 = sa[i01];
+            m2_ip<MS_DENSE>(m, a00, a10);
+            m2_ip<MS_DENSE>(m, a01, a11);
+        } else {  // control hi, then the 2x2 on hi: pairs (00, 01), (10, 11)
+            a01 = sa[i11];
+            a11 = sa[i01];
+            a10 = sa[i10];
+            m2_ip<MS_DENSE>(m, a00, a01);
+            m2_ip<MS_DENSE>(m, a10, a11);
+        }
+        sa[i00] = a00;
+        sa[i10] = a10;
+        sa[i01] = a01;
+        sa[i11] = a11;
+    }
 }
 
 // MK_HAM: B += Dm A with Dm = M + M^dagger (op.mat2); acc += sum conj(A) (M A)
@@ -728,11 +770,16 @@ __device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const M
         case MK_BLOCK:
             if constexpr ((KS & (1 << MK_BLOCK)) != 0) {
                 // forward micro-ops act on A alone; reverse ones on A and B
-                constexpr bool two = KS != KS_FWD;
+                constexpr bool two = KS != KS_FWD && KS != KS_FWDH;
                 if constexpr (KS == KS_REV) {  // state-vector reverse: the ansatz shape first
                     const int shape = hea_rev_shape(op);
                     if (shape == 1) mop_hea_rev<true>(sa, sb, t, op, pool, red, buf);
                     else if (shape == 2) mop_hea_rev<false>(sa, sb, t, op, pool, red, buf);
+                    else mop_block(sa, sb, t, op, pool, red, buf, two);
+                } else if constexpr (KS == KS_FWDH) {
+                    const int shape = hea_fwd_shape(op);
+                    if (shape == 1) mop_hea_fwd<true>(sa, t, op, pool);
+                    else if (shape == 2) mop_hea_fwd<false>(sa, t, op, pool);
                     else mop_block(sa, sb, t, op, pool, red, buf, two);
                 } else {
                     mop_block(sa, sb, t, op, pool, red, buf, two);
