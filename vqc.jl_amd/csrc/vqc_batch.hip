@@ -199,7 +199,7 @@ __device__ __forceinline__ void batch_loop(const BatchArgs& a, int& s_item, int&
                 double2* B = a.arena + in.b_off;
                 if (p.kind == PH_FWD) {
                     for (int64_t tile = it.tile_begin; tile < it.tile_end; ++tile)
-                        phase_tile<MAXKB, KS_FWD>(p, uint64_t(tile), A, A, B, a.mops, a.pool, sa, sb, red, gacc, st);
+                        phase_tile<MAXKB, KS_FWDH>(p, uint64_t(tile), A, A, B, a.mops, a.pool, sa, sb, red, gacc, st);
                 } else if (p.kind == PH_HAM) {
                     for (int64_t tile = it.tile_begin; tile < it.tile_end; ++tile)
                         phase_tile<MAXKB, KS_HAM>(p, uint64_t(tile), A, A, B, a.mops, a.pool, sa, sb, red, gacc, st);
