@@ -662,6 +662,9 @@ int vqc_plan_reverse(PlanBuilder& pb, int nbits, int n, int is_dm, int nops, con
                     g.ng = 1;
                     g.gen[0] = side ? xp.variant(gens[j], D, MF_CONJ) : gens[j];
                     g.acc[0] = accs[j];
+                    // the generator's structure (its conjugate keeps it), for one-bit
+                    // terms that become gradient-only steps of a register block
+                    if (o.nq == 1) g.mstr = mstruct_of(xp, g.gen[0], D, hint_gen(o)) << 2;
                     pb.mops.push_back(g);
                 }
         });
