@@ -112,14 +112,20 @@ int vqc_get_stream(void** stream);
  * current value; vqc_get_tuning reports the values in force. The VQC_TILE_* /
  * VQC_DENSE_* environment variables give the initial values. */
 typedef struct vqc_tuning {
-    int32_t tile_fwd_bits;          /* tile bits of forward phases, 6..13 (default 12) */
-    int32_t tile_rev_bits;          /* tile bits of Hamiltonian / reverse phases, 6..12 (11) */
-    int32_t tile_small_bits;        /* whole evaluation in one workgroup up to this many bits, 1..12 (12) */
+    int32_t tile_fwd_bits;          /* tile bits of forward phases, 6..13, or 0 (default): by size, below */
+    int32_t tile_rev_bits;          /* tile bits of Hamiltonian / reverse phases, 6..12, or 0 (default) */
+    int32_t tile_small_bits;        /* whole evaluation in one workgroup up to this many bits, 1..12, or 0 (default) */
     int32_t tile_low_bits;          /* contiguous low index bits in every tile, 0..8 (4) */
     int32_t dense_groups[3];        /* amplitude groups per thread, 1-, 2-, 3-qubit sweeps: 1, 2, 4, 8 (1) */
     int32_t dense_nontemporal[3];   /* non-temporal loads/stores, 1-, 2-, 3-qubit sweeps: 0/1 (1, 1, 1);
                                        used only when every target is at bit 3 or above */
 } vqc_tuning_t;
+/* Tile bits left at 0 follow what is evaluated. A single state of b index bits
+ * (2n for an n-qubit density matrix) runs as one workgroup up to b = 9, else in
+ * tiles of 9 bits up to b = 14, 10 up to 18, 11 up to 20 and 12 forward / 11
+ * reverse above: smaller tiles put more CUs on a small state, larger ones put
+ * more gates in each pass. Batches (vqc_batch_*, the serving loop's batches)
+ * take 12 / 11 and one workgroup per instance up to 12 bits. */
 int vqc_set_tuning(const vqc_tuning_t* t);
 int vqc_get_tuning(vqc_tuning_t* t);
 
@@ -227,10 +233,22 @@ int vqc_apply_circuit(vqc_state* s, int nops, const vqc_op_t* ops, const double*
 /* Shape of the most recent fused plan run by the calling thread (no reference
  * counterpart): LDS tile passes by kind (each forward pass reads and writes the
  * state once, each Hamiltonian / reverse pass two states) and micro-ops by kind
- * (apply, apply2, apply2x, grad, ham, chain, block). */
+ * (apply, apply2, apply2x, grad, ham, chain, block). reused is 1 when the plan
+ * came from the library's plan cache.
+ *
+ * Plan cache: vqc_apply_circuit, vqc_sv_energy_grad, vqc_dm_energy_grad and
+ * vqc_circuit_vjp keep the plans of the 8 most recent programs. A call whose op
+ * and term tables, pool lengths, state shape and tile tuning equal a cached
+ * plan's reuses it with the new matrix values: the matrices the plan derives
+ * (adjoints, conjugates, products of runs on one qubit) are recomputed from them
+ * and every structure class the plan relies on (diagonal, real, X-like, CNOT) is
+ * checked again; a class that no longer holds (a zero angle that makes a
+ * rotation diagonal, say) plans the program again. VQC_PLAN_CACHE=0 in the
+ * environment turns the cache off. */
 typedef struct vqc_plan_info {
     int32_t phases_fwd, phases_ham, phases_rev;
     int32_t mops[8];
+    int32_t reused;
 } vqc_plan_info_t;
 int vqc_plan_info(vqc_plan_info_t* out);
 

@@ -128,7 +128,7 @@ def test_tuning_roundtrip_and_validation():
         assert t["tile_fwd_bits"] == 11 and t["tile_low_bits"] == 4 and t["dense_groups"][0] == 2
         assert t["tile_rev_bits"] == before["tile_rev_bits"]  # -1 keeps the value
         for bad in ({"tile_fwd_bits": 14}, {"tile_rev_bits": 13}, {"tile_low_bits": 12}, {"dense_groups": [3, -1, -1]},
-                    {"dense_nontemporal": [2, -1, -1]}, {"tile_small_bits": 0}):
+                    {"dense_nontemporal": [2, -1, -1]}, {"tile_small_bits": 13}, {"tile_fwd_bits": 5}):
             with pytest.raises(_lib.VQCError):
                 autotune.set_tuning(**bad)
         assert autotune.get_tuning()["tile_fwd_bits"] == 11  # a rejected call changes nothing

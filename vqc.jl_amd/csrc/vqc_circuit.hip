@@ -11,6 +11,7 @@
 // (circuitdiff.jl:23-32, 61-78); it is the per-circuit baseline the fused and
 // batched forms are measured against.
 #include <cstdlib>
+#include <memory>
 
 #include "vqc_tile.h"
 
@@ -22,12 +23,26 @@ int vqc_dm_expec_sweep(const double2* rho, int n, int k, const int32_t* pos, con
                        unsigned* grid_out, hipStream_t st);
 void vqc_zero(double2* v, uint64_t len, hipStream_t st);
 
-TileTuning vqc_tile_tuning() {
+// Tile shape. Tile-bit fields left at 0 resolve by what is evaluated. One
+// state of nbits index bits (nbits >= 0) runs on one workgroup up to 9 bits,
+// else in tiles of 9 bits up to 14, 10 up to 18, 11 up to 20 and 12 (forward) /
+// 11 (reverse) above: 2^(nbits - t) tiles must keep enough CUs busy, while
+// each phase, one launch, covers fewer gates the smaller its tiles (the sizes
+// measured fastest for layered circuits of 8 to 20 qubits on MI355X). A batch
+// (nbits < 0), whose instances fill the device together, takes 12 / 11 and a
+// whole-state workgroup up to 12 bits.
+TileTuning vqc_tile_tuning(int nbits) {
     const vqc_tuning_t g = vqc_tuning_now();
+    int fwd = 12, rev = 11, small = 12;
+    if (nbits >= 0) {
+        small = 9;
+        fwd = nbits <= 14 ? 9 : (nbits <= 18 ? 10 : (nbits <= 20 ? 11 : 12));
+        rev = std::min(fwd, 11);
+    }
     TileTuning t;
-    t.t_fwd = std::min(kMaxTileBits, std::max(6, int(g.tile_fwd_bits)));
-    t.t_rev = std::min(kMaxTileBits2, std::max(6, int(g.tile_rev_bits)));
-    t.t_small = std::min(kMaxTileBits2, std::max(1, int(g.tile_small_bits)));
+    t.t_fwd = std::min(kMaxTileBits, std::max(6, g.tile_fwd_bits ? int(g.tile_fwd_bits) : fwd));
+    t.t_rev = std::min(kMaxTileBits2, std::max(6, g.tile_rev_bits ? int(g.tile_rev_bits) : rev));
+    t.t_small = std::min(kMaxTileBits2, std::max(1, g.tile_small_bits ? int(g.tile_small_bits) : small));
     t.mlow = std::min(8, std::max(0, int(g.tile_low_bits)));
     return t;
 }
@@ -139,9 +154,9 @@ static int host_mstruct(const double* M, int D) {
     return lo ? MS_CX_LO : (hi ? MS_CX_HI : MS_DENSE);
 }
 
-static int mstruct_of(const XPool& xp, int64_t off, int D, int hint_ms) {
+static int mstruct_of(XPool& xp, int64_t off, int D, int hint_ms) {
     if (off < 0) return MS_DENSE;
-    if (!xp.deferred) return host_mstruct(xp.h.data() + 2 * off, D);
+    if (!xp.deferred) return xp.classify(off, D);
     return hint_ms;
 }
 static int hint_mat(const vqc_op_t& o) {
@@ -225,21 +240,8 @@ static void plan_groups(PlanBuilder& pb, int kind, int nbits, int nitems, int t,
     }
 }
 
-int64_t XPool::variant(int64_t off, int D, int v) {
-    if (v == 0) return off;
-    const uint64_t key = (uint64_t(off) << 4) | uint64_t(v);
-    const auto hit = cache.find(key);
-    if (hit != cache.end()) return hit->second;
-    const int64_t out = size();
-    cache.emplace(key, out);
-    if (deferred) {
-        len += int64_t(D) * D;
-        table.push_back({off, out, D, v});
-        return out;
-    }
-    h.resize(h.size() + size_t(2) * D * D);
-    const double* M = h.data() + 2 * off;
-    double* R = h.data() + 2 * out;
+// R <- the variant v of the D x D matrix M (R and M do not overlap)
+static void make_variant(const double* M, int D, int v, double* R) {
     for (int r = 0; r < D; ++r)
         for (int c = 0; c < D; ++c) {
             const size_t dst = 2 * size_t(r + c * D);
@@ -255,27 +257,13 @@ int64_t XPool::variant(int64_t off, int D, int v) {
             R[dst] = re;
             R[dst + 1] = im;
         }
-    return out;
 }
-
-int64_t XPool::product(const int64_t* srcs, int n, int D) {
-    const int64_t out = size();
-    if (deferred) {
-        len += int64_t(D) * D;
-        XPoolEntry e{};
-        e.src = srcs[0];
-        e.dst = out;
-        e.D = D;
-        e.v = 8;
-        e.nsrc = n;
-        for (int i = 0; i < 4; ++i) e.srcs[i] = i < n ? srcs[i] : -1;
-        table.push_back(e);
-        return out;
-    }
+// out <- U_n ... U_1 for the factors at h + 2 srcs[f], in application order
+static void make_product(const double* h, const int64_t* srcs, int n, int D, double* out) {
     std::vector<double> P(size_t(2) * D * D, 0.0), T(size_t(2) * D * D);
     for (int i = 0; i < D; ++i) P[2 * (i + i * D)] = 1.0;
     for (int f = 0; f < n; ++f) {  // P <- U_f P
-        const double* U = h.data() + 2 * srcs[f];
+        const double* U = h + 2 * srcs[f];
         for (int r = 0; r < D; ++r)
             for (int c = 0; c < D; ++c) {
                 double re = 0, im = 0;
@@ -290,8 +278,61 @@ int64_t XPool::product(const int64_t* srcs, int n, int D) {
             }
         P.swap(T);
     }
-    h.insert(h.end(), P.begin(), P.end());
+    std::copy(P.begin(), P.end(), out);
+}
+
+int64_t XPool::variant(int64_t off, int D, int v) {
+    if (v == 0) return off;
+    const uint64_t key = (uint64_t(off) << 4) | uint64_t(v);
+    const auto hit = cache.find(key);
+    if (hit != cache.end()) return hit->second;
+    const int64_t out = size();
+    cache.emplace(key, out);
+    table.push_back({off, out, D, v});
+    if (deferred) {
+        len += int64_t(D) * D;
+        return out;
+    }
+    h.resize(h.size() + size_t(2) * D * D);
+    make_variant(h.data() + 2 * off, D, v, h.data() + 2 * out);
     return out;
+}
+
+int64_t XPool::product(const int64_t* srcs, int n, int D) {
+    const int64_t out = size();
+    XPoolEntry e{};
+    e.src = srcs[0];
+    e.dst = out;
+    e.D = D;
+    e.v = 8;
+    e.nsrc = n;
+    for (int i = 0; i < 4; ++i) e.srcs[i] = i < n ? srcs[i] : -1;
+    table.push_back(e);
+    if (deferred) {
+        len += int64_t(D) * D;
+        return out;
+    }
+    h.resize(h.size() + size_t(2) * D * D);
+    make_product(h.data(), srcs, n, D, h.data() + 2 * out);
+    return out;
+}
+
+int XPool::classify(int64_t off, int D) {
+    const int ms = host_mstruct(h.data() + 2 * off, D);
+    claims.push_back({off, D, ms});
+    return ms;
+}
+
+bool XPool::reload(const double* pool, int64_t plen, const double* hpool, int64_t hlen) {
+    if (plen) std::copy(pool, pool + 2 * plen, h.begin());
+    if (hlen) std::copy(hpool, hpool + 2 * hlen, h.begin() + 2 * plen);
+    for (const XPoolEntry& e : table) {
+        if (e.v == 8) make_product(h.data(), e.srcs, e.nsrc, e.D, h.data() + 2 * e.dst);
+        else make_variant(h.data() + 2 * e.src, e.D, e.v, h.data() + 2 * e.dst);
+    }
+    for (const XPoolClaim& c : claims)
+        if (host_mstruct(h.data() + 2 * c.off, c.D) != c.ms) return false;
+    return true;
 }
 
 // Register blocks on a pair of bits (MK_BLOCK). Within one phase's micro-ops,
@@ -477,7 +518,7 @@ void vqc_fuse_chains(PlanBuilder& pb, size_t phase_begin, size_t phase_end) {
                     for (int j = 0; j < run; ++j) srcs[j] = pb.mops[o + j].mat;
                     MicroOp f = m;
                     f.mat = xp.product(srcs, run, 2);
-                    f.mstr = xp.deferred ? MS_DENSE : host_mstruct(xp.h.data() + 2 * f.mat, 2);
+                    f.mstr = xp.deferred ? MS_DENSE : xp.classify(f.mat, 2);
                     out.push_back(f);
                     o += run;
                     continue;
@@ -882,6 +923,8 @@ static int run_forward(const PlanBuilder& pb, size_t ph_begin, size_t ph_end, co
 
 // shape of the most recent plan uploaded by this thread (vqc_plan_info)
 static thread_local vqc_plan_info_t g_last_plan{};
+// whether the plan being run came from the plan cache
+static bool g_plan_reused = false;
 
 static void record_plan(const PlanBuilder& pb) {
     vqc_plan_info_t& r = g_last_plan;
@@ -893,6 +936,7 @@ static void record_plan(const PlanBuilder& pb) {
     }
     for (const MicroOp& m : pb.mops)
         if (m.kind >= 0 && m.kind < 8) ++r.mops[m.kind];
+    r.reused = g_plan_reused;
 }
 
 extern "C" int vqc_plan_info(vqc_plan_info_t* out) {
@@ -901,18 +945,97 @@ extern "C" int vqc_plan_info(vqc_plan_info_t* out) {
     return VQC_OK;
 }
 
-static int upload_program(const PlanBuilder& pb, hipStream_t st) {
+// cached plan whose micro-ops and phases the device buffers hold (0: unknown)
+static uint64_t g_uploaded_plan = 0;
+
+// The matrix pool always goes up; the micro-ops and phases only when the device
+// does not hold them already (plan_id: the cached plan they belong to, 0 for an
+// uncached one).
+static int upload_program(const PlanBuilder& pb, hipStream_t st, uint64_t plan_id = 0) {
     record_plan(pb);
     const std::vector<double>& h = pb.xp->h;
     VQC_TRY(ensure(g_wpool, h.size() * 8 + 16));
-    VQC_TRY(ensure(g_wmops, std::max<size_t>(pb.mops.size(), 1) * sizeof(MicroOp)));
     if (!h.empty()) VQC_HIP(hipMemcpyAsync(g_wpool.buf, h.data(), h.size() * 8, hipMemcpyHostToDevice, st));
+    if (plan_id != 0 && plan_id == g_uploaded_plan) return VQC_OK;
+    g_uploaded_plan = 0;
+    VQC_TRY(ensure(g_wmops, std::max<size_t>(pb.mops.size(), 1) * sizeof(MicroOp)));
     if (!pb.mops.empty())
         VQC_HIP(hipMemcpyAsync(g_wmops.buf, pb.mops.data(), pb.mops.size() * sizeof(MicroOp), hipMemcpyHostToDevice, st));
     VQC_TRY(ensure(g_wphases, std::max<size_t>(pb.phases.size(), 1) * sizeof(PhaseDesc)));
     if (!pb.phases.empty())
         VQC_HIP(hipMemcpyAsync(g_wphases.buf, pb.phases.data(), pb.phases.size() * sizeof(PhaseDesc), hipMemcpyHostToDevice, st));
+    g_uploaded_plan = plan_id;
     return VQC_OK;
+}
+
+// ------------------------------- plan cache ---------------------------------
+// Plans of the most recent programs. An optimizer evaluates one program again
+// and again with new angles. A plan depends on the op and term tables (kinds,
+// positions, pool offsets, hints), the pool lengths, the tile tuning and the
+// entry point, and on the matrix values only through the derived matrices it
+// made (XPool::table) and the structure classes it read (XPool::claims). A hit
+// loads the new values, recomputes the derived matrices and checks every class
+// again; when one no longer holds (an angle that makes a rotation diagonal, say)
+// the program is planned again. The cache lives under the library lock.
+struct PlanEntry {
+    std::string key;
+    XPool xp;
+    PlanBuilder pb;
+    int info[8] = {};  // entry-point data (launch shape, phase boundaries)
+    uint64_t id = 0;
+};
+constexpr size_t kPlanCacheSize = 8;
+static std::vector<std::unique_ptr<PlanEntry>> g_plans;  // most recent first
+static uint64_t g_plan_ids = 0;
+static bool g_plan_cache_on = std::getenv("VQC_PLAN_CACHE") == nullptr || std::atoi(std::getenv("VQC_PLAN_CACHE")) != 0;
+
+template <class T>
+static void key_put(std::string& k, const T& v) {
+    k.append(reinterpret_cast<const char*>(&v), sizeof v);
+}
+static std::string plan_key(char entry, int nbits, int n, int extra, int nparams, int nops, const vqc_op_t* ops, int64_t plen,
+                            int nterms, const vqc_term_t* terms, int64_t hlen) {
+    const TileTuning tt = vqc_tile_tuning(nbits);
+    std::string k;
+    k.reserve(64 + size_t(nops) * sizeof(vqc_op_t) + size_t(nterms) * sizeof(vqc_term_t));
+    k.push_back(entry);
+    for (int v : {nbits, n, extra, nparams, nops, nterms, tt.t_fwd, tt.t_rev, tt.t_small, tt.mlow}) key_put(k, v);
+    key_put(k, plen);
+    key_put(k, hlen);
+    if (nops) k.append(reinterpret_cast<const char*>(ops), size_t(nops) * sizeof(vqc_op_t));
+    if (nterms) k.append(reinterpret_cast<const char*>(terms), size_t(nterms) * sizeof(vqc_term_t));
+    return k;
+}
+// The cached plan for key with the caller's matrices loaded, or null.
+static PlanEntry* plan_find(const std::string& key, const double* pool, int64_t plen, const double* hpool, int64_t hlen) {
+    if (!g_plan_cache_on) return nullptr;
+    for (size_t i = 0; i < g_plans.size(); ++i) {
+        if (g_plans[i]->key != key) continue;
+        std::unique_ptr<PlanEntry> e = std::move(g_plans[i]);
+        g_plans.erase(g_plans.begin() + i);
+        if (!e->xp.reload(pool, plen, hpool, hlen)) return nullptr;
+        g_plans.insert(g_plans.begin(), std::move(e));
+        g_plan_reused = true;
+        return g_plans.front().get();
+    }
+    return nullptr;
+}
+// A new, empty plan whose pool holds the caller's matrices (pool, then hpool).
+static std::unique_ptr<PlanEntry> plan_make(std::string key, const double* pool, int64_t plen, const double* hpool, int64_t hlen) {
+    std::unique_ptr<PlanEntry> e(new PlanEntry);
+    g_plan_reused = false;
+    e->key = std::move(key);
+    if (plen) e->xp.h.assign(pool, pool + 2 * plen);
+    if (hlen) e->xp.h.insert(e->xp.h.end(), hpool, hpool + 2 * hlen);
+    e->pb.xp = &e->xp;
+    e->id = ++g_plan_ids;
+    return e;
+}
+// Keep a finished plan (the least recent one beyond the cache size goes).
+static PlanEntry* plan_store(std::unique_ptr<PlanEntry> e) {
+    g_plans.insert(g_plans.begin(), std::move(e));
+    if (g_plans.size() > kPlanCacheSize) g_plans.pop_back();
+    return g_plans.front().get();
 }
 
 extern "C" int vqc_apply_circuit(vqc_state* s, int nops, const vqc_op_t* ops, const double* pool) {
@@ -924,14 +1047,17 @@ extern "C" int vqc_apply_circuit(vqc_state* s, int nops, const vqc_op_t* ops, co
     if (!pool) return vqc_fail(VQC_EINVAL, "null matrix pool");
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));
-    const TileTuning tt = vqc_tile_tuning();
-    XPool xp;
-    xp.h.assign(pool, pool + 2 * plen);
-    PlanBuilder pb;
-    pb.xp = &xp;
-    VQC_TRY(vqc_plan_forward(pb, s->nbits, s->nqubits, s->is_dm, nops, ops, tt.t_fwd, tt.mlow));
-    vqc_fuse_chains(pb, 0, pb.phases.size());
-    VQC_TRY(upload_program(pb, c->stream));
+    std::string key = plan_key('F', s->nbits, s->nqubits, s->is_dm, 0, nops, ops, plen, 0, nullptr, 0);
+    PlanEntry* e = plan_find(key, pool, plen, nullptr, 0);
+    if (!e) {
+        const TileTuning tt = vqc_tile_tuning(s->nbits);
+        std::unique_ptr<PlanEntry> ne = plan_make(std::move(key), pool, plen, nullptr, 0);
+        VQC_TRY(vqc_plan_forward(ne->pb, s->nbits, s->nqubits, s->is_dm, nops, ops, tt.t_fwd, tt.mlow));
+        vqc_fuse_chains(ne->pb, 0, ne->pb.phases.size());
+        e = plan_store(std::move(ne));
+    }
+    const PlanBuilder& pb = e->pb;
+    VQC_TRY(upload_program(pb, c->stream, e->id));
     return run_forward(pb, 0, pb.phases.size(), s->d, s->d, (const MicroOp*)g_wmops.buf, (const double2*)g_wpool.buf,
                        (const PhaseDesc*)g_wphases.buf, c->stream);
 }
@@ -1040,6 +1166,57 @@ static int max_small_rows(int nbits) {
     return used >= 160 * 1024 ? 0 : int((160 * 1024 - used) / 16);
 }
 
+// Plan of the state-vector adjoint into e. The whole evaluation runs in one
+// workgroup when allow_small and the state and the accumulators fit in LDS
+// (info: 1, Hamiltonian start, reverse start, micro-op width, reverse kind set);
+// otherwise in tile phases (info: 0, forward phases, forward + Hamiltonian phases).
+static int plan_sv(PlanEntry& e, int nb, int nops, const vqc_op_t* ops, int nterms, const vqc_term_t* terms, int64_t plen,
+                   int64_t hlen, bool allow_small) {
+    const TileTuning tt = vqc_tile_tuning(nb);
+    PlanBuilder& pb = e.pb;
+    if (allow_small && nb <= tt.t_small) {
+        VQC_TRY(vqc_plan_forward(pb, nb, nb, 0, nops, ops, nb, 0));
+        const size_t nf = pb.phases.size();
+        VQC_TRY(vqc_plan_ham(pb, nb, nterms, terms, plen, nb, 0));
+        const size_t nh = pb.phases.size();
+        VQC_TRY(vqc_plan_reverse(pb, nb, nb, 0, nops, ops, nb, 0));
+        vqc_fuse_chains(pb, 0, pb.phases.size());
+        if (pb.nrows <= max_small_rows(nb)) {
+            // accumulator index := global row
+            for (const PhaseDesc& p : pb.phases)
+                for (int o = p.op_begin; o < p.op_end; ++o)
+                    for (int q = 0; q < 4; ++q) pb.mops[o].acc[q] += p.row_begin;
+            const int rev_begin = vqc_phase_op_start(pb, nh);
+            const int mkb = vqc_max_kb(pb.mops, 0, int(pb.mops.size()));
+            bool rev_simple = true;
+            for (size_t o = size_t(rev_begin); o < pb.mops.size(); ++o) rev_simple &= vqc_rev_simple(pb.mops[o]);
+            e.info[0] = 1;
+            e.info[1] = vqc_phase_op_start(pb, nf);
+            e.info[2] = rev_begin;
+            e.info[3] = mkb <= 2 ? 2 : (mkb <= 3 ? 3 : 5);
+            e.info[4] = rev_simple ? KS_REV : KS_REVX;
+            return VQC_OK;
+        }
+        // too many accumulators for LDS: plan again in tiles, from the caller's matrices only
+        pb = PlanBuilder();
+        pb.xp = &e.xp;
+        e.xp.h.resize(size_t(2) * (plen + hlen));
+        e.xp.cache.clear();
+        e.xp.table.clear();
+        e.xp.claims.clear();
+    }
+    VQC_TRY(vqc_plan_forward(pb, nb, nb, 0, nops, ops, tt.t_fwd, tt.mlow));
+    const size_t nfwd = pb.phases.size();
+    VQC_TRY(vqc_plan_ham(pb, nb, nterms, terms, plen, tt.t_rev, tt.mlow));
+    const size_t nham_end = pb.phases.size();
+    VQC_TRY(vqc_plan_reverse(pb, nb, nb, 0, nops, ops, tt.t_rev, tt.mlow));
+    vqc_fuse_chains(pb, 0, pb.phases.size());
+    e.info[0] = 0;
+    e.info[1] = int(nfwd);
+    e.info[2] = int(nham_end);
+    return VQC_OK;
+}
+
 // The state-vector adjoint sweep. The reverse buffer B starts as
 // (H + H^dagger) psi (the energy's cotangent) or, when `seed` is given (no
 // terms then), as a copy of it; `seed_out` (may be null) receives B after the
@@ -1069,86 +1246,59 @@ static int sv_adjoint(const vqc_state* psi0, int nops, const vqc_op_t* ops, cons
         if (seed) return vqc_fail(VQC_EUNSUPPORTED, "vqc_circuit_vjp runs on the fused path only (flags must be 0)");
         VQC_TRY(sv_energy_grad_plain(psi0, nops, ops, pool, plen, nterms, terms, hpool, nparams, energy2, grad, A, B, fwd_out, st));
     } else {
-        const TileTuning tt = vqc_tile_tuning();
-        XPool xp;
-        xp.h.assign(pool, pool + 2 * plen);
-        xp.h.insert(xp.h.end(), hpool, hpool + 2 * hlen);
-        PlanBuilder pb;
-        pb.xp = &xp;
-        const bool small = nb <= tt.t_small && !seed;
-        if (small) {
-            VQC_TRY(vqc_plan_forward(pb, nb, nb, 0, nops, ops, nb, 0));
-            const size_t nf = pb.phases.size();
-            VQC_TRY(vqc_plan_ham(pb, nb, nterms, terms, plen, nb, 0));
-            const size_t nh = pb.phases.size();
-            VQC_TRY(vqc_plan_reverse(pb, nb, nb, 0, nops, ops, nb, 0));
-            vqc_fuse_chains(pb, 0, pb.phases.size());
-            const int ham_begin = vqc_phase_op_start(pb, nf);
-            const int rev_begin = vqc_phase_op_start(pb, nh);
-            if (pb.nrows <= max_small_rows(nb)) {
-                // accumulator index := global row
-                for (const PhaseDesc& p : pb.phases)
-                    for (int o = p.op_begin; o < p.op_end; ++o)
-                        for (int q = 0; q < 4; ++q) pb.mops[o].acc[q] += p.row_begin;
-                VQC_TRY(upload_program(pb, st));
-                VQC_TRY(ensure(g_wrows, size_t(std::max(pb.nrows, 1)) * 16));
-                SmallArgs a{nb, int(pb.mops.size()), ham_begin, rev_begin, pb.nrows, psi0->d, fwd_out, (const MicroOp*)g_wmops.buf,
-                            (const double2*)g_wpool.buf, (double2*)g_wrows.buf};
-                VQC_TRY(set_lds_attrs());
-                const size_t lds = (2 * L + kRedDoubles2 + size_t(std::max(pb.nrows, 1))) * 16;
-                const int mkb = vqc_max_kb(pb.mops, 0, int(pb.mops.size()));
-                const int mk = mkb <= 2 ? 2 : (mkb <= 3 ? 3 : 5);
-                bool rev_simple = true;
-                for (size_t o = size_t(rev_begin); o < pb.mops.size(); ++o) rev_simple &= vqc_rev_simple(pb.mops[o]);
-                const int ks = rev_simple ? KS_REV : KS_REVX;
+        std::string key = plan_key('V', nb, nb, seed ? 1 : 0, nparams, nops, ops, plen, nterms, terms, hlen);
+        PlanEntry* e = plan_find(key, pool, plen, hpool, hlen);
+        if (!e) {
+            std::unique_ptr<PlanEntry> ne = plan_make(std::move(key), pool, plen, hpool, hlen);
+            VQC_TRY(plan_sv(*ne, nb, nops, ops, nterms, terms, plen, hlen, !seed));
+            e = plan_store(std::move(ne));
+        }
+        const PlanBuilder& pb = e->pb;
+        VQC_TRY(upload_program(pb, st, e->id));
+        if (e->info[0]) {
+            const int ham_begin = e->info[1], rev_begin = e->info[2], mk = e->info[3], ks = e->info[4];
+            VQC_TRY(ensure(g_wrows, size_t(std::max(pb.nrows, 1)) * 16));
+            SmallArgs a{nb, int(pb.mops.size()), ham_begin, rev_begin, pb.nrows, psi0->d, fwd_out, (const MicroOp*)g_wmops.buf,
+                        (const double2*)g_wpool.buf, (double2*)g_wrows.buf};
+            VQC_TRY(set_lds_attrs());
+            const size_t lds = (2 * L + kRedDoubles2 + size_t(std::max(pb.nrows, 1))) * 16;
 #define VQC_SMALL(M, K) \
     if (mk == M && ks == K) hipLaunchKernelGGL((k_small_eval<M, K>), dim3(1), dim3(kTileThreads), lds, st, a);
-                VQC_SMALL(2, KS_REV) VQC_SMALL(2, KS_REVX) VQC_SMALL(3, KS_REV) VQC_SMALL(3, KS_REVX) VQC_SMALL(5, KS_REV)
-                VQC_SMALL(5, KS_REVX)
+            VQC_SMALL(2, KS_REV) VQC_SMALL(2, KS_REVX) VQC_SMALL(3, KS_REV) VQC_SMALL(3, KS_REVX) VQC_SMALL(5, KS_REV)
+            VQC_SMALL(5, KS_REVX)
 #undef VQC_SMALL
-                VQC_HIP(hipGetLastError());
-                std::vector<double> rows(2 * size_t(std::max(pb.nrows, 1)));
-                if (pb.nrows) VQC_TRY(vqc_fetch((double2*)g_wrows.buf, pb.nrows, rows.data()));
-                else VQC_HIP(hipStreamSynchronize(st));
-                finish_rows(rows, pb.row_slot, nparams, energy2, grad);
-                return VQC_OK;
-            }
-            pb = PlanBuilder();
-            pb.xp = &xp;
-        }
-        {
-            VQC_TRY(vqc_plan_forward(pb, nb, nb, 0, nops, ops, tt.t_fwd, tt.mlow));
-            const size_t nfwd = pb.phases.size();
-            VQC_TRY(vqc_plan_ham(pb, nb, nterms, terms, plen, tt.t_rev, tt.mlow));
-            const size_t nham_end = pb.phases.size();
-            VQC_TRY(vqc_plan_reverse(pb, nb, nb, 0, nops, ops, tt.t_rev, tt.mlow));
-            vqc_fuse_chains(pb, 0, pb.phases.size());
-            VQC_TRY(upload_program(pb, st));
-            const MicroOp* dm = (const MicroOp*)g_wmops.buf;
-            const double2* dp = (const double2*)g_wpool.buf;
-            const int stride = 1024;
-            const int nrows = std::max(pb.nrows, 1);
-            VQC_TRY(ensure(g_wparts, size_t(nrows) * (stride + 1) * 16));
-            double2* parts = (double2*)g_wparts.buf;
-            double2* rowsum = parts + size_t(nrows) * stride;
-            vqc_zero(parts, size_t(nrows) * stride, st);
-            if (nfwd == 0) VQC_HIP(hipMemcpyAsync(A, psi0->d, L * 16, hipMemcpyDeviceToDevice, st));
-            else VQC_TRY(run_forward(pb, 0, nfwd, psi0->d, A, dm, dp, (const PhaseDesc*)g_wphases.buf, st));
-            if (fwd_out) VQC_HIP(hipMemcpyAsync(fwd_out, A, L * 16, hipMemcpyDeviceToDevice, st));
-            if (seed) VQC_HIP(hipMemcpyAsync(B, seed->d, L * 16, hipMemcpyDeviceToDevice, st));
-            else if (nham_end == nfwd) vqc_zero(B, L, st);
-            for (size_t i = nfwd; i < pb.phases.size(); ++i) {
-                PhaseBufs b{A, A, B, dm, dp, parts, stride};
-                VQC_TRY(launch_phase(pb, i, (const PhaseDesc*)g_wphases.buf, b, phase_grid(pb.phases[i], stride), st));
-            }
-            if (seed_out) VQC_HIP(hipMemcpyAsync(seed_out->d, B, L * 16, hipMemcpyDeviceToDevice, st));
-            if (pb.nrows) vqc_reduce_partials(parts, pb.nrows, stride, rowsum, st);
             VQC_HIP(hipGetLastError());
-            std::vector<double> rows(2 * size_t(nrows));
-            if (pb.nrows) VQC_TRY(vqc_fetch(rowsum, pb.nrows, rows.data()));
+            std::vector<double> rows(2 * size_t(std::max(pb.nrows, 1)));
+            if (pb.nrows) VQC_TRY(vqc_fetch((double2*)g_wrows.buf, pb.nrows, rows.data()));
             else VQC_HIP(hipStreamSynchronize(st));
             finish_rows(rows, pb.row_slot, nparams, energy2, grad);
+            return VQC_OK;
         }
+        const size_t nfwd = size_t(e->info[1]), nham_end = size_t(e->info[2]);
+        const MicroOp* dm = (const MicroOp*)g_wmops.buf;
+        const double2* dp = (const double2*)g_wpool.buf;
+        const int stride = 1024;
+        const int nrows = std::max(pb.nrows, 1);
+        VQC_TRY(ensure(g_wparts, size_t(nrows) * (stride + 1) * 16));
+        double2* parts = (double2*)g_wparts.buf;
+        double2* rowsum = parts + size_t(nrows) * stride;
+        vqc_zero(parts, size_t(nrows) * stride, st);
+        if (nfwd == 0) VQC_HIP(hipMemcpyAsync(A, psi0->d, L * 16, hipMemcpyDeviceToDevice, st));
+        else VQC_TRY(run_forward(pb, 0, nfwd, psi0->d, A, dm, dp, (const PhaseDesc*)g_wphases.buf, st));
+        if (fwd_out) VQC_HIP(hipMemcpyAsync(fwd_out, A, L * 16, hipMemcpyDeviceToDevice, st));
+        if (seed) VQC_HIP(hipMemcpyAsync(B, seed->d, L * 16, hipMemcpyDeviceToDevice, st));
+        else if (nham_end == nfwd) vqc_zero(B, L, st);
+        for (size_t i = nfwd; i < pb.phases.size(); ++i) {
+            PhaseBufs b{A, A, B, dm, dp, parts, stride};
+            VQC_TRY(launch_phase(pb, i, (const PhaseDesc*)g_wphases.buf, b, phase_grid(pb.phases[i], stride), st));
+        }
+        if (seed_out) VQC_HIP(hipMemcpyAsync(seed_out->d, B, L * 16, hipMemcpyDeviceToDevice, st));
+        if (pb.nrows) vqc_reduce_partials(parts, pb.nrows, stride, rowsum, st);
+        VQC_HIP(hipGetLastError());
+        std::vector<double> rows(2 * size_t(nrows));
+        if (pb.nrows) VQC_TRY(vqc_fetch(rowsum, pb.nrows, rows.data()));
+        else VQC_HIP(hipStreamSynchronize(st));
+        finish_rows(rows, pb.row_slot, nparams, energy2, grad);
     }
     return VQC_OK;
 }
@@ -1226,37 +1376,47 @@ static int dm_adjoint(const vqc_state* rho0, int nops, const vqc_op_t* ops, cons
     double2* A = (double2*)g_wA.buf;  // the reverse sweep takes A back to rho0
     double2* B = (double2*)g_wB.buf;
     const bool plain = flags & VQC_FLAG_UNFUSED;
-    const TileTuning tt = vqc_tile_tuning();
     // device pool [ops pool | hpool | variants]
-    XPool xp;
-    xp.h.assign(pool, pool + 2 * plen);
-    xp.h.insert(xp.h.end(), hpool, hpool + 2 * hlen);
-    PlanBuilder pb;
-    pb.xp = &xp;
+    std::unique_ptr<PlanEntry> local;
+    PlanEntry* e = nullptr;
     std::vector<int64_t> conj_gen(size_t(nops) * 3, -1);
     if (plain) {
+        local = plan_make(std::string(), pool, plen, hpool, hlen);
+        local->id = 0;
+        e = local.get();
+        PlanBuilder& lp = e->pb;
         for (int i = 0; i < nops; ++i)
             for (int j = 0; j < ops[i].nparams; ++j)
-                if (ops[i].slot[j] >= 0) conj_gen[size_t(i) * 3 + j] = xp.variant(ops[i].gen[j], 1 << ops[i].nq, MF_CONJ);
-    } else {
-        VQC_TRY(vqc_plan_forward(pb, nb, n, 1, nops, ops, tt.t_fwd, tt.mlow));
-        vqc_fuse_chains(pb, 0, pb.phases.size());
-    }
-    const size_t nfwd = pb.phases.size();
-    pb.nrows = nterms;  // rows 0..nterms-1: energies (DM energies come from the trace kernels)
-    for (int t = 0; t < nterms; ++t) pb.row_slot.push_back(-1 - t);
-    if (!plain) {
-        VQC_TRY(vqc_plan_reverse(pb, nb, n, 1, nops, ops, tt.t_rev, tt.mlow));
-        vqc_fuse_chains(pb, nfwd, pb.phases.size());
-    } else {
+                if (ops[i].slot[j] >= 0) conj_gen[size_t(i) * 3 + j] = e->xp.variant(ops[i].gen[j], 1 << ops[i].nq, MF_CONJ);
+        lp.nrows = nterms;  // rows 0..nterms-1: energies (DM energies come from the trace kernels)
+        for (int t = 0; t < nterms; ++t) lp.row_slot.push_back(-1 - t);
         for (int i = nops - 1; i >= 0; --i)
             for (int j = 0; j < ops[i].nparams; ++j)
                 if (ops[i].slot[j] >= 0) {
-                    pb.row_slot.push_back(ops[i].slot[j]);
-                    pb.nrows++;
+                    lp.row_slot.push_back(ops[i].slot[j]);
+                    lp.nrows++;
                 }
+    } else {
+        std::string key = plan_key('D', nb, n, seed ? 1 : 0, nparams, nops, ops, plen, nterms, terms, hlen);
+        e = plan_find(key, pool, plen, hpool, hlen);
+        if (!e) {
+            const TileTuning tt = vqc_tile_tuning(nb);
+            std::unique_ptr<PlanEntry> ne = plan_make(std::move(key), pool, plen, hpool, hlen);
+            PlanBuilder& np = ne->pb;
+            VQC_TRY(vqc_plan_forward(np, nb, n, 1, nops, ops, tt.t_fwd, tt.mlow));
+            vqc_fuse_chains(np, 0, np.phases.size());
+            const size_t nf = np.phases.size();
+            np.nrows = nterms;  // rows 0..nterms-1: energies (DM energies come from the trace kernels)
+            for (int t = 0; t < nterms; ++t) np.row_slot.push_back(-1 - t);
+            VQC_TRY(vqc_plan_reverse(np, nb, n, 1, nops, ops, tt.t_rev, tt.mlow));
+            vqc_fuse_chains(np, nf, np.phases.size());
+            ne->info[0] = int(nf);
+            e = plan_store(std::move(ne));
+        }
     }
-    VQC_TRY(upload_program(pb, st));
+    const PlanBuilder& pb = e->pb;
+    const size_t nfwd = plain ? 0 : size_t(e->info[0]);
+    VQC_TRY(upload_program(pb, st, e->id));
     const MicroOp* dm = (const MicroOp*)g_wmops.buf;
     const double2* dp = (const double2*)g_wpool.buf;
     const int stride = 2048;  // row and column gradient halves in the plain path
@@ -1389,7 +1549,7 @@ int vqc_solo_build(int n, int nops, const vqc_op_t* ops, const double* pool, int
     VQC_TRY(check_terms(n, nterms, terms, &hlen));
     if (plen > pool_len || hlen > hpool_len) return vqc_fail(VQC_EINVAL, "vqc_solo: matrix offsets outside the pools");
     if ((plen && !pool) || (hlen && !hpool)) return vqc_fail(VQC_EINVAL, "vqc_solo: null matrix pool");
-    const TileTuning tt = vqc_tile_tuning();
+    const TileTuning tt = vqc_tile_tuning(n);
     vqc_solo* s = new vqc_solo;
     s->nbits = n;
     s->nparams = nparams;

@@ -397,9 +397,10 @@ int env_or(const char* name, int def) {
 
 void tune_init_locked() {
     if (g_tune_init) return;
-    g_tune.tile_fwd_bits = env_or("VQC_TILE_FWD", 12);
-    g_tune.tile_rev_bits = env_or("VQC_TILE_REV", 11);
-    g_tune.tile_small_bits = env_or("VQC_TILE_SMALL", 12);
+    // 0: chosen by the size of what is evaluated
+    g_tune.tile_fwd_bits = env_or("VQC_TILE_FWD", 0);
+    g_tune.tile_rev_bits = env_or("VQC_TILE_REV", 0);
+    g_tune.tile_small_bits = env_or("VQC_TILE_SMALL", 0);
     g_tune.tile_low_bits = env_or("VQC_TILE_MLOW", 4);
     const int r = env_or("VQC_DENSE_R", 1), nt = env_or("VQC_DENSE_NT", -1);
     for (int k = 0; k < 3; ++k) {
@@ -435,9 +436,11 @@ extern "C" int vqc_set_tuning(const vqc_tuning_t* t) {
         take(t->dense_groups[k], n.dense_groups[k]);
         take(t->dense_nontemporal[k], n.dense_nontemporal[k]);
     }
-    if (n.tile_fwd_bits < 6 || n.tile_fwd_bits > 13 || n.tile_rev_bits < 6 || n.tile_rev_bits > 12 ||
-        n.tile_small_bits < 1 || n.tile_small_bits > 12 || n.tile_low_bits < 0 || n.tile_low_bits > 8 ||
-        n.tile_low_bits > std::min(n.tile_fwd_bits, n.tile_rev_bits))
+    // tile bits: 0 (by size: at least 9 bits) or a fixed count
+    const int fwd_min = n.tile_fwd_bits ? n.tile_fwd_bits : 9, rev_min = n.tile_rev_bits ? n.tile_rev_bits : 9;
+    if ((n.tile_fwd_bits != 0 && (n.tile_fwd_bits < 6 || n.tile_fwd_bits > 13)) ||
+        (n.tile_rev_bits != 0 && (n.tile_rev_bits < 6 || n.tile_rev_bits > 12)) || n.tile_small_bits < 0 ||
+        n.tile_small_bits > 12 || n.tile_low_bits < 0 || n.tile_low_bits > 8 || n.tile_low_bits > std::min(fwd_min, rev_min))
         return vqc_fail(VQC_EINVAL, "vqc_set_tuning: tile bits out of range");
     for (int k = 0; k < 3; ++k)
         if (!valid_groups(n.dense_groups[k]) || (n.dense_nontemporal[k] != 0 && n.dense_nontemporal[k] != 1))

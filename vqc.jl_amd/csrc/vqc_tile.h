@@ -902,7 +902,7 @@ struct TileTuning {
     int t_small = 12;   // whole state in one tile up to this many bits (both buffers)
     int mlow = 4;       // contiguous low bits kept in every tile
 };
-TileTuning vqc_tile_tuning();
+TileTuning vqc_tile_tuning(int nbits = -1);  // nbits: one state of that size; -1: a batch
 
 struct OpBits {
     std::vector<int> bits;  // global index bits the op touches
@@ -917,19 +917,32 @@ struct XPoolEntry {
     int32_t nsrc;
     int32_t pad;
 };
+struct XPoolClaim {
+    int64_t off;
+    int32_t D, ms;  // the structure class found for the D x D matrix at off
+};
 struct XPool {
     std::vector<double> h;  // interleaved complex
     std::unordered_map<uint64_t, int64_t> cache;  // (source offset, variant) -> offset
-    // deferred: only offsets are assigned and each variant is recorded in
-    // `table`, to be computed on the device once the caller's matrices arrive
+    // deferred: only offsets are assigned, to be computed on the device once the
+    // caller's matrices arrive
     bool deferred = false;
     int64_t len = 0;
+    // every derived matrix in the order it was made (deferred or not)
     std::vector<XPoolEntry> table;
+    // every structure class the planner read from values (not deferred)
+    std::vector<XPoolClaim> claims;
     int64_t size() const { return deferred ? len : int64_t(h.size() / 2); }
     // variant: MF_CONJ / MF_TRANS combination, or 4 for M + M^dagger
     int64_t variant(int64_t off, int D, int v);
     // U_n ... U_2 U_1 for D x D factors given in application order (n <= 4)
     int64_t product(const int64_t* srcs, int n, int D);
+    // the structure class (MS_*) of the D x D matrix at off, from its values, recorded
+    int classify(int64_t off, int D);
+    // New caller matrices (pool, then hpool, the lengths the pool was made with):
+    // every derived matrix is recomputed from them in the order it was made.
+    // False when a structure class the planner read no longer holds.
+    bool reload(const double* pool, int64_t plen, const double* hpool, int64_t hlen);
 };
 
 struct PlanBuilder {

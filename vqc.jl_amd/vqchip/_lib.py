@@ -91,7 +91,7 @@ class Tuning(ctypes.Structure):
 
 class PlanInfo(ctypes.Structure):
     _fields_ = [("phases_fwd", ctypes.c_int32), ("phases_ham", ctypes.c_int32), ("phases_rev", ctypes.c_int32),
-                ("mops", ctypes.c_int32 * 8)]
+                ("mops", ctypes.c_int32 * 8), ("reused", ctypes.c_int32)]
 
 
 _P = ctypes.c_void_p

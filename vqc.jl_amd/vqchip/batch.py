@@ -28,7 +28,7 @@ def instance_bytes(n, t_small=None):
     tile, none when the whole evaluation stays in LDS."""
     if t_small is None:
         from .autotune import get_tuning
-        t_small = get_tuning()["tile_small_bits"]
+        t_small = get_tuning()["tile_small_bits"] or 12  # 0 (by size): batches keep up to 12 bits in LDS
     return 0 if n <= t_small else 32 << n
 
 
