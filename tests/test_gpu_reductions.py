@@ -35,6 +35,16 @@ def test_partial_trace_state_vector(sites):
     assert np.max(np.abs(out.storage() - A @ A.conj().T)) < 1e-12
 
 
+@pytest.mark.parametrize("sites", [[1, 2], [2, 5], [6, 3]])
+def test_partial_trace_state_equals_its_density_matrix(sites):
+    # check_utility.jl:4-13: partial_tr(a, sites) == partial_tr(DensityMatrix(a), sites)
+    psi = rand_state(6, np.random.default_rng(sites[0] * 10 + sites[1]))
+    a = R.partial_tr(vq.StateVector(psi), sites)
+    b = R.partial_tr(vq.DensityMatrix(vq.StateVector(psi)), sites)
+    assert a.nqubits() == b.nqubits() == 4
+    assert np.max(np.abs(a.storage() - b.storage())) < 1e-12
+
+
 def test_partial_trace_density_matrix_and_scalars():
     n = 6
     rng = np.random.default_rng(12)

@@ -49,6 +49,34 @@ def test_generators_are_udagger_du():
             assert np.allclose(U.conj().T @ dU, G, atol=1e-8), (g, j)
 
 
+def test_adjoints_and_controlled_gates():
+    # gate' is the conjugate transpose; the named gates keep a named kernel (check_gateop.jl:68-158)
+    import vqchip as vq
+    rng = np.random.default_rng(8)
+    U = np.linalg.qr(rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2)))[0]
+    gates = [vq.XGate(2), vq.YGate(1), vq.ZGate(3), vq.TGate(2), vq.HGate(1), vq.PHASEGate(2, 0.3), vq.CNOTGate(3, 1),
+             vq.CZGate(1, 2), vq.CPHASEGate(4, 1, -0.5), vq.SWAPGate(2, 5), vq.iSWAPGate(2, 5), vq.FSIMGate(1, 3, 0.2, 0.9),
+             vq.CRxGate(2, 1, 0.4), vq.TOFFOLIGate(3, 1, 2), vq.FREDKINGate(1, 2, 3), vq.CCPHASEGate(1, 2, 3, 0.7),
+             vq.CONTROLGate((2, 1), U), vq.CONTROLCONTROLGate(1, 3, 2, U)]
+    for g in gates:
+        a = g.adjoint()
+        assert a.ordered_positions() == g.ordered_positions()
+        assert np.array_equal(a.ordered_mat(), g.ordered_mat().conj().T), g
+        assert (a.named_kind is None) == (g.named_kind is None), g
+        assert np.array_equal(a.adjoint().ordered_mat(), g.ordered_mat()), g
+    assert vq.iSWAPGate(1, 2).adjoint().named_kind == _lib.GATE_ISWAPDAG
+    assert vq.PHASEGate(1, 0.3).adjoint().params == (-0.3,) and vq.PHASEGate(1, 0.3).adjoint().nparameters() == 0
+    X = np.array([[0, 1], [1, 0]])
+    assert np.array_equal(vq.CONTROLGate((3, 1), X).ordered_mat(), vq.CNOTGate(3, 1).ordered_mat())
+    assert np.array_equal(vq.CONTROLCONTROLGate(2, 4, 1, X).ordered_mat(), vq.TOFFOLIGate(2, 4, 1).ordered_mat())
+    # CONTROLCONTROL(U) acts as U on the target exactly when both controls are 1
+    M = vq.CONTROLCONTROLGate(1, 2, 3, U).mat()
+    for a in (0, 1):
+        for b in (0, 1):
+            blk = M[np.ix_([a + 2 * b, a + 2 * b + 4], [a + 2 * b, a + 2 * b + 4])]
+            assert np.array_equal(blk, U if a and b else np.eye(2))
+
+
 def test_inactive_parameters_and_slots():
     # test/parameter.jl: isparas=false parameters are not counted and get no slot
     c = QCircuit([RxGate(1, 0.1), RxGate(1, 1.1, isparas=False), CNOTGate(1, 2), RyGate(2, 0.2)])

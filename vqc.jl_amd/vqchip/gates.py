@@ -154,28 +154,43 @@ class _Param(Gate):
         self.__dict__.update(new.__dict__)
 
 
-class XGate(Gate):
+class _SelfAdjoint(Gate):
+    """Gates equal to their adjoint: gate' keeps the gate (and its named kernel)."""
+
+    def adjoint(self):
+        return type(self)(*self.positions)
+
+
+class _PhaseFamily(_Param):
+    """PHASE, CPHASE, CCPHASE: gate' is the same gate at -phi (its named kernel
+    too), with the parameter inactive as for any adjoint gate."""
+
+    def adjoint(self):
+        return type(self)(*self.positions, -self.params[0], isparas=False)
+
+
+class XGate(_SelfAdjoint):
     name, named_kind = "X", _lib.GATE_X
 
     def __init__(self, q):
         super().__init__((q,), X)
 
 
-class YGate(Gate):
+class YGate(_SelfAdjoint):
     name = "Y"
 
     def __init__(self, q):
         super().__init__((q,), Y)
 
 
-class ZGate(Gate):
+class ZGate(_SelfAdjoint):
     name = "Z"
 
     def __init__(self, q):
         super().__init__((q,), Z)
 
 
-class HGate(Gate):
+class HGate(_SelfAdjoint):
     name = "H"
 
     def __init__(self, q):
@@ -217,7 +232,7 @@ class RzGate(_Param):
         super().__init__((q,), _rot(Z, theta), (theta,), isparas, [-0.5j * Z])
 
 
-class PHASEGate(_Param):
+class PHASEGate(_PhaseFamily):
     """|1> -> e^{i phi}|1> (specific/phasegate.jl:6-22)."""
 
     name, named_kind = "PHASE", _lib.GATE_PHASE
@@ -226,7 +241,7 @@ class PHASEGate(_Param):
         super().__init__((q,), np.diag([1, np.exp(1j * phi)]), (phi,), isparas, [1j * P1])
 
 
-class CNOTGate(Gate):
+class CNOTGate(_SelfAdjoint):
     """control = first position (specific/czgate.jl:60-64)."""
 
     name, named_kind = "CNOT", _lib.GATE_CNOT
@@ -235,14 +250,14 @@ class CNOTGate(Gate):
         super().__init__((c, t), kron_le(P0, I2) + kron_le(P1, X))
 
 
-class CZGate(Gate):
+class CZGate(_SelfAdjoint):
     name, named_kind = "CZ", _lib.GATE_CZ
 
     def __init__(self, a, b):
         super().__init__((a, b), np.diag([1, 1, 1, -1]))
 
 
-class CPHASEGate(_Param):
+class CPHASEGate(_PhaseFamily):
     name, named_kind = "CPHASE", _lib.GATE_CPHASE
 
     def __init__(self, a, b, phi, isparas=True):
@@ -250,7 +265,7 @@ class CPHASEGate(_Param):
         super().__init__((a, b), np.diag([1, 1, 1, np.exp(1j * phi)]), (phi,), isparas, [1j * P11])
 
 
-class SWAPGate(Gate):
+class SWAPGate(_SelfAdjoint):
     name, named_kind = "SWAP", _lib.GATE_SWAP
 
     def __init__(self, a, b):
@@ -264,6 +279,21 @@ class iSWAPGate(Gate):
 
     def __init__(self, a, b):
         super().__init__((a, b), np.array([[1, 0, 0, 0], [0, 0, 1j, 0], [0, 1j, 0, 0], [0, 0, 0, 1]]))
+
+    def adjoint(self):
+        return iSWAPdagGate(*self.positions)
+
+
+class iSWAPdagGate(Gate):
+    """iSWAPGate': |01>,|10> swapped with factor -i (specific/swapgates.jl:47-49)."""
+
+    name, named_kind = "iSWAP'", _lib.GATE_ISWAPDAG
+
+    def __init__(self, a, b):
+        super().__init__((a, b), np.array([[1, 0, 0, 0], [0, 0, -1j, 0], [0, -1j, 0, 0], [0, 0, 0, 1]]))
+
+    def adjoint(self):
+        return iSWAPGate(*self.positions)
 
 
 def _controlled(P, theta):
@@ -309,7 +339,7 @@ class FSIMGate(_Param):
         super().__init__((a, b), m, (theta, phi), isparas, [-1j * A, -1j * P11])
 
 
-class TOFFOLIGate(Gate):
+class TOFFOLIGate(_SelfAdjoint):
     name = "TOFFOLI"
 
     def __init__(self, a, b, c):
@@ -318,7 +348,7 @@ class TOFFOLIGate(Gate):
         super().__init__((a, b, c), m)
 
 
-class FREDKINGate(Gate):
+class FREDKINGate(_SelfAdjoint):
     """control = first position, swaps the other two (specific/fredkingate.jl:65-74)."""
 
     name, named_kind = "FREDKIN", _lib.GATE_FREDKIN
@@ -329,7 +359,7 @@ class FREDKINGate(Gate):
         super().__init__((c, a, b), m)
 
 
-class CCPHASEGate(_Param):
+class CCPHASEGate(_PhaseFamily):
     name, named_kind = "CCPHASE", _lib.GATE_CCPHASE
 
     def __init__(self, a, b, c, phi, isparas=True):
@@ -338,6 +368,30 @@ class CCPHASEGate(_Param):
         g = np.zeros((8, 8), dtype=np.complex128)
         g[7, 7] = 1j
         super().__init__((a, b, c), np.diag(d), (phi,), isparas, [g])
+
+
+def CONTROLGate(positions, U):
+    """Controlled-U on positions (control, targets...): CONTROL(U) over the given
+    order, the control first (test/check_gateop.jl:104)."""
+    positions = tuple(positions)
+    U = np.asarray(U, dtype=np.complex128)
+    if U.shape != (1 << (len(positions) - 1),) * 2:
+        raise ValueError("U must act on the positions after the control")
+    g = Gate(positions, CONTROL(U))
+    g.name = "CONTROL"
+    return g
+
+
+def CONTROLCONTROLGate(a, b, c, U):
+    """U on c when both controls a and b are 1 (test/check_gateop.jl:144)."""
+    U = np.asarray(U, dtype=np.complex128)
+    if U.shape != (2, 2):
+        raise ValueError("U must be a 2 x 2 matrix")
+    m = np.eye(8, dtype=np.complex128)
+    m[np.ix_([3, 7], [3, 7])] = U  # over (a, b, c): a = b = 1 is index 3 (c = 0) and 7 (c = 1)
+    g = Gate((a, b, c), m)
+    g.name = "CONTROLCONTROL"
+    return g
 
 
 def QuantumGate(positions, mat):
