@@ -77,6 +77,23 @@ def test_adjoints_and_controlled_gates():
             assert np.array_equal(blk, U if a and b else np.eye(2))
 
 
+def test_from_external_is_the_term_operator():
+    # check_high_qubit_gate.jl:7-9: from_external(pos, kron(m_1, ..., m_k)) == QubitsTerm(pos[i] => m_i)
+    import vqchip as vq
+    rng = np.random.default_rng(9)
+    for pos in [(3, 1, 4, 2), (2, 5, 1, 4, 3), (2, 1)]:
+        ms = [np.linalg.qr(rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2)))[0] for _ in pos]
+        K = ms[0]
+        for m in ms[1:]:
+            K = np.kron(K, m)
+        g = vq.from_external(pos, K)
+        t = QubitsTerm(dict(zip(pos, ms)))
+        n = max(pos)
+        A = full_op(n, [q - 1 for q in g.ordered_positions()], g.ordered_mat())
+        B = full_op(n, [q - 1 for q in t.positions], t.matrix())
+        assert np.max(np.abs(A - B)) < 1e-14
+
+
 def test_inactive_parameters_and_slots():
     # test/parameter.jl: isparas=false parameters are not counted and get no slot
     c = QCircuit([RxGate(1, 0.1), RxGate(1, 1.1, isparas=False), CNOTGate(1, 2), RyGate(2, 0.2)])

@@ -8,7 +8,7 @@ from .circuit import Program, QCircuit
 from .gates import (CONTROL, CONTROLCONTROLGate, CONTROLGate, R, AmplitudeDamping, CCPHASEGate, CNOTGate, CPHASEGate,
                     CRxGate, CRyGate, CRzGate, CZGate, Depolarizing, FREDKINGate, FSIMGate, Gate, HGate, PHASEGate,
                     QuantumGate, QuantumMap, RxGate, RyGate, RzGate, SGate, SWAPGate, TGate, TOFFOLIGate, XGate, YGate,
-                    ZGate, iSWAPdagGate, iSWAPGate)
+                    ZGate, from_external, iSWAPdagGate, iSWAPGate)
 from .operators import QubitsOperator, QubitsTerm, pack_pauli, pack_terms
 from .state import DensityMatrix, StateVector, onehot_encoding, qubit_encoding, rand_state
 from .utility import QFT

@@ -370,6 +370,15 @@ class CCPHASEGate(_PhaseFamily):
         super().__init__((a, b, c), np.diag(d), (phi,), isparas, [g])
 
 
+def from_external(positions, mat):
+    """A dense gate from a matrix in the external order: kron(m_1, ..., m_k) puts m_1
+    on positions[0] (the first factor of the kron is the first position), the order
+    test/check_high_qubit_gate.jl:7-8 uses against QubitsTerm(j => m_1, ...). It is
+    QuantumGate over the reversed positions."""
+    positions = tuple(positions) if not isinstance(positions, int) else (positions,)
+    return QuantumGate(tuple(reversed(positions)), mat)
+
+
 def CONTROLGate(positions, U):
     """Controlled-U on positions (control, targets...): CONTROL(U) over the given
     order, the control first (test/check_gateop.jl:104)."""
