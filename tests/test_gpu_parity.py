@@ -346,3 +346,35 @@ def test_ansatz_block_shapes_energy_gradient(n, reverse_ladder):
     assert np.max(np.abs(psi - psi_r)) < 1e-12
     assert abs(E - Er) < 1e-10 * max(1, abs(Er))
     assert np.max(np.abs(g - gr)) < 1e-10 * max(1, np.max(np.abs(gr)))
+
+
+@pytest.mark.parametrize("n,reverse_ladder", [(5, False), (5, True), (7, True)])
+def test_dm_ansatz_block_shapes_energy_gradient(n, reverse_ladder):
+    # the noisy layer of the density-matrix workload (Depolarizing after each CNOT,
+    # AmplitudeDamping per qubit per layer): its reverse sweep runs the straight-line
+    # block for a CNOT row / column uncompute, Rz^dagger, the Rz gradient term and
+    # Ry^dagger on the CNOT's target, in both orientations of the ladder
+    rng = np.random.default_rng(1300 + n + reverse_ladder)
+    c = vq.QCircuit()
+    for _ in range(2):
+        for q in range(1, n + 1):
+            c.append(vq.RxGate(q, float(rng.uniform(0, 6.3))))
+            c.append(vq.RyGate(q, float(rng.uniform(0, 6.3))))
+            c.append(vq.RzGate(q, float(rng.uniform(0, 6.3))))
+        ladder = [(q + 1, q) for q in range(n - 1, 0, -1)] if reverse_ladder else [(q, q + 1) for q in range(1, n)]
+        for a, b in ladder:
+            c.append(vq.CNOTGate(a, b))
+            c.append(vq.Depolarizing(b, 0.02))
+        for q in range(1, n + 1):
+            c.append(vq.AmplitudeDamping(q, 0.01))
+    H = heisenberg(n, rng.standard_normal(n - 1))
+    rho0 = vec(rand_density(n, rng))
+    d = vq.DensityMatrix(rho0)
+    E, g, out = vq.energy_and_gradient(H, c, d, fused=True, return_state=True)
+    prog = Program(c, n=n, dm=True)
+    terms, nt, hpool = pack_terms(H)
+    Er, gr, rho_r = oracle.energy_grad(rho0, n, 1, prog.op_array(), len(prog.ops), prog.pool(), terms, nt, hpool,
+                                       prog.nparams)
+    assert np.max(np.abs(vec(out.storage()) - rho_r)) < 1e-12
+    assert abs(E - Er) < 1e-10 * max(1, abs(Er))
+    assert np.max(np.abs(g - gr)) < 1e-10 * max(1, np.max(np.abs(gr)))
