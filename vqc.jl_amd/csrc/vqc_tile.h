@@ -601,6 +601,25 @@ __device__ __forceinline__ int dm_rev_shape(const MicroOp& op) {
     return 0;
 }
 
+// Two more density-matrix reverse blocks on a qubit's row and column bits (66 each
+// in the same circuit): the Ry gradient terms on both bits, then Rx^dagger on both
+// (kDmRyRx), and the Rx gradient terms on both bits alone (kDmRx).
+constexpr int kDmRyRxSmask = BS_LO | (BS_HI << 2) | (BS_LO << 4) | (BS_HI << 6);
+constexpr int kDmRyRxMstr = (MS_REAL << 2) | ((MS_REAL << 2) << 4) | (MS_XLIKE << 8) | (MS_XLIKE << 12);
+constexpr int kDmRxSmask = BS_LO | (BS_HI << 2);
+constexpr int kDmRxMstr = (MS_XLIKE << 2) | ((MS_XLIKE << 2) << 4);
+
+// 1: kDmRyRx, 2: kDmRx, 0: neither
+__device__ __forceinline__ int dm_pair_shape(const MicroOp& op) {
+    if (op.nchain == 4 && op.smask == kDmRyRxSmask && op.mstr == kDmRyRxMstr && op.cmat[0] < 0 && op.cmat[1] < 0 &&
+        op.gen[0] >= 0 && op.gen[1] >= 0 && op.cmat[2] >= 0 && op.cmat[3] >= 0 && op.gen[2] < 0 && op.gen[3] < 0)
+        return 1;
+    if (op.nchain == 2 && op.smask == kDmRxSmask && op.mstr == kDmRxMstr && op.cmat[0] < 0 && op.cmat[1] < 0 &&
+        op.gen[0] >= 0 && op.gen[1] >= 0)
+        return 2;
+    return 0;
+}
+
 __device__ __forceinline__ int hea_fwd_shape(const MicroOp& op) {
     if (op.nchain != 2 || op.cmat[0] < 0 || op.cmat[1] < 0) return 0;
     if (op.smask == kHeaFwdSmask && op.mstr == kHeaFwdMstr) return 1;
@@ -729,6 +748,52 @@ __device__ __forceinline__ void mop_dm_rev(double2* __restrict__ sa, double2* __
     wave_partial_re(red, buf, 0, g);
 }
 
+// kDmRyRx (RYRX) or kDmRx on the quad x00, x10, x01, x11 over (lo, hi): the
+// gradient term of each bit's generator, lo then hi, then (RYRX) the X-like
+// matrix on lo and on hi. kDmRx leaves the tile unchanged, so it stores nothing.
+template <bool RYRX>
+__device__ __forceinline__ void mop_dm_pair(double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
+                                            const double2* __restrict__ pool, double2* red, int buf) {
+    constexpr int GS = RYRX ? MS_REAL : MS_XLIKE;
+    double g0 = 0.0, g1 = 0.0;
+    const uint32_t ng = 1u << (t - 2);
+    const uint32_t olo = 1u << op.lpos[0], ohi = 1u << op.lpos[1];
+    const Mat2 e0 = load_mat2(pool + op.gen[0]), e1 = load_mat2(pool + op.gen[1]);
+    Mat2 m2{}, m3{};
+    if (RYRX) {
+        m2 = load_mat2(pool + op.cmat[2]);
+        m3 = load_mat2(pool + op.cmat[3]);
+    }
+    for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
+        const uint32_t base = ins32<2>(g, op.lpos);
+        const uint32_t i00 = base, i10 = base + olo, i01 = base + ohi, i11 = base + olo + ohi;
+        double2 a00 = sa[i00], a10 = sa[i10], a01 = sa[i01], a11 = sa[i11];
+        double2 b00 = sb[i00], b10 = sb[i10], b01 = sb[i01], b11 = sb[i11];
+        g0 += gdot2<GS>(e0, a00, a10, b00, b10) + gdot2<GS>(e0, a01, a11, b01, b11);  // lo pairs
+        g1 += gdot2<GS>(e1, a00, a01, b00, b01) + gdot2<GS>(e1, a10, a11, b10, b11);  // hi pairs
+        if (RYRX) {
+            m2_ip<MS_XLIKE>(m2, a00, a10);
+            m2_ip<MS_XLIKE>(m2, a01, a11);
+            m2_ip<MS_XLIKE>(m2, b00, b10);
+            m2_ip<MS_XLIKE>(m2, b01, b11);
+            m2_ip<MS_XLIKE>(m3, a00, a01);
+            m2_ip<MS_XLIKE>(m3, a10, a11);
+            m2_ip<MS_XLIKE>(m3, b00, b01);
+            m2_ip<MS_XLIKE>(m3, b10, b11);
+            sa[i00] = a00;
+            sa[i10] = a10;
+            sa[i01] = a01;
+            sa[i11] = a11;
+            sb[i00] = b00;
+            sb[i10] = b10;
+            sb[i01] = b01;
+            sb[i11] = b11;
+        }
+    }
+    wave_partial_re(red, buf, 0, g0);
+    wave_partial_re(red, buf, 1, g1);
+}
+
 template <bool CXLO>
 __device__ __forceinline__ void mop_hea_fwd(double2* __restrict__ sa, int t, const MicroOp& op, const double2* __restrict__ pool) {
     const uint32_t ng = 1u << (t - 2);
@@ -850,8 +915,11 @@ __device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const M
                     else mop_block(sa, sb, t, op, pool, red, buf, two);
                 } else if constexpr (KS == KS_REVX) {  // general reverse: the density-matrix shape first
                     const int shape = dm_rev_shape(op);
+                    const int pshape = shape ? 0 : dm_pair_shape(op);
                     if (shape == 1) mop_dm_rev<true>(sa, sb, t, op, pool, red, buf);
                     else if (shape == 2) mop_dm_rev<false>(sa, sb, t, op, pool, red, buf);
+                    else if (pshape == 1) mop_dm_pair<true>(sa, sb, t, op, pool, red, buf);
+                    else if (pshape == 2) mop_dm_pair<false>(sa, sb, t, op, pool, red, buf);
                     else mop_block(sa, sb, t, op, pool, red, buf, two);
                 } else if constexpr (KS == KS_FWDH) {
                     const int shape = hea_fwd_shape(op);
