@@ -378,3 +378,41 @@ def test_dm_ansatz_block_shapes_energy_gradient(n, reverse_ladder):
     assert np.max(np.abs(vec(out.storage()) - rho_r)) < 1e-12
     assert abs(E - Er) < 1e-10 * max(1, abs(Er))
     assert np.max(np.abs(g - gr)) < 1e-10 * max(1, np.max(np.abs(gr)))
+
+
+def mixed_unitary_channel(positions, rng, k=3):
+    """sum_i w_i U_i rho U_i^dagger with random unitaries: a dense superoperator, so
+    it has no parity-block form (unlike depolarizing and amplitude damping)."""
+    w = rng.uniform(0.2, 1.0, k)
+    w /= w.sum()
+    return vq.QuantumMap(positions, [np.sqrt(x) * rand_unitary(len(positions), rng) for x in w])
+
+
+@pytest.mark.parametrize("n", [4, 6])
+def test_dm_general_and_parity_channels(n):
+    # the fused path takes the parity-block sweep for depolarizing / amplitude damping
+    # and the general 4 x 4 and 16 x 16 superoperator sweeps for the other channels
+    rng = np.random.default_rng(1400 + n)
+    c = vq.QCircuit()
+    for layer in range(2):
+        for q in range(1, n + 1):
+            c.append(vq.RyGate(q, float(rng.uniform(0, 6.3))))
+            c.append(vq.RzGate(q, float(rng.uniform(0, 6.3))))
+        for q in range(1, n):
+            c.append(vq.CNOTGate(q, q + 1))
+            c.append(vq.Depolarizing(q + 1, 0.03) if q % 2 else mixed_unitary_channel((q + 1,), rng))
+        c.append(mixed_unitary_channel((1, n), rng))
+        c.append(vq.AmplitudeDamping(2, 0.05))
+    H = heisenberg(n, rng.standard_normal(n - 1))
+    rho0 = vec(rand_density(n, rng))
+    prog = Program(c, n=n, dm=True)
+    terms, nt, hpool = pack_terms(H)
+    Er, gr, rho_r = oracle.energy_grad(rho0, n, 1, prog.op_array(), len(prog.ops), prog.pool(), terms, nt, hpool,
+                                       prog.nparams)
+    d = vq.DensityMatrix(rho0)
+    vq.apply_(c, d)
+    assert np.max(np.abs(vec(d.storage()) - rho_r)) < 1e-12
+    E, g, out = vq.energy_and_gradient(H, c, vq.DensityMatrix(rho0), fused=True, return_state=True)
+    assert np.max(np.abs(vec(out.storage()) - rho_r)) < 1e-12
+    assert abs(E - Er) < 1e-10 * max(1, abs(Er))
+    assert np.max(np.abs(g - gr)) < 1e-10 * max(1, np.max(np.abs(gr)))

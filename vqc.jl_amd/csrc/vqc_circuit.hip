@@ -319,8 +319,24 @@ int64_t XPool::product(const int64_t* srcs, int n, int D) {
 
 int XPool::classify(int64_t off, int D) {
     const int ms = host_mstruct(h.data() + 2 * off, D);
-    claims.push_back({off, D, ms});
+    claims.push_back({off, D, ms, 0, 0});
     return ms;
+}
+
+// entries (r, c) of a 4 x 4 with parity(r) != parity(c) are all exactly zero
+static bool host_parity4(const double* M) {
+    for (int c = 0; c < 4; ++c)
+        for (int r = 0; r < 4; ++r) {
+            const bool same = ((r ^ (r >> 1)) & 1) == ((c ^ (c >> 1)) & 1);
+            if (!same && (M[2 * (r + 4 * c)] != 0.0 || M[2 * (r + 4 * c) + 1] != 0.0)) return false;
+        }
+    return true;
+}
+
+bool XPool::parity_blocks(int64_t off) {
+    const bool p = host_parity4(h.data() + 2 * off);
+    claims.push_back({off, 4, p ? 1 : 0, 1, 0});
+    return p;
 }
 
 bool XPool::reload(const double* pool, int64_t plen, const double* hpool, int64_t hlen) {
@@ -330,8 +346,10 @@ bool XPool::reload(const double* pool, int64_t plen, const double* hpool, int64_
         if (e.v == 8) make_product(h.data(), e.srcs, e.nsrc, e.D, h.data() + 2 * e.dst);
         else make_variant(h.data() + 2 * e.src, e.D, e.v, h.data() + 2 * e.dst);
     }
-    for (const XPoolClaim& c : claims)
-        if (host_mstruct(h.data() + 2 * c.off, c.D) != c.ms) return false;
+    for (const XPoolClaim& c : claims) {
+        const int now = c.test == 1 ? int(host_parity4(h.data() + 2 * c.off)) : host_mstruct(h.data() + 2 * c.off, c.D);
+        if (now != c.ms) return false;
+    }
     return true;
 }
 
@@ -579,6 +597,7 @@ int vqc_plan_forward(PlanBuilder& pb, int nbits, int n, int is_dm, int nops, con
                 set_lpos(m, p, b.data(), int(b.size()));
                 m.mat = o.mat;
                 if (o.kind != VQC_OP_CHANNEL) m.mstr = mstruct_of(xp, m.mat, 1 << o.nq, hint_mat(o));
+                else if (o.nq == 1 && !xp.deferred && xp.parity_blocks(m.mat)) m.mstr = kMsParity;
                 pb.mops.push_back(m);
             } else {
                 int cols[5];
@@ -640,6 +659,7 @@ int vqc_plan_reverse(PlanBuilder& pb, int nbits, int n, int is_dm, int nops, con
                 set_lpos(m, p, b.data(), int(b.size()));
                 m.mat = o.aux;                                           // state: S^-1
                 m.mat2 = xp.variant(o.mat, D * D, MF_CONJ | MF_TRANS);   // adjoint: S^dagger
+                if (o.nq == 1 && !xp.deferred && xp.parity_blocks(m.mat) && xp.parity_blocks(m.mat2)) m.mstr = kMsParity;
                 pb.mops.push_back(m);
                 return;
             }

@@ -58,6 +58,11 @@ constexpr int kLoadBatch = 8;      // global loads in flight per thread when a t
 constexpr int kMaxTileBits = 13;   // one tile buffer in LDS (forward phases): 128 KiB
 constexpr int kMaxTileBits2 = 12;  // two tile buffers (Hamiltonian / reverse phases, whole small evaluations)
 constexpr int kMaxAcc = 48;  // accumulator entries per phase
+// MicroOp::mstr flag of a 2-bit MK_APPLY / MK_APPLY2X whose 4 x 4 matrices only
+// couple indices of equal parity (b0 xor b1): entries {00, 11} x {00, 11} and
+// {01, 10} x {01, 10}. Depolarizing and amplitude-damping superoperators have this
+// form, and so do their inverses and adjoints.
+constexpr int kMsParity = 1 << 16;
 
 struct MicroOp {
     int32_t kind;
@@ -253,6 +258,50 @@ __device__ __forceinline__ void apply1(double2* __restrict__ s, uint32_t ng, con
         const Pair2 r = m2<MS>(m, s[base], s[base + o1]);
         s[base] = r.x0;
         s[base + o1] = r.x1;
+    }
+}
+
+// y = M x for a 4 x 4 M with the parity-block form (kMsParity), x over (lpos[0],
+// lpos[1]): two 2 x 2 products, on (x0, x3) and on (x1, x2)
+struct Par4 {
+    double2 m00, m30, m03, m33, m11, m21, m12, m22;
+};
+__device__ __forceinline__ Par4 load_par4(const double2* __restrict__ M) {
+    return Par4{M[0], M[3], M[12], M[15], M[5], M[6], M[9], M[10]};
+}
+__device__ __forceinline__ void par4_ip(const Par4& m, double2& x0, double2& x1, double2& x2, double2& x3) {
+    const double2 y0 = cfma(m.m03, x3, cmul(m.m00, x0)), y3 = cfma(m.m33, x3, cmul(m.m30, x0));
+    const double2 y1 = cfma(m.m12, x2, cmul(m.m11, x1)), y2 = cfma(m.m22, x2, cmul(m.m21, x1));
+    x0 = y0;
+    x1 = y1;
+    x2 = y2;
+    x3 = y3;
+}
+// MK_APPLY (one buffer) or MK_APPLY2X (A <- M A, B <- M2 B) of a parity-block channel
+template <bool TWO>
+__device__ __forceinline__ void mop_parity(double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
+                                           const double2* __restrict__ pool) {
+    const uint32_t ng = 1u << (t - 2);
+    const uint32_t o0 = 1u << op.lpos[0], o1 = 1u << op.lpos[1];
+    const Par4 m = load_par4(pool + op.mat);
+    const Par4 m2 = TWO ? load_par4(pool + op.mat2) : m;
+    for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
+        const uint32_t base = ins32<2>(g, op.lpos);
+        const uint32_t i1 = base + o0, i2 = base + o1, i3 = base + o0 + o1;
+        double2 x0 = sa[base], x1 = sa[i1], x2 = sa[i2], x3 = sa[i3];
+        par4_ip(m, x0, x1, x2, x3);
+        sa[base] = x0;
+        sa[i1] = x1;
+        sa[i2] = x2;
+        sa[i3] = x3;
+        if (TWO) {
+            double2 y0 = sb[base], y1 = sb[i1], y2 = sb[i2], y3 = sb[i3];
+            par4_ip(m2, y0, y1, y2, y3);
+            sb[base] = y0;
+            sb[i1] = y1;
+            sb[i2] = y2;
+            sb[i3] = y3;
+        }
     }
 }
 
@@ -872,6 +921,7 @@ __device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const M
             if constexpr ((KS & (1 << MK_APPLY)) != 0) {
             double2* s = op.buf ? sb : sa;
             if (kb == 1) mop_apply<1>(s, t, op, pool);
+            else if (kb == 2 && (op.mstr & kMsParity)) mop_parity<false>(s, sb, t, op, pool);
             else if (kb == 2) mop_apply<2>(s, t, op, pool);
             else if constexpr (MAXKB >= 3) {
                 if (kb == 3) mop_apply<3>(s, t, op, pool);
@@ -886,7 +936,15 @@ __device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const M
         case MK_APPLY2X:
         case MK_GRAD:
             if constexpr ((KS & ((1 << MK_APPLY2) | (1 << MK_APPLY2X) | (1 << MK_GRAD))) != 0) {
-                if (kb == 1) mop_pair<1>(sa, sb, t, op, pool, red, buf);
+                bool done = false;
+                if constexpr ((KS & (1 << MK_APPLY2X)) != 0) {
+                    if (kb == 2 && op.kind == MK_APPLY2X && op.ng == 0 && (op.mstr & kMsParity)) {
+                        mop_parity<true>(sa, sb, t, op, pool);
+                        done = true;
+                    }
+                }
+                if (done) {
+                } else if (kb == 1) mop_pair<1>(sa, sb, t, op, pool, red, buf);
                 else if (kb == 2) mop_pair<2>(sa, sb, t, op, pool, red, buf);
                 else if constexpr (MAXKB >= 3) {
                     if (kb == 3) mop_pair<3>(sa, sb, t, op, pool, red, buf);
@@ -1064,7 +1122,9 @@ struct XPoolEntry {
 };
 struct XPoolClaim {
     int64_t off;
-    int32_t D, ms;  // the structure class found for the D x D matrix at off
+    int32_t D, ms;  // what was found for the D x D matrix at off: its class (MS_*), or for
+    int32_t test;   // test 1, whether it is a parity-block channel (1) or not (0)
+    int32_t pad;
 };
 struct XPool {
     std::vector<double> h;  // interleaved complex
@@ -1084,6 +1144,8 @@ struct XPool {
     int64_t product(const int64_t* srcs, int n, int D);
     // the structure class (MS_*) of the D x D matrix at off, from its values, recorded
     int classify(int64_t off, int D);
+    // whether the 4 x 4 matrix at off only couples indices of equal bit parity, recorded
+    bool parity_blocks(int64_t off);
     // New caller matrices (pool, then hpool, the lengths the pool was made with):
     // every derived matrix is recomputed from them in the order it was made.
     // False when a structure class the planner read no longer holds.
