@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle
-from brute import rand_state, vec
+from brute import rand_density, rand_state, vec
 import vqchip as vq
 from vqchip import _lib
 from vqchip.circuit import Program
@@ -148,3 +148,38 @@ def test_density_matrix_and_vjp_reuse():
         gr, dxr, _ = oracle.circuit_vjp(x, n, 0, prog.op_array(), len(prog.ops), prog.pool(), prog.nparams, dy)
         assert np.max(np.abs(g - gr)) < 1e-10 * max(1, np.max(np.abs(gr)))
         assert np.max(np.abs(dx.storage() - dxr)) < 1e-12
+
+
+def test_channel_form_change_plans_again():
+    # the same op table with a depolarizing channel (parity-block superoperator) and
+    # then a mixed-unitary one (dense): the parity claim fails and the plan is redone
+    from test_gpu_parity import mixed_unitary_channel
+    n = 4
+    rng = np.random.default_rng(21)
+    rho0 = vec(rand_density(n, rng))
+    H = tfim(n, 0.6)
+
+    def circ(ch):
+        c = vq.QCircuit()
+        for q in range(1, n + 1):
+            c.append(vq.RyGate(q, 0.3 * q))
+        c.append(vq.CNOTGate(1, 2))
+        c.append(ch)
+        c.append(vq.RzGate(2, 0.7))
+        return c
+
+    expected = [None, 0, 1, 0, 1]
+    channels = [vq.Depolarizing(2, 0.1), mixed_unitary_channel((2,), rng), mixed_unitary_channel((2,), rng),
+                vq.AmplitudeDamping(2, 0.2), vq.Depolarizing(2, 0.05)]
+    for ch, want in zip(channels, expected):
+        c = circ(ch)
+        E, g, out = vq.energy_and_gradient(H, c, vq.DensityMatrix(rho0), fused=True, return_state=True)
+        r = reused()
+        prog = Program(c, n=n, dm=True)
+        terms, nt, hpool = pack_terms(H)
+        Er, gr, rho_r = oracle.energy_grad(rho0, n, 1, prog.op_array(), len(prog.ops), prog.pool(), terms, nt, hpool,
+                                           prog.nparams)
+        assert np.max(np.abs(vec(out.storage()) - rho_r)) < 1e-12
+        assert abs(E - Er) < 1e-10 and np.max(np.abs(g - gr)) < 1e-10
+        if want is not None:
+            assert r == want, (ch, r)
