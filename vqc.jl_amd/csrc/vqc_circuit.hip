@@ -521,6 +521,21 @@ void vqc_fuse_chains(PlanBuilder& pb, size_t phase_begin, size_t phase_end) {
     for (size_t i = phase_begin; i < phase_end; ++i) {
         PhaseDesc& p = pb.phases[i];
         const int b = int(out.size());
+        if (p.kind == PH_FWD) {
+            // One-bit steps on different bits commute exactly. Within each run of
+            // consecutive one-bit MK_APPLY on one buffer, order them by bit (the order
+            // on any one bit kept), so that each bit's steps form one run for the
+            // product below: the row and column steps of a density-matrix gate
+            // alternate between two bits, and so do gates on different qubits.
+            for (int o = p.op_begin; o < p.op_end;) {
+                int e = o;
+                while (e < p.op_end && pb.mops[e].kind == MK_APPLY && pb.mops[e].kb == 1 && pb.mops[e].buf == pb.mops[o].buf) ++e;
+                if (e - o > 1)
+                    std::stable_sort(pb.mops.begin() + o, pb.mops.begin() + e,
+                                     [](const MicroOp& x, const MicroOp& y) { return x.lpos[0] < y.lpos[0]; });
+                o = std::max(e, o + 1);
+            }
+        }
         for (int o = p.op_begin; o < p.op_end;) {
             const MicroOp& m = pb.mops[o];
             int run = 1;
