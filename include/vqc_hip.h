@@ -124,8 +124,11 @@ typedef struct vqc_tuning {
  * (2n for an n-qubit density matrix) runs as one workgroup up to b = 9, else in
  * tiles of 9 bits up to b = 14, 10 up to 18, 11 up to 20 and 12 forward / 11
  * reverse above: smaller tiles put more CUs on a small state, larger ones put
- * more gates in each pass. Batches (vqc_batch_*, the serving loop's batches)
- * take 12 / 11 and one workgroup per instance up to 12 bits. */
+ * more gates in each pass. A batch (vqc_batch_*) of at least 16 instances takes
+ * 13 / 12, the largest tiles, since its instances fill the device together; a
+ * smaller batch (the serving loop's usual one or two requests) gives each tiled
+ * instance the single-state shape of its size. Batch instances of up to 12 qubits
+ * run in one workgroup each. */
 int vqc_set_tuning(const vqc_tuning_t* t);
 int vqc_get_tuning(vqc_tuning_t* t);
 

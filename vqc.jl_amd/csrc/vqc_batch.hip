@@ -378,7 +378,12 @@ int vqc_batch_build(int ninst, const vqc_instance_t* inst, const vqc_op_t* ops, 
         return vqc_fail(VQC_EINVAL, "vqc_batch_create: bad arguments");
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));
+    // A wide batch fills the device with its instances and takes the batch tile
+    // shape; a narrow one (the serving loop's usual single request) gives each
+    // tiled instance the shape of a lone state of its size. Instances of up to
+    // tt.t_small qubits run in one workgroup either way.
     const TileTuning tt = vqc_tile_tuning();
+    const bool wide = ninst >= kWideBatch;
     vqc_batch* b = new vqc_batch;
     b->opts = opts;
     b->stream = opts.stream ? opts.stream : c->stream;
@@ -467,18 +472,19 @@ int vqc_batch_build(int ninst, const vqc_instance_t* inst, const vqc_op_t* ops, 
             }
         }
         if (!small) {
+            const TileTuning ti = wide ? tt : vqc_tile_tuning(n);
             // phases: forward (first reads |0..0>), Hamiltonian, reverse; rows are global
-            vqc_plan_forward(pb, n, n, 0, I.nops, iops, tt.t_fwd, tt.mlow);
+            vqc_plan_forward(pb, n, n, 0, I.nops, iops, ti.t_fwd, ti.mlow);
             if (pb.phases.size() == ph0) {  // no ops: one empty forward phase writes |0..0> into A
                 PhaseDesc p;
-                vqc_make_phase(p, PH_FWD, n, tt.t_fwd, {});
+                vqc_make_phase(p, PH_FWD, n, ti.t_fwd, {});
                 p.op_begin = p.op_end = int(pb.mops.size());
                 p.row_begin = pb.nrows;
                 pb.phases.push_back(p);
             }
             pb.phases[ph0].flags |= PF_INIT_ZERO;
-            vqc_plan_ham(pb, n, I.nterms, iterms, pool_len, tt.t_rev, tt.mlow);
-            vqc_plan_reverse(pb, n, n, 0, I.nops, iops, tt.t_rev, tt.mlow);
+            vqc_plan_ham(pb, n, I.nterms, iterms, pool_len, ti.t_rev, ti.mlow);
+            vqc_plan_reverse(pb, n, n, 0, I.nops, iops, ti.t_rev, ti.mlow);
             vqc_fuse_chains(pb, ph0, pb.phases.size());
             bi.nrows = pb.nrows - rows0;
             bi.a_off = arena;

@@ -28,12 +28,14 @@ void vqc_zero(double2* v, uint64_t len, hipStream_t st);
 // else in tiles of 9 bits up to 14, 10 up to 18, 11 up to 20 and 12 (forward) /
 // 11 (reverse) above: 2^(nbits - t) tiles must keep enough CUs busy, while
 // each phase, one launch, covers fewer gates the smaller its tiles (the sizes
-// measured fastest for layered circuits of 8 to 20 qubits on MI355X). A batch
-// (nbits < 0), whose instances fill the device together, takes 12 / 11 and a
-// whole-state workgroup up to 12 bits.
+// measured fastest for layered circuits of 8 to 20 qubits on MI355X). A wide
+// batch (nbits < 0), whose instances fill the device together, takes the
+// largest tiles, 13 / 12: fewer phases and HBM passes per instance (256 mixed
+// instances: 90 ms at 12 / 11, 66 ms at 13 / 12), and a whole-state workgroup
+// up to 12 bits.
 TileTuning vqc_tile_tuning(int nbits) {
     const vqc_tuning_t g = vqc_tuning_now();
-    int fwd = 12, rev = 11, small = 12;
+    int fwd = kMaxTileBits, rev = kMaxTileBits2, small = 12;
     if (nbits >= 0) {
         small = 9;
         fwd = nbits <= 14 ? 9 : (nbits <= 18 ? 10 : (nbits <= 20 ? 11 : 12));

@@ -58,6 +58,7 @@ constexpr int kLoadBatch = 8;      // global loads in flight per thread when a t
 constexpr int kMaxTileBits = 13;   // one tile buffer in LDS (forward phases): 128 KiB
 constexpr int kMaxTileBits2 = 12;  // two tile buffers (Hamiltonian / reverse phases, whole small evaluations)
 constexpr int kMaxAcc = 48;  // accumulator entries per phase
+constexpr int kWideBatch = 16;  // batches of at least this many instances take the batch tile shape
 // MicroOp::mstr flag of a 2-bit MK_APPLY / MK_APPLY2X whose 4 x 4 matrices only
 // couple indices of equal parity (b0 xor b1): entries {00, 11} x {00, 11} and
 // {01, 10} x {01, 10}. Depolarizing and amplitude-damping superoperators have this
