@@ -352,8 +352,9 @@ def test_ansatz_block_shapes_energy_gradient(n, reverse_ladder):
 def test_dm_ansatz_block_shapes_energy_gradient(n, reverse_ladder):
     # the noisy layer of the density-matrix workload (Depolarizing after each CNOT,
     # AmplitudeDamping per qubit per layer): its reverse sweep runs the straight-line
-    # block for a CNOT row / column uncompute, Rz^dagger, the Rz gradient term and
-    # Ry^dagger on the CNOT's target, in both orientations of the ladder
+    # block for a CNOT row (or column) uncompute, then Rz^dagger, Ry^dagger and
+    # Rx^dagger with their gradient terms on the CNOT's target, in both orientations
+    # of the ladder
     rng = np.random.default_rng(1300 + n + reverse_ladder)
     c = vq.QCircuit()
     for _ in range(2):

@@ -731,6 +731,25 @@ int vqc_plan_reverse(PlanBuilder& pb, int nbits, int n, int is_dm, int nops, con
             }
             int cols[5];
             for (int j = 0; j < o.nq; ++j) cols[j] = o.pos[j] + n;
+            if (o.nq == 1 && na == 1) {
+                // A one-qubit, one-parameter gate: each side's gradient term right after
+                // that side's uncompute, in the same micro-op. The other side's uncompute
+                // acts on another bit and is unitary, so it leaves the term unchanged:
+                // <C D| G_row C y> = <D| G_row y> for C on the column bit.
+                auto side = [&](const int* bits, int64_t mat, int64_t gen) {
+                    MicroOp m = mop_init(MK_APPLY2);
+                    set_lpos(m, p, bits, 1);
+                    m.mat = m.mat2 = mat;
+                    m.mstr = mstruct_of(xp, mat, D, hint_mat(o)) | (mstruct_of(xp, gen, D, hint_gen(o)) << 2);
+                    m.ng = 1;
+                    m.gen[0] = gen;
+                    m.acc[0] = accs[0];
+                    pb.mops.push_back(m);
+                };
+                side(o.pos, udag, gens[0]);                                                  // rows
+                side(cols, xp.variant(o.mat, D, MF_TRANS), xp.variant(gens[0], D, MF_CONJ));  // columns
+                return;
+            }
             uncompute(o.pos, udag, false);                             // U^dagger on rows
             uncompute(cols, xp.variant(o.mat, D, MF_TRANS), false);    // conj(U^dagger) = U^T on columns
             for (int j = 0; j < na; ++j)                               // <D|G_row y> + <D|conj(G)_col y>

@@ -618,7 +618,9 @@ __device__ __forceinline__ void mop_block(double2* __restrict__ sa, double2* __r
 // Ry^dagger, Rx^dagger on the CNOT's target, each with its gradient (diagonal,
 // real, X-like classes), and its forward counterpart, a CNOT followed by a fused
 // 2x2 on its control (KS_FWDH kernels only). CXLO: the control is the pair's
-// low bit.
+// low bit. Density matrices run the same reverse shape twice per CNOT, on its row
+// bits and on its column bits (each side's gradient term rides with that side's
+// uncompute).
 constexpr int kHeaRevSmask = (BS_HI << 2) | (BS_HI << 4) | (BS_HI << 6);  // BOTH, then the target three times
 constexpr int kHeaRevSmaskHi = (BS_LO << 2) | (BS_LO << 4) | (BS_LO << 6);
 constexpr int kHeaRevMstr = MS_CX_LO | ((MS_DIAG | MS_DIAG << 2) << 4) | ((MS_REAL | MS_REAL << 2) << 8) |
@@ -632,41 +634,6 @@ __device__ __forceinline__ int hea_rev_shape(const MicroOp& op) {
     if (op.nchain != 4 || op.cmat[0] < 0 || op.gen[0] >= 0 || op.gen[1] < 0 || op.gen[2] < 0 || op.gen[3] < 0) return 0;
     if (op.smask == kHeaRevSmask && op.mstr == kHeaRevMstr) return 1;
     if (op.smask == kHeaRevSmaskHi && op.mstr == kHeaRevMstrHi) return 2;
-    return 0;
-}
-
-// The density-matrix reverse block the same layers produce most often (132 of 282
-// reverse blocks of the 12-qubit noisy circuit): the row (or column) uncompute of a
-// CNOT, then on its target the Rz^dagger uncompute, the Rz gradient term (a
-// gradient-only step with the diagonal generator) and the Ry^dagger uncompute.
-constexpr int kDmRevMstr = MS_CX_LO | (MS_DIAG << 4) | ((MS_DIAG << 2) << 8) | (MS_REAL << 12);
-constexpr int kDmRevMstrHi = MS_CX_HI | (kDmRevMstr & ~3);
-
-__device__ __forceinline__ int dm_rev_shape(const MicroOp& op) {
-    if (op.nchain != 4 || op.cmat[0] < 0 || op.gen[0] >= 0 || op.cmat[1] < 0 || op.gen[1] >= 0 || op.cmat[2] >= 0 ||
-        op.gen[2] < 0 || op.cmat[3] < 0 || op.gen[3] >= 0)
-        return 0;
-    if (op.smask == kHeaRevSmask && op.mstr == kDmRevMstr) return 1;
-    if (op.smask == kHeaRevSmaskHi && op.mstr == kDmRevMstrHi) return 2;
-    return 0;
-}
-
-// Two more density-matrix reverse blocks on a qubit's row and column bits (66 each
-// in the same circuit): the Ry gradient terms on both bits, then Rx^dagger on both
-// (kDmRyRx), and the Rx gradient terms on both bits alone (kDmRx).
-constexpr int kDmRyRxSmask = BS_LO | (BS_HI << 2) | (BS_LO << 4) | (BS_HI << 6);
-constexpr int kDmRyRxMstr = (MS_REAL << 2) | ((MS_REAL << 2) << 4) | (MS_XLIKE << 8) | (MS_XLIKE << 12);
-constexpr int kDmRxSmask = BS_LO | (BS_HI << 2);
-constexpr int kDmRxMstr = (MS_XLIKE << 2) | ((MS_XLIKE << 2) << 4);
-
-// 1: kDmRyRx, 2: kDmRx, 0: neither
-__device__ __forceinline__ int dm_pair_shape(const MicroOp& op) {
-    if (op.nchain == 4 && op.smask == kDmRyRxSmask && op.mstr == kDmRyRxMstr && op.cmat[0] < 0 && op.cmat[1] < 0 &&
-        op.gen[0] >= 0 && op.gen[1] >= 0 && op.cmat[2] >= 0 && op.cmat[3] >= 0 && op.gen[2] < 0 && op.gen[3] < 0)
-        return 1;
-    if (op.nchain == 2 && op.smask == kDmRxSmask && op.mstr == kDmRxMstr && op.cmat[0] < 0 && op.cmat[1] < 0 &&
-        op.gen[0] >= 0 && op.gen[1] >= 0)
-        return 2;
     return 0;
 }
 
@@ -740,108 +707,6 @@ __device__ __forceinline__ void mop_hea_rev(double2* __restrict__ sa, double2* _
     wave_partial_re(red, buf, 0, g1);
     wave_partial_re(red, buf, 1, g2);
     wave_partial_re(red, buf, 2, g3);
-}
-
-// the density-matrix shape above on the pairs (p0, p1), (q0, q1) of A and (r0, r1),
-// (s0, s1) of B: diagonal step, gradient term, real step
-__device__ __forceinline__ double dm_steps(const Mat2& m1, const Mat2& g2, const Mat2& m3, double2& p0, double2& p1,
-                                           double2& q0, double2& q1, double2& r0, double2& r1, double2& s0, double2& s1) {
-    m2_ip<MS_DIAG>(m1, p0, p1);
-    m2_ip<MS_DIAG>(m1, q0, q1);
-    m2_ip<MS_DIAG>(m1, r0, r1);
-    m2_ip<MS_DIAG>(m1, s0, s1);
-    const double g = gdot2<MS_DIAG>(g2, p0, p1, r0, r1) + gdot2<MS_DIAG>(g2, q0, q1, s0, s1);
-    m2_ip<MS_REAL>(m3, p0, p1);
-    m2_ip<MS_REAL>(m3, q0, q1);
-    m2_ip<MS_REAL>(m3, r0, r1);
-    m2_ip<MS_REAL>(m3, s0, s1);
-    return g;
-}
-
-template <bool CXLO>
-__device__ __forceinline__ void mop_dm_rev(double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
-                                           const double2* __restrict__ pool, double2* red, int buf) {
-    double g = 0.0;
-    const uint32_t ng = 1u << (t - 2);
-    const uint32_t olo = 1u << op.lpos[0], ohi = 1u << op.lpos[1];
-    const Mat2 m1 = load_mat2(pool + op.cmat[1]), g2 = load_mat2(pool + op.gen[2]), m3 = load_mat2(pool + op.cmat[3]);
-    for (uint32_t g0 = threadIdx.x; g0 < ng; g0 += kTileThreads) {
-        const uint32_t base = ins32<2>(g0, op.lpos);
-        const uint32_t i00 = base, i10 = base + olo, i01 = base + ohi, i11 = base + olo + ohi;
-        double2 a00 = sa[i00], a10, a01, a11, b00 = sb[i00], b10, b01, b11;
-        if (CXLO) {  // the CNOT as a renaming, as in mop_hea_rev
-            a10 = sa[i11];
-            a11 = sa[i10];
-            a01 = sa[i01];
-            b10 = sb[i11];
-            b11 = sb[i10];
-            b01 = sb[i01];
-            g += dm_steps(m1, g2, m3, a00, a01, a10, a11, b00, b01, b10, b11);
-        } else {
-            a01 = sa[i11];
-            a11 = sa[i01];
-            a10 = sa[i10];
-            b01 = sb[i11];
-            b11 = sb[i01];
-            b10 = sb[i10];
-            g += dm_steps(m1, g2, m3, a00, a10, a01, a11, b00, b10, b01, b11);
-        }
-        sa[i00] = a00;
-        sa[i10] = a10;
-        sa[i01] = a01;
-        sa[i11] = a11;
-        sb[i00] = b00;
-        sb[i10] = b10;
-        sb[i01] = b01;
-        sb[i11] = b11;
-    }
-    wave_partial_re(red, buf, 0, g);
-}
-
-// kDmRyRx (RYRX) or kDmRx on the quad x00, x10, x01, x11 over (lo, hi): the
-// gradient term of each bit's generator, lo then hi, then (RYRX) the X-like
-// matrix on lo and on hi. kDmRx leaves the tile unchanged, so it stores nothing.
-template <bool RYRX>
-__device__ __forceinline__ void mop_dm_pair(double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
-                                            const double2* __restrict__ pool, double2* red, int buf) {
-    constexpr int GS = RYRX ? MS_REAL : MS_XLIKE;
-    double g0 = 0.0, g1 = 0.0;
-    const uint32_t ng = 1u << (t - 2);
-    const uint32_t olo = 1u << op.lpos[0], ohi = 1u << op.lpos[1];
-    const Mat2 e0 = load_mat2(pool + op.gen[0]), e1 = load_mat2(pool + op.gen[1]);
-    Mat2 m2{}, m3{};
-    if (RYRX) {
-        m2 = load_mat2(pool + op.cmat[2]);
-        m3 = load_mat2(pool + op.cmat[3]);
-    }
-    for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
-        const uint32_t base = ins32<2>(g, op.lpos);
-        const uint32_t i00 = base, i10 = base + olo, i01 = base + ohi, i11 = base + olo + ohi;
-        double2 a00 = sa[i00], a10 = sa[i10], a01 = sa[i01], a11 = sa[i11];
-        double2 b00 = sb[i00], b10 = sb[i10], b01 = sb[i01], b11 = sb[i11];
-        g0 += gdot2<GS>(e0, a00, a10, b00, b10) + gdot2<GS>(e0, a01, a11, b01, b11);  // lo pairs
-        g1 += gdot2<GS>(e1, a00, a01, b00, b01) + gdot2<GS>(e1, a10, a11, b10, b11);  // hi pairs
-        if (RYRX) {
-            m2_ip<MS_XLIKE>(m2, a00, a10);
-            m2_ip<MS_XLIKE>(m2, a01, a11);
-            m2_ip<MS_XLIKE>(m2, b00, b10);
-            m2_ip<MS_XLIKE>(m2, b01, b11);
-            m2_ip<MS_XLIKE>(m3, a00, a01);
-            m2_ip<MS_XLIKE>(m3, a10, a11);
-            m2_ip<MS_XLIKE>(m3, b00, b01);
-            m2_ip<MS_XLIKE>(m3, b10, b11);
-            sa[i00] = a00;
-            sa[i10] = a10;
-            sa[i01] = a01;
-            sa[i11] = a11;
-            sb[i00] = b00;
-            sb[i10] = b10;
-            sb[i01] = b01;
-            sb[i11] = b11;
-        }
-    }
-    wave_partial_re(red, buf, 0, g0);
-    wave_partial_re(red, buf, 1, g1);
 }
 
 template <bool CXLO>
@@ -972,13 +837,10 @@ __device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const M
                     if (shape == 1) mop_hea_rev<true>(sa, sb, t, op, pool, red, buf);
                     else if (shape == 2) mop_hea_rev<false>(sa, sb, t, op, pool, red, buf);
                     else mop_block(sa, sb, t, op, pool, red, buf, two);
-                } else if constexpr (KS == KS_REVX) {  // general reverse: the density-matrix shape first
-                    const int shape = dm_rev_shape(op);
-                    const int pshape = shape ? 0 : dm_pair_shape(op);
-                    if (shape == 1) mop_dm_rev<true>(sa, sb, t, op, pool, red, buf);
-                    else if (shape == 2) mop_dm_rev<false>(sa, sb, t, op, pool, red, buf);
-                    else if (pshape == 1) mop_dm_pair<true>(sa, sb, t, op, pool, red, buf);
-                    else if (pshape == 2) mop_dm_pair<false>(sa, sb, t, op, pool, red, buf);
+                } else if constexpr (KS == KS_REVX) {  // general reverse (density matrices): the same shape
+                    const int shape = hea_rev_shape(op);
+                    if (shape == 1) mop_hea_rev<true>(sa, sb, t, op, pool, red, buf);
+                    else if (shape == 2) mop_hea_rev<false>(sa, sb, t, op, pool, red, buf);
                     else mop_block(sa, sb, t, op, pool, red, buf, two);
                 } else if constexpr (KS == KS_FWDH) {
                     const int shape = hea_fwd_shape(op);
