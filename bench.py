@@ -74,7 +74,7 @@ def _eval(vq, _lib, state, args, fused):
     return complex(e[0], e[1]), g[:prog.nparams]
 
 
-def _time_evals(fn, min_s=1.0, max_n=200):
+def _time_evals(fn, min_s=1.0, max_n=5000):
     fn()  # warm
     n, t0 = 0, time.perf_counter()
     while n < max_n and (n < 3 or time.perf_counter() - t0 < min_s):
@@ -185,7 +185,7 @@ def bench_config3(vq, _lib, oracle, timer):
     from vqchip import autotune
     Ef, gf = _eval(vq, _lib, rho0, args, True)
     Ep, gp = _eval(vq, _lib, rho0, args, False)
-    tf_default = 1.0 / _time_evals(lambda: _eval(vq, _lib, rho0, args, True), min_s=1.0, max_n=20)
+    tf_default = 1.0 / _time_evals(lambda: _eval(vq, _lib, rho0, args, True), min_s=1.0, max_n=60)
     tp = 1.0 / _time_evals(lambda: _eval(vq, _lib, rho0, args, False), min_s=1.0, max_n=5)
     # the autotuner picks the tile shape for this workload (tile bits, contiguous low bits)
     before = autotune.get_tuning()
@@ -193,7 +193,7 @@ def bench_config3(vq, _lib, oracle, timer):
     tuned = autotune.tune_energy_grad(H, c, rho0, fwd=(11, 12), rev=(10, 11, 12), low=(2, 3, 4, 6), reps=2)
     tune_s = time.perf_counter() - t0
     Et, gt = _eval(vq, _lib, rho0, args, True)
-    tf = 1.0 / _time_evals(lambda: _eval(vq, _lib, rho0, args, True), min_s=1.0, max_n=20)
+    tf = 1.0 / _time_evals(lambda: _eval(vq, _lib, rho0, args, True), min_s=1.0, max_n=60)
     autotune.set_tuning(**{k: before[k] for k in autotune.FIELDS})
     out = {
         "workload": "12-qubit DM (2^24 entries), 6 layers, depolarizing p=0.01 after each CNOT, "
