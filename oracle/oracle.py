@@ -1,7 +1,8 @@
 """ctypes wrapper of liboracle.so, the CPU restatement of VQC.jl's kernels.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker. The product path never calls it.
+bench.py's cpu_baseline leg, to compare the device results against. The product
+path never calls it.
 Every function here forwards to vqc_oracle.cpp, whose routines cite the
 reference file:line they restate. Pinning (see tests/test_oracle.py): dense
 gates, expectations, Pauli strings and DM maps against numpy brute force on the

@@ -1,6 +1,7 @@
 // vqc_oracle.cpp -- CPU restatement of VQC.jl's gate / expectation / adjoint
-// kernels in FP64, used ONLY as test infrastructure: the parity checker for
-// libvqc_hip.so and the timed CPU baseline ("reference algorithm, restated").
+// kernels in FP64, used ONLY as test infrastructure: the parity reference that
+// libvqc_hip.so's results are compared against, and the timed CPU baseline
+// ("reference algorithm, restated").
 // Nothing in the product library links or calls this file.
 //
 // The reference is Julia (no Julia toolchain in this image), so each routine

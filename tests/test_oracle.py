@@ -1,4 +1,4 @@
-"""Pin the CPU oracle (oracle/vqc_oracle.cpp) before trusting it as the checker.
+"""Pin the CPU oracle (oracle/vqc_oracle.cpp) before comparing the device results against it.
 
 Every oracle routine is checked against something it does not share code with:
 numpy brute force on the full 2^n operator, closed forms, the reference's
