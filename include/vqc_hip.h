@@ -115,7 +115,8 @@ typedef struct vqc_tuning {
     int32_t tile_fwd_bits;          /* tile bits of forward phases, 6..13, or 0 (default): by size, below */
     int32_t tile_rev_bits;          /* tile bits of Hamiltonian / reverse phases, 6..12, or 0 (default) */
     int32_t tile_small_bits;        /* whole evaluation in one workgroup up to this many bits, 1..12, or 0 (default) */
-    int32_t tile_low_bits;          /* contiguous low index bits in every tile, 0..8 (4) */
+    int32_t tile_low_bits;          /* contiguous low index bits in every tile, 0..8 (4; wide
+                                       batches take 5 unless this field was set) */
     int32_t dense_groups[3];        /* amplitude groups per thread, 1-, 2-, 3-qubit sweeps: 1, 2, 4, 8 (1) */
     int32_t dense_nontemporal[3];   /* non-temporal loads/stores, 1-, 2-, 3-qubit sweeps: 0/1 (1, 1, 1);
                                        used only when every target is at bit 3 or above */

@@ -31,8 +31,9 @@ void vqc_zero(double2* v, uint64_t len, hipStream_t st);
 // measured fastest for layered circuits of 8 to 20 qubits on MI355X). A wide
 // batch (nbits < 0), whose instances fill the device together, takes the
 // largest tiles, 13 / 12: fewer phases and HBM passes per instance (256 mixed
-// instances: 90 ms at 12 / 11, 66 ms at 13 / 12), and a whole-state workgroup
-// up to 12 bits.
+// instances: 90 ms at 12 / 11, 66 ms at 13 / 12), a whole-state workgroup up to
+// 12 bits, and 5 contiguous low bits unless tile_low_bits was given (64.0 ms
+// against 65.8 at 4).
 TileTuning vqc_tile_tuning(int nbits) {
     const vqc_tuning_t g = vqc_tuning_now();
     int fwd = kMaxTileBits, rev = kMaxTileBits2, small = 12;
@@ -46,6 +47,7 @@ TileTuning vqc_tile_tuning(int nbits) {
     t.t_rev = std::min(kMaxTileBits2, std::max(6, g.tile_rev_bits ? int(g.tile_rev_bits) : rev));
     t.t_small = std::min(kMaxTileBits2, std::max(1, g.tile_small_bits ? int(g.tile_small_bits) : small));
     t.mlow = std::min(8, std::max(0, int(g.tile_low_bits)));
+    if (nbits < 0 && !vqc_tuning_low_bits_given()) t.mlow = 5;
     return t;
 }
 

@@ -389,6 +389,7 @@ namespace {
 std::mutex g_tune_mu;
 bool g_tune_init = false;
 vqc_tuning_t g_tune;
+bool g_low_bits_given = false;
 
 int env_or(const char* name, int def) {
     const char* v = getenv(name);
@@ -402,6 +403,7 @@ void tune_init_locked() {
     g_tune.tile_rev_bits = env_or("VQC_TILE_REV", 0);
     g_tune.tile_small_bits = env_or("VQC_TILE_SMALL", 0);
     g_tune.tile_low_bits = env_or("VQC_TILE_MLOW", 4);
+    g_low_bits_given = getenv("VQC_TILE_MLOW") != nullptr && *getenv("VQC_TILE_MLOW");
     const int r = env_or("VQC_DENSE_R", 1), nt = env_or("VQC_DENSE_NT", -1);
     for (int k = 0; k < 3; ++k) {
         g_tune.dense_groups[k] = r;
@@ -413,6 +415,12 @@ void tune_init_locked() {
 
 bool valid_groups(int r) { return r == 1 || r == 2 || r == 4 || r == 8; }
 }  // namespace
+
+bool vqc_tuning_low_bits_given() {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    tune_init_locked();
+    return g_low_bits_given;
+}
 
 vqc_tuning_t vqc_tuning_now() {
     std::lock_guard<std::mutex> g(g_tune_mu);
@@ -446,6 +454,7 @@ extern "C" int vqc_set_tuning(const vqc_tuning_t* t) {
         if (!valid_groups(n.dense_groups[k]) || (n.dense_nontemporal[k] != 0 && n.dense_nontemporal[k] != 1))
             return vqc_fail(VQC_EINVAL, "vqc_set_tuning: dense groups must be 1, 2, 4 or 8 and non-temporal 0 or 1");
     g_tune = n;
+    if (t->tile_low_bits != -1) g_low_bits_given = true;
     return VQC_OK;
 }
 

@@ -53,6 +53,8 @@ struct VqcCtx {
 int vqc_ctx(VqcCtx** out);
 // the tuning in force (vqc_set_tuning; initial values from the environment)
 vqc_tuning_t vqc_tuning_now();
+// whether tile_low_bits was given (vqc_set_tuning or VQC_TILE_MLOW) rather than defaulted
+bool vqc_tuning_low_bits_given();
 int vqc_scratch(size_t bytes, void** out);      // device scratch, valid until the next call
 int vqc_pinned(size_t bytes, double** out);     // pinned host staging
 // Upload host data into a dedicated device buffer of at least `bytes` bytes (grown on demand).
