@@ -130,3 +130,17 @@ def test_batch_beyond_memory_limit_runs_in_waves():
     for x, y in zip(g1, g2):
         assert x.shape == y.shape and (x.size == 0 or np.max(np.abs(x - y)) < 1e-12)
     check(inst, E2, g2)
+
+
+def test_wide_batch_matches_oracle():
+    # 16 or more instances take the wide-batch tile shape (13 forward / 12 reverse
+    # tile bits, 5 contiguous low bits); instances above 12 qubits run in tile phases
+    ns = [13, 5, 14, 9, 15, 12, 16, 7, 13, 11, 14, 8, 15, 10, 13, 6, 16, 12, 14, 9]
+    inst = make_instances(11, ns)
+    b = Batch(inst)
+    E, grads = b.run()
+    check(inst, E, grads)
+    E1, g1 = Batch(inst[:15]).run()  # the same instances in a narrow batch
+    assert np.max(np.abs(np.asarray(E[:15]) - np.asarray(E1))) < 1e-12
+    for x, y in zip(grads[:15], g1):
+        assert x.size == 0 or np.max(np.abs(x - y)) < 1e-12
