@@ -417,3 +417,18 @@ def test_dm_general_and_parity_channels(n):
     assert np.max(np.abs(vec(out.storage()) - rho_r)) < 1e-12
     assert abs(E - Er) < 1e-10 * max(1, abs(Er))
     assert np.max(np.abs(g - gr)) < 1e-10 * max(1, np.max(np.abs(gr)))
+
+
+@pytest.mark.parametrize("n", [20, 21])
+def test_energy_gradient_in_the_larger_tile_bands(n):
+    # single states of 19-20 qubits take 11-bit tiles, above that 12 forward / 11
+    # reverse: one shallow layered circuit in each band against the oracle
+    rng = np.random.default_rng(1500 + n)
+    c = hea_circuit(n, 1, rng.uniform(0, 6.3, 3 * n))
+    c.append(vq.QuantumGate((2, n // 2, n), rand_unitary(3, rng)))
+    H = tfim(n, 0.7)
+    psi0 = rand_state(n, rng)
+    E, g, psi, Er, gr, psi_r = _sv_energy_grad(c, H, n, psi0, True)
+    assert np.max(np.abs(psi - psi_r)) < 1e-12
+    assert abs(E - Er) < 1e-10 * max(1, abs(Er))
+    assert np.max(np.abs(g - gr)) < 1e-10 * max(1, np.max(np.abs(gr)))
