@@ -432,3 +432,20 @@ def test_energy_gradient_in_the_larger_tile_bands(n):
     assert np.max(np.abs(psi - psi_r)) < 1e-12
     assert abs(E - Er) < 1e-10 * max(1, abs(Er))
     assert np.max(np.abs(g - gr)) < 1e-10 * max(1, np.max(np.abs(gr)))
+
+
+@pytest.mark.parametrize("n", [9, 10])
+def test_dm_noisy_energy_gradient_in_the_larger_tile_bands(n):
+    # vec(rho) of 18 bits takes 10-bit tiles, of 20 bits 11-bit tiles; a mixed
+    # initial state so every entry of the reverse pass is non-trivial
+    rng = np.random.default_rng(1600 + n)
+    c, H, _ = config3(seed=n, n=n, depth=2, p=0.05, gamma=0.03)
+    rho0 = vec(rand_density(n, rng))
+    E, g, out = vq.energy_and_gradient(H, c, vq.DensityMatrix(rho0), fused=True, return_state=True)
+    prog = Program(c, n=n, dm=True)
+    terms, nt, hpool = pack_terms(H)
+    Er, gr, rho_r = oracle.energy_grad(rho0, n, 1, prog.op_array(), len(prog.ops), prog.pool(), terms, nt, hpool,
+                                       prog.nparams)
+    assert np.max(np.abs(vec(out.storage()) - rho_r)) < 1e-12
+    assert abs(E - Er) < 1e-10
+    assert np.max(np.abs(g - gr)) < 1e-10 * max(1, np.max(np.abs(gr)))
