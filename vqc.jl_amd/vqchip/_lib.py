@@ -45,6 +45,27 @@ class Term(ctypes.Structure):
     _fields_ = [("nq", ctypes.c_int32), ("pos", ctypes.c_int32 * 5), ("mat", ctypes.c_int64)]
 
 
+class MatrixPool(list):
+    """Flattened complex matrices shared by several programs (the pool that
+    vqc_op_t.mat / vqc_term_t.mat offsets index), with its running length kept
+    so that packing one more program does not re-count the chunks before it."""
+
+    def __init__(self):
+        super().__init__()
+        self.size = 0
+
+    def add(self, flat):
+        off = self.size
+        self.append(flat)
+        self.size += len(flat)
+        return off
+
+    def array(self):
+        if not self:
+            return np.zeros(1, dtype=np.complex128)
+        return np.ascontiguousarray(np.concatenate(self))
+
+
 class Pauli(ctypes.Structure):
     _fields_ = [("x", ctypes.c_uint64), ("z", ctypes.c_uint64), ("re", ctypes.c_double), ("im", ctypes.c_double)]
 

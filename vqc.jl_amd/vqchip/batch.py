@@ -108,7 +108,7 @@ class _Part:
         inst = (_lib.Instance * self.n_inst)()
         self._ops = []
         self._terms = []
-        pool_chunks, hpool_chunks = [], []
+        pool_chunks, hpool_chunks = _lib.MatrixPool(), _lib.MatrixPool()
         self._nparams = []
         grad = 0
         for i, (circuit, H, n) in enumerate(instances):
@@ -129,8 +129,8 @@ class _Part:
             grad += prog.nparams
         self.grad_total = grad
         self._inst = inst
-        self._pool = np.ascontiguousarray(np.concatenate(pool_chunks)) if pool_chunks else np.zeros(1, complex)
-        self._hpool = np.ascontiguousarray(np.concatenate(hpool_chunks)) if hpool_chunks else np.zeros(1, complex)
+        self._pool = pool_chunks.array()
+        self._hpool = hpool_chunks.array()
         ops = (_lib.Op * max(len(self._ops), 1))(*self._ops)
         terms = (_lib.Term * max(len(self._terms), 1))(*self._terms)
         h = ctypes.c_void_p()
@@ -142,10 +142,10 @@ class _Part:
 
     def update(self, circuits):
         """New parameters for the same circuit structures (matrix pool rebuilt)."""
-        chunks = []
+        chunks = _lib.MatrixPool()
         for c, n in zip(circuits, [self._inst[i].nqubits for i in range(self.n_inst)]):
             Program(c, n=n, pool=chunks)
-        pool = np.ascontiguousarray(np.concatenate(chunks))
+        pool = chunks.array()
         if pool.size != self._pool.size:
             raise ValueError("circuit structure changed; create a new Batch")
         self._pool = pool

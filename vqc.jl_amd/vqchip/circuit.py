@@ -65,9 +65,7 @@ class Program:
 
     def __init__(self, circuit, n=None, dm=False, slot_base=0, pool=None):
         self.ops = []
-        chunks = [] if pool is None else pool
-        self._chunks = chunks
-        self._len = sum(len(c) for c in chunks)
+        self._chunks = _lib.MatrixPool() if pool is None else pool
         slot = slot_base
         for g in circuit:
             op = _lib.Op()
@@ -114,16 +112,10 @@ class Program:
         self.slot_end = slot
 
     def _add(self, m):
-        m = np.ascontiguousarray(m, dtype=np.complex128).reshape(-1, order="F")
-        off = self._len
-        self._chunks.append(m)
-        self._len += len(m)
-        return off
+        return self._chunks.add(np.ascontiguousarray(m, dtype=np.complex128).reshape(-1, order="F"))
 
     def pool(self):
-        if not self._chunks:
-            return np.zeros(1, dtype=np.complex128)
-        return np.ascontiguousarray(np.concatenate(self._chunks))
+        return self._chunks.array()
 
     def op_array(self):
         arr = (_lib.Op * max(len(self.ops), 1))()

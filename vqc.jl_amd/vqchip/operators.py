@@ -93,10 +93,12 @@ class QubitsOperator:
 
 
 def pack_terms(op, pool=None):
-    """QubitsOperator (all terms <= 5 sites) -> (Term array, complex pool)."""
+    """QubitsOperator (all terms <= 5 sites) -> (Term array, number of terms,
+    complex pool). With a shared `pool` (a MatrixPool) the matrices are appended
+    to it, the offsets index it, and the third value is None: the pool's owner
+    concatenates it once."""
     data = op.data()
-    chunks = [] if pool is None else pool
-    length = sum(len(c) for c in chunks)
+    chunks = _lib.MatrixPool() if pool is None else pool
     terms = (_lib.Term * max(len(data), 1))()
     for i, (key, m) in enumerate(data.items()):
         if len(key) > LARGEST_SUPPORTED_NTERMS:
@@ -104,12 +106,8 @@ def pack_terms(op, pool=None):
         terms[i].nq = len(key)
         for j, p in enumerate(key):
             terms[i].pos[j] = p - 1
-        flat = np.ascontiguousarray(m, dtype=np.complex128).reshape(-1, order="F")
-        terms[i].mat = length
-        chunks.append(flat)
-        length += len(flat)
-    pool_arr = np.concatenate(chunks) if chunks else np.zeros(1, dtype=np.complex128)
-    return terms, len(data), np.ascontiguousarray(pool_arr)
+        terms[i].mat = chunks.add(np.ascontiguousarray(m, dtype=np.complex128).reshape(-1, order="F"))
+    return terms, len(data), (chunks.array() if pool is None else None)
 
 
 def pack_pauli(op):
