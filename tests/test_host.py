@@ -250,3 +250,31 @@ def test_batch_wave_grouping():
         tot = sum(instance_bytes(n) for _, _, n in g)
         assert tot <= 3 << 19 or len(g) == 1
     assert [len(g) for g in w][0] >= 2
+
+
+def test_shared_matrix_pool_offsets():
+    # programs packed into one shared pool (a batch) point at the same matrices as
+    # the same programs packed alone, shifted by the pool's length before them
+    pool, hpool = _lib.MatrixPool(), _lib.MatrixPool()
+    insts = config2_instances(count=6, seed=5)
+    for c, H, n in insts:
+        start, hstart = pool.size, hpool.size
+        shared = Program(c, n=n, pool=pool)
+        alone = Program(c, n=n)
+        big = pool.array()
+        own = alone.pool()
+        assert alone.ops and pool.size == start + own.size
+        for a, b in zip(shared.ops, alone.ops):
+            assert a.mat == b.mat + start
+            nm = 1 << (2 * a.nq)
+            assert np.array_equal(big[a.mat:a.mat + nm], own[b.mat:b.mat + nm])
+            for j in range(a.nparams):
+                assert a.gen[j] == b.gen[j] + start
+        terms, nt, arr = pack_terms(H, pool=hpool)
+        terms1, nt1, arr1 = pack_terms(H)
+        assert arr is None and nt == nt1
+        hbig = hpool.array()
+        for i in range(nt):
+            nm = 1 << (2 * terms[i].nq)
+            assert terms[i].mat == terms1[i].mat + hstart
+            assert np.array_equal(hbig[terms[i].mat:terms[i].mat + nm], arr1[terms1[i].mat:terms1[i].mat + nm])
