@@ -2,7 +2,7 @@
 import numpy as np
 
 from . import _lib
-from .gates import Gate, QuantumMap
+from .gates import CNOTGate, Gate, PHASEGate, QuantumMap, RxGate, RyGate, RzGate, SGate, TGate, ZGate
 
 
 class QCircuit(list):
@@ -42,7 +42,6 @@ def _structure_hint(g):
     off-diagonal). Batches plan before their angles arrive, so the planner takes
     a gate's structure from its type; every angle keeps it (Ry(t) is real and
     its generator -iY/2 too; Rx(t) and -iX/2 are X-like)."""
-    from .gates import CNOTGate, PHASEGate, RxGate, RyGate, RzGate, SGate, TGate, ZGate
     if isinstance(g, (RzGate, PHASEGate, ZGate, SGate, TGate)):
         return 1 | 2
     if isinstance(g, RyGate):
