@@ -449,3 +449,30 @@ def test_dm_noisy_energy_gradient_in_the_larger_tile_bands(n):
     assert np.max(np.abs(vec(out.storage()) - rho_r)) < 1e-12
     assert abs(E - Er) < 1e-10
     assert np.max(np.abs(g - gr)) < 1e-10 * max(1, np.max(np.abs(gr)))
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_dm_energy_gradient_with_large_and_multi_parameter_gates(fused):
+    # 3-, 4- and 5-qubit dense gates act on both sides of vec(rho) (up to 10 of its
+    # 12 bits), between parametric gates with 1-3 generators and a channel
+    n = 6
+    rng = np.random.default_rng(1700 + fused)
+    c = hea_circuit(n, 1, rng.uniform(0, 6.3, 3 * n))
+    c.append(vq.QuantumGate((1, 3, 6), rand_unitary(3, rng)))
+    c.append(vq.FSIMGate(2, 5, *rng.uniform(0, 6.3, 2)))
+    c.append(vq.QuantumGate((1, 2, 4, 6), rand_unitary(4, rng)))
+    c.append(vq.Depolarizing(3, 0.07))
+    c.append(vq.CRxGate(6, 1, float(rng.uniform(0, 6.3))))
+    c.append(vq.QuantumGate((1, 2, 3, 5, 6), rand_unitary(5, rng)))
+    c.append(vq.RyGate(4, float(rng.uniform(0, 6.3))))
+    H = heisenberg(n, rng.standard_normal(n - 1)) + vq.QubitsOperator([vq.QubitsTerm({1: "X", 4: "Y", 6: "Z"}, coeff=0.4)])
+    rho0 = vec(rand_density(n, rng))
+    E, g, out = vq.energy_and_gradient(H, c, vq.DensityMatrix(rho0), fused=fused, return_state=True)
+    prog = Program(c, n=n, dm=True)
+    terms, nt, hpool = pack_terms(H)
+    Er, gr, rho_r = oracle.energy_grad(rho0, n, 1, prog.op_array(), len(prog.ops), prog.pool(), terms, nt, hpool,
+                                       prog.nparams)
+    assert prog.nparams == 3 * n + 2 + 1 + 1
+    assert np.max(np.abs(vec(out.storage()) - rho_r)) < 1e-12
+    assert abs(E - Er) < 1e-10
+    assert np.max(np.abs(g - gr)) < 1e-10 * max(1, np.max(np.abs(gr)))
