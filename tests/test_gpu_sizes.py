@@ -1,7 +1,11 @@
 """Edge sizes. Tiny states (1 and 2 qubits) against the oracle; the full
 28-qubit workload through size-independent properties: the fused and the
 one-kernel-per-gate paths agree (|a - b|^2 = 2 - 2 Re<a|b> from a device dot),
-the norm is kept, and the circuit followed by its inverse returns |0...0>."""
+the norm is kept, and the circuit followed by its inverse returns |0...0>.
+A 24-qubit energy and gradient: the fused and plain paths agree, and the
+gradient matches central differences of the energy."""
+import copy
+
 import numpy as np
 import pytest
 
@@ -11,7 +15,7 @@ import vqchip as vq
 from vqchip import _lib
 from vqchip.circuit import Program
 from vqchip.operators import pack_terms
-from vqchip.workloads import config1_circuit, tfim
+from vqchip.workloads import config1_circuit, hea_circuit, tfim
 
 pytestmark = pytest.mark.gpu
 
@@ -64,3 +68,28 @@ def test_full_size_28_qubits_properties():
     e0 = vq.StateVector(n)
     assert abs(a.dot(e0) - 1) < 1e-12
     del a, b, e0
+
+
+def test_energy_gradient_24_qubits_properties():
+    # beyond the oracle's sizes: the fused tile engine against the plain path (one
+    # kernel per gate and per reduction), and central differences of the energy
+    n = 24
+    rng = np.random.default_rng(2400)
+    c = hea_circuit(n, 1, rng.uniform(0, 6.3, 3 * n))
+    H = tfim(n, 0.8)
+    Ef, gf = vq.energy_and_gradient(H, c, vq.StateVector(n), fused=True)
+    Ep, gp = vq.energy_and_gradient(H, c, vq.StateVector(n), fused=False)
+    assert len(gf) == 3 * n
+    assert abs(Ef - Ep) < 1e-10 * max(1, abs(Ep))
+    assert np.max(np.abs(gf - gp)) < 1e-10 * max(1, np.max(np.abs(gp)))
+    x = np.array(c.active_parameters(), dtype=float)
+    h = 1e-4
+    for j in (0, 3 * n // 2 + 1, 3 * n - 1):
+        ends = []
+        for s in (h, -h):
+            cj = copy.deepcopy(c)
+            xj = x.copy()
+            xj[j] += s
+            cj.reset_parameters(xj)
+            ends.append(vq.energy_and_gradient(H, cj, vq.StateVector(n), fused=True)[0].real)
+        assert abs((ends[0] - ends[1]) / (2 * h) - gf[j]) < 1e-6 * max(1, np.max(np.abs(gf)))
