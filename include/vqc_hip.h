@@ -96,6 +96,19 @@ typedef struct vqc_pauli {
     double   im;
 } vqc_pauli_t;
 
+/* One product term c * m_1 (x) ... (x) m_w on w ascending sites: a QubitsTerm of any
+ * length, which the reference applies one site at a time (_apply_qterm_util!,
+ * src/hamiltonian/apply_qterms/apply_qterms.jl:47-59) whenever an operator holds a
+ * term of more than 5 sites. Positions live in a caller array of int32 (0-based
+ * qubits), factors in a pool of interleaved complex doubles. */
+typedef struct vqc_qterm {
+    int32_t nsites;      /* 0..64 (0: c times the identity) */
+    int32_t reserved;
+    int64_t pos;         /* offset of the nsites ascending positions in the position array */
+    int64_t mat;         /* pool offset (complex elements) of nsites consecutive 2x2 column-major factors */
+    double  re, im;      /* the coefficient c */
+} vqc_qterm_t;
+
 /* ---- library ----------------------------------------------------------- */
 const char* vqc_last_error(void);
 int vqc_version(void);
@@ -156,6 +169,12 @@ int vqc_state_scale(vqc_state* s, double re, double im);      /* x * y, statevec
 /* y += a x (x + y and x - y, statevector.jl:44-45; the vector updates of the Krylov solvers
  * that groundstate.jl delegates to KrylovKit). */
 int vqc_state_axpy(vqc_state* y, double re, double im, const vqc_state* x);
+/* rho = |psi><psi|, i.e. vec(rho) = kron(conj(psi), psi): DensityMatrix(x::StateVector)
+ * (densitymatrix.jl:27). rho must have psi's qubit count. */
+int vqc_dm_from_sv(vqc_state* rho, const vqc_state* psi);
+/* rho = the identity matrix (not normalised): DensityMatrix(one(storage(state)), n), on
+ * which the expectation pullback of a density matrix applies H^dagger (expecdiff.jl:24-25). */
+int vqc_dm_identity(vqc_state* rho);
 /* <a|b> = sum conj(a_i) b_i (LinearAlgebra.dot, statevector.jl:58). */
 int vqc_state_dot(const vqc_state* a, const vqc_state* b, double* out2);
 
@@ -186,6 +205,22 @@ int vqc_sv_apply_pauli_sum(const vqc_state* in, vqc_state* out, int nterms, cons
 /* out = sum_keys M_key in; energy2 = <in|out> (fused H*psi and expectation). */
 int vqc_sv_apply_terms(const vqc_state* in, vqc_state* out, int nterms, const vqc_term_t* terms,
                        const double* pool, double* energy2);
+
+/* out = sum_t T_t in, or out += sum_t T_t in when accumulate is 1: the per-term
+ * route of (m::QubitsOperator)(v) for operators with terms of more than 5 sites
+ * (apply_qterms.jl:20-25 over _apply_qterm_util!, :47-59). On a density matrix
+ * the terms act on the row bits of vec(rho) (apply_qterms_dm.jl:11-27). */
+int vqc_sv_apply_qterms(const vqc_state* in, vqc_state* out, int nterms, const vqc_qterm_t* terms,
+                        const int32_t* pos, const double* pool, int accumulate);
+/* <bra|T_t|ket> for every term (2*nterms doubles in out, may be NULL) and their sum
+ * (bra = ket when bra is NULL): the dot(psi, m * psi) fallback of
+ * expecs.jl:17,40,50,72 for long terms. */
+int vqc_sv_expec_qterms(const vqc_state* ket, const vqc_state* bra, int nterms, const vqc_qterm_t* terms,
+                        const int32_t* pos, const double* pool, double* out, double* sum2);
+/* tr(T_t rho) for every term and their sum: tr(m * rho) of expecs_dm.jl:10-26
+ * for terms beyond the dense-key kernels. */
+int vqc_dm_expec_qterms(const vqc_state* rho, int nterms, const vqc_qterm_t* terms, const int32_t* pos,
+                        const double* pool, double* out, double* sum2);
 
 /* ---- density matrices ----------------------------------------------------
  * rho <- U rho U^dagger: U on pos, conj(U) on pos+n (generic.jl:54-64). */

@@ -114,6 +114,55 @@ def apply_pauli_sum(v, paulis, nterms):
     return out
 
 
+def _qterm_args(qterms):
+    """(terms array, positions, pool) of a list of (coef, [0-based sites], [2x2 factors])."""
+    import vqchip._lib as vl
+    arr = (vl.QTerm * max(len(qterms), 1))()
+    pos, pool = [], []
+    for i, (c, sites, mats) in enumerate(qterms):
+        arr[i].nsites = len(sites)
+        arr[i].pos = len(pos)
+        arr[i].mat = sum(len(m) for m in pool)
+        arr[i].re, arr[i].im = complex(c).real, complex(c).imag
+        pos.extend(int(s) for s in sites)
+        for m in mats:
+            pool.append(_mat(m))
+    pool = np.ascontiguousarray(np.concatenate(pool) if pool else np.zeros(1, dtype=np.complex128))
+    return arr, (ctypes.c_int * max(len(pos), 1))(*pos), pool
+
+
+def apply_qterms(v, qterms):
+    """sum_t T_t v (apply_qterms.jl:20-25 over _apply_qterm_util!, :47-59)."""
+    v = np.ascontiguousarray(v, dtype=np.complex128)
+    out = np.zeros_like(v)
+    arr, pos, pool = _qterm_args(qterms)
+    nbits = int(round(np.log2(v.size)))
+    _rc(lib().orc_apply_qterms(nbits, _c(v), _c(out), len(qterms), arr, pos, _c(pool)))
+    return out
+
+
+def expec_qterms(v, qterms, vc=None):
+    """(per-term <vc|T_t|v>, their sum) (expecs.jl:17,40,50,72)."""
+    v = np.ascontiguousarray(v, dtype=np.complex128)
+    arr, pos, pool = _qterm_args(qterms)
+    out = np.zeros(2 * max(len(qterms), 1))
+    s2 = np.zeros(2)
+    nbits = int(round(np.log2(v.size)))
+    vcp = _c(np.ascontiguousarray(vc, dtype=np.complex128)) if vc is not None else None
+    _rc(lib().orc_expec_qterms(nbits, _c(v), vcp, len(qterms), arr, pos, _c(pool), out.ctypes.data_as(_D), s2.ctypes.data_as(_D)))
+    return out[0:2 * len(qterms):2] + 1j * out[1:2 * len(qterms):2], complex(s2[0], s2[1])
+
+
+def dm_expec_qterms(rho, n, qterms):
+    """(per-term tr(T_t rho), their sum) (expecs_dm.jl:10-26)."""
+    rho = np.ascontiguousarray(np.asarray(rho, dtype=np.complex128).reshape(-1, order="F"))
+    arr, pos, pool = _qterm_args(qterms)
+    out = np.zeros(2 * max(len(qterms), 1))
+    s2 = np.zeros(2)
+    _rc(lib().orc_dm_expec_qterms(n, _c(rho), len(qterms), arr, pos, _c(pool), out.ctypes.data_as(_D), s2.ctypes.data_as(_D)))
+    return out[0:2 * len(qterms):2] + 1j * out[1:2 * len(qterms):2], complex(s2[0], s2[1])
+
+
 def dm_apply_gate(rho, n, k, pos0, U):
     m = _mat(U)
     _rc(lib().orc_dm_apply_gate(n, _c(rho), k, _ipos(pos0), _c(m)))

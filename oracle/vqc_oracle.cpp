@@ -287,6 +287,70 @@ static void apply_qterm(int nbits, const cd* v, cd* vout, int nsites, const int*
     for (int s = 0; s < nsites; ++s) orc_apply_dense(nbits, vout, 1, &sites[s], ops + 4 * s);
 }
 
+// Product terms of any length (vqc_qterm_t: c, sites, 2x2 factors in a complex
+// pool). out = sum_t T_t v with the workspace of (m::QubitsOperator)(v) for
+// long terms: vout = 0, then _apply_qterm_util!(term, v, workspace) and
+// vout .+= workspace per term (apply_qterms.jl:20-25). On a density matrix the
+// vector is vec(rho) and the sites are row bits (apply_qterms_dm.jl:11-27).
+extern "C" int orc_apply_qterms(int nbits, const cd* v, cd* vout, int nterms, const vqc_qterm_t* terms, const int* pos,
+                                const cd* pool) {
+    const int64_t L = int64_t(1) << nbits;
+    std::vector<cd> work(L);
+    std::fill(vout, vout + L, cd(0));
+    for (int t = 0; t < nterms; ++t) {
+        apply_qterm(nbits, v, work.data(), terms[t].nsites, pos + terms[t].pos, pool + terms[t].mat, cd(terms[t].re, terms[t].im));
+        parallel_run(L, [&](int64_t ist, int64_t ifn) {
+            for (int64_t i = ist; i <= ifn; ++i) vout[i] += work[i];
+        });
+    }
+    return 0;
+}
+
+// <vc|T_t|v> per term: dot(state_c, m * state) (expecs.jl:17,50 for one long
+// term; :40,72 for the operator, summed in term order).
+extern "C" int orc_expec_qterms(int nbits, const cd* v, const cd* vc, int nterms, const vqc_qterm_t* terms, const int* pos,
+                                const cd* pool, double* out, double* sum2) {
+    if (!vc) vc = v;
+    const int64_t L = int64_t(1) << nbits;
+    std::vector<cd> work(L);
+    cd total = 0;
+    for (int t = 0; t < nterms; ++t) {
+        apply_qterm(nbits, v, work.data(), terms[t].nsites, pos + terms[t].pos, pool + terms[t].mat, cd(terms[t].re, terms[t].im));
+        double r2[2];
+        orc_dot(nbits, vc, work.data(), r2);
+        if (out) {
+            out[2 * t] = r2[0];
+            out[2 * t + 1] = r2[1];
+        }
+        total += cd(r2[0], r2[1]);
+    }
+    sum2[0] = total.real();
+    sum2[1] = total.imag();
+    return 0;
+}
+
+// tr(T_t rho) per term: tr(m * rho) (expecs_dm.jl:10-26), m * rho through the
+// row-bit route above, then the trace of the result.
+extern "C" int orc_dm_expec_qterms(int n, const cd* rho, int nterms, const vqc_qterm_t* terms, const int* pos, const cd* pool,
+                                   double* out, double* sum2) {
+    const int64_t N = int64_t(1) << n, L = N * N;
+    std::vector<cd> work(L);
+    cd total = 0;
+    for (int t = 0; t < nterms; ++t) {
+        apply_qterm(2 * n, rho, work.data(), terms[t].nsites, pos + terms[t].pos, pool + terms[t].mat, cd(terms[t].re, terms[t].im));
+        cd tr = 0;
+        for (int64_t i = 0; i < N; ++i) tr += work[i + i * N];
+        if (out) {
+            out[2 * t] = tr.real();
+            out[2 * t + 1] = tr.imag();
+        }
+        total += tr;
+    }
+    sum2[0] = total.real();
+    sum2[1] = total.imag();
+    return 0;
+}
+
 // Pauli strings through the reference's >5-site route: state_2 = sum_t P_t v
 // term by term (apply_qterms.jl:13-28), then dot(bra, state_2) (expecs.jl:40,68).
 // out gets one complex per term; sum2 the total.

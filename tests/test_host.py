@@ -278,3 +278,28 @@ def test_shared_matrix_pool_offsets():
             nm = 1 << (2 * terms[i].nq)
             assert terms[i].mat == terms1[i].mat + hstart
             assert np.array_equal(hbig[terms[i].mat:terms[i].mat + nm], arr1[terms1[i].mat:terms1[i].mat + nm])
+
+
+def test_batch_repack_rejects_structure_changes():
+    """A batch is planned from its op records; new angles may change only the
+    matrix values (vqchip.batch.repack), anything else raises."""
+    from vqchip.batch import repack
+    n = 4
+    base = hea_circuit(n, 2, np.linspace(0.1, 2.0, 3 * n * 2))
+    old = Program(base, n=n)
+    size = old.pool().size
+    fresh = hea_circuit(n, 2, np.linspace(0.5, 2.5, 3 * n * 2))
+    pool = repack([fresh], [n], [len(old.ops)], old.ops, size)
+    assert pool.size == size
+    # Ry replaced by Rx: the same pool length, another structure hint
+    swapped = QCircuit(RxGate(g.positions[0], 0.3) if isinstance(g, RyGate) else g for g in base)
+    with pytest.raises(ValueError):
+        repack([swapped], [n], [len(old.ops)], old.ops, size)
+    # a rotation moved to another qubit
+    moved = QCircuit(base)
+    moved[0] = RxGate(2, 0.7)
+    with pytest.raises(ValueError):
+        repack([moved], [n], [len(old.ops)], old.ops, size)
+    # one gate fewer
+    with pytest.raises(ValueError):
+        repack([QCircuit(base[:-1])], [n], [len(old.ops)], old.ops, size)

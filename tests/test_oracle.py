@@ -412,3 +412,60 @@ def test_select_brute_force():
     v = np.zeros(8, complex)
     v[1] = 1.0
     assert np.array_equal(oracle.select(v, 3, 2, 0), np.eye(1, 4, 2, dtype=complex).reshape(-1))
+
+
+def _rand_factor(kind, rng):
+    from vqchip.gates import SIGMA_MINUS, SIGMA_PLUS
+    X = np.array([[0, 1], [1, 0]], dtype=complex)
+    Y = np.array([[0, -1j], [1j, 0]])
+    Z = np.diag([1.0 + 0j, -1])
+    if kind == "dense":
+        return rand_matrix(1, rng)
+    if kind == "diag":
+        return np.diag(rng.standard_normal(2) + 1j * rng.standard_normal(2))
+    if kind == "anti":
+        return np.array([[0, rng.standard_normal() + 1j], [rng.standard_normal() - 0.5j, 0]])
+    return {"+": SIGMA_PLUS, "-": SIGMA_MINUS, "X": X, "Y": Y, "Z": Z, "I": np.eye(2, dtype=complex),
+            "P0": np.diag([1.0 + 0j, 0]), "P1": np.diag([0j, 1.0])}[kind]
+
+
+def _rand_qterms(n, rng, kinds, nterms=3, wmin=6):
+    out = []
+    for _ in range(nterms):
+        w = int(rng.integers(wmin, n + 1))
+        sites = sorted(rng.choice(n, size=w, replace=False).tolist())
+        mats = [_rand_factor(kinds[int(rng.integers(len(kinds)))], rng) for _ in sites]
+        out.append((complex(rng.standard_normal(), rng.standard_normal()), sites, mats))
+    return out
+
+
+def _qterm_full(n, c, sites, mats):
+    from vqchip.gates import kron_le
+    return c * full_op(n, sites, kron_le(*mats))
+
+
+@pytest.mark.parametrize("kinds", [("+", "-"), ("+", "-", "X", "Y", "Z", "I", "P0", "P1"), ("dense", "diag", "anti", "Z"),
+                                   ("dense",)])
+def test_qterms_brute_force(kinds):
+    """Product terms of 6..n sites (_apply_qterm_util!, apply_qterms.jl:47-59) against
+    the dense operator built site by site (test/check_qubits_ham.jl:73-101 method:
+    sigma+ / sigma- strings, m * state and expectation against matrix(L, m))."""
+    rng = np.random.default_rng(sum(map(ord, "".join(kinds))))
+    for n in (7, 9):
+        qt = _rand_qterms(n, rng, kinds)
+        v = rand_state(n, rng)
+        vc = rand_state(n, rng)
+        H = sum(_qterm_full(n, c, s, m) for c, s, m in qt)
+        assert np.allclose(oracle.apply_qterms(v, qt), H @ v, atol=1e-12)
+        per, tot = oracle.expec_qterms(v, qt)
+        assert np.allclose(per, [v.conj() @ _qterm_full(n, c, s, m) @ v for c, s, m in qt], atol=1e-12)
+        assert abs(tot - v.conj() @ H @ v) < 1e-12
+        _, tot2 = oracle.expec_qterms(v, qt, vc=vc)
+        assert abs(tot2 - vc.conj() @ H @ v) < 1e-12
+    # density matrices: tr(T rho) with T on the row bits (expecs_dm.jl:25, apply_qterms_dm.jl:11-27)
+    n = 6
+    qt = _rand_qterms(n, rng, kinds, wmin=4)
+    rho = rand_density(n, rng)
+    per, tot = oracle.dm_expec_qterms(rho, n, qt)
+    assert np.allclose(per, [np.trace(_qterm_full(n, c, s, m) @ rho) for c, s, m in qt], atol=1e-12)
+    assert abs(tot - sum(per)) < 1e-12

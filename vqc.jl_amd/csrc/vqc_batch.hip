@@ -422,6 +422,7 @@ int vqc_batch_build(int ninst, const vqc_instance_t* inst, const vqc_op_t* ops, 
                 return fail(VQC_EINVAL, "bad op (batched instances are state vectors)", i);
             const int64_t D2 = int64_t(1) << (2 * op.nq);
             if (op.mat < 0 || op.mat + D2 > pool_len) return fail(VQC_EINVAL, "op matrix outside the pool", i);
+            if (op.nparams < 0 || op.nparams > 3) return fail(VQC_EINVAL, "an op has more than 3 parameters", i);
             for (int j = 0; j < op.nparams; ++j)
                 if (op.slot[j] >= I.nparams || (op.slot[j] >= 0 && (op.gen[j] < 0 || op.gen[j] + D2 > pool_len)))
                     return fail(VQC_EINVAL, "bad parameter slot or generator", i);

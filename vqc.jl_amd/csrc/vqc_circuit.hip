@@ -998,6 +998,7 @@ static void record_plan(const PlanBuilder& pb) {
 }
 
 extern "C" int vqc_plan_info(vqc_plan_info_t* out) {
+    VQC_API_LOCK;  // g_last_plan is per thread; the lock orders this read after the calling thread's last plan
     if (!out) return vqc_fail(VQC_EINVAL, "null plan info");
     *out = g_last_plan;
     return VQC_OK;

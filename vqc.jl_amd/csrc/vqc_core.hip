@@ -354,6 +354,46 @@ extern "C" int vqc_state_axpy(vqc_state* y, double re, double im, const vqc_stat
     return VQC_OK;
 }
 
+// rho[i + j 2^n] = psi_i conj(psi_j): DensityMatrix(x::StateVector) = kron(conj(x), x)
+// (densitymatrix.jl:27), one output entry per thread, columns j walked by the grid
+__global__ void __launch_bounds__(256) k_dm_from_sv(double2* __restrict__ rho, const double2* __restrict__ psi, int n) {
+    const uint64_t len = uint64_t(1) << (2 * n), mask = (uint64_t(1) << n) - 1;
+    for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < len; e += uint64_t(gridDim.x) * blockDim.x)
+        rho[e] = cmul(psi[e & mask], cconj(psi[e >> n]));
+}
+
+extern "C" int vqc_dm_from_sv(vqc_state* rho, const vqc_state* psi) {
+    VQC_API_LOCK;
+    VQC_TRY(vqc_check_state(rho, "vqc_dm_from_sv"));
+    VQC_TRY(vqc_check_qubits(psi, "vqc_dm_from_sv"));
+    if (!rho->is_dm || psi->is_dm || rho->nqubits != psi->nqubits)
+        return vqc_fail(VQC_EINVAL, "vqc_dm_from_sv: needs a density matrix and a state vector of the same qubit count");
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    hipLaunchKernelGGL(k_dm_from_sv, dim3(stream_grid(rho->len())), dim3(256), 0, c->stream, rho->d, psi->d, psi->nqubits);
+    VQC_HIP(hipGetLastError());
+    return VQC_OK;
+}
+
+// vec(I): ones at i + i 2^n (the DensityMatrix(one(storage(state)), n) that the
+// expectation pullback of a density matrix applies H^dagger to, expecdiff.jl:24-25,64-65)
+__global__ void __launch_bounds__(256) k_dm_identity(double2* __restrict__ rho, int n) {
+    const uint64_t len = uint64_t(1) << (2 * n), mask = (uint64_t(1) << n) - 1;
+    for (uint64_t e = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < len; e += uint64_t(gridDim.x) * blockDim.x)
+        rho[e] = make_double2((e & mask) == (e >> n) ? 1.0 : 0.0, 0.0);
+}
+
+extern "C" int vqc_dm_identity(vqc_state* rho) {
+    VQC_API_LOCK;
+    VQC_TRY(vqc_check_state(rho, "vqc_dm_identity"));
+    if (!rho->is_dm) return vqc_fail(VQC_EINVAL, "vqc_dm_identity: not a density matrix");
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    hipLaunchKernelGGL(k_dm_identity, dim3(stream_grid(rho->len())), dim3(256), 0, c->stream, rho->d, rho->nqubits);
+    VQC_HIP(hipGetLastError());
+    return VQC_OK;
+}
+
 extern "C" int vqc_state_scale(vqc_state* s, double re, double im) {
     VQC_API_LOCK;
     VQC_TRY(vqc_check_state(s, "vqc_state_scale"));

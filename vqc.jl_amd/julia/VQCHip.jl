@@ -150,17 +150,47 @@ function pack(circuit::QCircuit, isdm::Bool)
     return ops, pool, slot
 end
 
-# QubitsOperator -> merged keys (hamiltonian/util.jl:6-15)
+# QubitsOperator -> merged keys of at most 5 sites (hamiltonian/util.jl:6-15), and
+# the longer terms as product terms (vqc_qterm_t: coefficient, sites, 2x2 factors)
+struct CQTerm
+    nsites::Int32
+    reserved::Int32
+    pos::Int64
+    mat::Int64
+    re::Float64
+    im::Float64
+end
+
 function pack(h::QubitsOperator)
     terms = CTerm[]
     pool = ComplexF64[]
     for (k, v) in h.data
-        length(k) <= 5 || throw(ArgumentError("terms longer than 5 sites are not supported on this path"))
+        length(k) <= 5 || continue
         m = VQC._get_mat(length(k), v)
         push!(terms, CTerm(length(k), pad5(collect(k) .- 1), length(pool)))
         append!(pool, vec(Matrix{ComplexF64}(m)))
     end
     return terms, pool
+end
+
+# the terms of more than 5 sites, one by one (the per-term route of
+# _apply_qterm_util!, apply_qterms.jl:20-25,47-59)
+function pack_long(h::QubitsOperator)
+    qt = CQTerm[]
+    pos = Int32[]
+    pool = ComplexF64[]
+    for (k, v) in h.data
+        length(k) > 5 || continue
+        for item in v  # (oplist, coeff), as apply_qterms.jl:22 reads them
+            c = ComplexF64(item[2])
+            push!(qt, CQTerm(length(k), 0, length(pos), length(pool), real(c), imag(c)))
+            append!(pos, Int32.(collect(k) .- 1))
+            for m in item[1]
+                append!(pool, vec(Matrix{ComplexF64}(m)))
+            end
+        end
+    end
+    return qt, pos, pool
 end
 
 function apply!(circuit::QCircuit, s::GPUState)
@@ -173,10 +203,14 @@ end
 Base.:*(circuit::QCircuit, s::GPUState) = apply!(circuit, copy(s))
 
 # ---- expectation (expecs.jl:12-42, expecs_dm.jl:7-27) -------------------------
+# Keys of at most 5 sites take one dense sweep each; longer terms the per-term
+# route (dot(psi, m * psi), expecs.jl:40; tr(m * rho), expecs_dm.jl:25) in the
+# product-term kernels, which take Pauli and sigma+- strings in one pass each.
 function VQC.expectation(h::QubitsOperator, s::GPUState)
     r = zero(ComplexF64)
     out = zeros(Float64, 2)
     for (k, v) in h.data
+        length(k) <= 5 || continue
         pos = Int32.(collect(k) .- 1)
         m = Matrix{ComplexF64}(VQC._get_mat(length(k), v))
         if s.isdm
@@ -189,18 +223,73 @@ function VQC.expectation(h::QubitsOperator, s::GPUState)
         end
         r += complex(out[1], out[2])
     end
+    qt, qpos, qpool = pack_long(h)
+    if !isempty(qt)
+        if s.isdm
+            check(ccall((:vqc_dm_expec_qterms, libvqc), Cint,
+                        (Ptr{Cvoid}, Cint, Ptr{CQTerm}, Ptr{Int32}, Ptr{ComplexF64}, Ptr{Float64}, Ptr{Float64}),
+                        s.h, length(qt), qt, qpos, qpool, C_NULL, out))
+        else
+            check(ccall((:vqc_sv_expec_qterms, libvqc), Cint,
+                        (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{CQTerm}, Ptr{Int32}, Ptr{ComplexF64}, Ptr{Float64}, Ptr{Float64}),
+                        s.h, C_NULL, length(qt), qt, qpos, qpool, C_NULL, out))
+        end
+        r += complex(out[1], out[2])
+    end
     return r
 end
+VQC.expectation(h::QubitsTerm, s::GPUState) = VQC.expectation(QubitsOperator(h), s)
 
 # H * psi ((m::QubitsOperator)(v), apply_qterms.jl:13-28; on a density matrix the
-# keys act on the row bits, apply_qterms_dm.jl:11-27)
+# keys act on the row bits, apply_qterms_dm.jl:11-27); long terms accumulate on
+# top of the dense keys
 function Base.:*(h::QubitsOperator, s::GPUState)
     terms, pool = pack(h)
     out = GPUState(s.nqubits, s.isdm)
     check(ccall((:vqc_sv_apply_terms, libvqc), Cint,
                 (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{CTerm}, Ptr{ComplexF64}, Ptr{Float64}),
                 s.h, out.h, length(terms), terms, pool, C_NULL))
+    qt, qpos, qpool = pack_long(h)
+    if !isempty(qt)
+        check(ccall((:vqc_sv_apply_qterms, libvqc), Cint,
+                    (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{CQTerm}, Ptr{Int32}, Ptr{ComplexF64}, Cint),
+                    s.h, out.h, length(qt), qt, qpos, qpool, 1))
+    end
     return out
+end
+Base.:*(h::QubitsTerm, s::GPUState) = QubitsOperator(h) * s
+
+# y <- a x + y on the device (vqc_state_axpy), and the sum Zygote takes of two
+# tangents of one state
+axpy!(a::Number, x::GPUState, y::GPUState) = (c = ComplexF64(a);
+    check(ccall((:vqc_state_axpy, libvqc), Cint, (Ptr{Cvoid}, Cdouble, Cdouble, Ptr{Cvoid}), y.h, real(c), imag(c), x.h)); y)
+scale!(s::GPUState, a::Number) = (c = ComplexF64(a);
+    check(ccall((:vqc_state_scale, libvqc), Cint, (Ptr{Cvoid}, Cdouble, Cdouble), s.h, real(c), imag(c))); s)
+Base.:+(a::GPUState, b::GPUState) = axpy!(1.0, b, copy(a))
+
+# The expectation pullback (@adjoint expectation, expecdiff.jl:4-68): for a state
+# vector (conj(z) H + z H') psi (expecdiff.jl:37-56 for long terms: the same
+# per-term sweeps); for a density matrix z H' applied to the identity
+# (expecdiff.jl:24-25,64-65). With the rrule of * below, Zygote differentiates
+# the reference's own loss c -> real(expectation(H, c * psi0)) unchanged.
+function expectation_pullback(h::QubitsOperator, s::GPUState, z::Number)
+    z = ComplexF64(z)
+    if s.isdm
+        ident = GPUState(s.nqubits, true)
+        check(ccall((:vqc_dm_identity, libvqc), Cint, (Ptr{Cvoid},), ident.h))
+        return scale!(h' * ident, z)
+    end
+    out = scale!(h * s, conj(z))
+    return axpy!(z, h' * s, out)
+end
+
+function ChainRulesCore.rrule(::typeof(VQC.expectation), h::QubitsOperator, s::GPUState)
+    pullback(z) = (NoTangent(), NoTangent(), expectation_pullback(h, s, unthunk(z)))
+    return VQC.expectation(h, s), pullback
+end
+function ChainRulesCore.rrule(::typeof(VQC.expectation), h::QubitsTerm, s::GPUState)
+    pullback(z) = (NoTangent(), NoTangent(), expectation_pullback(QubitsOperator(h), s, unthunk(z)))
+    return VQC.expectation(h, s), pullback
 end
 
 # Pauli strings beyond 5 sites (the dot(psi, H*psi) fallback of expecs.jl:40)
@@ -232,6 +321,13 @@ end
 # ---- energy and adjoint gradient -----------------------------------------------
 "E = expectation(h, circuit * s0) and dRe(E)/dθ for the active parameters, in circuit order."
 function energy_and_gradient(h::QubitsOperator, circuit::QCircuit, s0::GPUState)
+    if !isempty(pack_long(h)[1])
+        # terms of more than 5 sites: the reference's chain, y = circuit * s0, the
+        # expectation pullback with z = 1 and the circuit pullback of it
+        y = circuit * s0
+        g, _ = circuit_vjp(circuit, s0, expectation_pullback(h, y, 1.0))
+        return VQC.expectation(h, y), g
+    end
     ops, pool, np = pack(circuit, s0.isdm)
     terms, hpool = pack(h)
     e = zeros(Float64, 2)

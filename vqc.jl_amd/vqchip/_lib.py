@@ -70,6 +70,11 @@ class Pauli(ctypes.Structure):
     _fields_ = [("x", ctypes.c_uint64), ("z", ctypes.c_uint64), ("re", ctypes.c_double), ("im", ctypes.c_double)]
 
 
+class QTerm(ctypes.Structure):
+    _fields_ = [("nsites", ctypes.c_int32), ("reserved", ctypes.c_int32), ("pos", ctypes.c_int64), ("mat", ctypes.c_int64),
+                ("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+
 class Instance(ctypes.Structure):
     _fields_ = [
         ("nqubits", ctypes.c_int32),
@@ -142,6 +147,8 @@ SIGNATURES = {
     "vqc_state_scale": [_P, ctypes.c_double, ctypes.c_double],
     "vqc_state_axpy": [_P, ctypes.c_double, ctypes.c_double, _P],
     "vqc_state_dot": [_P, _P, _D],
+    "vqc_dm_from_sv": [_P, _P],
+    "vqc_dm_identity": [_P],
     "vqc_sv_apply_dense": [_P, _I, _I32, _D],
     "vqc_sv_apply_named": [_P, _I, _I32, ctypes.c_double],
     "vqc_sv_accum_dense": [_P, _P, _I, _I32, _D],
@@ -149,6 +156,9 @@ SIGNATURES = {
     "vqc_sv_expec_pauli": [_P, _P, _I, ctypes.POINTER(Pauli), _D, _D],
     "vqc_sv_apply_pauli_sum": [_P, _P, _I, ctypes.POINTER(Pauli)],
     "vqc_sv_apply_terms": [_P, _P, _I, ctypes.POINTER(Term), _D, _D],
+    "vqc_sv_apply_qterms": [_P, _P, _I, ctypes.POINTER(QTerm), _I32, _D, _I],
+    "vqc_sv_expec_qterms": [_P, _P, _I, ctypes.POINTER(QTerm), _I32, _D, _D, _D],
+    "vqc_dm_expec_qterms": [_P, _I, ctypes.POINTER(QTerm), _I32, _D, _D, _D],
     "vqc_dm_apply_gate": [_P, _I, _I32, _D],
     "vqc_dm_apply_superop": [_P, _I, _I32, _D, _I],
     "vqc_dm_expec_dense": [_P, _I, _I32, _D, _D],

@@ -15,7 +15,8 @@ import numpy as np
 from . import _lib
 from .circuit import Program, QCircuit
 from .gates import Gate, QuantumMap
-from .operators import LARGEST_SUPPORTED_NTERMS, QubitsOperator, QubitsTerm, pack_pauli, pack_terms
+from .operators import (LARGEST_SUPPORTED_NTERMS, QubitsOperator, QubitsTerm, pack_pauli, pack_qterms, pack_terms,
+                        split_long)
 from .state import DensityMatrix, StateVector
 
 
@@ -36,6 +37,15 @@ def apply_(x, state, named=True):
     if isinstance(x, (list, QCircuit)):
         if len(x) == 0:
             return state
+        if not state.is_dm:
+            first = next((i for i, g in enumerate(x) if isinstance(g, QuantumMap)), None)
+            if first is not None:
+                # apply!(circuit, state) applies element by element (generic.jl:26-31), and a
+                # channel on a state vector continues on DensityMatrix(state) (apply_qmaps.jl:19):
+                # the gates before the first channel run on the state vector, the rest on rho
+                if first:
+                    apply_(QCircuit(x[:first]), state)
+                return apply_(QCircuit(x[first:]), DensityMatrix(state))
         prog = Program(x, n=state.nqubits(), dm=bool(state.is_dm))
         ops = prog.op_array()
         pool = prog.pool()
@@ -80,8 +90,44 @@ def _as_operator(m):
     return QubitsOperator([m]) if isinstance(m, QubitsTerm) else m
 
 
+def _expec_keys(m, state, bra):
+    """sum over the merged keys (<= 5 sites) of <bra|M_key|state> or tr(M_key rho)."""
+    total = 0j
+    for key, M in m.data().items():
+        Mf = np.ascontiguousarray(M.reshape(-1, order="F"))
+        out = np.zeros(2)
+        p = _lib.i32ptr([q - 1 for q in key])
+        if state.is_dm:
+            _lib.call("vqc_dm_expec_dense", state.handle, len(key), p, _lib.dptr(Mf), _lib.dptr(out))
+        else:
+            _lib.call("vqc_sv_expec_dense", state.handle, bra.handle if bra else None, len(key), p, _lib.dptr(Mf),
+                      _lib.dptr(out))
+        total += complex(out[0], out[1])
+    return total
+
+
+def _expec_products(terms, state, bra):
+    """sum_t <bra|T_t|state> (or tr(T_t rho)) for product terms of any length."""
+    if not terms:
+        return 0j
+    arr, nt, pos, pool = pack_qterms(terms)
+    s2 = np.zeros(2)
+    if state.is_dm:
+        _lib.call("vqc_dm_expec_qterms", state.handle, nt, arr, pos, _lib.dptr(pool), None, _lib.dptr(s2))
+    else:
+        _lib.call("vqc_sv_expec_qterms", state.handle, bra.handle if bra else None, nt, arr, pos, _lib.dptr(pool), None,
+                  _lib.dptr(s2))
+    return complex(s2[0], s2[1])
+
+
 def expectation(*args):
-    """expectation(H, state) or expectation(bra, H, state); returns a complex."""
+    """expectation(H, state) or expectation(bra, H, state); returns a complex.
+
+    Keys of at most 5 sites take one dense sweep each (expecs.jl:22-28; on a
+    density matrix tr(M rho) over the diagonal band, expecs_dm.jl:18-23). Longer
+    terms take the reference's per-term route, dot(psi, m * psi) (expecs.jl:40,72)
+    or tr(m * rho) (expecs_dm.jl:25): Pauli strings through the bitmask kernels,
+    other products through the product-term kernels."""
     if len(args) == 2:
         m, state, bra = args[0], args[1], None
     else:
@@ -93,44 +139,50 @@ def expectation(*args):
         _lib.call("vqc_sv_expec_dense", state.handle, bra.handle if bra else None, len(pos),
                   _lib.i32ptr([q - 1 for q in pos]), _lib.dptr(M), _lib.dptr(out))
         return complex(out[0], out[1])
+    if isinstance(m, QubitsTerm) and not m.positions:  # isempty(m) (expecs.jl:13, expecs_dm.jl:8)
+        if state.is_dm:
+            return state.tr()
+        return (bra or state).dot(state)
     m = _as_operator(m)
-    if state.is_dm:
-        if bra is not None:
-            raise NotImplementedError("bra-ket expectation on density matrices")
-        total = 0j
-        for key, M in m.data().items():
-            Mf = np.ascontiguousarray(M.reshape(-1, order="F"))
-            out = np.zeros(2)
-            _lib.call("vqc_dm_expec_dense", state.handle, len(key), _lib.i32ptr([q - 1 for q in key]), _lib.dptr(Mf),
-                      _lib.dptr(out))
-            total += complex(out[0], out[1])
-        return total
+    if state.is_dm and bra is not None:
+        raise NotImplementedError("bra-ket expectation on density matrices")
     if m.largest_nterm() <= LARGEST_SUPPORTED_NTERMS:
-        total = 0j
-        for key, M in m.data().items():
-            Mf = np.ascontiguousarray(M.reshape(-1, order="F"))
-            out = np.zeros(2)
-            _lib.call("vqc_sv_expec_dense", state.handle, bra.handle if bra else None, len(key),
-                      _lib.i32ptr([q - 1 for q in key]), _lib.dptr(Mf), _lib.dptr(out))
-            total += complex(out[0], out[1])
-        return total
-    arr, nt = pack_pauli(m)
-    s2 = np.zeros(2)
-    _lib.call("vqc_sv_expec_pauli", state.handle, bra.handle if bra else None, nt, arr, None, _lib.dptr(s2))
-    return complex(s2[0], s2[1])
+        return _expec_keys(m, state, bra)
+    short, pauli, other = split_long(m)
+    total = _expec_keys(short, state, bra) if short.terms else 0j
+    if state.is_dm:
+        return total + _expec_products(pauli.terms + other, state, None)
+    if pauli.terms:
+        arr, nt = pack_pauli(pauli)
+        s2 = np.zeros(2)
+        _lib.call("vqc_sv_expec_pauli", state.handle, bra.handle if bra else None, nt, arr, None, _lib.dptr(s2))
+        total += complex(s2[0], s2[1])
+    return total + _expec_products(other, state, bra)
 
 
 def apply_operator(m, state):
     """H * state as a new state ((m::QubitsOperator)(v), apply_qterms.jl:13-28). On a
-    density matrix the keys act on the row bits of vec(rho) (apply_qterms_dm.jl:11-27)."""
+    density matrix the keys act on the row bits of vec(rho) (apply_qterms_dm.jl:11-27).
+    Terms of more than 5 sites take the per-term route (_apply_qterm_util!,
+    apply_qterms.jl:47-59)."""
     m = _as_operator(m)
     out = DensityMatrix(state.nqubits()) if state.is_dm else StateVector(state.nqubits())
     if m.largest_nterm() <= LARGEST_SUPPORTED_NTERMS:
         terms, nt, pool = pack_terms(m)
         _lib.call("vqc_sv_apply_terms", state.handle, out.handle, nt, terms, _lib.dptr(pool), None)
-    else:
-        arr, nt = pack_pauli(m)
+        return out
+    short, pauli, other = split_long(m)
+    if not short.terms and not other and not state.is_dm:
+        arr, nt = pack_pauli(pauli)
         _lib.call("vqc_sv_apply_pauli_sum", state.handle, out.handle, nt, arr)
+        return out
+    accumulate = 0
+    if short.terms:
+        terms, nt, pool = pack_terms(short)
+        _lib.call("vqc_sv_apply_terms", state.handle, out.handle, nt, terms, _lib.dptr(pool), None)
+        accumulate = 1
+    arr, nt, pos, pool = pack_qterms(pauli.terms + other)
+    _lib.call("vqc_sv_apply_qterms", state.handle, out.handle, nt, arr, pos, _lib.dptr(pool), accumulate)
     return out
 
 
@@ -151,7 +203,8 @@ def expectation_vjp(H, state, z=1.0):
     z = complex(z)
     n = state.nqubits()
     if state.is_dm:
-        ident = DensityMatrix(np.eye(1 << n, dtype=np.complex128))
+        ident = DensityMatrix(n)
+        _lib.call("vqc_dm_identity", ident.handle)
         out = apply_operator(H.adjoint(), ident)
         out.scale_(z)
         return out
@@ -197,11 +250,26 @@ def circuit_vjp(circuit, x, delta_y, return_state=False):
 
 def energy_and_gradient(H, circuit, state0, fused=True, return_state=False):
     """E = expectation(H, circuit * state0) and dRe(E)/dtheta for every active
-    parameter of the circuit, in forward order (circuitdiff.jl:61-75)."""
+    parameter of the circuit, in forward order (circuitdiff.jl:61-75).
+
+    A circuit holding a channel, applied to a state vector, runs on
+    DensityMatrix(state0) (apply_qmaps.jl:19). Hamiltonians of keys of at most 5
+    sites run in one fused call (forward, the seed (H + H^dagger)psi or H^dagger
+    with the energy, reverse). With longer terms the reference's own chain is
+    taken: y = circuit * state0, E = expectation(H, y), the expectation pullback
+    (expecdiff.jl:37-56: the per-term route of _apply_qterm_util!) and the circuit
+    pullback of that cotangent (circuitdiff.jl:2-8)."""
     H = _as_operator(H)
+    if not state0.is_dm and any(isinstance(g, QuantumMap) for g in circuit):
+        state0 = DensityMatrix(state0)
     dm = bool(state0.is_dm)
     if H.largest_nterm() > LARGEST_SUPPORTED_NTERMS:
-        raise NotImplementedError("adjoint with terms longer than 5 sites")
+        y = apply_(circuit, state0.copy())
+        E = expectation(H, y)
+        delta = expectation_vjp(H, y, 1.0)
+        grad, _ = circuit_vjp(circuit, state0, delta)
+        res = (E, grad)
+        return res + (y,) if return_state else res
     prog = Program(circuit, n=state0.nqubits(), dm=dm)
     pool = prog.pool()
     terms, nt, hpool = pack_terms(H)

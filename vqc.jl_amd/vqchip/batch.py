@@ -32,6 +32,38 @@ def instance_bytes(n, t_small=None):
     return 0 if n <= t_small else 32 << n
 
 
+def _op_key(o):
+    """Every field of a vqc_op_t record the batch plan depends on."""
+    return (o.kind, o.nq, tuple(o.pos), o.nparams, tuple(o.slot), o.reserved, o.mat, o.aux, tuple(o.gen))
+
+
+def repack(circuits, nqubits, nops, old_ops, pool_size):
+    """The matrix pool of new circuits for a batch planned from old_ops.
+
+    A batch is planned from its op records (kinds, positions, parameter slots,
+    matrix offsets and the structure hints the device planner used), so every
+    record of the new circuits must equal the stored one; only the matrix values
+    may change. Anything else raises ValueError."""
+    circuits = list(circuits)
+    if len(circuits) != len(nqubits):
+        raise ValueError(f"{len(circuits)} circuits for a batch of {len(nqubits)}")
+    chunks = _lib.MatrixPool()
+    k = 0
+    for i, c in enumerate(circuits):
+        prog = Program(c, n=nqubits[i], pool=chunks)
+        if len(prog.ops) != nops[i]:
+            raise ValueError(f"instance {i}: {len(prog.ops)} ops instead of {nops[i]}; create a new Batch")
+        for j, o in enumerate(prog.ops):
+            if _op_key(o) != _op_key(old_ops[k + j]):
+                raise ValueError(f"instance {i}, op {j}: kind, qubits, parameters, matrix offsets or structure "
+                                 "changed; create a new Batch")
+        k += len(prog.ops)
+    pool = chunks.array()
+    if pool.size != pool_size:
+        raise ValueError("circuit structure changed; create a new Batch")
+    return pool
+
+
 def waves(instances, max_bytes):
     """Consecutive groups of instances whose states fit max_bytes (an instance
     larger than that forms a wave of its own)."""
@@ -141,14 +173,10 @@ class _Part:
         self._grads = np.zeros(max(grad, 1))
 
     def update(self, circuits):
-        """New parameters for the same circuit structures (matrix pool rebuilt)."""
-        chunks = _lib.MatrixPool()
-        for c, n in zip(circuits, [self._inst[i].nqubits for i in range(self.n_inst)]):
-            Program(c, n=n, pool=chunks)
-        pool = chunks.array()
-        if pool.size != self._pool.size:
-            raise ValueError("circuit structure changed; create a new Batch")
-        self._pool = pool
+        """New parameters for the same circuit structures (matrix pool rebuilt;
+        see repack)."""
+        self._pool = repack(circuits, [self._inst[i].nqubits for i in range(self.n_inst)],
+                            [self._inst[i].nops for i in range(self.n_inst)], self._ops, self._pool.size)
 
     def launch(self):
         _lib.call("vqc_batch_launch", self._h, _lib.dptr(self._pool), _lib.dptr(self._hpool))

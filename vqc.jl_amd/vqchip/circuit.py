@@ -70,7 +70,8 @@ class Program:
             op = _lib.Op()
             if isinstance(g, QuantumMap):
                 if not dm:
-                    raise ValueError("channels act on density matrices only (apply_qmaps.jl:19)")
+                    raise ValueError("a channel in a state-vector program: the reference continues on "
+                                     "DensityMatrix(state) (apply_qmaps.jl:19); pack it for a density matrix")
                 S = g.ordered_supermat()
                 op.kind = _lib.OP_CHANNEL
                 op.mat = self._add(S)

@@ -2,9 +2,13 @@
 
 expectation(H, psi) merges all terms that share a position tuple into one
 dense matrix and does one sweep per key when every term spans at most 5
-sites (hamiltonian/expecs/expecs.jl:22-28, hamiltonian/util.jl:6-15);
-longer terms go through the Pauli-string kernels.
+sites (hamiltonian/expecs/expecs.jl:22-28, hamiltonian/util.jl:6-15).
+Longer terms take the reference's per-term route (_apply_qterm_util!,
+apply_qterms.jl:47-59): Pauli strings through the bitmask kernels, any other
+product of 2x2 factors through the product-term kernels (vqc_qterm_t).
 """
+import ctypes
+
 import numpy as np
 
 from . import _lib
@@ -129,3 +133,34 @@ def pack_pauli(op):
         arr[i].x, arr[i].z = x, z
         arr[i].re, arr[i].im = t.coeff.real, t.coeff.imag
     return arr, len(op.terms)
+
+
+def split_long(op):
+    """(the terms of at most 5 sites as a QubitsOperator, the longer Pauli strings
+    as a QubitsOperator, the longer other terms as a list)."""
+    short, pauli, other = [], [], []
+    for t in op.terms:
+        if len(t.positions) <= LARGEST_SUPPORTED_NTERMS:
+            short.append(t)
+        elif t.is_pauli():
+            pauli.append(t)
+        else:
+            other.append(t)
+    return QubitsOperator(short), QubitsOperator(pauli), other
+
+
+def pack_qterms(terms):
+    """QubitsTerms -> (vqc_qterm_t array, count, int32 positions, complex pool):
+    each term's 0-based sites and its 2x2 factors in site order."""
+    arr = (_lib.QTerm * max(len(terms), 1))()
+    pos, chunks = [], _lib.MatrixPool()
+    for i, t in enumerate(terms):
+        arr[i].nsites = len(t.positions)
+        arr[i].pos = len(pos)
+        arr[i].re, arr[i].im = t.coeff.real, t.coeff.imag
+        pos.extend(p - 1 for p in t.positions)
+        arr[i].mat = chunks.size
+        for m in t.oplist:
+            chunks.add(np.ascontiguousarray(np.asarray(m, dtype=np.complex128).reshape(-1, order="F")))
+    parr = (ctypes.c_int32 * max(len(pos), 1))(*pos)
+    return arr, len(terms), parr, chunks.array()

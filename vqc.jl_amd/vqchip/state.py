@@ -104,9 +104,12 @@ class DensityMatrix(_DeviceState):
             _lib.call("vqc_dm_create", n, ctypes.byref(h))
             super().__init__(2 * n, h)
             return
-        if isinstance(data_or_n, StateVector):  # DensityMatrix(psi) = kron(conj psi, psi)
-            psi = data_or_n.storage()
-            data = np.kron(psi.conj(), psi)
+        if isinstance(data_or_n, StateVector):  # DensityMatrix(psi) = kron(conj psi, psi), on the device
+            n = data_or_n.nqubits()
+            _lib.call("vqc_dm_create", n, ctypes.byref(h))
+            super().__init__(2 * n, h)
+            _lib.call("vqc_dm_from_sv", self._h, data_or_n.handle)
+            return
         else:
             data = np.asarray(data_or_n, dtype=np.complex128)
             if data.ndim == 2:
