@@ -7,14 +7,26 @@ kernel per gate (the unfused drop-in path). `value` is the algorithmic HBM
 bandwidth of those sweeps: every gate reads and writes all 2^28 amplitudes once,
 2 * 16 * 2^28 bytes per gate.
 
+Every workload also reports its error against an FP64 reference: the oracle's
+values (computed in the CPU leg, which times them as the baseline, or frozen in
+tests/golden/ for the workloads that take the oracle minutes), closed forms
+(the Pauli energy of a product state), or the plain path where the oracle is out
+of reach (served requests above 16 qubits).
+
 Prints one JSON line. --gpus N > 1 is refused: this build runs on one GPU.
 """
-import argparse
-import ctypes
-import json
 import os
-import sys
-import time
+
+# CPU baseline threads: one per core this process may run on, pinned in order
+# (read by the OpenMP runtime of the CPU reference when it loads)
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
+
+import argparse  # noqa: E402
+import ctypes  # noqa: E402
+import json  # noqa: E402
+import sys  # noqa: E402
+import time  # noqa: E402
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "vqc.jl_amd"))
@@ -23,7 +35,9 @@ sys.path.insert(0, os.path.join(ROOT, "oracle"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
 METRIC = "28q gate HBM GB/s; energy+grad evals/s and p99 latency, batched vs plain"
+GOLDEN = os.path.join(ROOT, "tests", "golden")
 
 
 class HipTimer:
@@ -49,6 +63,25 @@ class HipTimer:
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    if a.size == 0:
+        return 0.0
+    return float(np.max(np.abs(a - b)) / max(1.0, float(np.max(np.abs(b)))))
+
+
+def median_s(fn, reps, warm=1):
+    """Median wall time of `reps` calls after `warm` untimed ones."""
+    for _ in range(warm):
+        fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts))
 
 
 def unfused_sweep(vq, _lib, state, mats):
@@ -83,7 +116,44 @@ def _time_evals(fn, min_s=1.0, max_n=5000):
     return n / (time.perf_counter() - t0)
 
 
-def bench_config0(vq, _lib, oracle, timer):
+def eval_flops(circuit, H, n, dm=False):
+    """Algorithmic FP64 flops of one energy + adjoint-gradient evaluation: a dense
+    k-qubit matrix costs 8 * 2^k flops per amplitude it sweeps; the forward pass
+    applies every gate once, the seed applies every key twice (the energy and
+    (M + M^dagger) psi), the reverse pass applies every gate to psi and lambda and
+    every generator once. A density-matrix gate sweeps the rows and the columns."""
+    amps = 4 ** n if dm else 2 ** n
+    side = 2 if dm else 1
+    f = 0.0
+    for g in circuit:
+        k = len(g.ordered_positions())
+        if hasattr(g, "kraus"):
+            f += 3 * 8 * 4 ** k  # superoperator on 2k bits, forward and reverse (S^-1 and S^dagger)
+            continue
+        np_ = g.nparameters()
+        f += side * 8 * 2 ** k * (3 + np_)
+    for key in H.data():
+        f += 2 * 8 * 2 ** len(key)
+    return f * amps
+
+
+def fused_hbm_bytes(info, nbits):
+    """Physical HBM bytes of one fused evaluation from its plan: a forward pass reads
+    and writes the state, the Hamiltonian passes read psi and write lambda (and read
+    it back after the first), a reverse pass reads and writes psi and lambda."""
+    sb = 16 * 2 ** nbits
+    ham = 2 * min(info.phases_ham, 1) + 3 * max(info.phases_ham - 1, 0)
+    return sb * (2 * info.phases_fwd + ham + 4 * info.phases_rev)
+
+
+def roofline_of(seconds, flops, hbm_bytes):
+    fl = flops / seconds / 1e12
+    gb = hbm_bytes / seconds / 1e9
+    return {"fp64_TFps": round(fl, 3), "fp64_frac": round(fl / FP64_PEAK_TFS, 4), "hbm_GBps": round(gb, 1),
+            "hbm_frac": round(gb / HBM_PEAK_GBS, 4), "flops_per_eval": flops, "hbm_bytes_per_eval": hbm_bytes}
+
+
+def bench_config0(vq, _lib):
     """12 qubits, 4 HEA layers (144 angles), TFIM: energy + full adjoint gradient."""
     from vqchip.workloads import config0
     c, H, _ = config0()
@@ -91,27 +161,24 @@ def bench_config0(vq, _lib, oracle, timer):
     args = _prog_args(vq, c, H, n)
     psi0 = vq.StateVector(n)
     Ef, gf = _eval(vq, _lib, psi0, args, True)
+    info = _lib.PlanInfo()
+    _lib.call("vqc_plan_info", ctypes.byref(info))
     Ep, gp = _eval(vq, _lib, psi0, args, False)
-    v0 = np.zeros(1 << n, dtype=complex)
-    v0[0] = 1
-    prog, ops, pool, terms, nt, hpool = args
-    Er, gr, _ = oracle.energy_grad(v0, n, 0, ops, len(prog.ops), pool, terms, nt, hpool, prog.nparams)
+    rate = _time_evals(lambda: _eval(vq, _lib, psi0, args, True))
     out = {
         "workload": "12 qubits, 4 layers Rx/Ry/Rz + CNOT ladder, 144 angles, TFIM (23 keys)",
-        "fused_evals_per_s": round(_time_evals(lambda: _eval(vq, _lib, psi0, args, True)), 1),
+        "fused_evals_per_s": round(rate, 1),
         "plain_evals_per_s": round(_time_evals(lambda: _eval(vq, _lib, psi0, args, False)), 1),
-        "cpu_evals_per_s": round(_time_evals(lambda: oracle.energy_grad(v0, n, 0, ops, len(prog.ops), pool, terms, nt,
-                                                                        hpool, prog.nparams)), 1),
         "energy": Ef.real,
-        "max_abs_err_energy_vs_oracle": abs(Ef - Er),
-        "max_abs_err_grad_vs_oracle": float(np.max(np.abs(gf - gr))),
-        "max_abs_err_grad_plain_vs_oracle": float(np.max(np.abs(gp - gr))),
+        "max_abs_diff_plain_vs_fused": max(abs(Ep - Ef), float(np.max(np.abs(gp - gf)))),
+        "roofline_fused": roofline_of(1.0 / rate, eval_flops(c, H, n), fused_hbm_bytes(info, n)),
+        "tile_passes": {"fwd": info.phases_fwd, "ham": info.phases_ham, "rev": info.phases_rev},
     }
     log(f"config0: {out}")
-    return out
+    return out, (Ef, gf, Ep, gp, args)
 
 
-def bench_config2(vq, _lib, oracle, timer):
+def bench_config2(vq, _lib):
     """256 instances, n ~ U{8..20}, depth ~ U{2..12}, Heisenberg: batched graph vs plain per-instance loop."""
     from vqchip.batch import Batch
     from vqchip.workloads import config2_instances
@@ -120,11 +187,8 @@ def bench_config2(vq, _lib, oracle, timer):
     b = Batch(inst)
     plan_s = time.perf_counter() - t0
     E_b, g_b = b.run()
-    reps = 5
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        b.run()
-    t_batch = (time.perf_counter() - t0) / reps
+    binfo = b.info()
+    t_batch = median_s(b.run, 7)
     # plain: one circuit per call, one kernel per gate and per reduction
     prepared = [(_prog_args(vq, c, H, n), n) for c, H, n in inst]
     states = {}
@@ -137,25 +201,20 @@ def bench_config2(vq, _lib, oracle, timer):
         E_p.append(E)
         g_p.append(g)
     t_plain = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    for a, n in prepared:
-        _eval(vq, _lib, states[n], a, True)
-    t_fused = time.perf_counter() - t0
-    err_e = max(abs(x - y) for x, y in zip(E_b, E_p))
-    err_g = max(float(np.max(np.abs(x - y))) if x.size else 0.0 for x, y in zip(g_b, g_p))
-    # oracle on the small instances (n <= 14) of the batch
-    err_o, n_o, t_o, cnt = 0.0, 0, 0.0, 0
-    for i, ((a, n), (c, H, _)) in enumerate(zip(prepared, inst)):
-        if n > 14:
-            continue
-        prog, ops, pool, terms, nt, hpool = a
-        v0 = np.zeros(1 << n, dtype=complex)
-        v0[0] = 1
-        t1 = time.perf_counter()
-        Er, gr, _ = oracle.energy_grad(v0, n, 0, ops, len(prog.ops), pool, terms, nt, hpool, prog.nparams)
-        t_o += time.perf_counter() - t1
-        cnt += 1
-        err_o = max(err_o, abs(E_b[i] - Er), float(np.max(np.abs(g_b[i] - gr))) if gr.size else 0.0)
+
+    def fused_loop():
+        for a, n in prepared:
+            _eval(vq, _lib, states[n], a, True)
+
+    t_fused = median_s(fused_loop, 3)
+    err_p = max(max(abs(x - y) for x, y in zip(E_b, E_p)),
+                max(float(np.max(np.abs(x - y))) if x.size else 0.0 for x, y in zip(g_b, g_p)))
+    # every instance against the oracle's values (frozen: minutes of CPU work)
+    fx = np.load(os.path.join(GOLDEN, "config2_energy_grad.npz"), allow_pickle=False)
+    off = fx["grad_offsets"]
+    err_o = max(max(rel_err(E_b[i], fx["energy"][i]), rel_err(g_b[i], fx["grad"][off[i]:off[i + 1]]))
+                for i in range(len(inst)))
+    flops = sum(eval_flops(c, H, n) for c, H, n in inst)
     out = {
         "workload": "256 instances, n~U{8..20}, depth~U{2..12}, Heisenberg J~N(0,1)",
         "batched_evals_per_s": round(len(inst) / t_batch, 1),
@@ -163,19 +222,21 @@ def bench_config2(vq, _lib, oracle, timer):
         "plain_evals_per_s": round(len(inst) / t_plain, 1),
         "plain_ms": round(t_plain * 1e3, 1),
         "fused_per_instance_evals_per_s": round(len(inst) / t_fused, 1),
+        "fused_per_instance_ms": round(t_fused * 1e3, 1),
         "speedup_batched_vs_plain": round(t_plain / t_batch, 2),
+        "speedup_batched_vs_fused_per_instance": round(t_fused / t_batch, 2),
         "plan_ms": round(plan_s * 1e3, 1),
         "arena_MiB": round(b.arena_bytes() / 2**20, 1),
-        "max_abs_diff_batched_vs_plain": max(err_e, err_g),
-        "max_abs_err_vs_oracle_n_le_14": err_o,
-        "oracle_checked_instances": cnt,
-        "cpu_evals_per_s_n_le_14_sample": round(cnt / t_o, 1) if t_o else None,
+        "batch_plan": binfo,
+        "roofline_batched": roofline_of(t_batch, flops, binfo["hbm_bytes"]),
+        "max_abs_diff_batched_vs_plain": err_p,
+        "max_rel_err_vs_oracle_all_256": err_o,
     }
     log(f"config2: {out}")
     return out
 
 
-def bench_config3(vq, _lib, oracle, timer):
+def bench_config3(vq, _lib):
     """12-qubit density matrix, 6 noisy layers, sum Z_i: energy + 216-angle gradient."""
     from vqchip.workloads import config3
     c, H, _ = config3()
@@ -184,17 +245,22 @@ def bench_config3(vq, _lib, oracle, timer):
     rho0 = vq.DensityMatrix(n)
     from vqchip import autotune
     Ef, gf = _eval(vq, _lib, rho0, args, True)
+    info = _lib.PlanInfo()
+    _lib.call("vqc_plan_info", ctypes.byref(info))
     Ep, gp = _eval(vq, _lib, rho0, args, False)
-    tf_default = 1.0 / _time_evals(lambda: _eval(vq, _lib, rho0, args, True), min_s=1.0, max_n=60)
-    tp = 1.0 / _time_evals(lambda: _eval(vq, _lib, rho0, args, False), min_s=1.0, max_n=5)
+    tf_default = median_s(lambda: _eval(vq, _lib, rho0, args, True), 9)
+    tp = median_s(lambda: _eval(vq, _lib, rho0, args, False), 3)
     # the autotuner picks the tile shape for this workload (tile bits, contiguous low bits)
     before = autotune.get_tuning()
     t0 = time.perf_counter()
     tuned = autotune.tune_energy_grad(H, c, rho0, fwd=(11, 12), rev=(10, 11, 12), low=(2, 3, 4, 6), reps=2)
     tune_s = time.perf_counter() - t0
     Et, gt = _eval(vq, _lib, rho0, args, True)
-    tf = 1.0 / _time_evals(lambda: _eval(vq, _lib, rho0, args, True), min_s=1.0, max_n=60)
+    info_t = _lib.PlanInfo()
+    _lib.call("vqc_plan_info", ctypes.byref(info_t))
+    tf = median_s(lambda: _eval(vq, _lib, rho0, args, True), 9)
     autotune.set_tuning(**{k: before[k] for k in autotune.FIELDS})
+    fx = np.load(os.path.join(GOLDEN, "config3_energy_grad.npz"), allow_pickle=False)
     out = {
         "workload": "12-qubit DM (2^24 entries), 6 layers, depolarizing p=0.01 after each CNOT, "
                     "amplitude damping 0.005 per qubit per layer, sum Z_i, 216 angles",
@@ -205,9 +271,12 @@ def bench_config3(vq, _lib, oracle, timer):
         "autotune_s": round(tune_s, 2),
         "plain_evals_per_s": round(1 / tp, 3),
         "plain_ms": round(tp * 1e3, 2),
+        "speedup_fused_vs_plain": round(tp / tf, 2),
         "energy": Ef.real,
-        "max_abs_diff_fused_vs_plain": max(abs(Ef - Ep), float(np.max(np.abs(gf - gp)))),
-        "max_abs_diff_autotuned_vs_plain": max(abs(Et - Ep), float(np.max(np.abs(gt - gp)))),
+        "roofline_fused": roofline_of(tf, eval_flops(c, H, n, dm=True), fused_hbm_bytes(info_t, 2 * n)),
+        "max_rel_err_vs_oracle": max(rel_err(Ef, fx["energy"]), rel_err(gf, fx["grad"]), rel_err(Et, fx["energy"]),
+                                     rel_err(gt, fx["grad"])),
+        "max_rel_err_plain_vs_oracle": max(rel_err(Ep, fx["energy"]), rel_err(gp, fx["grad"])),
     }
     log(f"config3: {out}")
     return out
@@ -224,12 +293,16 @@ def bench_config4(vq, _lib, count, seed=20261203):
     """Serving: open-loop Poisson stream of energy + gradient requests, n log-uniform
     in 10..26, depth U{4..40}, TFIM h ~ U(0.5, 1.5). The rate is set so that the
     plain path (one request per call, one kernel per gate) is ~70% busy; the
-    continuous-batching server gets the same arrival trace."""
-    from vqchip.serve import Request, Server, poisson_arrivals, run_plain_fifo, run_server_open_loop, plain_eval
+    continuous-batching server gets the same arrival trace. The server's closed-loop
+    saturation rate is measured too, and open-loop streams at 50%, 80% and 100% of
+    it, with every request packed at its arrival (packing counts in its latency)."""
+    from vqchip.serve import (Request, Server, poisson_arrivals, run_plain_fifo, run_server_open_loop, plain_eval,
+                              run_server_closed_loop)
     from vqchip.workloads import config4_request
     rng = np.random.default_rng(seed)
+    specs = [config4_request(rng) for _ in range(count)]
     t0 = time.perf_counter()
-    reqs = [Request(*config4_request(rng)) for _ in range(count)]
+    reqs = [Request(*s) for s in specs]
     pack_s = time.perf_counter() - t0
     # service time of every request on the plain path, back to back (also its closed-loop throughput)
     states = {}
@@ -246,21 +319,39 @@ def bench_config4(vq, _lib, count, seed=20261203):
     arr = poisson_arrivals(rate, count, np.random.default_rng(seed + 1))
     lat_p, busy_p, span_p = run_plain_fifo(reqs, arr, fused=False)
     log(f"config4: plain open loop done ({span_p:.1f} s)")
-    with Server(hbm_budget=288e9) as srv:
-        lat_s, span_s, res_s = run_server_open_loop(srv, reqs, arr)
-        st = srv.stats()
     err = 0.0
-    for (E, g), (Ep, gp) in zip(res_s, plain_res):
-        err = max(err, abs(E - Ep), float(np.max(np.abs(g - gp))) if gp.size else 0.0)
-    # the same requests at 10x the rate: far beyond the plain path's capacity, so
-    # requests pile up and the server batches them (continuous batching under load)
-    arr_hi = poisson_arrivals(10 * rate, count, np.random.default_rng(seed + 2))
-    with Server(hbm_budget=288e9) as srv:
-        lat_h, span_h, res_h = run_server_open_loop(srv, reqs, arr_hi)
-        st_h = srv.stats()
-    for (E, g), (Ep, gp) in zip(res_h, plain_res):
-        err = max(err, abs(E - Ep), float(np.max(np.abs(g - gp))) if gp.size else 0.0)
+
+    def check(res):
+        nonlocal err
+        for (E, g), (Ep, gp) in zip(res, plain_res):
+            err = max(err, abs(E - Ep), float(np.max(np.abs(g - gp))) if gp.size else 0.0)
+
     pct = lambda v, q: round(float(np.percentile(v, q)) * 1e3, 2)
+
+    def hist(st):
+        return {int(k): int(v) for k, v in sorted(st["batch_size_hist"].items())}
+
+    with Server(hbm_budget=288e9) as srv:
+        lat_s, span_s, res_s = run_server_open_loop(srv, reqs, arr, pack=specs)
+        st = srv.stats()
+    check(res_s)
+    # closed loop: every request queued at once, the server's saturation rate
+    with Server(hbm_budget=288e9) as srv:
+        span_c, res_c = run_server_closed_loop(srv, reqs)
+        st_c = srv.stats()
+    check(res_c)
+    sat = count / span_c
+    loads = {}
+    for frac in (0.5, 0.8, 1.0):
+        arr_f = poisson_arrivals(frac * sat, count, np.random.default_rng(seed + int(frac * 10)))
+        with Server(hbm_budget=288e9) as srv:
+            lat_f, span_f, res_f = run_server_open_loop(srv, reqs, arr_f, pack=specs)
+            st_f = srv.stats()
+        check(res_f)
+        loads[f"{int(frac * 100)}pct"] = {"rate_per_s": round(frac * sat, 2), "served_evals_per_s": round(count / span_f, 2),
+                                         "p50_ms": pct(lat_f, 50), "p99_ms": pct(lat_f, 99),
+                                         "mean_batch_size": round(st_f["batched_requests"] / max(st_f["batches"], 1), 2),
+                                         "batch_size_hist": hist(st_f)}
     out = {
         "workload": f"{count} requests, n=round(exp(U(ln10, ln26))), depth~U{{4..40}}, HEA layer, TFIM h~U(0.5,1.5), "
                     "Poisson arrivals",
@@ -272,22 +363,135 @@ def bench_config4(vq, _lib, count, seed=20261203):
         "plain_closed_loop_evals_per_s": round(count / float(np.sum(svc)), 3),
         "served_p50_ms": pct(lat_s, 50), "served_p99_ms": pct(lat_s, 99), "served_mean_ms": round(float(np.mean(lat_s)) * 1e3, 2),
         "served_evals_per_s": round(count / span_s, 3),
+        "latency_includes_packing": True,
         "p99_speedup": round(float(np.percentile(lat_p, 99) / np.percentile(lat_s, 99)), 2),
         "p50_speedup": round(float(np.percentile(lat_p, 50) / np.percentile(lat_s, 50)), 2),
         "server_batches": st["batches"],
         "mean_batch_size": round(st["batched_requests"] / max(st["batches"], 1), 2),
+        "batch_size_hist": hist(st),
+        "saturation_closed_loop_evals_per_s": round(sat, 2),
+        "saturation_batch_size_hist": hist(st_c),
+        "load_sweep": loads,
         "peak_live_GiB": round(st["peak_live_bytes"] / 2**30, 3),
         "budget_GiB": round(st["budget_bytes"] / 2**30, 1),
         "max_abs_diff_served_vs_plain": err,
-        "at_10x_rate": {"rate_per_s": round(10 * rate, 3), "served_evals_per_s": round(count / span_h, 3),
-                        "served_p50_ms": pct(lat_h, 50), "served_p99_ms": pct(lat_h, 99),
-                        "server_batches": st_h["batches"],
-                        "mean_batch_size": round(st_h["batched_requests"] / max(st_h["batches"], 1), 2)},
         "qubits_histogram": {int(k): int(v) for k, v in zip(*np.unique([r.n for r in reqs], return_counts=True))},
         "pack_s": round(pack_s, 2),
     }
     log(f"config4: {out}")
-    return out
+    return out, reqs, plain_res
+
+
+def pauli_product_state_check(vq, _lib, n, Hp, arr, nt, seed=281):
+    """200 Pauli strings on a product state: <P> = c prod_q <u_q|sigma_q|u_q>, the
+    FP64 closed form of config 1's energy sweep."""
+    rng = np.random.default_rng(seed)
+    prod = vq.StateVector(n)
+    us = []
+    for q in range(n):
+        Z = rng.standard_normal((2, 2)) + 1j * rng.standard_normal((2, 2))
+        Q, R = np.linalg.qr(Z)
+        U = Q * (np.diag(R) / np.abs(np.diag(R)))
+        us.append(U[:, 0])
+        _lib.call("vqc_sv_apply_dense", prod.handle, 1, _lib.i32ptr([q]), _lib.dptr(np.ascontiguousarray(U.reshape(-1, order="F"))))
+    per = np.zeros(2 * nt)
+    s2 = np.zeros(2)
+    _lib.call("vqc_sv_expec_pauli", prod.handle, None, nt, arr, _lib.dptr(per), _lib.dptr(s2))
+    sig = {"X": np.array([[0, 1], [1, 0]], complex), "Y": np.array([[0, -1j], [1j, 0]]), "Z": np.diag([1.0 + 0j, -1]),
+           "I": np.eye(2, dtype=complex)}
+    closed = []
+    for t in Hp.terms:
+        v = t.coeff
+        for p, l in zip(t.positions, t.labels):
+            v *= us[p - 1].conj() @ sig[l] @ us[p - 1]
+        closed.append(v)
+    return rel_err(per[0::2] + 1j * per[1::2], np.array(closed))
+
+
+def cpu_baseline_leg(oracle, args, circuit, c0, c4):
+    """The CPU reference (the oracle: a C++/OpenMP restatement of the reference's
+    threaded kernels) on this host's cores, pinned, one thread per core: 1 warm-up,
+    then the median of at least 5 repetitions of a fixed sample of each workload.
+    It also yields the oracle's values that the device results are checked against."""
+    cores = len(os.sched_getaffinity(0))
+    oracle.set_threads(cores)
+    out = {"threads": cores, "nproc": os.cpu_count(), "omp_proc_bind": os.environ.get("OMP_PROC_BIND"),
+           "omp_places": os.environ.get("OMP_PLACES")}
+    n = args.qubits
+    # config 1: the first layer (14 + 7 + 3 gates) of the 28-qubit circuit
+    v = np.zeros(1 << n, dtype=np.complex128)
+    v[0] = 1
+    layer = list(circuit[:24])
+
+    def sweep_layer():
+        for g in layer:
+            pos = [q - 1 for q in g.ordered_positions()]
+            oracle.apply_dense(v, len(pos), pos, g.ordered_mat())
+
+    t1 = median_s(sweep_layer, args.cpu_reps)
+    gbs = 2 * 16 * (1 << n) * len(layer) / t1 / 1e9
+    out["config1"] = {"GBps": round(gbs, 2), "sample": f"the first layer ({len(layer)} gates) of the {n}-qubit circuit",
+                      "median_s": round(t1, 3)}
+    log(f"cpu config1: {gbs:.1f} GB/s on {cores} threads")
+    del v
+    # config 0: the whole evaluation
+    Ef, gf, Ep, gp, a0 = c0
+    prog, ops, pool, terms, nt, hpool = a0
+    v0 = np.zeros(1 << 12, dtype=complex)
+    v0[0] = 1
+    res = {}
+
+    def e0():
+        res["r"] = oracle.energy_grad(v0, 12, 0, ops, len(prog.ops), pool, terms, nt, hpool, prog.nparams)
+
+    t0 = median_s(e0, max(args.cpu_reps, 15))
+    Er, gr, _ = res["r"]
+    out["config0"] = {"evals_per_s": round(1 / t0, 2), "sample": "the whole 12-qubit evaluation", "median_s": round(t0, 5)}
+    errs0 = {"max_rel_err_fused_vs_oracle": max(rel_err(Ef, Er), rel_err(gf, gr)),
+             "max_rel_err_plain_vs_oracle": max(rel_err(Ep, Er), rel_err(gp, gr))}
+    # config 2: the instances of up to 12 qubits of the batch (the whole evaluation each)
+    from vqchip.workloads import config2_instances
+    import vqchip as vq
+    sample = [(vq.Program(c, n=nn), H, nn) for c, H, nn in config2_instances() if nn <= 12]
+    from vqchip.operators import pack_terms
+    packed = []
+    for p, H, nn in sample:
+        t, k, hp = pack_terms(H)
+        z = np.zeros(1 << nn, dtype=complex)
+        z[0] = 1
+        packed.append((p, p.op_array(), p.pool(), t, k, hp, nn, z))
+
+    def e2():
+        for p, o, pl, t, k, hp, nn, z in packed:
+            oracle.energy_grad(z, nn, 0, o, len(p.ops), pl, t, k, hp, p.nparams)
+
+    t2 = median_s(e2, args.cpu_reps)
+    out["config2"] = {"evals_per_s": round(len(packed) / t2, 2), "median_s": round(t2, 3),
+                      "sample": f"the {len(packed)} instances of up to 12 qubits of the 256-instance batch"}
+    # config 3: one noisy layer of the 12-qubit density-matrix circuit (the full six take ~25 s per evaluation)
+    from vqchip.workloads import config3
+    c3, H3, _ = config3(depth=1)
+    p3 = vq.Program(c3, n=12, dm=True)
+    t3_, k3, hp3 = pack_terms(H3)
+    rho0 = np.zeros(1 << 24, dtype=complex)
+    rho0[0] = 1
+    o3, pl3 = p3.op_array(), p3.pool()
+    t3 = median_s(lambda: oracle.energy_grad(rho0, 12, 1, o3, len(p3.ops), pl3, t3_, k3, hp3, p3.nparams), args.cpu_reps)
+    out["config3"] = {"evals_per_s_one_layer": round(1 / t3, 3), "median_s": round(t3, 3),
+                      "sample": "one of the six noisy layers (72 of the 432 ops), energy + 36 gradients"}
+    # config 4: the oracle's values of the requests of up to 16 qubits (checks the served results)
+    errs4 = 0.0
+    if c4 is not None:
+        reqs, plain_res = c4
+        for r, (Ep4, gp4) in zip(reqs, plain_res):
+            if r.n > 16:
+                continue
+            z = np.zeros(1 << r.n, dtype=complex)
+            z[0] = 1
+            Er4, gr4, _ = oracle.energy_grad(z, r.n, 0, r.ops, r.nops, r.pool, r.terms, r.nterms, r.hpool, r.nparams)
+            errs4 = max(errs4, rel_err(Ep4, Er4), rel_err(gp4, gr4))
+    log(f"cpu leg: {out}")
+    return out, errs0, errs4
 
 
 def main():
@@ -297,9 +501,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--qubits", type=int, default=28)
     ap.add_argument("--layers", type=int, default=20)
-    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--cpu-reps", type=int, default=5)
     ap.add_argument("--skip-extra", action="store_true")
-    ap.add_argument("--only", default="", help="comma list of parts to run: c0,c1,c2,c3,c4 (default all)")
+    ap.add_argument("--only", default="", help="comma list of parts to run: c0,c2,c3,c4,cpu (default all)")
     ap.add_argument("--serve-requests", type=int, default=100)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -369,12 +573,17 @@ def main():
         _lib.call("vqc_state_copy", other.handle, state.handle)
     ms_copy = timer.stop_ms() / 3
     extra["copy_GBps"] = round(2 * 16 * (1 << n) / (ms_copy * 1e-3) / 1e9, 1)
-    del other
     log(f"copy {extra['copy_GBps']} GB/s")
-    # fused (LDS tile phases)
+    # fused (LDS tile phases), from |0..0>, against the per-gate result of the same circuit
     prog = vq.Program(circuit, n=n)
     ops, pool = prog.op_array(), prog.pool()
+    _lib.call("vqc_state_reset", other.handle)
+    unfused_sweep(vq, _lib, other, mats)
+    _lib.call("vqc_state_reset", state.handle)
     _lib.call("vqc_apply_circuit", state.handle, len(prog.ops), ops, _lib.dptr(pool))
+    _lib.call("vqc_state_axpy", other.handle, -1.0, 0.0, state.handle)
+    extra["fused_vs_plain_l2_diff"] = float(np.sqrt(other.dot(other).real))
+    del other
     _lib.call("vqc_sync")
     timer.start()
     for _ in range(args.steps):
@@ -387,8 +596,11 @@ def main():
     # physical HBM bandwidth of the fused path: each forward tile pass reads and writes the state once
     extra["fused_tile_passes"] = info.phases_fwd
     extra["fused_physical_GBps"] = round(bytes_per_gate * info.phases_fwd / (msf * 1e-3) / 1e9, 1)
+    fl = sum(8 * 2 ** len(g.ordered_positions()) for g in circuit) * 2.0 ** n
+    extra["fused_fp64_TFps"] = round(fl / (msf * 1e-3) / 1e12, 2)
+    extra["fused_bound_ms"] = round(max(bytes_per_gate * info.phases_fwd / HBM_PEAK_GBS / 1e6, fl / FP64_PEAK_TFS / 1e9), 2)
     log(f"fused: {msf:.1f} ms/step, effective {extra['fused_effective_gate_GBps']} GB/s, "
-        f"{info.phases_fwd} tile passes = {extra['fused_physical_GBps']} GB/s physical")
+        f"{info.phases_fwd} tile passes = {extra['fused_physical_GBps']} GB/s physical, {extra['fused_fp64_TFps']} TF/s")
 
     if not args.skip_extra:
         Hp = random_pauli_operator(n, 200)
@@ -400,35 +612,36 @@ def main():
         extra["pauli200_energy"] = s2[0]
         extra["pauli200_ms"] = round(mse, 2)
         extra["pauli200_read_GBps"] = round(16 * (1 << n) * nt / (mse * 1e-3) / 1e9, 1)
-        log(f"pauli energy {s2[0]:.6f} in {mse:.1f} ms")
+        del state
+        extra["pauli200_product_state_rel_err_vs_closed_form"] = pauli_product_state_check(vq, _lib, n, Hp, arr, nt)
+        log(f"pauli energy {s2[0]:.6f} in {mse:.1f} ms, closed-form error "
+            f"{extra['pauli200_product_state_rel_err_vs_closed_form']:.2e}")
+    else:
+        del state
 
-    parts = set(args.only.split(",")) if args.only else {"c0", "c2", "c3", "c4"}
-    import oracle
-    cores = int(os.environ.get("VQC_CPU_THREADS", min(16, os.cpu_count() or 1)))
-    oracle.set_threads(cores)
-    del state  # free the 28-qubit state before the other workloads
+    parts = set(args.only.split(",")) if args.only else {"c0", "c2", "c3", "c4", "cpu"}
+    c0 = c4 = None
     if "c0" in parts:
-        extra["config0"] = bench_config0(vq, _lib, oracle, timer)
+        extra["config0"], c0 = bench_config0(vq, _lib)
     if "c2" in parts:
-        extra["config2"] = bench_config2(vq, _lib, oracle, timer)
+        extra["config2"] = bench_config2(vq, _lib)
     if "c3" in parts:
-        extra["config3"] = bench_config3(vq, _lib, oracle, timer)
+        extra["config3"] = bench_config3(vq, _lib)
     if "c4" in parts:
-        extra["config4"] = bench_config4(vq, _lib, args.serve_requests)
+        extra["config4"], reqs4, plain4 = bench_config4(vq, _lib, args.serve_requests)
+        c4 = (reqs4, plain4)
 
-    # CPU baseline: the oracle (C++/OpenMP restatement of the reference's
-    # threaded kernels) on the host cores, on the first gates of the same circuit
-    v = np.zeros(1 << n, dtype=np.complex128)
-    v[0] = 1
-    done, t0 = 0, time.perf_counter()
-    while done < G and time.perf_counter() - t0 < args.cpu_seconds:
-        g = circuit[done]
-        pos = [q - 1 for q in g.ordered_positions()]
-        oracle.apply_dense(v, len(pos), pos, g.ordered_mat())
-        done += 1
-    cpu_t = time.perf_counter() - t0
-    cpu_gbs = bytes_per_gate * done / cpu_t / 1e9
-    log(f"cpu oracle: {done} gates in {cpu_t:.1f}s = {cpu_gbs:.1f} GB/s on {cores} threads")
+    cpu = {"value": None, "unit": "GB/s", "cores": len(os.sched_getaffinity(0)), "kind": "port", "sample": "not run"}
+    if "cpu" in parts and c0 is not None:
+        import oracle
+        leg, errs0, errs4 = cpu_baseline_leg(oracle, args, circuit, c0, c4)
+        extra["config0"].update(errs0)
+        if c4 is not None:
+            extra["config4"]["max_rel_err_served_vs_oracle_n_le_16"] = errs4
+        extra["cpu_baseline_all"] = leg
+        cpu = {"value": leg["config1"]["GBps"], "unit": "GB/s", "cores": leg["threads"], "kind": "port",
+               "sample": leg["config1"]["sample"] + f", median of {args.cpu_reps} after 1 warm-up, oracle restatement, "
+                         "threads pinned (OMP_PROC_BIND=close, OMP_PLACES=cores)"}
 
     out = {
         "metric": METRIC,
@@ -450,11 +663,10 @@ def main():
                      "frac": round(ach1 / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_1Q_BYTES if n == 28 else None,
                      "kernel": "k_dense_small<1, 1, true> (1-qubit dense sweep)",
                      "avg_launch_ms": round(avg1_ms, 4), "bytes_per_launch": bytes_per_gate},
-        "cpu_baseline": {"value": round(cpu_gbs, 2), "unit": "GB/s", "cores": cores, "kind": "port",
-                         "sample": f"first {done} gates of the same {n}-qubit circuit, oracle restatement"},
+        "cpu_baseline": cpu,
         "extra": extra,
     }
-    print(json.dumps(out), flush=True)
+    print(json.dumps(out, default=float), flush=True)
 
 
 if __name__ == "__main__":

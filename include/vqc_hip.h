@@ -262,6 +262,12 @@ int vqc_partial_trace(const vqc_state* s, int nsites, const int32_t* sites, vqc_
 /* <psi|rho|psi>: fidelity(::DensityMatrix, ::StateVector) (densitymatrix.jl:70). */
 int vqc_dm_sandwich(const vqc_state* rho, const vqc_state* psi, double* out2);
 
+/* real(tr(sqrt(x) sqrt(y))): fidelity(x::DensityMatrix, y::DensityMatrix)
+ * (densitymatrix.jl:65), from Hermitian eigendecompositions of (x + x^dagger)/2 and
+ * (y + y^dagger)/2 (square roots as Julia's sqrt of a Hermitian matrix takes them).
+ * At most 14 qubits. */
+int vqc_dm_fidelity(const vqc_state* x, const vqc_state* y, double* out);
+
 /* ---- circuits -----------------------------------------------------------
  * apply!(circuit, state) for SV or DM (generic.jl:26-31, apply_qmaps.jl:8-16),
  * fused into LDS-tiled passes. */
@@ -345,6 +351,16 @@ int vqc_batch_run(vqc_batch* b, const double* pool, const double* hpool, double*
 int vqc_batch_launch(vqc_batch* b, const double* pool, const double* hpool);
 int vqc_batch_wait(vqc_batch* b, double* energies2, double* grads);
 int vqc_batch_arena_bytes(const vqc_batch* b, uint64_t* bytes);
+/* Plan shape of a batch: instances run in tile phases and in one workgroup's LDS,
+ * phases and work items of the persistent queue, its grid, and the HBM bytes the
+ * phases move per launch (every phase reads and writes its states once). */
+typedef struct vqc_batch_info {
+    int32_t tiled_instances, small_instances;
+    int64_t phases, items;
+    int32_t grid, reserved;
+    uint64_t hbm_bytes, arena_bytes;
+} vqc_batch_info_t;
+int vqc_batch_info(const vqc_batch* b, vqc_batch_info_t* out);
 int vqc_batch_destroy(vqc_batch* b);
 
 /* ---- serving loop (no counterpart in the reference) ------------------------
@@ -371,6 +387,7 @@ typedef struct vqc_server_config {
 typedef struct vqc_server_stats {
     int64_t completed, batches, batched_requests, pending, running;
     uint64_t budget_bytes, reserved_bytes, peak_live_bytes;
+    int64_t batch_hist[8];  /* batches of 1, 2, 3-4, 5-8, 9-16, 17-32, 33-64, 65+ requests */
 } vqc_server_stats_t;
 
 typedef struct vqc_server vqc_server;

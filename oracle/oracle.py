@@ -239,3 +239,20 @@ def energy_grad(state0, n, is_dm, ops, nops, pool, terms, nterms, hpool, nparams
     _rc(fn(n, _c(np.ascontiguousarray(state0)), nops, ops, _c(pool), nterms, terms, _c(hpool), nparams,
            e2.ctypes.data_as(_D), g.ctypes.data_as(_D), _c(out)))
     return complex(e2[0], e2[1]), g[:nparams], out
+
+
+def _julia_hermitian_sqrt(A):
+    """sqrt of a Hermitian matrix as Julia's LinearAlgebra takes it: the
+    eigendecomposition, roots sqrt(max(l, 0)) when every eigenvalue is at least
+    -max|l| * eps * N (round-off), else sqrt(complex(l))."""
+    A = 0.5 * (A + A.conj().T)
+    l, V = np.linalg.eigh(A)
+    l0 = -np.max(np.abs(l)) * np.finfo(float).eps * A.shape[0]
+    r = np.sqrt(np.maximum(l, 0)) if np.all(l >= l0) else np.sqrt(l.astype(complex))
+    return (V * r) @ V.conj().T
+
+
+def fidelity_dm(x, y):
+    """fidelity(x::DensityMatrix, y::DensityMatrix) = real(tr(sqrt(x) * sqrt(y)))
+    (densitymatrix.jl:65) for two 2^n x 2^n matrices."""
+    return float(np.real(np.trace(_julia_hermitian_sqrt(np.asarray(x)) @ _julia_hermitian_sqrt(np.asarray(y)))))

@@ -12,7 +12,8 @@
 Inputs are stored explicitly (states, matrices, the packed op and term tables),
 so a fixture does not depend on the generators that made it.
 
-    python tests/golden/make_golden.py      # from the repository root
+    python tests/golden/make_golden.py           # from the repository root
+    python tests/golden/make_golden.py configs   # the measured workloads (minutes)
 """
 import os
 import sys
@@ -143,5 +144,54 @@ def main():
     np.savez(os.path.join(HERE, "measure_n6.npz"), **m)
 
 
+def inputs_digest(progs_and_terms):
+    """sha256 over the packed op tables, term tables and pools of a workload: a
+    fixture of outputs alone states which inputs it belongs to."""
+    import hashlib
+    h = hashlib.sha256()
+    for prog, terms, nt, hpool in progs_and_terms:
+        h.update(op_table(prog).tobytes())
+        h.update(np.ascontiguousarray(prog.pool()).tobytes())
+        h.update(term_table(terms, nt).tobytes())
+        h.update(np.ascontiguousarray(hpool).tobytes())
+    return h.hexdigest()
+
+
+def config_fixtures():
+    """Oracle outputs at the measured workload sizes (minutes of CPU work; run with
+    the argument `configs`): every instance of the 256-instance uneven batch, and the
+    12-qubit noisy density matrix of 6 layers (216 angles). Inputs come from the
+    seeded generators of vqchip.workloads; the fixture holds their digest."""
+    from vqchip.workloads import config2_instances
+    oracle.set_threads(os.cpu_count() or 1)
+    inst = config2_instances()
+    packed, E, grads, ns = [], [], [], []
+    for c, H, n in inst:
+        prog = Program(c, n=n)
+        terms, nt, hpool = pack_terms(H)
+        packed.append((prog, terms, nt, hpool))
+        psi0 = np.zeros(1 << n, complex)
+        psi0[0] = 1
+        e, g, _ = oracle.energy_grad(psi0, n, 0, prog.op_array(), len(prog.ops), prog.pool(), terms, nt, hpool,
+                                     prog.nparams)
+        E.append(e)
+        grads.append(g)
+        ns.append(n)
+    off = np.concatenate([[0], np.cumsum([len(g) for g in grads])])
+    np.savez_compressed(os.path.join(HERE, "config2_energy_grad.npz"), nqubits=np.array(ns), energy=np.array(E),
+                        grad=np.concatenate(grads), grad_offsets=off, inputs_sha256=np.array(inputs_digest(packed)))
+    c, H, _ = config3()
+    prog = Program(c, n=12, dm=True)
+    terms, nt, hpool = pack_terms(H)
+    rho0 = np.zeros(1 << 24, complex)
+    rho0[0] = 1
+    e, g, _ = oracle.energy_grad(rho0, 12, 1, prog.op_array(), len(prog.ops), prog.pool(), terms, nt, hpool, prog.nparams)
+    np.savez_compressed(os.path.join(HERE, "config3_energy_grad.npz"), energy=np.array(e), grad=g,
+                        inputs_sha256=np.array(inputs_digest([(prog, terms, nt, hpool)])))
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["configs"]:
+        config_fixtures()
+    else:
+        main()

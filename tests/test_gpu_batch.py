@@ -144,3 +144,22 @@ def test_wide_batch_matches_oracle():
     assert np.max(np.abs(np.asarray(E[:15]) - np.asarray(E1))) < 1e-12
     for x, y in zip(grads[:15], g1):
         assert x.size == 0 or np.max(np.abs(x - y)) < 1e-12
+
+
+def test_batch_rejects_more_than_three_parameters():
+    """vqc_batch_create validates every op as the single-state paths do: an op
+    with nparams = 4 (slot[] holds 3) is rejected with VQC_EINVAL."""
+    import ctypes
+    from vqchip import _lib
+    c = hea_circuit(3, 1, np.linspace(0.1, 0.9, 9))
+    prog = Program(c, n=3)
+    ops = prog.op_array()
+    ops[0].nparams = 4
+    terms, nt, hpool = pack_terms(tfim(3))
+    inst = (_lib.Instance * 1)()
+    inst[0].nqubits, inst[0].nops, inst[0].nterms, inst[0].nparams = 3, len(prog.ops), nt, prog.nparams
+    h = ctypes.c_void_p()
+    with pytest.raises(_lib.VQCError) as e:
+        _lib.call("vqc_batch_create", 1, inst, ops, len(prog.ops), terms, nt, len(prog.pool()), len(hpool), 0,
+                  ctypes.byref(h))
+    assert e.value.code == -1

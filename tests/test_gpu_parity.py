@@ -476,3 +476,26 @@ def test_dm_energy_gradient_with_large_and_multi_parameter_gates(fused):
     assert np.max(np.abs(vec(out.storage()) - rho_r)) < 1e-12
     assert abs(E - Er) < 1e-10
     assert np.max(np.abs(g - gr)) < 1e-10 * max(1, np.max(np.abs(gr)))
+
+
+@pytest.mark.parametrize("n", [3, 5])
+def test_channel_on_state_vector_continues_on_its_density_matrix(n):
+    """apply!(circuit, psi) applies element by element (generic.jl:26-31) and a
+    channel on a state vector continues on DensityMatrix(psi) (apply_qmaps.jl:19):
+    the result, and the energy gradient, equal the whole circuit on |psi><psi|."""
+    rng = np.random.default_rng(60 + n)
+    c, H, _ = config3(seed=n, n=n, depth=2, p=0.05, gamma=0.03)
+    first = next(i for i, g in enumerate(c) if isinstance(g, vq.QuantumMap))
+    assert first > 0
+    psi = rand_state(n, rng)
+    rho = np.outer(psi, psi.conj())
+    out = vq.apply_(c, vq.StateVector(psi))
+    assert out.is_dm
+    prog = Program(c, n=n, dm=True)
+    ref = oracle.apply_circuit(vec(rho), n, 1, prog.op_array(), len(prog.ops), prog.pool())
+    assert np.max(np.abs(vec(out.storage()) - ref)) < 1e-12
+    # the device DensityMatrix(psi) itself: kron(conj(psi), psi) (densitymatrix.jl:27)
+    assert np.max(np.abs(vec(vq.DensityMatrix(vq.StateVector(psi)).storage()) - np.kron(psi.conj(), psi))) < 1e-15
+    E, g = vq.energy_and_gradient(H, c, vq.StateVector(psi))
+    E2, g2 = vq.energy_and_gradient(H, c, vq.DensityMatrix(rho))
+    assert abs(E - E2) < 1e-12 and np.max(np.abs(g - g2)) < 1e-12

@@ -86,3 +86,20 @@ def test_fidelity_and_distance():
     assert abs(R.fidelity(sy, d) - np.vdot(y, rho @ y).real) < 1e-13
     rho2 = rand_density(n, rng)
     assert abs(R.distance(d, vq.DensityMatrix(vec(rho2))) - np.linalg.norm(rho - rho2)) < 1e-12
+
+
+@pytest.mark.parametrize("n", [1, 4, 7, 10])
+def test_fidelity_of_two_density_matrices(n):
+    """fidelity(x::DensityMatrix, y::DensityMatrix) = real(tr(sqrt(x) sqrt(y)))
+    (densitymatrix.jl:65) on the device against the oracle (pinned to scipy's
+    sqrtm and closed forms); full-rank inputs to 1e-10, pure states (square roots
+    of round-off eigenvalues, ~1e-8 in any implementation of the formula) to 1e-7."""
+    import oracle
+    rng = np.random.default_rng(650 + n)
+    x, y = rand_density(n, rng), rand_density(n, rng)
+    f = R.fidelity(vq.DensityMatrix(x), vq.DensityMatrix(y))
+    assert abs(f - oracle.fidelity_dm(x, y)) < 1e-10
+    assert abs(R.fidelity(vq.DensityMatrix(x), vq.DensityMatrix(x)) - 1.0) < 1e-10
+    psi, phi = rand_state(n, rng), rand_state(n, rng)
+    fp = R.fidelity(vq.DensityMatrix(vq.StateVector(psi)), vq.DensityMatrix(vq.StateVector(phi)))
+    assert abs(fp - abs(psi.conj() @ phi) ** 2) < 1e-7

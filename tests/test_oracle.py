@@ -469,3 +469,24 @@ def test_qterms_brute_force(kinds):
     per, tot = oracle.dm_expec_qterms(rho, n, qt)
     assert np.allclose(per, [np.trace(_qterm_full(n, c, s, m) @ rho) for c, s, m in qt], atol=1e-12)
     assert abs(tot - sum(per)) < 1e-12
+
+
+def test_fidelity_dm_pins():
+    """fidelity(x::DensityMatrix, y::DensityMatrix) (densitymatrix.jl:65) in the oracle:
+    against scipy's Schur-based sqrtm, and the closed forms tr(sqrt(rho) sqrt(rho)) =
+    tr(rho) = 1, pure states |<psi|phi>|^2 (sqrt(rho) = rho), diagonal matrices
+    sum sqrt(p_i q_i)."""
+    from scipy.linalg import sqrtm
+    rng = np.random.default_rng(65)
+    for n in (1, 3, 5):
+        x, y = rand_density(n, rng), rand_density(n, rng)
+        f = oracle.fidelity_dm(x, y)
+        assert abs(f - np.real(np.trace(sqrtm(x) @ sqrtm(y)))) < 1e-10
+        assert abs(oracle.fidelity_dm(x, x) - 1.0) < 1e-12
+        psi, phi = rand_state(n, rng), rand_state(n, rng)
+        fp = oracle.fidelity_dm(np.outer(psi, psi.conj()), np.outer(phi, phi.conj()))
+        # rank-deficient: round-off eigenvalues of order eps enter as sqrt(eps) ~ 1e-8,
+        # in the reference's formula as here
+        assert abs(fp - abs(psi.conj() @ phi) ** 2) < 1e-7
+        p, q = rng.dirichlet(np.ones(1 << n)), rng.dirichlet(np.ones(1 << n))
+        assert abs(oracle.fidelity_dm(np.diag(p), np.diag(q)) - np.sum(np.sqrt(p * q))) < 1e-12

@@ -299,6 +299,11 @@ struct vqc_batch {
     int64_t nparts = 0;
     unsigned* d_counters = nullptr;
     uint64_t arena_bytes = 0;
+    // plan shape: tiled and LDS-resident instances, phases, and the HBM bytes the
+    // phases move per launch (each phase reads and writes its states once)
+    int ntiled = 0, nsmall = 0;
+    int64_t nphases = 0;
+    uint64_t hbm_bytes = 0;
     // host pinned staging
     double* h_pool = nullptr;
     double* h_rows = nullptr;
@@ -470,6 +475,7 @@ int vqc_batch_build(int ninst, const vqc_instance_t* inst, const vqc_op_t* ops, 
                 bi.a_off = bi.b_off = -1;
                 b->lds = std::max(b->lds, need);
                 chains[i] = {-1, 0, true, double(T) * double(bi.mop_end - bi.mop_begin)};
+                b->nsmall += 1;
             }
         }
         if (!small) {
@@ -493,8 +499,14 @@ int vqc_batch_build(int ninst, const vqc_instance_t* inst, const vqc_op_t* ops, 
             arena += int64_t(2) << n;
             bi.mop_begin = bi.mop_end = bi.ham_begin = bi.rev_begin = 0;
             double cost = 0;
+            b->ntiled += 1;
+            b->nphases += int64_t(pb.phases.size() - ph0);
             for (size_t p = ph0; p < pb.phases.size(); ++p) {
                 const PhaseDesc& ph = pb.phases[p];
+                const uint64_t sb = uint64_t(16) << n;  // one state, read or written once
+                if (ph.kind == PH_FWD) b->hbm_bytes += (ph.flags & PF_INIT_ZERO) ? sb : 2 * sb;
+                else if (ph.kind == PH_HAM) b->hbm_bytes += (ph.ham_accum ? 3 : 2) * sb;
+                else b->hbm_bytes += 4 * sb;
                 const double nb = ph.kind == PH_FWD ? 2 : 4;
                 cost += double(int64_t(1) << n) * (nb + 0.25 * (ph.op_end - ph.op_begin));
                 b->lds = std::max(b->lds, ((ph.kind == PH_FWD ? 1 : 2) * (size_t(1) << ph.t) + kRedDoubles2 + size_t(std::max(ph.nacc, 1))) * 16);
@@ -811,6 +823,18 @@ extern "C" int vqc_batch_run(vqc_batch* b, const double* pool, const double* hpo
 extern "C" int vqc_batch_arena_bytes(const vqc_batch* b, uint64_t* bytes) {
     if (!b || !bytes) return vqc_fail(VQC_EINVAL, "null argument");
     *bytes = b->arena_bytes;
+    return VQC_OK;
+}
+
+extern "C" int vqc_batch_info(const vqc_batch* b, vqc_batch_info_t* out) {
+    if (!b || !out) return vqc_fail(VQC_EINVAL, "null argument");
+    out->tiled_instances = b->ntiled;
+    out->small_instances = b->nsmall;
+    out->phases = b->nphases;
+    out->items = b->nitems;
+    out->grid = int32_t(b->grid);
+    out->hbm_bytes = b->hbm_bytes;
+    out->arena_bytes = b->arena_bytes;
     return VQC_OK;
 }
 

@@ -152,6 +152,14 @@ struct vqc_server {
     std::thread worker;
     // statistics
     int64_t completed = 0, batches = 0, batched_requests = 0;
+    int64_t batch_hist[8] = {};  // batches of 1, 2, 3-4, 5-8, 9-16, 17-32, 33-64, 65+ requests
+    void count_batch(int ni) {
+        int b = 0;
+        while (b < 7 && (1 << b) < ni) ++b;
+        ++batch_hist[b];
+        ++batches;
+        batched_requests += ni;
+    }
 
     double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
     bool is_solo(const Request* r) const { return cfg.solo_qubits > 0 && r->inst.nqubits >= cfg.solo_qubits; }
@@ -317,8 +325,7 @@ bool vqc_server::launch(Lane& L) {
             r->status = 1;
             r->t_launch = t;
         }
-        ++batches;
-        batched_requests += ni;
+        count_batch(ni);
     }
     L.batch = b;
     L.reqs = std::move(sel);
@@ -375,8 +382,7 @@ bool vqc_server::launch_solo(Lane& L, Request* r) {
         std::lock_guard<std::mutex> g(mu);
         r->status = 1;
         r->t_launch = now();
-        ++batches;
-        batched_requests += 1;
+        count_batch(1);
     }
     L.solo = so;
     L.solo_block = block;
@@ -623,6 +629,7 @@ extern "C" int vqc_server_stats(vqc_server* s, vqc_server_stats_t* out) {
     out->budget_bytes = s->mem.budget;
     out->reserved_bytes = s->mem.reserved;
     out->peak_live_bytes = s->mem.peak_live;
+    for (int i = 0; i < 8; ++i) out->batch_hist[i] = s->batch_hist[i];
     return VQC_OK;
 }
 

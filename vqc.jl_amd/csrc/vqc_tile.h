@@ -19,6 +19,9 @@
 //   MK_HAM     B += (M + M^dagger) A, acc += sum conj(A) (M A)   (adjoint seed, energy)
 //   MK_CHAIN   up to 4 one-bit MK_APPLY2 steps on the same bit, done in registers
 //              (one LDS round trip for e.g. the Rz, Ry, Rx of one qubit)
+//   MK_DENSE   A <- M A for a dense 2^K x 2^K matrix on K = 3..5 bits, the product
+//              of a run of gates fused by the planner, as FP64 MFMA products
+//              (v_mfma_f64_16x16x4f64) over the tile's groups
 //   MK_BLOCK   up to 4 steps on one pair of bits, done in registers: each step a
 //              4x4 on both bits or a 2x2 on either bit (or no matrix), on A alone
 //              (forward phases) or on A and B with an optional generator per step
@@ -33,7 +36,7 @@
 
 #include "vqc_internal.h"
 
-enum { MK_APPLY = 0, MK_APPLY2 = 1, MK_APPLY2X = 2, MK_GRAD = 3, MK_HAM = 4, MK_CHAIN = 5, MK_BLOCK = 6 };
+enum { MK_APPLY = 0, MK_APPLY2 = 1, MK_APPLY2X = 2, MK_GRAD = 3, MK_HAM = 4, MK_CHAIN = 5, MK_BLOCK = 6, MK_DENSE = 7 };
 enum { BS_BOTH = 0, BS_LO = 1, BS_HI = 2 };  // MK_BLOCK step targets
 // matrix structure; codes 2 and 3 name CNOT patterns for 4x4 matrices and
 // real / "X-like" (real diagonal, imaginary off-diagonal: Rx and its generator)
@@ -49,7 +52,9 @@ enum {
     KS_REVX = (1 << MK_APPLY) | (1 << MK_APPLY2) | (1 << MK_APPLY2X) | (1 << MK_GRAD) | (1 << MK_CHAIN) | (1 << MK_BLOCK),
     // forward, with the straight-line block for the ansatz shape (chosen per
     // phase by the host, so other forward phases keep the plain forward kernel)
-    KS_FWDH = KS_FWD | (1 << 7),
+    KS_FWDH = KS_FWD | (1 << 9),
+    // forward, with fused dense blocks of 3 to 5 bits on the matrix cores
+    KS_FWDD = KS_FWD | (1 << MK_DENSE),
 };
 
 constexpr int kTileThreads = 512;
@@ -771,6 +776,70 @@ __device__ __forceinline__ void mop_ham(const double2* __restrict__ sa, double2*
     wave_partial(red, buf, 0, er, ei);
 }
 
+// MK_DENSE: A <- M A with the 2^K x 2^K complex M (column-major over lpos[0..K),
+// ascending) as a real GEMM on the matrix cores. A group (2^K amplitudes, the
+// other tile bits fixed) is one column of X; in real form M is 2D x 2D with
+// entries (row 2i + p, column 2j + q) = Re M_ij, -Im M_ij, Im M_ij, Re M_ij for
+// (p, q) = (0,0), (0,1), (1,0), (1,1), which acts on the interleaved (re, im)
+// amplitudes as stored. v_mfma_f64_16x16x4f64 takes a 16 x 4 slice of that
+// matrix (lane l: row l % 16, k = l / 16), a 4 x 16 slice of X (lane l: k = l / 16,
+// column l % 16) and accumulates a 16 x 16 block (lane l, register i: row
+// 4 (l / 16) + i, column l % 16). The k order is chosen so that one 16-byte LDS
+// read serves two steps: steps 2s and 2s + 1 take, in lane group g = l / 16,
+// the real and imaginary parts of amplitude j = 4s + g. A wave owns whole
+// column tiles (16 groups), computes every row tile of them, and only then
+// writes them back: the reads and writes of one column stay in one wave, so the
+// in-place update needs no extra barrier. Output registers (0, 1) and (2, 3) of
+// row tile rt are amplitudes 8 rt + 2 g and 8 rt + 2 g + 1 of the group.
+typedef double vqc_v4d __attribute__((ext_vector_type(4)));
+template <int K>
+__device__ __forceinline__ void mop_dense(double2* __restrict__ s, int t, const MicroOp& op, const double2* __restrict__ pool) {
+    constexpr int D = 1 << K;          // amplitudes per group
+    constexpr int RT = (2 * D) / 16;   // 16-row tiles of the real matrix
+    constexpr int NS = D / 4;          // pairs of k steps
+    const double2* __restrict__ M = pool + op.mat;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int col = lane & 15, g = lane >> 4;
+    const uint32_t ncols = 1u << (t - K);  // groups
+    const uint32_t nct = ncols >> 4;       // column tiles (the planner keeps t - K >= 7)
+    // offsets of amplitude 4s + g of a group, and of the two output amplitudes per row tile
+    uint32_t offin[NS], offout[RT][2];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) offin[q] = off32<K>(4 * q + g, op.lpos);
+#pragma unroll
+    for (int rt = 0; rt < RT; ++rt) {
+        offout[rt][0] = off32<K>(8 * rt + 2 * g, op.lpos);
+        offout[rt][1] = off32<K>(8 * rt + 2 * g + 1, op.lpos);
+    }
+    // A-operand entries of this lane: real row r = 16 rt + (lane % 16) is complex row
+    // r / 2, part p = r % 2; step 2q + h reads complex column 4q + g, part h
+    const int prow = col & 1;
+    for (uint32_t ct = wave; ct < nct; ct += kTileWaves) {
+        const uint32_t base = ins32<K>(ct * 16 + col, op.lpos);
+        vqc_v4d acc[RT];
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) acc[rt] = vqc_v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            const double2 x = s[base + offin[q]];
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                const double2 m = M[(8 * rt + (col >> 1)) + (4 * q + g) * D];
+                // (p, h) = (0, 0): Re m, (0, 1): -Im m, (1, 0): Im m, (1, 1): Re m
+                const double a0 = prow ? m.y : m.x;
+                const double a1 = prow ? m.x : -m.y;
+                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, x.x, acc[rt], 0, 0, 0);
+                acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, x.y, acc[rt], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int rt = 0; rt < RT; ++rt) {
+            s[base + offout[rt][0]] = make_double2(acc[rt][0], acc[rt][1]);
+            s[base + offout[rt][1]] = make_double2(acc[rt][2], acc[rt][3]);
+        }
+    }
+}
+
 // One micro-op followed by a barrier. `red` holds kRedDoubles2 entries and
 // `gacc` the accumulator rows addressed by op.acc[]. MAXKB is the widest
 // micro-op the calling kernel is built for: kernels for phases of at most 3
@@ -831,7 +900,7 @@ __device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const M
         case MK_BLOCK:
             if constexpr ((KS & (1 << MK_BLOCK)) != 0) {
                 // forward micro-ops act on A alone; reverse ones on A and B
-                constexpr bool two = KS != KS_FWD && KS != KS_FWDH;
+                constexpr bool two = KS != KS_FWD && KS != KS_FWDH && KS != KS_FWDD;
                 if constexpr (KS == KS_REV) {  // state-vector reverse: the ansatz shape first
                     const int shape = hea_rev_shape(op);
                     if (shape == 1) mop_hea_rev<true>(sa, sb, t, op, pool, red, buf);
@@ -851,6 +920,13 @@ __device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const M
                     mop_block(sa, sb, t, op, pool, red, buf, two);
                 }
                 nred = two ? op.ng : 0;
+            }
+            break;
+        case MK_DENSE:
+            if constexpr ((KS & (1 << MK_DENSE)) != 0) {
+                if (kb == 3) mop_dense<3>(sa, t, op, pool);
+                else if (kb == 4) mop_dense<4>(sa, t, op, pool);
+                else mop_dense<5>(sa, t, op, pool);
             }
             break;
         case MK_HAM:

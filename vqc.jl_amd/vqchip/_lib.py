@@ -87,6 +87,12 @@ class Instance(ctypes.Structure):
     ]
 
 
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("tiled_instances", ctypes.c_int32), ("small_instances", ctypes.c_int32), ("phases", ctypes.c_int64),
+                ("items", ctypes.c_int64), ("grid", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("hbm_bytes", ctypes.c_uint64), ("arena_bytes", ctypes.c_uint64)]
+
+
 class ServerConfig(ctypes.Structure):
     _fields_ = [
         ("hbm_budget", ctypes.c_uint64),
@@ -101,7 +107,8 @@ class ServerConfig(ctypes.Structure):
 
 class ServerStats(ctypes.Structure):
     _fields_ = [(k, ctypes.c_int64) for k in ("completed", "batches", "batched_requests", "pending", "running")] + [
-        (k, ctypes.c_uint64) for k in ("budget_bytes", "reserved_bytes", "peak_live_bytes")]
+        (k, ctypes.c_uint64) for k in ("budget_bytes", "reserved_bytes", "peak_live_bytes")] + [
+        ("batch_hist", ctypes.c_int64 * 8)]
 
 
 class Tuning(ctypes.Structure):
@@ -168,6 +175,7 @@ SIGNATURES = {
     "vqc_sv_select": [_P, _I, _I, _P, _D],
     "vqc_partial_trace": [_P, _I, _I32, _P],
     "vqc_dm_sandwich": [_P, _P, _D],
+    "vqc_dm_fidelity": [_P, _P, _D],
     "vqc_apply_circuit": [_P, _I, ctypes.POINTER(Op), _D],
     "vqc_plan_info": [ctypes.POINTER(PlanInfo)],
     "vqc_sv_energy_grad": [_P, _I, ctypes.POINTER(Op), _D, _I, ctypes.POINTER(Term), _D, _I, _D, _D, _P, _I],
@@ -179,6 +187,7 @@ SIGNATURES = {
     "vqc_batch_launch": [_P, _D, _D],
     "vqc_batch_wait": [_P, _D, _D],
     "vqc_batch_arena_bytes": [_P, ctypes.POINTER(ctypes.c_uint64)],
+    "vqc_batch_info": [_P, ctypes.POINTER(BatchInfo)],
     "vqc_batch_destroy": [_P],
     "vqc_server_default_config": [ctypes.POINTER(ServerConfig)],
     "vqc_server_create": [ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)],

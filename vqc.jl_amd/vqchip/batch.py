@@ -128,6 +128,12 @@ class Batch:
             return self._part.arena_bytes()
         return max(sum(instance_bytes(n) for _, _, n in w) for w in self._waves)
 
+    def info(self):
+        """Plan shape of a one-part batch (vqc_batch_info)."""
+        if self._part is None:
+            raise RuntimeError("a batch in waves has one plan per wave")
+        return self._part.info()
+
     def nwaves(self):
         return 1 if self._part is not None else len(self._waves)
 
@@ -202,6 +208,11 @@ class _Part:
         v = ctypes.c_uint64()
         _lib.call("vqc_batch_arena_bytes", self._h, ctypes.byref(v))
         return v.value
+
+    def info(self):
+        v = _lib.BatchInfo()
+        _lib.call("vqc_batch_info", self._h, ctypes.byref(v))
+        return {k: getattr(v, k) for k, _ in v._fields_ if k != "reserved"}
 
     def close(self):
         h = getattr(self, "_h", None)

@@ -35,9 +35,9 @@ def tr(rho):
 
 def fidelity(x, y):
     """|<x|y>|^2 for state vectors (statevector.jl:63); <y|rho|y> for a density
-    matrix and a state vector (densitymatrix.jl:70-71). The density-matrix pair
-    tr(sqrt(rho) sqrt(sigma)) (densitymatrix.jl:69) needs matrix square roots
-    and is not provided on the device."""
+    matrix and a state vector (densitymatrix.jl:70-71); real(tr(sqrt(x) sqrt(y)))
+    for two density matrices (densitymatrix.jl:65), from Hermitian
+    eigendecompositions on the device (rocSOLVER)."""
     if isinstance(x, StateVector) and isinstance(y, StateVector):
         return abs(dot(x, y)) ** 2
     if isinstance(x, StateVector):
@@ -46,7 +46,9 @@ def fidelity(x, y):
         out = np.zeros(2)
         _lib.call("vqc_dm_sandwich", x.handle, y.handle, _lib.dptr(out))
         return out[0]
-    raise NotImplementedError("fidelity of two density matrices needs matrix square roots")
+    out = ctypes.c_double()
+    _lib.call("vqc_dm_fidelity", x.handle, y.handle, ctypes.byref(out))
+    return out.value
 
 
 def partial_tr(state, sites):

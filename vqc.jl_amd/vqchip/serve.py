@@ -95,7 +95,10 @@ class Server:
     def stats(self):
         s = _lib.ServerStats()
         _lib.call("vqc_server_stats", self._h, ctypes.byref(s))
-        return {k: getattr(s, k) for k, _ in s._fields_}
+        d = {k: getattr(s, k) for k, _ in s._fields_ if k != "batch_hist"}
+        # batches by size: the upper end of each bin (1, 2, 4, ..., 64, and 65 for larger)
+        d["batch_size_hist"] = {(1 << i) if i < 7 else 65: int(s.batch_hist[i]) for i in range(8) if s.batch_hist[i]}
+        return d
 
     def close(self):
         if self._h:
@@ -151,10 +154,14 @@ def run_plain_fifo(requests, arrivals, fused=False):
     return lat, busy, time.perf_counter() - t0
 
 
-def run_server_open_loop(server, requests, arrivals):
+def run_server_open_loop(server, requests, arrivals, pack=None):
     """Submit each request at its arrival time from a feeder thread; wait for all.
+    With `pack` (the (circuit, H, n) of every request) each request is packed at
+    its arrival, so its latency counts the host packing: arrival -> packed and
+    submitted (host clock) plus submit -> done (the server's clock).
     Returns (latencies, span, results)."""
     tickets = [None] * len(requests)
+    lead = np.zeros(len(requests))  # arrival -> submit
     ready = threading.Semaphore(0)
     err = []
     t0 = time.perf_counter()
@@ -165,6 +172,9 @@ def run_server_open_loop(server, requests, arrivals):
                 wait = t0 + arrivals[i] - time.perf_counter()
                 if wait > 0:
                     time.sleep(wait)
+                if pack is not None:
+                    req = Request(*pack[i])
+                lead[i] = time.perf_counter() - (t0 + arrivals[i])
                 tickets[i] = server.submit(req)
                 ready.release()
         except Exception as ex:  # surfaced by the caller below
@@ -181,9 +191,18 @@ def run_server_open_loop(server, requests, arrivals):
         if err:
             break
         E, g, (ts, tl, td) = server.wait(tickets[i])
-        lat[i] = td - ts
+        lat[i] = max(lead[i], 0.0) + (td - ts)
         results.append((E, g))
     th.join()
     if err:
         raise err[0]
     return lat, time.perf_counter() - t0, results
+
+
+def run_server_closed_loop(server, requests):
+    """Every (packed) request queued at once; the span from the first submit to
+    the last result, i.e. the server's saturation throughput. Returns (span, results)."""
+    t0 = time.perf_counter()
+    tickets = [server.submit(r) for r in requests]
+    res = [server.wait(t) for t in tickets]
+    return time.perf_counter() - t0, [(E, g) for E, g, _ in res]
