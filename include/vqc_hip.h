@@ -200,6 +200,17 @@ int vqc_sv_expec_dense(const vqc_state* ket, const vqc_state* bra, int k, const 
  * expecs.jl:40 / apply_qterms.jl:20-25 for Pauli strings. */
 int vqc_sv_expec_pauli(const vqc_state* ket, const vqc_state* bra, int nterms,
                        const vqc_pauli_t* terms, double* out, double* sum2);
+/* vqc_sv_expec_pauli without any allocation (no counterpart in the reference): for
+ * callers that cannot allocate device or pinned memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The same kernels and
+ * summation order, bit-identical results. The partial sums live in a 4 MiB device
+ * array of the library that each GPU holds until the process ends; host work arrays
+ * are static. At most 256 terms (VQC_EINVAL beyond). Every call on one GPU shares
+ * that array, so calls must not overlap on a GPU: the library's calls are ordered
+ * on its one stream and serialised by its lock, which keeps them apart. The first
+ * call of any entry point creates the library's stream. */
+int vqc_sv_expec_pauli_noalloc(const vqc_state* ket, const vqc_state* bra, int nterms,
+                               const vqc_pauli_t* terms, double* out, double* sum2);
 /* out = sum_t c_t P_t in  ((m::QubitsOperator)(v), apply_qterms.jl:13-28). */
 int vqc_sv_apply_pauli_sum(const vqc_state* in, vqc_state* out, int nterms, const vqc_pauli_t* terms);
 /* out = sum_keys M_key in; energy2 = <in|out> (fused H*psi and expectation). */

@@ -373,6 +373,131 @@ extern "C" int vqc_sv_expec_pauli(const vqc_state* ket, const vqc_state* bra, in
     return VQC_OK;
 }
 
+// ---------------------------------------------------------------------------
+// vqc_sv_expec_pauli_noalloc: the same sweeps and reductions with the partial
+// sums in a file-scope device array instead of the library's growable scratch,
+// and the results copied straight to the caller's arrays (no pinned staging),
+// so the call allocates nothing.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocDoubles2 = size_t(1) << 18;  // 4 MiB of partials and results per device
+static __device__ __align__(256) double2 scratch[kNoAllocDoubles2];
+constexpr int kMaxDevices = 64;
+static double2* g_noalloc_ptr[kMaxDevices] = {};
+
+// The device address of `scratch` on the current device (looked up once per
+// device, with that device current; the array's host-side name is not it).
+static int noalloc_scratch(double2** out) {
+    int dev = 0;
+    VQC_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return vqc_fail(VQC_EUNSUPPORTED, "device %d beyond the no-allocation table", dev);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        VQC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_noalloc_ptr[dev] = (double2*)p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return VQC_OK;
+}
+
+constexpr int kNoAllocMaxTerms = 256;
+
+// pauli_chunks without heap use: a stable insertion sort of the terms by x mask
+// into caller arrays of kNoAllocMaxTerms entries
+static int pauli_chunks_fixed(int nterms, const vqc_pauli_t* terms, PauliChunk* chunks, int* order, int* slot_of) {
+    for (int i = 0; i < nterms; ++i) {
+        int j = i;
+        while (j > 0 && terms[order[j - 1]].x > terms[i].x) {
+            order[j] = order[j - 1];
+            --j;
+        }
+        order[j] = i;
+    }
+    int nch = 0, slot = 0;
+    for (int i = 0; i < nterms;) {
+        PauliChunk& pc = chunks[nch++];
+        pc.x = terms[order[i]].x;
+        pc.hbit = pc.x ? 63 - __builtin_clzll(pc.x) : -1;
+        pc.nt = 0;
+        while (i < nterms && terms[order[i]].x == pc.x && pc.nt < kPauliChunk) {
+            const vqc_pauli_t& t = terms[order[i]];
+            const int ny = __builtin_popcountll(t.x & t.z) & 3;
+            double re = t.re, im = t.im;
+            for (int r = 0; r < ny; ++r) {  // multiply by i
+                const double tmp = re;
+                re = -im;
+                im = tmp;
+            }
+            pc.z[pc.nt] = t.z;
+            pc.c[pc.nt] = make_double2(re, im);
+            slot_of[order[i]] = slot++;
+            ++pc.nt;
+            ++i;
+        }
+        for (int t = pc.nt; t < kPauliChunk; ++t) {
+            pc.z[t] = 0;
+            pc.c[t] = make_double2(0, 0);
+        }
+    }
+    return nch;
+}
+
+// Host-side work arrays of the call, used under the library lock (no heap use per call).
+static PauliChunk g_na_chunks[kNoAllocMaxTerms];
+static int g_na_order[kNoAllocMaxTerms], g_na_slot[kNoAllocMaxTerms];
+static double g_na_res[2 * kNoAllocMaxTerms];
+
+extern "C" int vqc_sv_expec_pauli_noalloc(const vqc_state* ket, const vqc_state* bra, int nterms, const vqc_pauli_t* terms,
+                                          double* out, double* sum2) {
+    VQC_API_LOCK;
+    VQC_TRY(vqc_check_qubits(ket, "vqc_sv_expec_pauli_noalloc"));
+    if (bra && bra->nbits != ket->nbits) return vqc_fail(VQC_EINVAL, "bra and ket differ in size");
+    VQC_TRY(check_paulis(ket->nbits, nterms, terms));
+    if (nterms > kNoAllocMaxTerms)
+        return vqc_fail(VQC_EINVAL, "vqc_sv_expec_pauli_noalloc: %d terms, the maximum is %d", nterms, kNoAllocMaxTerms);
+    const unsigned nparts = parts_grid(ket->len() >> 1);
+    const size_t need = size_t(nterms) * (nparts + 1);
+    if (need > kNoAllocDoubles2)  // cannot happen for kNoAllocMaxTerms * 1025 <= 2^18; kept as the stated limit
+        return vqc_fail(VQC_EINVAL, "vqc_sv_expec_pauli_noalloc: %zu partial sums, the maximum is %zu", need, kNoAllocDoubles2);
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    double2* parts;
+    VQC_TRY(noalloc_scratch(&parts));
+    const int nch = pauli_chunks_fixed(nterms, terms, g_na_chunks, g_na_order, g_na_slot);
+    double2* res = parts + size_t(nterms) * nparts;
+    int first = 0;
+    for (int k = 0; k < nch; ++k) {
+        const PauliChunk& pc = g_na_chunks[k];
+        if (bra)
+            hipLaunchKernelGGL(k_pauli_expec<true>, dim3(nparts), dim3(kEB), 0, c->stream, ket->d, bra->d, ket->nbits, pc, parts,
+                               int(nparts), first);
+        else
+            hipLaunchKernelGGL(k_pauli_expec<false>, dim3(nparts), dim3(kEB), 0, c->stream, ket->d, ket->d, ket->nbits, pc, parts,
+                               int(nparts), first);
+        first += pc.nt;
+    }
+    if (nterms) {
+        vqc_reduce_partials(parts, nterms, nparts, res, c->stream);
+        VQC_HIP(hipGetLastError());
+        VQC_HIP(hipMemcpyAsync(g_na_res, res, size_t(nterms) * 16, hipMemcpyDeviceToHost, c->stream));
+        VQC_HIP(hipStreamSynchronize(c->stream));
+    }
+    double sr = 0, si = 0;
+    for (int i = 0; i < nterms; ++i) {  // per term in the caller's order, summed in that order (as vqc_sv_expec_pauli)
+        const int sl = g_na_slot[i];
+        if (out) {
+            out[2 * i] = g_na_res[2 * sl];
+            out[2 * i + 1] = g_na_res[2 * sl + 1];
+        }
+        sr += g_na_res[2 * sl];
+        si += g_na_res[2 * sl + 1];
+    }
+    if (sum2) {
+        sum2[0] = sr;
+        sum2[1] = si;
+    }
+    return VQC_OK;
+}
+
 __global__ void __launch_bounds__(256) k_zero(double2* v, uint64_t len) {
     for (uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < len; i += uint64_t(gridDim.x) * blockDim.x)
         v[i] = make_double2(0.0, 0.0);

@@ -161,6 +161,7 @@ SIGNATURES = {
     "vqc_sv_accum_dense": [_P, _P, _I, _I32, _D],
     "vqc_sv_expec_dense": [_P, _P, _I, _I32, _D, _D],
     "vqc_sv_expec_pauli": [_P, _P, _I, ctypes.POINTER(Pauli), _D, _D],
+    "vqc_sv_expec_pauli_noalloc": [_P, _P, _I, ctypes.POINTER(Pauli), _D, _D],
     "vqc_sv_apply_pauli_sum": [_P, _P, _I, ctypes.POINTER(Pauli)],
     "vqc_sv_apply_terms": [_P, _P, _I, ctypes.POINTER(Term), _D, _D],
     "vqc_sv_apply_qterms": [_P, _P, _I, ctypes.POINTER(QTerm), _I32, _D, _I],
