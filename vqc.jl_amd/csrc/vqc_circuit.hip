@@ -321,6 +321,63 @@ int64_t XPool::product(const int64_t* srcs, int n, int D) {
     return out;
 }
 
+// out <- the product of b's factors (application order), each embedded in the
+// 2^K block space
+static void make_block_product(const double* h, const XPoolBlock& b, double* out) {
+    const int D = 1 << b.K;
+    std::vector<double> P(size_t(2) * D * D, 0.0), col(size_t(2) * D);
+    for (int i = 0; i < D; ++i) P[2 * (i + i * D)] = 1.0;
+    for (size_t f = 0; f < b.mats.size(); ++f) {
+        const int k = b.nb[f], d = 1 << k;
+        const double* U = h + 2 * b.mats[f];
+        const int* mp = b.map.data() + 5 * f;
+        int fmask = 0;
+        for (int j = 0; j < k; ++j) fmask |= 1 << mp[j];
+        for (int c = 0; c < D; ++c) {  // column c of P <- U_f (column c of P)
+            double* pc = P.data() + 2 * size_t(c) * D;
+            std::copy(pc, pc + 2 * D, col.begin());
+            for (int r = 0; r < D; ++r) {
+                if (r & fmask) continue;  // r: the group base (factor bits zero)
+                double in[64], o[64];
+                for (int a = 0; a < d; ++a) {
+                    int idx = r;
+                    for (int j = 0; j < k; ++j)
+                        if (a >> j & 1) idx |= 1 << mp[j];
+                    in[2 * a] = col[2 * idx];
+                    in[2 * a + 1] = col[2 * idx + 1];
+                }
+                for (int a = 0; a < d; ++a) {
+                    double re = 0, im = 0;
+                    for (int e = 0; e < d; ++e) {
+                        const double ur = U[2 * (a + e * d)], ui = U[2 * (a + e * d) + 1];
+                        re += ur * in[2 * e] - ui * in[2 * e + 1];
+                        im += ur * in[2 * e + 1] + ui * in[2 * e];
+                    }
+                    o[2 * a] = re;
+                    o[2 * a + 1] = im;
+                }
+                for (int a = 0; a < d; ++a) {
+                    int idx = r;
+                    for (int j = 0; j < k; ++j)
+                        if (a >> j & 1) idx |= 1 << mp[j];
+                    pc[2 * idx] = o[2 * a];
+                    pc[2 * idx + 1] = o[2 * a + 1];
+                }
+            }
+        }
+    }
+    std::copy(P.begin(), P.end(), out);
+}
+
+int64_t XPool::dense_block(XPoolBlock b) {
+    const int64_t D = int64_t(1) << b.K;
+    b.dst = size();
+    h.resize(h.size() + size_t(2) * D * D);
+    make_block_product(h.data(), b, h.data() + 2 * b.dst);
+    blocks.push_back(std::move(b));
+    return blocks.back().dst;
+}
+
 int XPool::classify(int64_t off, int D) {
     const int ms = host_mstruct(h.data() + 2 * off, D);
     claims.push_back({off, D, ms, 0, 0});
@@ -350,6 +407,7 @@ bool XPool::reload(const double* pool, int64_t plen, const double* hpool, int64_
         if (e.v == 8) make_product(h.data(), e.srcs, e.nsrc, e.D, h.data() + 2 * e.dst);
         else make_variant(h.data() + 2 * e.src, e.D, e.v, h.data() + 2 * e.dst);
     }
+    for (const XPoolBlock& b : blocks) make_block_product(h.data(), b, h.data() + 2 * b.dst);
     for (const XPoolClaim& c : claims) {
         const int now = c.test == 1 ? int(host_parity4(h.data() + 2 * c.off)) : host_mstruct(h.data() + 2 * c.off, c.D);
         if (now != c.ms) return false;
@@ -513,6 +571,156 @@ void form_blocks(std::vector<MicroOp>& out, size_t b, int phase_kind) {
 }
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// Dense-block fusion of forward phases (MK_DENSE). The phase's MK_APPLY
+// micro-ops are grouped greedily, in order, into blocks of at most kmax bits:
+// an op joins the blocks that last touched its bits when their union fits and
+// every older block among them is still the last one on all of its bits (so it
+// commutes with everything it is moved past); blocks keep the position of the
+// newest. A block of 3 to 5 bits becomes one MK_DENSE micro-op (its product
+// matrix, run on the matrix cores) when that is cheaper by a cost per amplitude
+// of 8 * 2^k flops scaled for each engine plus a fixed cost per micro-op;
+// otherwise its ops stay as they are. kmax = 5, 4 and 3 are tried and the
+// cheapest grouping is kept. The tile must hold at least 128 groups of the
+// block (t - K >= 7: 8 column tiles of 16 groups, one per wave).
+// ---------------------------------------------------------------------------
+namespace {
+struct DenseCost {
+    double av = 2.0, cv = 64.0, am = 1.25, cm = 64.0;  // VALU / MFMA scale and per-micro-op cost
+};
+DenseCost dense_cost() {
+    DenseCost c;
+    if (const char* e = getenv("VQC_DENSE_COST")) sscanf(e, "%lf,%lf,%lf,%lf", &c.av, &c.cv, &c.am, &c.cm);
+    return c;
+}
+struct DBlock {
+    std::vector<int> bits;  // local bits
+    std::vector<int> ops;   // indices into the phase's op list, in order
+    bool fusable = true;
+};
+double op_cost_valu(const MicroOp& m, const DenseCost& c) { return c.av * 8.0 * double(1 << m.kb) + c.cv; }
+
+std::vector<DBlock> group_dense(const std::vector<MicroOp>& ph, int kmax) {
+    std::vector<DBlock> bl;
+    std::vector<int> last(64, -1);
+    for (size_t i = 0; i < ph.size(); ++i) {
+        const MicroOp& m = ph[i];
+        const bool fusable = m.kind == MK_APPLY && m.buf == 0;
+        std::vector<int> cands;
+        for (int j = 0; j < m.kb; ++j)
+            if (last[m.lpos[j]] >= 0 && std::find(cands.begin(), cands.end(), last[m.lpos[j]]) == cands.end())
+                cands.push_back(last[m.lpos[j]]);
+        std::sort(cands.begin(), cands.end());
+        bool ok = fusable && !cands.empty();
+        std::vector<int> u(m.lpos, m.lpos + m.kb);
+        if (ok) {
+            for (int cidx : cands) {
+                if (!bl[cidx].fusable) ok = false;
+                for (int b : bl[cidx].bits)
+                    if (std::find(u.begin(), u.end(), b) == u.end()) u.push_back(b);
+            }
+            ok = ok && int(u.size()) <= kmax;
+            for (size_t q = 0; ok && q + 1 < cands.size(); ++q)
+                for (int b : bl[cands[q]].bits)
+                    if (last[b] != cands[q]) ok = false;
+        }
+        if (ok) {
+            const int keep = cands.back();
+            std::vector<int> ops;
+            for (int cidx : cands) {
+                ops.insert(ops.end(), bl[cidx].ops.begin(), bl[cidx].ops.end());
+                if (cidx != keep) {
+                    // an older block was the last one on all its bits: they move with it
+                    for (int b : bl[cidx].bits) last[b] = keep;
+                    bl[cidx].ops.clear();
+                    bl[cidx].bits.clear();
+                }
+            }
+            // merged ops: the older blocks' ops commute with the newer ones'
+            // (disjoint bits), so concatenation in block order is exact
+            ops.push_back(int(i));
+            DBlock& nb = bl[keep];
+            nb.ops = ops;
+            std::sort(u.begin(), u.end());
+            nb.bits = u;
+            // only the new op's bits: a bit of the kept block that a later block
+            // touched keeps that block as its last
+            for (int j = 0; j < m.kb; ++j) last[m.lpos[j]] = keep;
+        } else {
+            DBlock b;
+            b.bits.assign(m.lpos, m.lpos + m.kb);
+            std::sort(b.bits.begin(), b.bits.end());
+            b.ops = {int(i)};
+            b.fusable = fusable;
+            bl.push_back(b);
+            for (int x : b.bits) last[x] = int(bl.size()) - 1;
+        }
+    }
+    return bl;
+}
+
+bool block_dense(const DBlock& b, const std::vector<MicroOp>& ph, int t, const DenseCost& c, double* cost) {
+    const int K = int(b.bits.size());
+    double split = 0;
+    for (int o : b.ops) split += op_cost_valu(ph[o], c);
+    const double dense = c.am * 8.0 * double(1 << K) + c.cm;
+    const bool use = b.fusable && K >= 3 && K <= 5 && t - K >= 7 && b.ops.size() >= 1 && dense < split;
+    *cost = use ? dense : split;
+    return use;
+}
+}  // namespace
+
+static std::vector<MicroOp> fuse_dense(const std::vector<MicroOp>& ph, XPool& xp, int t) {
+    if (ph.size() < 1 || getenv("VQC_NO_DENSE")) return ph;
+    const DenseCost c = dense_cost();
+    std::vector<DBlock> best;
+    double best_cost = 0;
+    for (int kmax : {5, 4, 3}) {
+        std::vector<DBlock> bl = group_dense(ph, kmax);
+        double tot = 0;
+        for (const DBlock& b : bl) {
+            if (b.ops.empty()) continue;
+            double cst;
+            block_dense(b, ph, t, c, &cst);
+            tot += cst;
+        }
+        if (best.empty() || tot < best_cost) {
+            best = bl;
+            best_cost = tot;
+        }
+    }
+    bool any = false;
+    for (const DBlock& b : best) {
+        double cst;
+        if (!b.ops.empty() && block_dense(b, ph, t, c, &cst)) any = true;
+    }
+    if (!any) return ph;  // no block pays: the phase keeps its order
+    std::vector<MicroOp> out;
+    for (const DBlock& b : best) {
+        if (b.ops.empty()) continue;
+        double cst;
+        if (!block_dense(b, ph, t, c, &cst)) {
+            for (int o : b.ops) out.push_back(ph[o]);
+            continue;
+        }
+        XPoolBlock xb;
+        xb.K = int(b.bits.size());
+        for (int o : b.ops) {
+            const MicroOp& m = ph[o];
+            xb.mats.push_back(m.mat);
+            xb.nb.push_back(m.kb);
+            for (int j = 0; j < 5; ++j)
+                xb.map.push_back(j < m.kb ? int(std::find(b.bits.begin(), b.bits.end(), m.lpos[j]) - b.bits.begin()) : 0);
+        }
+        MicroOp d = mop_init(MK_DENSE);
+        d.kb = xb.K;
+        for (int j = 0; j < 5; ++j) d.lpos[j] = j < xb.K ? b.bits[j] : 0;
+        d.mat = xp.dense_block(std::move(xb));
+        out.push_back(d);
+    }
+    return out;
+}
+
 void vqc_fuse_chains(PlanBuilder& pb, size_t phase_begin, size_t phase_end) {
     // Phases own contiguous, ascending micro-op ranges; the range's phases are
     // rewritten in place and later phases are shifted.
@@ -525,34 +733,37 @@ void vqc_fuse_chains(PlanBuilder& pb, size_t phase_begin, size_t phase_end) {
     for (size_t i = phase_begin; i < phase_end; ++i) {
         PhaseDesc& p = pb.phases[i];
         const int b = int(out.size());
+        std::vector<MicroOp> ph(pb.mops.begin() + p.op_begin, pb.mops.begin() + p.op_end);
+        if (p.kind == PH_FWD && pb.dense_fwd && !xp.deferred) ph = fuse_dense(ph, xp, p.t);
+        const int np = int(ph.size());
         if (p.kind == PH_FWD) {
             // One-bit steps on different bits commute exactly. Within each run of
             // consecutive one-bit MK_APPLY on one buffer, order them by bit (the order
             // on any one bit kept), so that each bit's steps form one run for the
             // product below: the row and column steps of a density-matrix gate
             // alternate between two bits, and so do gates on different qubits.
-            for (int o = p.op_begin; o < p.op_end;) {
+            for (int o = 0; o < np;) {
                 int e = o;
-                while (e < p.op_end && pb.mops[e].kind == MK_APPLY && pb.mops[e].kb == 1 && pb.mops[e].buf == pb.mops[o].buf) ++e;
+                while (e < np && ph[e].kind == MK_APPLY && ph[e].kb == 1 && ph[e].buf == ph[o].buf) ++e;
                 if (e - o > 1)
-                    std::stable_sort(pb.mops.begin() + o, pb.mops.begin() + e,
+                    std::stable_sort(ph.begin() + o, ph.begin() + e,
                                      [](const MicroOp& x, const MicroOp& y) { return x.lpos[0] < y.lpos[0]; });
                 o = std::max(e, o + 1);
             }
         }
-        for (int o = p.op_begin; o < p.op_end;) {
-            const MicroOp& m = pb.mops[o];
+        for (int o = 0; o < np;) {
+            const MicroOp& m = ph[o];
             int run = 1;
             if (p.kind == PH_FWD && m.kind == MK_APPLY && m.kb == 1) {
                 // U_k ... U_1 on one bit of one buffer -> one 2x2 product
-                while (o + run < p.op_end && run < 4) {
-                    const MicroOp& x = pb.mops[o + run];
+                while (o + run < np && run < 4) {
+                    const MicroOp& x = ph[o + run];
                     if (x.kind != MK_APPLY || x.kb != 1 || x.lpos[0] != m.lpos[0] || x.buf != m.buf) break;
                     ++run;
                 }
                 if (run > 1) {
                     int64_t srcs[4];
-                    for (int j = 0; j < run; ++j) srcs[j] = pb.mops[o + j].mat;
+                    for (int j = 0; j < run; ++j) srcs[j] = ph[o + j].mat;
                     MicroOp f = m;
                     f.mat = xp.product(srcs, run, 2);
                     f.mstr = xp.deferred ? MS_DENSE : xp.classify(f.mat, 2);
@@ -563,8 +774,8 @@ void vqc_fuse_chains(PlanBuilder& pb, size_t phase_begin, size_t phase_end) {
             } else if (p.kind == PH_REV && m.kind == MK_APPLY2 && m.kb == 1 && m.ng <= 1) {
                 // reverse steps on one bit stay separate (each gradient needs its
                 // intermediate pair) but share one pass over the tile
-                while (o + run < p.op_end && run < 4) {
-                    const MicroOp& x = pb.mops[o + run];
+                while (o + run < np && run < 4) {
+                    const MicroOp& x = ph[o + run];
                     if (x.kind != MK_APPLY2 || x.kb != 1 || x.lpos[0] != m.lpos[0] || x.ng > 1) break;
                     ++run;
                 }
@@ -577,7 +788,7 @@ void vqc_fuse_chains(PlanBuilder& pb, size_t phase_begin, size_t phase_end) {
                     f.ng = 0;
                     f.mstr = 0;
                     for (int j = 0; j < run; ++j) {
-                        const MicroOp& x = pb.mops[o + j];
+                        const MicroOp& x = ph[o + j];
                         f.cmat[j] = x.mat;
                         f.gen[j] = x.ng ? x.gen[0] : -1;
                         if (x.ng) f.acc[f.ng++] = x.acc[0];
@@ -897,15 +1108,18 @@ static size_t phase_lds(const PhaseDesc& p) {
     return ((p.kind == PH_FWD ? T : 2 * T) + kRedDoubles2 + size_t(std::max(p.nacc, 1))) * 16;
 }
 
+// widest micro-op a phase kernel must carry code for (MK_DENSE blocks have their own)
 int vqc_max_kb(const std::vector<MicroOp>& mops, int begin, int end) {
     int m = 1;
-    for (int o = begin; o < end; ++o) m = std::max(m, int(mops[o].kb));
+    for (int o = begin; o < end; ++o)
+        if (mops[o].kind != MK_DENSE) m = std::max(m, int(mops[o].kb));
     return m;
 }
 
 #define VQC_FOR_KERNELS(X)                                                                                     \
-    X(2, KS_FWD) X(2, KS_FWDH) X(2, KS_HAM) X(2, KS_REV) X(2, KS_REVX) X(3, KS_FWD) X(3, KS_FWDH) X(3, KS_HAM)     \
-    X(3, KS_REV) X(3, KS_REVX) X(5, KS_FWD) X(5, KS_HAM) X(5, KS_REV) X(5, KS_REVX)
+    X(2, KS_FWD) X(2, KS_FWDH) X(2, KS_FWDD) X(2, KS_HAM) X(2, KS_REV) X(2, KS_REVX) X(3, KS_FWD) X(3, KS_FWDH)     \
+    X(3, KS_FWDD) X(3, KS_HAM) X(3, KS_REV) X(3, KS_REVX) X(5, KS_FWD) X(5, KS_FWDD) X(5, KS_HAM) X(5, KS_REV)     \
+    X(5, KS_REVX)
 
 static int set_lds_attrs() {
     static bool done = false;
@@ -925,6 +1139,8 @@ static int set_lds_attrs() {
 int vqc_phase_kindset(const PlanBuilder& pb, size_t i) {
     const PhaseDesc& p = pb.phases[i];
     if (p.kind == PH_FWD) {
+        for (int o = p.op_begin; o < p.op_end; ++o)
+            if (pb.mops[o].kind == MK_DENSE) return KS_FWDD;
         // the ansatz-shape kernel when a block of that shape is present (and the
         // phase needs no 4- or 5-bit micro-op kernel)
         if (vqc_max_kb(pb.mops, p.op_begin, p.op_end) > 3) return KS_FWD;
@@ -1111,6 +1327,7 @@ extern "C" int vqc_apply_circuit(vqc_state* s, int nops, const vqc_op_t* ops, co
     if (!e) {
         const TileTuning tt = vqc_tile_tuning(s->nbits);
         std::unique_ptr<PlanEntry> ne = plan_make(std::move(key), pool, plen, nullptr, 0);
+        ne->pb.dense_fwd = true;
         VQC_TRY(vqc_plan_forward(ne->pb, s->nbits, s->nqubits, s->is_dm, nops, ops, tt.t_fwd, tt.mlow));
         vqc_fuse_chains(ne->pb, 0, ne->pb.phases.size());
         e = plan_store(std::move(ne));
@@ -1264,6 +1481,7 @@ static int plan_sv(PlanEntry& e, int nb, int nops, const vqc_op_t* ops, int nter
         e.xp.table.clear();
         e.xp.claims.clear();
     }
+    pb.dense_fwd = true;
     VQC_TRY(vqc_plan_forward(pb, nb, nb, 0, nops, ops, tt.t_fwd, tt.mlow));
     const size_t nfwd = pb.phases.size();
     VQC_TRY(vqc_plan_ham(pb, nb, nterms, terms, plen, tt.t_rev, tt.mlow));
@@ -1462,6 +1680,7 @@ static int dm_adjoint(const vqc_state* rho0, int nops, const vqc_op_t* ops, cons
             const TileTuning tt = vqc_tile_tuning(nb);
             std::unique_ptr<PlanEntry> ne = plan_make(std::move(key), pool, plen, hpool, hlen);
             PlanBuilder& np = ne->pb;
+            np.dense_fwd = true;
             VQC_TRY(vqc_plan_forward(np, nb, n, 1, nops, ops, tt.t_fwd, tt.mlow));
             vqc_fuse_chains(np, 0, np.phases.size());
             const size_t nf = np.phases.size();

@@ -784,13 +784,16 @@ __device__ __forceinline__ void mop_ham(const double2* __restrict__ sa, double2*
 // amplitudes as stored. v_mfma_f64_16x16x4f64 takes a 16 x 4 slice of that
 // matrix (lane l: row l % 16, k = l / 16), a 4 x 16 slice of X (lane l: k = l / 16,
 // column l % 16) and accumulates a 16 x 16 block (lane l, register i: row
-// 4 (l / 16) + i, column l % 16). The k order is chosen so that one 16-byte LDS
-// read serves two steps: steps 2s and 2s + 1 take, in lane group g = l / 16,
-// the real and imaginary parts of amplitude j = 4s + g. A wave owns whole
-// column tiles (16 groups), computes every row tile of them, and only then
-// writes them back: the reads and writes of one column stay in one wave, so the
-// in-place update needs no extra barrier. Output registers (0, 1) and (2, 3) of
-// row tile rt are amplitudes 8 rt + 2 g and 8 rt + 2 g + 1 of the group.
+// l / 16 + 4 i, column l % 16; checked on gfx950 with a lane-layout probe). The
+// k order is chosen so that one 16-byte LDS read serves two steps: steps 2s and
+// 2s + 1 take, in lane group g = l / 16, the real and imaginary parts of
+// amplitude j = 4s + g. The row order is chosen so that each lane's output is
+// whole amplitudes: block row rho = g + 4 i of row tile rt is amplitude
+// 8 rt + 2 g + i / 2, part i % 2, so registers (0, 1) and (2, 3) are amplitudes
+// 8 rt + 2 g and 8 rt + 2 g + 1. A wave owns whole column tiles (16 groups),
+// computes every row tile of them, and only then writes them back: the reads and
+// writes of one column stay in one wave, so the in-place update needs no extra
+// barrier.
 typedef double vqc_v4d __attribute__((ext_vector_type(4)));
 template <int K>
 __device__ __forceinline__ void mop_dense(double2* __restrict__ s, int t, const MicroOp& op, const double2* __restrict__ pool) {
@@ -802,29 +805,25 @@ __device__ __forceinline__ void mop_dense(double2* __restrict__ s, int t, const 
     const int col = lane & 15, g = lane >> 4;
     const uint32_t ncols = 1u << (t - K);  // groups
     const uint32_t nct = ncols >> 4;       // column tiles (the planner keeps t - K >= 7)
-    // offsets of amplitude 4s + g of a group, and of the two output amplitudes per row tile
-    uint32_t offin[NS], offout[RT][2];
-#pragma unroll
-    for (int q = 0; q < NS; ++q) offin[q] = off32<K>(4 * q + g, op.lpos);
-#pragma unroll
-    for (int rt = 0; rt < RT; ++rt) {
-        offout[rt][0] = off32<K>(8 * rt + 2 * g, op.lpos);
-        offout[rt][1] = off32<K>(8 * rt + 2 * g + 1, op.lpos);
-    }
-    // A-operand entries of this lane: real row r = 16 rt + (lane % 16) is complex row
-    // r / 2, part p = r % 2; step 2q + h reads complex column 4q + g, part h
-    const int prow = col & 1;
+    // A-operand entries of this lane: block row rho = lane % 16 = g' + 4 i' is complex
+    // row 8 rt + 2 g' + i' / 2, part i' % 2; step 2q + h reads complex column 4q + g, part h
+    const int prow = (col >> 2) & 1, rrow = 2 * (col & 3) + (col >> 3);
+    const double2* __restrict__ Mrow = M + rrow + size_t(g) * D;  // + 8 rt + 4 q D
+    const uint32_t offg = off32<K>(g, op.lpos);
     for (uint32_t ct = wave; ct < nct; ct += kTileWaves) {
-        const uint32_t base = ins32<K>(ct * 16 + col, op.lpos);
+        const uint32_t ob = ins32<K>(ct * 16 + col, op.lpos);
+        const uint32_t base = ob + offg;
         vqc_v4d acc[RT];
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt) acc[rt] = vqc_v4d{0.0, 0.0, 0.0, 0.0};
-#pragma unroll
+        // the k loop stays rolled for the larger blocks, so that the matrix entries
+        // are read per step (from the cache) instead of all held in registers
+#pragma unroll(K <= 3 ? NS : 2)
         for (int q = 0; q < NS; ++q) {
-            const double2 x = s[base + offin[q]];
+            const double2 x = s[base + off32<K>(4 * q, op.lpos)];
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) {
-                const double2 m = M[(8 * rt + (col >> 1)) + (4 * q + g) * D];
+                const double2 m = Mrow[8 * rt + size_t(4 * q) * D];
                 // (p, h) = (0, 0): Re m, (0, 1): -Im m, (1, 0): Im m, (1, 1): Re m
                 const double a0 = prow ? m.y : m.x;
                 const double a1 = prow ? m.x : -m.y;
@@ -834,8 +833,8 @@ __device__ __forceinline__ void mop_dense(double2* __restrict__ s, int t, const 
         }
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt) {
-            s[base + offout[rt][0]] = make_double2(acc[rt][0], acc[rt][1]);
-            s[base + offout[rt][1]] = make_double2(acc[rt][2], acc[rt][3]);
+            s[ob + off32<K>(8 * rt + 2 * g, op.lpos)] = make_double2(acc[rt][0], acc[rt][1]);
+            s[ob + off32<K>(8 * rt + 2 * g + 1, op.lpos)] = make_double2(acc[rt][2], acc[rt][3]);
         }
     }
 }
@@ -1059,6 +1058,17 @@ struct XPoolEntry {
     int32_t nsrc;
     int32_t pad;
 };
+// The product of a run of gates as one dense matrix on the union of their bits
+// (MK_DENSE): factor f acts on nb[f] of the K block bits, its matrix bit j on
+// block bit map[5 f + j]; the product is column-major over the block bits in
+// ascending order.
+struct XPoolBlock {
+    int64_t dst;
+    int32_t K;
+    std::vector<int64_t> mats;  // factor matrices, in application order
+    std::vector<int32_t> nb;
+    std::vector<int32_t> map;
+};
 struct XPoolClaim {
     int64_t off;
     int32_t D, ms;  // what was found for the D x D matrix at off: its class (MS_*), or for
@@ -1076,6 +1086,10 @@ struct XPool {
     std::vector<XPoolEntry> table;
     // every structure class the planner read from values (not deferred)
     std::vector<XPoolClaim> claims;
+    // dense block products (not deferred), remade after `table` on reload
+    std::vector<XPoolBlock> blocks;
+    // the product of a dense block, recorded (host pools only)
+    int64_t dense_block(XPoolBlock b);
     int64_t size() const { return deferred ? len : int64_t(h.size() / 2); }
     // variant: MF_CONJ / MF_TRANS combination, or 4 for M + M^dagger
     int64_t variant(int64_t off, int D, int v);
@@ -1093,6 +1107,9 @@ struct XPool {
 
 struct PlanBuilder {
     XPool* xp = nullptr;
+    // forward phases may hold MK_DENSE micro-ops (the caller runs them with the
+    // tile-phase kernels, not the whole-state small kernels)
+    bool dense_fwd = false;
     std::vector<MicroOp> mops;
     std::vector<PhaseDesc> phases;
     int nrows = 0;
