@@ -75,7 +75,8 @@ typedef struct vqc_op {
                             off-diagonal, as Rx); 0 when unknown. A wrong hint gives wrong
                             results. */
     int64_t mat;         /* U (2^nq square) or superoperator S (4^nq square) */
-    int64_t aux;         /* channels: S^-1 for the reverse sweep; unitaries: -1 */
+    int64_t aux;         /* channels: S^-1 for the reverse sweep, or -1 when S is singular (the
+                            reverse then restores a checkpoint); unitaries: -1 */
     int64_t gen[3];      /* G_j = U^dagger dU/dtheta_j (2^nq square), one per parameter */
 } vqc_op_t;
 
@@ -285,6 +286,15 @@ int vqc_dm_fidelity(const vqc_state* x, const vqc_state* y, double* out);
 int vqc_apply_circuit(vqc_state* s, int nops, const vqc_op_t* ops, const double* pool);
 
 #define VQC_FLAG_UNFUSED 1    /* one kernel per gate and per reduction (the plain path) */
+/* Forward-state checkpoints every ceil(sqrt(nops)) ops (fused energy gradients and
+ * vjps): the reverse sweep restores each stored state instead of trusting the
+ * uncomputed one. Density matrices also checkpoint, whatever the flags, before
+ * every channel given without an inverse (vqc_op_t::aux = -1) and wherever the
+ * accumulated ||S^-1||_1 of the channels uncomputed since the last restore would
+ * pass VQC_CKPT_AMP (1e3); such a channel's reverse applies S^dagger to the
+ * cotangent only (circuitdiff.jl:53-59 takes S^-1 of the state). The checkpoints
+ * take one state of device memory each. */
+#define VQC_FLAG_CHECKPOINT 2
 
 /* Shape of the most recent fused plan run by the calling thread (no reference
  * counterpart): LDS tile passes by kind (each forward pass reads and writes the

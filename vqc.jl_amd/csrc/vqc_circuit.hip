@@ -1075,10 +1075,7 @@ static int check_program(int nbits_q, int is_dm, int nops, const vqc_op_t* ops, 
         const int64_t D2 = o.kind == VQC_OP_CHANNEL ? (int64_t(1) << (4 * o.nq)) : (int64_t(1) << (2 * o.nq));
         if (o.mat < 0) return vqc_fail(VQC_EINVAL, "op %d: negative matrix offset", i);
         len = std::max(len, o.mat + D2);
-        if (o.kind == VQC_OP_CHANNEL) {
-            if (o.aux < 0) return vqc_fail(VQC_EINVAL, "op %d: channel without an inverse superoperator", i);
-            len = std::max(len, o.aux + D2);
-        }
+        if (o.kind == VQC_OP_CHANNEL && o.aux >= 0) len = std::max(len, o.aux + D2);  // aux < 0: no inverse (checkpointed)
         for (int j = 0; j < o.nparams; ++j) {
             if (o.slot[j] >= nparams) return vqc_fail(VQC_EINVAL, "op %d: gradient slot %d >= nparams %d", i, o.slot[j], nparams);
             if (o.slot[j] >= 0) {
@@ -1195,6 +1192,13 @@ static int run_forward(const PlanBuilder& pb, size_t ph_begin, size_t ph_end, co
     return VQC_OK;
 }
 
+// The forward phases [0, nfwd) from src into A, storing checkpoint k (L
+// amplitudes at ck + k L) after phase e.ck_store[k] - 1.
+struct PlanEntry;
+static int run_forward_ckpt(const PlanEntry& e, size_t nfwd, const double2* src, double2* A, double2* ck, uint64_t L,
+                            const MicroOp* dmops, const double2* dpool, hipStream_t st);
+static int ckpt_restore(const PlanEntry& e, size_t phase, double2* A, const double2* ck, uint64_t L, hipStream_t st);
+
 // shape of the most recent plan uploaded by this thread (vqc_plan_info)
 static thread_local vqc_plan_info_t g_last_plan{};
 // whether the plan being run came from the plan cache
@@ -1258,6 +1262,9 @@ struct PlanEntry {
     PlanBuilder pb;
     int info[8] = {};  // entry-point data (launch shape, phase boundaries)
     uint64_t id = 0;
+    // forward-state checkpoints: checkpoint k is stored once the forward phases
+    // [0, ck_store[k]) have run and restored into A before phase ck_restore[k]
+    std::vector<int> ck_store, ck_restore;
 };
 constexpr size_t kPlanCacheSize = 8;
 static std::vector<std::unique_ptr<PlanEntry>> g_plans;  // most recent first
@@ -1442,15 +1449,160 @@ static int max_small_rows(int nbits) {
     return used >= 160 * 1024 ? 0 : int((160 * 1024 - used) / 16);
 }
 
+// ---------------------------------------------------------------------------
+// Forward-state checkpoints. The reverse sweep recovers each earlier state by
+// uncomputing (U^dagger for gates, S^-1 for channels, circuitdiff.jl:23-32,
+// 53-59). A unitary uncompute is exact up to rounding, but S^-1 amplifies the
+// error of what it is applied to by up to ||S^-1|| per channel, and a channel
+// without an inverse (depolarizing at p = 3/4) cannot be uncomputed at all.
+// The op list is then cut into segments; the forward pass stores the state
+// after each segment but the last in spare device memory, and the reverse pass
+// restores it at the start of that segment's reverse instead of trusting the
+// uncomputed one. A segment starts right before every channel whose inverse is
+// missing or would push the accumulated amplification of the channels
+// uncomputed since the last restore above VQC_CKPT_AMP (1e3, three digits), and,
+// with VQC_FLAG_CHECKPOINT, every ceil(sqrt(nops)) ops. Such a channel is the
+// last op of its segment's reverse, planned alone and with the identity in
+// place of S^-1: its input state is the next restore.
+// ---------------------------------------------------------------------------
+struct CkptCut {
+    std::vector<int> begin;  // segment starts, begin[0] = 0
+    std::vector<char> chan;  // the segment starts with a channel undone through the checkpoint
+};
+static double norm1_cols(const double* M, int D) {
+    double best = 0;
+    for (int c = 0; c < D; ++c) {
+        double s = 0;
+        for (int r = 0; r < D; ++r) s += std::hypot(M[2 * (r + c * D)], M[2 * (r + c * D) + 1]);
+        best = std::max(best, s);
+    }
+    return best;
+}
+static CkptCut ckpt_cut(int nops, const vqc_op_t* ops, const double* pool, int is_dm, int flags) {
+    std::vector<char> at(size_t(nops) + 1, 0), chan(size_t(nops) + 1, 0);
+    if (flags & VQC_FLAG_CHECKPOINT) {
+        const int seg = std::max(1, int(std::ceil(std::sqrt(double(nops)))));
+        for (int i = seg; i < nops; i += seg) at[i] = 1;
+    }
+    if (is_dm) {
+        double lim = 1e3;
+        if (const char* e = getenv("VQC_CKPT_AMP")) lim = std::max(1.0, atof(e));
+        double amp = 1.0;
+        for (int i = nops - 1; i >= 0; --i) {
+            const vqc_op_t& o = ops[i];
+            if (o.kind != VQC_OP_CHANNEL) continue;
+            const double a = o.aux < 0 ? INFINITY : norm1_cols(pool + 2 * o.aux, 1 << (2 * o.nq));
+            if (o.aux < 0 || amp * a > lim) {
+                at[i] = 1;
+                chan[i] = 1;
+                amp = 1.0;
+            } else {
+                amp *= std::max(a, 1.0);
+            }
+        }
+    }
+    CkptCut c;
+    c.begin.push_back(0);
+    c.chan.push_back(chan[0]);
+    for (int i = 1; i < nops; ++i)
+        if (at[i]) {
+            c.begin.push_back(i);
+            c.chan.push_back(chan[i]);
+        }
+    return c;
+}
+static std::string cut_key(const CkptCut& c) {
+    std::string k;
+    for (size_t i = 0; i < c.begin.size(); ++i) {
+        key_put(k, c.begin[i]);
+        k.push_back(c.chan[i]);
+    }
+    return k;
+}
+// The identity superoperator (4^N x 4^N) appended to a host pool, for the
+// checkpointed channels' reverse; it does not depend on the caller's matrices.
+static int64_t pool_identity(XPool& xp, int D) {
+    const int64_t off = xp.size();
+    xp.h.resize(xp.h.size() + size_t(2) * D * D, 0.0);
+    for (int i = 0; i < D; ++i) xp.h[2 * (off + i + int64_t(i) * D)] = 1.0;
+    return off;
+}
+static int run_forward_ckpt(const PlanEntry& e, size_t nfwd, const double2* src, double2* A, double2* ck, uint64_t L,
+                            const MicroOp* dmops, const double2* dpool, hipStream_t st) {
+    size_t k = 0;
+    for (size_t i = 0; i < nfwd; ++i) {
+        PhaseBufs b{i == 0 ? src : A, A, nullptr, dmops, dpool, nullptr, 0};
+        VQC_TRY(launch_phase(e.pb, i, (const PhaseDesc*)g_wphases.buf, b, phase_grid(e.pb.phases[i], 1), st));
+        while (k < e.ck_store.size() && size_t(e.ck_store[k]) == i + 1) {
+            VQC_HIP(hipMemcpyAsync(ck + k * L, A, L * 16, hipMemcpyDeviceToDevice, st));
+            ++k;
+        }
+    }
+    return VQC_OK;
+}
+static int ckpt_restore(const PlanEntry& e, size_t phase, double2* A, const double2* ck, uint64_t L, hipStream_t st) {
+    for (size_t k = 0; k < e.ck_restore.size(); ++k)
+        if (size_t(e.ck_restore[k]) == phase) VQC_HIP(hipMemcpyAsync(A, ck + k * L, L * 16, hipMemcpyDeviceToDevice, st));
+    return VQC_OK;
+}
+
+// Forward phases of every segment; checkpoint k after segment k's phases.
+static int plan_forward_cut(PlanEntry& e, const CkptCut& cut, int nbits, int n, int is_dm, int nops, const vqc_op_t* ops, int t,
+                            int mlow) {
+    const size_t S = cut.begin.size();
+    for (size_t s = 0; s < S; ++s) {
+        const int b = cut.begin[s], end = s + 1 < S ? cut.begin[s + 1] : nops;
+        VQC_TRY(vqc_plan_forward(e.pb, nbits, n, is_dm, end - b, ops + b, t, mlow));
+        if (s + 1 < S) e.ck_store.push_back(int(e.pb.phases.size()));
+    }
+    return VQC_OK;
+}
+// Reverse phases, last segment first; checkpoint k is restored before segment k's.
+static int plan_reverse_cut(PlanEntry& e, const CkptCut& cut, int nbits, int n, int is_dm, int nops, const vqc_op_t* ops, int t,
+                            int mlow) {
+    const size_t S = cut.begin.size();
+    e.ck_restore.assign(S > 0 ? S - 1 : 0, 0);
+    for (size_t s = S; s-- > 0;) {
+        const int b = cut.begin[s], end = s + 1 < S ? cut.begin[s + 1] : nops;
+        if (s + 1 < S) e.ck_restore[s] = int(e.pb.phases.size());
+        if (cut.chan[s]) {
+            // the channel opening the segment: its reverse runs last and alone, with
+            // the identity on the state (the restore that follows replaces it)
+            if (end - b > 1) VQC_TRY(vqc_plan_reverse(e.pb, nbits, n, is_dm, end - b - 1, ops + b + 1, t, mlow));
+            vqc_op_t ch = ops[b];
+            ch.aux = pool_identity(e.xp, 1 << (2 * ch.nq));
+            VQC_TRY(vqc_plan_reverse(e.pb, nbits, n, is_dm, 1, &ch, t, mlow));
+        } else {
+            VQC_TRY(vqc_plan_reverse(e.pb, nbits, n, is_dm, end - b, ops + b, t, mlow));
+        }
+    }
+    return VQC_OK;
+}
+// Checkpoint buffers of one call (grown on demand, kept for the next call)
+static Work g_wckpt;
+static int ckpt_buffers(const PlanEntry& e, uint64_t L, double2** out) {
+    const size_t k = e.ck_store.size();
+    if (!k) {
+        *out = nullptr;
+        return VQC_OK;
+    }
+    if (ensure(g_wckpt, k * L * 16) != VQC_OK)
+        return vqc_fail(VQC_ENOMEM, "forward-state checkpoints: %zu states of %llu bytes do not fit", k,
+                        (unsigned long long)(L * 16));
+    *out = (double2*)g_wckpt.buf;
+    return VQC_OK;
+}
+
 // Plan of the state-vector adjoint into e. The whole evaluation runs in one
 // workgroup when allow_small and the state and the accumulators fit in LDS
 // (info: 1, Hamiltonian start, reverse start, micro-op width, reverse kind set);
 // otherwise in tile phases (info: 0, forward phases, forward + Hamiltonian phases).
 static int plan_sv(PlanEntry& e, int nb, int nops, const vqc_op_t* ops, int nterms, const vqc_term_t* terms, int64_t plen,
-                   int64_t hlen, bool allow_small) {
+                   int64_t hlen, bool allow_small, const CkptCut& cut) {
     const TileTuning tt = vqc_tile_tuning(nb);
     PlanBuilder& pb = e.pb;
-    if (allow_small && nb <= tt.t_small) {
+    const bool cuts = cut.begin.size() > 1;
+    if (allow_small && !cuts && nb <= tt.t_small) {
         VQC_TRY(vqc_plan_forward(pb, nb, nb, 0, nops, ops, nb, 0));
         const size_t nf = pb.phases.size();
         VQC_TRY(vqc_plan_ham(pb, nb, nterms, terms, plen, nb, 0));
@@ -1482,11 +1634,13 @@ static int plan_sv(PlanEntry& e, int nb, int nops, const vqc_op_t* ops, int nter
         e.xp.claims.clear();
     }
     pb.dense_fwd = true;
-    VQC_TRY(vqc_plan_forward(pb, nb, nb, 0, nops, ops, tt.t_fwd, tt.mlow));
+    if (cuts) VQC_TRY(plan_forward_cut(e, cut, nb, nb, 0, nops, ops, tt.t_fwd, tt.mlow));
+    else VQC_TRY(vqc_plan_forward(pb, nb, nb, 0, nops, ops, tt.t_fwd, tt.mlow));
     const size_t nfwd = pb.phases.size();
     VQC_TRY(vqc_plan_ham(pb, nb, nterms, terms, plen, tt.t_rev, tt.mlow));
     const size_t nham_end = pb.phases.size();
-    VQC_TRY(vqc_plan_reverse(pb, nb, nb, 0, nops, ops, tt.t_rev, tt.mlow));
+    if (cuts) VQC_TRY(plan_reverse_cut(e, cut, nb, nb, 0, nops, ops, tt.t_rev, tt.mlow));
+    else VQC_TRY(vqc_plan_reverse(pb, nb, nb, 0, nops, ops, tt.t_rev, tt.mlow));
     vqc_fuse_chains(pb, 0, pb.phases.size());
     e.info[0] = 0;
     e.info[1] = int(nfwd);
@@ -1523,11 +1677,12 @@ static int sv_adjoint(const vqc_state* psi0, int nops, const vqc_op_t* ops, cons
         if (seed) return vqc_fail(VQC_EUNSUPPORTED, "vqc_circuit_vjp runs on the fused path only (flags must be 0)");
         VQC_TRY(sv_energy_grad_plain(psi0, nops, ops, pool, plen, nterms, terms, hpool, nparams, energy2, grad, A, B, fwd_out, st));
     } else {
-        std::string key = plan_key('V', nb, nb, seed ? 1 : 0, nparams, nops, ops, plen, nterms, terms, hlen);
+        const CkptCut cut = ckpt_cut(nops, ops, pool, 0, flags);
+        std::string key = plan_key('V', nb, nb, seed ? 1 : 0, nparams, nops, ops, plen, nterms, terms, hlen) + cut_key(cut);
         PlanEntry* e = plan_find(key, pool, plen, hpool, hlen);
         if (!e) {
             std::unique_ptr<PlanEntry> ne = plan_make(std::move(key), pool, plen, hpool, hlen);
-            VQC_TRY(plan_sv(*ne, nb, nops, ops, nterms, terms, plen, hlen, !seed));
+            VQC_TRY(plan_sv(*ne, nb, nops, ops, nterms, terms, plen, hlen, !seed, cut));
             e = plan_store(std::move(ne));
         }
         const PlanBuilder& pb = e->pb;
@@ -1560,12 +1715,15 @@ static int sv_adjoint(const vqc_state* psi0, int nops, const vqc_op_t* ops, cons
         double2* parts = (double2*)g_wparts.buf;
         double2* rowsum = parts + size_t(nrows) * stride;
         vqc_zero(parts, size_t(nrows) * stride, st);
+        double2* ck;
+        VQC_TRY(ckpt_buffers(*e, L, &ck));
         if (nfwd == 0) VQC_HIP(hipMemcpyAsync(A, psi0->d, L * 16, hipMemcpyDeviceToDevice, st));
-        else VQC_TRY(run_forward(pb, 0, nfwd, psi0->d, A, dm, dp, (const PhaseDesc*)g_wphases.buf, st));
+        else VQC_TRY(run_forward_ckpt(*e, nfwd, psi0->d, A, ck, L, dm, dp, st));
         if (fwd_out) VQC_HIP(hipMemcpyAsync(fwd_out, A, L * 16, hipMemcpyDeviceToDevice, st));
         if (seed) VQC_HIP(hipMemcpyAsync(B, seed->d, L * 16, hipMemcpyDeviceToDevice, st));
         else if (nham_end == nfwd) vqc_zero(B, L, st);
         for (size_t i = nfwd; i < pb.phases.size(); ++i) {
+            VQC_TRY(ckpt_restore(*e, i, A, ck, L, st));
             PhaseBufs b{A, A, B, dm, dp, parts, stride};
             VQC_TRY(launch_phase(pb, i, (const PhaseDesc*)g_wphases.buf, b, phase_grid(pb.phases[i], stride), st));
         }
@@ -1658,6 +1816,10 @@ static int dm_adjoint(const vqc_state* rho0, int nops, const vqc_op_t* ops, cons
     PlanEntry* e = nullptr;
     std::vector<int64_t> conj_gen(size_t(nops) * 3, -1);
     if (plain) {
+        for (int i = 0; i < nops; ++i)
+            if (ops[i].kind == VQC_OP_CHANNEL && ops[i].aux < 0)
+                return vqc_fail(VQC_EUNSUPPORTED, "op %d: a channel without an inverse needs the fused path (its reverse "
+                                "restores a forward-state checkpoint)", i);
         local = plan_make(std::string(), pool, plen, hpool, hlen);
         local->id = 0;
         e = local.get();
@@ -1674,19 +1836,23 @@ static int dm_adjoint(const vqc_state* rho0, int nops, const vqc_op_t* ops, cons
                     lp.nrows++;
                 }
     } else {
-        std::string key = plan_key('D', nb, n, seed ? 1 : 0, nparams, nops, ops, plen, nterms, terms, hlen);
+        const CkptCut cut = ckpt_cut(nops, ops, pool, 1, flags);
+        const bool cuts = cut.begin.size() > 1 || cut.chan[0];
+        std::string key = plan_key('D', nb, n, seed ? 1 : 0, nparams, nops, ops, plen, nterms, terms, hlen) + cut_key(cut);
         e = plan_find(key, pool, plen, hpool, hlen);
         if (!e) {
             const TileTuning tt = vqc_tile_tuning(nb);
             std::unique_ptr<PlanEntry> ne = plan_make(std::move(key), pool, plen, hpool, hlen);
             PlanBuilder& np = ne->pb;
             np.dense_fwd = true;
-            VQC_TRY(vqc_plan_forward(np, nb, n, 1, nops, ops, tt.t_fwd, tt.mlow));
+            if (cuts) VQC_TRY(plan_forward_cut(*ne, cut, nb, n, 1, nops, ops, tt.t_fwd, tt.mlow));
+            else VQC_TRY(vqc_plan_forward(np, nb, n, 1, nops, ops, tt.t_fwd, tt.mlow));
             vqc_fuse_chains(np, 0, np.phases.size());
             const size_t nf = np.phases.size();
             np.nrows = nterms;  // rows 0..nterms-1: energies (DM energies come from the trace kernels)
             for (int t = 0; t < nterms; ++t) np.row_slot.push_back(-1 - t);
-            VQC_TRY(vqc_plan_reverse(np, nb, n, 1, nops, ops, tt.t_rev, tt.mlow));
+            if (cuts) VQC_TRY(plan_reverse_cut(*ne, cut, nb, n, 1, nops, ops, tt.t_rev, tt.mlow));
+            else VQC_TRY(vqc_plan_reverse(np, nb, n, 1, nops, ops, tt.t_rev, tt.mlow));
             vqc_fuse_chains(np, nf, np.phases.size());
             ne->info[0] = int(nf);
             e = plan_store(std::move(ne));
@@ -1697,6 +1863,7 @@ static int dm_adjoint(const vqc_state* rho0, int nops, const vqc_op_t* ops, cons
     VQC_TRY(upload_program(pb, st, e->id));
     const MicroOp* dm = (const MicroOp*)g_wmops.buf;
     const double2* dp = (const double2*)g_wpool.buf;
+    double2* ck = nullptr;    // forward-state checkpoints (fused path)
     const int stride = 2048;  // row and column gradient halves in the plain path
     const int nrows = std::max(pb.nrows, 1);
     VQC_TRY(ensure(g_wparts, size_t(nrows) * (stride + 1) * 16));
@@ -1722,7 +1889,8 @@ static int dm_adjoint(const vqc_state* rho0, int nops, const vqc_op_t* ops, cons
     } else if (nfwd == 0) {
         VQC_HIP(hipMemcpyAsync(A, rho0->d, L * 16, hipMemcpyDeviceToDevice, st));
     } else {
-        VQC_TRY(run_forward(pb, 0, nfwd, rho0->d, A, dm, dp, (const PhaseDesc*)g_wphases.buf, st));
+        VQC_TRY(ckpt_buffers(*e, L, &ck));
+        VQC_TRY(run_forward_ckpt(*e, nfwd, rho0->d, A, ck, L, dm, dp, st));
     }
     if (rho_out) VQC_HIP(hipMemcpyAsync(rho_out->d, A, L * 16, hipMemcpyDeviceToDevice, st));
     // energies tr(M rho) and the seed Delta = H^dagger
@@ -1773,6 +1941,7 @@ static int dm_adjoint(const vqc_state* rho0, int nops, const vqc_op_t* ops, cons
         }
     } else {
         for (size_t i = nfwd; i < pb.phases.size(); ++i) {
+            VQC_TRY(ckpt_restore(*e, i, A, ck, L, st));
             PhaseBufs b{A, A, B, dm, dp, parts, stride};
             VQC_TRY(launch_phase(pb, i, (const PhaseDesc*)g_wphases.buf, b, phase_grid(pb.phases[i], stride / 2), st));
         }

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_PKG_DIR, "libvqc_hip.so")
 
 VQC_OK = 0
 VQC_FLAG_UNFUSED = 1
+VQC_FLAG_CHECKPOINT = 2
 
 GATE_X, GATE_PHASE, GATE_CZ, GATE_CNOT, GATE_CPHASE = 1, 2, 3, 4, 5
 GATE_SWAP, GATE_ISWAP, GATE_ISWAPDAG, GATE_FREDKIN, GATE_CCPHASE = 6, 7, 8, 9, 10

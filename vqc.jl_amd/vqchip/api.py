@@ -227,7 +227,7 @@ def qubit_encoding_vjp(thetas, delta):
     return g * np.pi
 
 
-def circuit_vjp(circuit, x, delta_y, return_state=False):
+def circuit_vjp(circuit, x, delta_y, return_state=False, checkpoint=False):
     """The pullback of y = circuit * x (@adjoint *(circuit, x), circuitdiff.jl:2-8,
     with back_propagate, :23-78) for the cotangent delta_y on y (a state of x's
     kind): (grad, delta_x[, y]). grad holds d loss / d theta for the circuit's
@@ -243,12 +243,12 @@ def circuit_vjp(circuit, x, delta_y, return_state=False):
     dx = new()
     y = new() if return_state else None
     _lib.call("vqc_circuit_vjp", x.handle, len(prog.ops), prog.op_array(), _lib.dptr(pool), prog.nparams, delta_y.handle,
-              _lib.dptr(grad), dx.handle, y.handle if y else None, 0)
+              _lib.dptr(grad), dx.handle, y.handle if y else None, _lib.VQC_FLAG_CHECKPOINT if checkpoint else 0)
     res = (grad[: prog.nparams], dx)
     return res + (y,) if return_state else res
 
 
-def energy_and_gradient(H, circuit, state0, fused=True, return_state=False):
+def energy_and_gradient(H, circuit, state0, fused=True, return_state=False, checkpoint=False):
     """E = expectation(H, circuit * state0) and dRe(E)/dtheta for every active
     parameter of the circuit, in forward order (circuitdiff.jl:61-75).
 
@@ -258,7 +258,12 @@ def energy_and_gradient(H, circuit, state0, fused=True, return_state=False):
     with the energy, reverse). With longer terms the reference's own chain is
     taken: y = circuit * state0, E = expectation(H, y), the expectation pullback
     (expecdiff.jl:37-56: the per-term route of _apply_qterm_util!) and the circuit
-    pullback of that cotangent (circuitdiff.jl:2-8)."""
+    pullback of that cotangent (circuitdiff.jl:2-8).
+
+    checkpoint=True stores forward states every ceil(sqrt(#ops)) ops for the
+    reverse sweep to restore (VQC_FLAG_CHECKPOINT); density matrices also
+    checkpoint around channels that are singular or whose inverses would
+    amplify the uncompute error beyond 1e3, with or without it."""
     H = _as_operator(H)
     if not state0.is_dm and any(isinstance(g, QuantumMap) for g in circuit):
         state0 = DensityMatrix(state0)
@@ -267,7 +272,7 @@ def energy_and_gradient(H, circuit, state0, fused=True, return_state=False):
         y = apply_(circuit, state0.copy())
         E = expectation(H, y)
         delta = expectation_vjp(H, y, 1.0)
-        grad, _ = circuit_vjp(circuit, state0, delta)
+        grad, _ = circuit_vjp(circuit, state0, delta, checkpoint=checkpoint)
         res = (E, grad)
         return res + (y,) if return_state else res
     prog = Program(circuit, n=state0.nqubits(), dm=dm)
@@ -279,6 +284,6 @@ def energy_and_gradient(H, circuit, state0, fused=True, return_state=False):
     fn = "vqc_dm_energy_grad" if dm else "vqc_sv_energy_grad"
     _lib.call(fn, state0.handle, len(prog.ops), prog.op_array(), _lib.dptr(pool), nt, terms, _lib.dptr(hpool),
               prog.nparams, _lib.dptr(e2), _lib.dptr(grad), out.handle if out else None,
-              0 if fused else _lib.VQC_FLAG_UNFUSED)
+              (0 if fused else _lib.VQC_FLAG_UNFUSED) | (_lib.VQC_FLAG_CHECKPOINT if checkpoint else 0))
     res = (complex(e2[0], e2[1]), grad[: prog.nparams])
     return res + (out,) if return_state else res

@@ -75,7 +75,12 @@ class Program:
                 S = g.ordered_supermat()
                 op.kind = _lib.OP_CHANNEL
                 op.mat = self._add(S)
-                op.aux = self._add(np.linalg.inv(S))
+                # S^-1 for the reverse sweep; a singular S (depolarizing at p = 3/4) has
+                # none, and its reverse restores a forward-state checkpoint instead
+                if np.linalg.cond(S) < 1e14:
+                    op.aux = self._add(np.linalg.inv(S))
+                else:
+                    op.aux = -1
                 op.nparams = 0
                 pos = g.ordered_positions()
             else:
