@@ -146,6 +146,17 @@ typedef struct vqc_tuning {
  * run in one workgroup each. */
 int vqc_set_tuning(const vqc_tuning_t* t);
 int vqc_get_tuning(vqc_tuning_t* t);
+/* Tile shapes by state size, the autotuner's results (vqchip.autotune): entry nbits[i]
+ * (index bits 1..34 of a single state, or of a tiled instance of a narrow batch or a
+ * served request; -1 for wide batches of at least 16 instances) takes entries[i]. A
+ * zero field keeps the built-in rule above for that size; process-wide fields set
+ * by vqc_set_tuning take precedence over the table. */
+typedef struct vqc_tile_entry {
+    int32_t fwd, rev, low, reserved;
+} vqc_tile_entry_t;
+int vqc_set_tile_table(int nentries, const int32_t* nbits, const vqc_tile_entry_t* entries);
+/* the 35 entries in force: index 0 is the wide-batch entry, index b the b-bit one */
+int vqc_get_tile_table(vqc_tile_entry_t* out35);
 
 /* ---- state handles (StateVector / DensityMatrix storage) ----------------
  * src/statevector.jl:1-21 (StateVector{T}(n) = |0..0>), src/densitymatrix.jl:38-42. */

@@ -51,3 +51,49 @@ def test_tune_dense(restore_tuning):
     s = vq.StateVector(np.ones(1 << 10, complex) / 32)
     vq.apply_(vq.HGate(3), s)  # still a valid sweep with the tuned shape
     assert abs(np.linalg.norm(s.storage()) - 1) < 1e-12
+
+
+@pytest.fixture
+def restore_table():
+    before = autotune.get_tile_table()
+    yield before
+    autotune.set_tile_table({n: (0, 0, 0) for n in list(range(1, 35)) + [-1]})
+    autotune.set_tile_table(before)
+
+
+def test_tune_sizes_sets_the_per_size_table(restore_tuning, restore_table):
+    """The per-size entries the library's own callers read (single states,
+    narrow batches, the serving loop): every candidate gives the same energy and
+    gradient, and the chosen entry is in force."""
+    res = autotune.tune_sizes([12, 15], reps=1, lows=(3, 4), spread=(0, 1))
+    assert set(res) == {12, 15}
+    table = autotune.get_tile_table()
+    for n, r in res.items():
+        assert table[n] == r["best"]
+    c = hea_circuit(15, 3, np.random.default_rng(8).uniform(0, 6.3, 9 * 15))
+    H = tfim(15, 0.7)
+    E, g = vq.energy_and_gradient(H, c, vq.StateVector(15))
+    autotune.set_tile_table({15: (0, 0, 0)})
+    E2, g2 = vq.energy_and_gradient(H, c, vq.StateVector(15))
+    assert abs(E - E2) < 1e-10 and np.max(np.abs(g - g2)) < 1e-10
+
+
+def test_tune_batch_sets_the_wide_batch_entry(restore_table):
+    from vqchip.batch import Batch
+    from vqchip.workloads import config2_instances
+    inst = config2_instances(count=24, nmin=12, nmax=16, dmax=4)
+    b = Batch(inst)
+    E0, g0 = b.run()
+    res = autotune.tune_batch(b, fwd=(12, 13), rev=(12,), lows=(4, 5), reps=1)
+    assert autotune.get_tile_table()[-1] == res["best"]
+    E1, g1 = b.run()
+    assert np.max(np.abs(E1 - E0)) < 1e-10
+    assert max(float(np.max(np.abs(x - y))) if x.size else 0.0 for x, y in zip(g0, g1)) < 1e-10
+
+
+def test_tune_forward(restore_table):
+    from vqchip.workloads import config1_circuit
+    n = 18
+    c = config1_circuit(n=n, layers=3)
+    res = autotune.tune_forward(c, vq.StateVector(n), fwd=(11, 12), lows=(3, 4), reps=1)
+    assert autotune.get_tile_table()[n][0] == res["best"][0]

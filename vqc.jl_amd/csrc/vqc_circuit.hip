@@ -42,12 +42,18 @@ TileTuning vqc_tile_tuning(int nbits) {
         fwd = nbits <= 14 ? 9 : (nbits <= 18 ? 10 : (nbits <= 20 ? 11 : 12));
         rev = std::min(fwd, 11);
     }
+    // the autotuner's entry for this size (vqc_set_tile_table) replaces the rule
+    // above; explicit process-wide fields (vqc_set_tuning) replace both
+    const vqc_tile_entry_t te = vqc_tile_entry(nbits);
+    if (te.fwd) fwd = te.fwd;
+    if (te.rev) rev = te.rev;
     TileTuning t;
     t.t_fwd = std::min(kMaxTileBits, std::max(6, g.tile_fwd_bits ? int(g.tile_fwd_bits) : fwd));
     t.t_rev = std::min(kMaxTileBits2, std::max(6, g.tile_rev_bits ? int(g.tile_rev_bits) : rev));
     t.t_small = std::min(kMaxTileBits2, std::max(1, g.tile_small_bits ? int(g.tile_small_bits) : small));
     t.mlow = std::min(8, std::max(0, int(g.tile_low_bits)));
     if (nbits < 0 && !vqc_tuning_low_bits_given()) t.mlow = 5;
+    if (te.low && !vqc_tuning_low_bits_given()) t.mlow = te.low;
     return t;
 }
 
@@ -190,6 +196,84 @@ static int hint_gen(const vqc_op_t& o) {
 // are summed) drops the dependence test.
 constexpr int kLookahead = 128;
 
+// One phase's scan with the high tile bits fixed to inS (bits below mlow are
+// always in the tile): an item is taken when its high bits are in the set, no
+// bit of it belongs to an item skipped before, and the accumulators allow it.
+static size_t scan_fixed(const std::vector<int>& rest, const std::vector<std::vector<int>>& bits, int nbits, int mlow, int a,
+                         const std::vector<char>& inS, std::vector<char>& blocked, std::vector<int>* take,
+                         std::vector<int>* keep, size_t* idx_out, std::vector<int>* cur) {
+    std::fill(blocked.begin(), blocked.end(), 0);
+    int nblocked = 0, acc = 0;
+    size_t cnt = 0, idx = 0;
+    for (; idx < rest.size() && int(idx) < kLookahead && nblocked < nbits; ++idx) {
+        const int j = rest[idx];
+        const std::vector<int>& b = bits[j];
+        bool ok = acc + a <= kMaxAcc;
+        for (int x : b)
+            if (blocked[x] || (x >= mlow && !inS[x])) ok = false;
+        if (ok) {
+            ++cnt;
+            if (take) take->push_back(j);
+            if (cur)
+                for (int x : b)
+                    if (std::find(cur->begin(), cur->end(), x) == cur->end()) cur->push_back(x);
+            acc += a;
+            if (a > 0 && acc + a > kMaxAcc) {
+                ++idx;
+                break;
+            }
+        } else {
+            if (keep) keep->push_back(j);
+            for (int x : b)
+                if (!blocked[x]) {
+                    blocked[x] = 1;
+                    ++nblocked;
+                }
+        }
+    }
+    if (idx_out) *idx_out = idx;
+    return cnt;
+}
+
+static size_t cover_scan(const std::vector<int>& rest, const std::vector<std::vector<int>>& bits, int nbits, int t, int mlow,
+                         int a, std::vector<char>& blocked, std::vector<int>& take, std::vector<int>& keep, size_t& idx,
+                         std::vector<int>& cur) {
+    const int budget = t - std::min(mlow, t);
+    std::vector<char> inS(size_t(nbits), 0);
+    int used = 0;
+    size_t base = scan_fixed(rest, bits, nbits, mlow, a, inS, blocked, nullptr, nullptr, nullptr, nullptr);
+    const size_t W = std::min<size_t>(rest.size(), 64);
+    for (;;) {
+        std::vector<int> best;
+        double best_score = 0;
+        size_t best_cnt = base;
+        std::vector<std::vector<int>> seen;
+        for (size_t w = 0; w < W; ++w) {
+            std::vector<int> mv;
+            for (int x : bits[rest[w]])
+                if (x >= mlow && !inS[x]) mv.push_back(x);
+            if (mv.empty() || used + int(mv.size()) > budget) continue;
+            std::sort(mv.begin(), mv.end());
+            if (std::find(seen.begin(), seen.end(), mv) != seen.end()) continue;
+            seen.push_back(mv);
+            for (int x : mv) inS[x] = 1;
+            const size_t c = scan_fixed(rest, bits, nbits, mlow, a, inS, blocked, nullptr, nullptr, nullptr, nullptr);
+            for (int x : mv) inS[x] = 0;
+            const double score = double(c - base) / double(mv.size());
+            if (c > base && score > best_score) {
+                best = mv;
+                best_score = score;
+                best_cnt = c;
+            }
+        }
+        if (best.empty()) break;
+        for (int x : best) inS[x] = 1;
+        used += int(best.size());
+        base = best_cnt;
+    }
+    return scan_fixed(rest, bits, nbits, mlow, a, inS, blocked, &take, &keep, &idx, &cur);
+}
+
 template <class BitsFn, class EmitFn>
 static void plan_groups(PlanBuilder& pb, int kind, int nbits, int nitems, int t, int mlow, int max_acc_per_item,
                         BitsFn bits_of, EmitFn emit, bool all_commute = false) {
@@ -229,6 +313,20 @@ static void plan_groups(PlanBuilder& pb, int kind, int nbits, int nitems, int t,
                             blocked[x] = 1;
                             ++nblocked;
                         }
+            }
+        }
+        if (!all_commute && nbits > t && !getenv("VQC_NO_COVER")) {
+            // Coverage choice of the tile bits: grow the set of high bits greedily by
+            // the item moves (an item's missing high bits) that make the most items of
+            // the window executable per added bit, under the same dependence rule.
+            // It replaces the in-order first fit when it runs more items.
+            std::vector<int> take2, keep2, cur2;
+            size_t idx2 = 0;
+            if (cover_scan(rest, bits, nbits, t, mlow, a, blocked, take2, keep2, idx2, cur2) > take.size()) {
+                take.swap(take2);
+                keep.swap(keep2);
+                cur.swap(cur2);
+                idx = idx2;
             }
         }
         keep.insert(keep.end(), rest.begin() + idx, rest.end());

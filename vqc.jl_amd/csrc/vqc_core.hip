@@ -498,6 +498,36 @@ extern "C" int vqc_set_tuning(const vqc_tuning_t* t) {
     return VQC_OK;
 }
 
+// Tile shapes by state size, set by the autotuner: entry b for states of b index
+// bits (1..34), entry 0 for wide batches; zero fields keep the built-in rule.
+static vqc_tile_entry_t g_tile_table[35];
+
+extern "C" int vqc_set_tile_table(int nentries, const int32_t* nbits, const vqc_tile_entry_t* entries) {
+    if (nentries < 0 || (nentries && (!nbits || !entries))) return vqc_fail(VQC_EINVAL, "vqc_set_tile_table: bad arrays");
+    for (int i = 0; i < nentries; ++i) {
+        const int b = nbits[i] < 0 ? 0 : nbits[i];
+        const vqc_tile_entry_t& e = entries[i];
+        if (b > 34 || (e.fwd && (e.fwd < 6 || e.fwd > 13)) || (e.rev && (e.rev < 6 || e.rev > 12)) || e.low < 0 || e.low > 8 ||
+            (e.fwd && e.low > e.fwd) || (e.rev && e.low > e.rev))
+            return vqc_fail(VQC_EINVAL, "vqc_set_tile_table: entry %d out of range", i);
+    }
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    for (int i = 0; i < nentries; ++i) g_tile_table[nbits[i] < 0 ? 0 : nbits[i]] = entries[i];
+    return VQC_OK;
+}
+
+extern "C" int vqc_get_tile_table(vqc_tile_entry_t* out35) {
+    if (!out35) return vqc_fail(VQC_EINVAL, "null table");
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    for (int i = 0; i < 35; ++i) out35[i] = g_tile_table[i];
+    return VQC_OK;
+}
+
+vqc_tile_entry_t vqc_tile_entry(int nbits) {
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    return g_tile_table[nbits < 0 ? 0 : std::min(nbits, 34)];
+}
+
 extern "C" int vqc_get_tuning(vqc_tuning_t* t) {
     if (!t) return vqc_fail(VQC_EINVAL, "null tuning");
     *t = vqc_tuning_now();

@@ -55,6 +55,8 @@ int vqc_ctx(VqcCtx** out);
 vqc_tuning_t vqc_tuning_now();
 // whether tile_low_bits was given (vqc_set_tuning or VQC_TILE_MLOW) rather than defaulted
 bool vqc_tuning_low_bits_given();
+// the autotuned tile shape for states of nbits index bits (nbits < 0: wide batches); zero fields: none
+vqc_tile_entry_t vqc_tile_entry(int nbits);
 int vqc_scratch(size_t bytes, void** out);      // device scratch, valid until the next call
 int vqc_pinned(size_t bytes, double** out);     // pinned host staging
 // Upload host data into a dedicated device buffer of at least `bytes` bytes (grown on demand).

@@ -123,6 +123,10 @@ class Tuning(ctypes.Structure):
     ]
 
 
+class TileEntry(ctypes.Structure):
+    _fields_ = [("fwd", ctypes.c_int32), ("rev", ctypes.c_int32), ("low", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 class PlanInfo(ctypes.Structure):
     _fields_ = [("phases_fwd", ctypes.c_int32), ("phases_ham", ctypes.c_int32), ("phases_rev", ctypes.c_int32),
                 ("mops", ctypes.c_int32 * 8), ("reused", ctypes.c_int32)]
@@ -142,6 +146,8 @@ SIGNATURES = {
     "vqc_get_stream": [ctypes.POINTER(_P)],
     "vqc_set_tuning": [ctypes.POINTER(Tuning)],
     "vqc_get_tuning": [ctypes.POINTER(Tuning)],
+    "vqc_set_tile_table": [_I, _I32, ctypes.POINTER(TileEntry)],
+    "vqc_get_tile_table": [ctypes.POINTER(TileEntry)],
     "vqc_sv_create": [_I, ctypes.POINTER(_P)],
     "vqc_dm_create": [_I, ctypes.POINTER(_P)],
     "vqc_state_wrap": [_I, _I, _P, ctypes.POINTER(_P)],

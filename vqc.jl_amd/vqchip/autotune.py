@@ -2,9 +2,15 @@
 path, and for the launch shape of the dense gate sweeps (no counterpart in the
 reference, whose kernels have fixed shapes).
 
-    best = tune_energy_grad(H, circuit, state0)     # sweeps tile bits / low bits
+    best = tune_energy_grad(H, circuit, state0)     # sweeps tile bits / low bits (process-wide)
+    table = tune_sizes([10, 12, ..., 20])           # per-size tile table (vqc_set_tile_table)
+    best = tune_batch(batch)                        # the wide-batch entry of that table
     best = tune_dense(28)                           # sweeps groups per thread / NT
     load_cached(key) / the JSON cache at ~/.cache/vqchip/tuning.json
+
+The per-size table is what the library's own callers read: single-state energy
+gradients, the tiled instances of narrow batches and the serving loop's
+requests take the entry of their size, wide batches the batch entry.
 
 Each candidate is applied with vqc_set_tuning, timed on the device (median of
 `reps` runs after one warm-up), and the fastest is left in force. Results are
@@ -151,3 +157,141 @@ def store_cached(key, params, path=CACHE):
     os.makedirs(os.path.dirname(path), exist_ok=True)
     with open(path, "w") as f:
         json.dump(d, f, indent=1, sort_keys=True)
+
+
+def get_tile_table():
+    """{nbits: (fwd, rev, low)} of the entries in force (nbits -1: wide batches)."""
+    arr = (_lib.TileEntry * 35)()
+    _lib.call("vqc_get_tile_table", arr)
+    return {(-1 if i == 0 else i): (arr[i].fwd, arr[i].rev, arr[i].low) for i in range(35)
+            if arr[i].fwd or arr[i].rev or arr[i].low}
+
+
+def set_tile_table(table):
+    """Set entries {nbits: (fwd, rev, low)} (0 keeps the rule for a field; nbits -1: wide batches)."""
+    items = sorted(table.items())
+    nb = (ctypes.c_int32 * max(len(items), 1))(*[int(k) for k, _ in items])
+    ents = (_lib.TileEntry * max(len(items), 1))()
+    for i, (_, (f, r, l)) in enumerate(items):
+        ents[i].fwd, ents[i].rev, ents[i].low = int(f), int(r), int(l)
+    _lib.call("vqc_set_tile_table", len(items), nb, ents)
+
+
+def _rule(nbits):
+    """The built-in tile rule for one state of nbits index bits (forward, reverse)."""
+    f = 9 if nbits <= 14 else (10 if nbits <= 18 else (11 if nbits <= 20 else 12))
+    return f, min(f, 11)
+
+
+def tune_sizes(sizes, workload=None, reps=3, lows=(3, 4, 5), spread=(-1, 0, 1), apply=True, cache_key=None):
+    """Tile shape per state size: for each n, energy + gradient of `workload(n)`
+    ((circuit, H, state0); by default a hardware-efficient ansatz of 4 layers with
+    a transverse-field Ising H) is timed for forward / reverse tile bits around the
+    built-in rule and the contiguous low bits given, and the fastest shape becomes
+    the size's entry of the library's tile table. Sizes of at most 9 bits run in
+    one workgroup and are left out. Returns {n: {"best": (fwd, rev, low), "table": [...]}}."""
+    from .api import energy_and_gradient
+    from .state import StateVector
+    from .workloads import hea_circuit, tfim
+    if workload is None:
+        def workload(n):
+            th = np.random.default_rng(n).uniform(0, 2 * np.pi, 3 * n * 4)
+            return hea_circuit(n, 4, th), tfim(n, 1.0), StateVector(n)
+    before = get_tuning()
+    set_tuning(tile_fwd_bits=0, tile_rev_bits=0)  # per-size entries apply only without a process-wide choice
+    out = {}
+    for n in sizes:
+        if n <= 9:
+            continue
+        c, H, s0 = workload(n)
+        f0, r0 = _rule(n)
+        table, ref = [], None
+        for df in spread:
+            for dr in spread:
+                f, r = f0 + df, r0 + dr
+                if not (6 <= f <= 13 and 6 <= r <= 12) or f > n or r > n:
+                    continue
+                for low in lows:
+                    if low > min(f, r):
+                        continue
+                    set_tile_table({n: (f, r, low)})
+                    res = {}
+
+                    def run():
+                        res["E"] = energy_and_gradient(H, c, s0)[0]
+
+                    t = _time(run, reps)
+                    if ref is None:
+                        ref = res["E"]
+                    elif abs(res["E"] - ref) > 1e-10 * max(1.0, abs(ref)):
+                        raise RuntimeError(f"tile shape {(f, r, low)} changed the energy at n = {n}")
+                    table.append(((f, r, low), t))
+        best = min(table, key=lambda x: x[1])[0]
+        set_tile_table({n: best if apply else (0, 0, 0)})
+        out[n] = {"best": best, "table": table}
+    set_tuning(tile_fwd_bits=before["tile_fwd_bits"], tile_rev_bits=before["tile_rev_bits"])
+    if cache_key is not None:
+        store_cached(cache_key, {"tile_table": {str(n): list(v["best"]) for n, v in out.items()}})
+    return out
+
+
+def tune_batch(batch, fwd=(12, 13), rev=(11, 12), lows=(4, 5, 6), reps=3, apply=True):
+    """The wide-batch tile shape: the batch is planned again for each candidate
+    (its packed records are kept) and replayed; the fastest becomes the table's
+    wide-batch entry, which vqc_batch_create and the serving loop's wide batches
+    read. Results are checked equal across candidates. Returns {"best": ..., "table": ...}."""
+    before = get_tile_table().get(-1, (0, 0, 0))
+    table, ref = [], None
+    for f in fwd:
+        for r in rev:
+            for low in lows:
+                set_tile_table({-1: (f, r, low)})
+                batch.rebuild()
+                E, _ = batch.run()
+                if ref is None:
+                    ref = E
+                elif np.max(np.abs(E - ref)) > 1e-10 * max(1.0, float(np.max(np.abs(ref)))):
+                    raise RuntimeError(f"batch tile shape {(f, r, low)} changed the energies")
+                table.append(((f, r, low), _time(batch.run, reps)))
+    best = min(table, key=lambda x: x[1])[0]
+    set_tile_table({-1: best if apply else before})
+    batch.rebuild()
+    return {"best": best, "table": table}
+
+
+def tune_forward(circuit, state, fwd=(11, 12, 13), lows=(3, 4, 5), reps=3, apply=True):
+    """Tile shape of the fused forward pass (apply!(circuit, state) on the tiled
+    path) for the state's size: each candidate's forward tile bits and contiguous
+    low bits are timed on a copy of the state and the fastest becomes the size's
+    entry of the tile table (its reverse field is kept). Results are checked equal
+    across candidates. Returns {"best": (fwd, low), "table": [...]}."""
+    from .api import apply_
+    nb = state.nqubits() * (2 if state.is_dm else 1)
+    keep = get_tile_table().get(nb, (0, 0, 0))
+    table, ref = [], None
+    for f in fwd:
+        for low in lows:
+            if low > f or f > nb:
+                continue
+            set_tile_table({nb: (f, keep[1], low)})
+            work = state.copy()
+            apply_(circuit, work)
+            v = work.dot(work)
+            if ref is None:
+                ref = work.copy()  # the timing below keeps applying the circuit to work
+            else:
+                d = work.copy()
+                from . import _lib
+                _lib.call("vqc_state_axpy", d.handle, -1.0, 0.0, ref.handle)
+                if abs(d.dot(d)) > 1e-20 * max(1.0, abs(v)):
+                    raise RuntimeError(f"forward tile shape {(f, low)} changed the state")
+
+            def run():
+                apply_(circuit, work)
+                from . import _lib
+                _lib.call("vqc_sync")
+
+            table.append(((f, low), _time(run, reps)))
+    best = min(table, key=lambda x: x[1])[0]
+    set_tile_table({nb: (best[0], keep[1], best[1]) if apply else keep})
+    return {"best": best, "table": table}

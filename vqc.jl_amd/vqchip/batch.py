@@ -115,6 +115,11 @@ class Batch:
             grads += g
         return np.concatenate(E), grads
 
+    def rebuild(self):
+        if self._part is None:
+            raise RuntimeError("a batch in waves is planned per run")
+        self._part.rebuild()
+
     def launch(self):
         if self._part is None:
             raise RuntimeError("a batch in waves runs with run()")
@@ -169,14 +174,21 @@ class _Part:
         self._inst = inst
         self._pool = pool_chunks.array()
         self._hpool = hpool_chunks.array()
-        ops = (_lib.Op * max(len(self._ops), 1))(*self._ops)
-        terms = (_lib.Term * max(len(self._terms), 1))(*self._terms)
-        h = ctypes.c_void_p()
-        _lib.call("vqc_batch_create", self.n_inst, inst, ops, len(self._ops), terms, len(self._terms),
-                  len(self._pool), len(self._hpool), 0, ctypes.byref(h))
-        self._h = h
+        self._op_arr = (_lib.Op * max(len(self._ops), 1))(*self._ops)
+        self._term_arr = (_lib.Term * max(len(self._terms), 1))(*self._terms)
+        self._h = None
+        self.rebuild()
         self._energies = np.zeros(2 * self.n_inst)
         self._grads = np.zeros(max(grad, 1))
+
+    def rebuild(self):
+        """Plan the batch again from its packed records (with the tile shapes in
+        force now) and capture a new graph; the autotuner times candidates so."""
+        self.close()
+        h = ctypes.c_void_p()
+        _lib.call("vqc_batch_create", self.n_inst, self._inst, self._op_arr, len(self._ops), self._term_arr,
+                  len(self._terms), len(self._pool), len(self._hpool), 0, ctypes.byref(h))
+        self._h = h
 
     def update(self, circuits):
         """New parameters for the same circuit structures (matrix pool rebuilt;
