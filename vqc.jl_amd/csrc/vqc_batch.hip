@@ -337,6 +337,8 @@ static int batch_enqueue(vqc_batch* b, hipStream_t st, const double* h_pool, dou
     vqc_zero(b->d_parts, uint64_t(std::max<int64_t>(b->nparts, 1)), st);
     hipLaunchKernelGGL(k_zero_u32, dim3(std::max(1, (b->ninst + 2 + 255) / 256)), dim3(256), 0, st, b->d_counters, b->ninst + 2);
     hipLaunchKernelGGL(batch_kernel(b), dim3(b->grid), dim3(kTileThreads), b->lds, st, a);
+    if (vqc_trace_on())
+        vqc_trace("k_batch maxkb %d instances %d items %d grid %u lds %zu", int(b->maxkb), int(b->ninst), int(b->nitems), unsigned(b->grid), size_t(b->lds));
     if (b->nrows)
         hipLaunchKernelGGL(k_reduce_rows, dim3(b->nrows), dim3(256), 0, st, b->d_parts, (const int64_t*)b->d_row_off,
                            (const int32_t*)b->d_row_cnt, b->d_rows);

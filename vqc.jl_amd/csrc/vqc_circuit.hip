@@ -1270,6 +1270,19 @@ static int launch_phase(const PlanBuilder& pb, size_t i, const PhaseDesc* dpd, c
     VQC_FOR_KERNELS(VQC_LAUNCH)
 #undef VQC_LAUNCH
     if (!launched) return vqc_fail(VQC_EINVAL, "no tile kernel for micro-op width %d and kind set %d", mk, ks);
+    if (vqc_trace_on()) {
+        // micro-ops as kind:bits, the MK_DENSE ones as D<bits>
+        std::string mo;
+        for (int o = p.op_begin; o < p.op_end; ++o) {
+            const MicroOp& op = pb.mops[o];
+            char w[24];
+            if (op.kind == MK_DENSE) std::snprintf(w, sizeof w, " D%d", op.kb);
+            else std::snprintf(w, sizeof w, " %d:%d", op.kind, op.kb);
+            mo += w;
+        }
+        vqc_trace("k_phase<%d,%d> phase %zu kind %d nbits %d t %d low %d grid %u lds %zu nacc %d ops%s", mk, ks, i, p.kind,
+                  p.nbits, p.t, p.m, grid, lds, p.nacc, mo.c_str());
+    }
     VQC_HIP(hipGetLastError());
     return VQC_OK;
 }

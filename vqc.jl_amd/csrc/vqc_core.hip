@@ -1,6 +1,7 @@
 // vqc_core.hip -- library context, errors, state handles and whole-state
 // elementwise kernels (StateVector / DensityMatrix storage on the device).
 #include <cstdarg>
+#include <cstdlib>
 #include <mutex>
 
 #include "vqc_internal.h"
@@ -15,6 +16,24 @@ int vqc_fail(int code, const char* fmt, ...) {
     va_end(ap);
     g_err = buf;
     return code;
+}
+
+bool vqc_trace_on() {
+    static const bool on = [] {
+        const char* e = std::getenv("VQC_TRACE");
+        return e && std::atoi(e) != 0;
+    }();
+    return on;
+}
+
+void vqc_trace(const char* fmt, ...) {
+    if (!vqc_trace_on()) return;
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    std::fprintf(stderr, "vqc-trace: %s\n", buf);
 }
 
 extern "C" const char* vqc_last_error(void) { return g_err.c_str(); }

@@ -141,10 +141,14 @@ static int launch_dense(double2* v, int nbits, const int32_t* pos, const double*
         VQC_DENSE_LAUNCH(1) else VQC_DENSE_LAUNCH(2) else VQC_DENSE_LAUNCH(4) else VQC_DENSE_LAUNCH(8) else return vqc_fail(
             VQC_EINVAL, "dense groups per thread must be 1, 2, 4 or 8");
 #undef VQC_DENSE_LAUNCH
+        if (vqc_trace_on())
+            vqc_trace("k_dense_small<%d,%d,%d> nbits %d lowest target %d grid %u", K, R, int(NT), nbits, gp.p[0],
+                      grid_for(ngroups, R * kBlock));
     } else {
         VQC_TRY(vqc_upload_buffer(&g_mat_dev, &g_mat_cap, Uh, size_t(D) * D * 16));
         hipLaunchKernelGGL((k_dense_big<K>), dim3(grid_for(ngroups, kBlock)), dim3(kBlock), 0, st, v, ngroups, gp,
                            (const double2*)g_mat_dev);
+        if (vqc_trace_on()) vqc_trace("k_dense_big<%d> nbits %d grid %u", K, nbits, grid_for(ngroups, kBlock));
     }
     VQC_HIP(hipGetLastError());
     return VQC_OK;

@@ -17,6 +17,10 @@
 // errors
 // ---------------------------------------------------------------------------
 int vqc_fail(int code, const char* fmt, ...);
+// Per-launch trace on stderr when the environment sets VQC_TRACE=1: one line per
+// kernel launch with its shape (kernel, grid, tile bits, micro-ops).
+bool vqc_trace_on();
+void vqc_trace(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
 #define VQC_HIP(call)                                                                      \
     do {                                                                                   \
         hipError_t e_ = (call);                                                            \

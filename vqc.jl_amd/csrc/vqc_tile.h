@@ -796,11 +796,10 @@ __device__ __forceinline__ void mop_ham(const double2* __restrict__ sa, double2*
 // barrier.
 typedef double vqc_v4d __attribute__((ext_vector_type(4)));
 template <int K>
-__device__ __forceinline__ void mop_dense(double2* __restrict__ s, int t, const MicroOp& op, const double2* __restrict__ pool) {
+__device__ __forceinline__ void mop_dense(double2* __restrict__ s, int t, const MicroOp& op, const double2* __restrict__ M) {
     constexpr int D = 1 << K;          // amplitudes per group
     constexpr int RT = (2 * D) / 16;   // 16-row tiles of the real matrix
     constexpr int NS = D / 4;          // pairs of k steps
-    const double2* __restrict__ M = pool + op.mat;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int col = lane & 15, g = lane >> 4;
     const uint32_t ncols = 1u << (t - K);  // groups
@@ -810,21 +809,76 @@ __device__ __forceinline__ void mop_dense(double2* __restrict__ s, int t, const 
     const int prow = (col >> 2) & 1, rrow = 2 * (col & 3) + (col >> 3);
     const double2* __restrict__ Mrow = M + rrow + size_t(g) * D;  // + 8 rt + 4 q D
     const uint32_t offg = off32<K>(g, op.lpos);
+    if constexpr (K <= 4) {
+        // A operands of every step in registers for the whole micro-op (K = 4: 32
+        // VGPRs); column tiles are taken CT at a time, so that the matrix cores see
+        // RT * CT independent accumulation chains
+        constexpr int CT = K == 3 ? 2 : 1;
+        double a0v[NS][RT], a1v[NS][RT];
+#pragma unroll
+        for (int q = 0; q < NS; ++q)
+#pragma unroll
+            for (int rt = 0; rt < RT; ++rt) {
+                const double2 m = Mrow[8 * rt + size_t(4 * q) * D];
+                // (p, h) = (0, 0): Re m, (0, 1): -Im m, (1, 0): Im m, (1, 1): Re m
+                a0v[q][rt] = prow ? m.y : m.x;
+                a1v[q][rt] = prow ? m.x : -m.y;
+            }
+        uint32_t xo[NS];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) xo[q] = offg + off32<K>(4 * q, op.lpos);
+        for (uint32_t ct0 = wave; ct0 < nct; ct0 += CT * kTileWaves) {
+            uint32_t ob[CT];
+            bool live[CT];
+            double2 x[CT][NS];
+#pragma unroll
+            for (int c = 0; c < CT; ++c) {
+                const uint32_t ct = ct0 + c * kTileWaves;
+                live[c] = ct < nct;  // uniform over the wave
+                ob[c] = ins32<K>((live[c] ? ct : ct0) * 16 + col, op.lpos);
+#pragma unroll
+                for (int q = 0; q < NS; ++q) x[c][q] = s[ob[c] + xo[q]];
+            }
+            vqc_v4d acc[CT][RT];
+#pragma unroll
+            for (int c = 0; c < CT; ++c)
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) acc[c][rt] = vqc_v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < NS; ++q)
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt)
+#pragma unroll
+                    for (int c = 0; c < CT; ++c) {
+                        acc[c][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0v[q][rt], x[c][q].x, acc[c][rt], 0, 0, 0);
+                        acc[c][rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1v[q][rt], x[c][q].y, acc[c][rt], 0, 0, 0);
+                    }
+#pragma unroll
+            for (int c = 0; c < CT; ++c) {
+                if (!live[c]) continue;
+#pragma unroll
+                for (int rt = 0; rt < RT; ++rt) {
+                    s[ob[c] + off32<K>(8 * rt + 2 * g, op.lpos)] = make_double2(acc[c][rt][0], acc[c][rt][1]);
+                    s[ob[c] + off32<K>(8 * rt + 2 * g + 1, op.lpos)] = make_double2(acc[c][rt][2], acc[c][rt][3]);
+                }
+            }
+        }
+        return;
+    }
     for (uint32_t ct = wave; ct < nct; ct += kTileWaves) {
         const uint32_t ob = ins32<K>(ct * 16 + col, op.lpos);
         const uint32_t base = ob + offg;
         vqc_v4d acc[RT];
 #pragma unroll
         for (int rt = 0; rt < RT; ++rt) acc[rt] = vqc_v4d{0.0, 0.0, 0.0, 0.0};
-        // the k loop stays rolled for the larger blocks, so that the matrix entries
-        // are read per step (from the cache) instead of all held in registers
-#pragma unroll(K <= 3 ? NS : 2)
+        // the k loop stays rolled for the largest blocks, so that the matrix entries
+        // are read per step instead of all held in registers
+#pragma unroll 2
         for (int q = 0; q < NS; ++q) {
             const double2 x = s[base + off32<K>(4 * q, op.lpos)];
 #pragma unroll
             for (int rt = 0; rt < RT; ++rt) {
                 const double2 m = Mrow[8 * rt + size_t(4 * q) * D];
-                // (p, h) = (0, 0): Re m, (0, 1): -Im m, (1, 0): Im m, (1, 1): Re m
                 const double a0 = prow ? m.y : m.x;
                 const double a1 = prow ? m.x : -m.y;
                 acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0, x.x, acc[rt], 0, 0, 0);
@@ -923,9 +977,10 @@ __device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const M
             break;
         case MK_DENSE:
             if constexpr ((KS & (1 << MK_DENSE)) != 0) {
-                if (kb == 3) mop_dense<3>(sa, t, op, pool);
-                else if (kb == 4) mop_dense<4>(sa, t, op, pool);
-                else mop_dense<5>(sa, t, op, pool);
+                const double2* M = pool + op.mat;
+                if (kb == 3) mop_dense<3>(sa, t, op, M);
+                else if (kb == 4) mop_dense<4>(sa, t, op, M);
+                else mop_dense<5>(sa, t, op, M);
             }
             break;
         case MK_HAM:
