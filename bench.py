@@ -236,6 +236,39 @@ def bench_config2(vq, _lib):
     return out
 
 
+def bench_config2_toffoli(vq, _lib):
+    """The uneven batch with one Toffoli per layer: batches with 3-qubit gates run
+    the persistent kernel built for 3-bit micro-ops (k_batch<3>)."""
+    from vqchip.batch import Batch
+    from vqchip.workloads import config2_toffoli_instances
+    inst = config2_toffoli_instances()
+    b = Batch(inst)
+    E_b, g_b = b.run()
+    t_batch = median_s(b.run, 5)
+    prepared = [(_prog_args(vq, c, H, n), n) for c, H, n in inst]
+    states = {n: vq.StateVector(n) for _, n in prepared}
+    res = [_eval(vq, _lib, states[n], a, True) for a, n in prepared]
+
+    def fused_loop():
+        for a, n in prepared:
+            _eval(vq, _lib, states[n], a, True)
+
+    t_fused = median_s(fused_loop, 3)
+    err = max(max(abs(x - y[0]) for x, y in zip(E_b, res)),
+              max(float(np.max(np.abs(x - y[1]))) if x.size else 0.0 for x, y in zip(g_b, res)))
+    out = {
+        "workload": "the 256-instance batch with one Toffoli (3-qubit gate) per layer",
+        "batched_evals_per_s": round(len(inst) / t_batch, 1),
+        "batched_ms": round(t_batch * 1e3, 2),
+        "fused_per_instance_evals_per_s": round(len(inst) / t_fused, 1),
+        "speedup_batched_vs_fused_per_instance": round(t_fused / t_batch, 2),
+        "batch_plan": b.info(),
+        "max_abs_diff_batched_vs_fused": err,
+    }
+    log(f"config2 toffoli: {out}")
+    return out
+
+
 def bench_config3(vq, _lib):
     """12-qubit density matrix, 6 noisy layers, sum Z_i: energy + 216-angle gradient."""
     from vqchip.workloads import config3
@@ -585,6 +618,16 @@ def main():
     extra["fused_vs_plain_l2_diff"] = float(np.sqrt(other.dot(other).real))
     del other
     _lib.call("vqc_sync")
+    # the autotuner picks this size's forward tile shape (tile bits, contiguous low
+    # bits) on a copy of the state; the choice goes into the library's tile table
+    from vqchip import autotune
+    t_tune = time.perf_counter()
+    tuned_fwd = autotune.tune_forward(circuit, state, fwd=(12,), lows=(3, 4), reps=1)
+    extra["fused_autotuned_tile"] = {"fwd_bits": tuned_fwd["best"][0], "low_bits": tuned_fwd["best"][1],
+                                     "tune_s": round(time.perf_counter() - t_tune, 2),
+                                     "candidates_ms": {f"{f}/{l}": round(t * 1e3, 2) for (f, l), t in tuned_fwd["table"]}}
+    _lib.call("vqc_apply_circuit", state.handle, len(prog.ops), ops, _lib.dptr(pool))
+    _lib.call("vqc_sync")
     timer.start()
     for _ in range(args.steps):
         _lib.call("vqc_apply_circuit", state.handle, len(prog.ops), ops, _lib.dptr(pool))
@@ -625,6 +668,7 @@ def main():
         extra["config0"], c0 = bench_config0(vq, _lib)
     if "c2" in parts:
         extra["config2"] = bench_config2(vq, _lib)
+        extra["config2_toffoli"] = bench_config2_toffoli(vq, _lib)
     if "c3" in parts:
         extra["config3"] = bench_config3(vq, _lib)
     if "c4" in parts:

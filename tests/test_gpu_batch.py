@@ -163,3 +163,17 @@ def test_batch_rejects_more_than_three_parameters():
         _lib.call("vqc_batch_create", 1, inst, ops, len(prog.ops), terms, nt, len(prog.pool()), len(hpool), 0,
                   ctypes.byref(h))
     assert e.value.code == -1
+
+
+def test_toffoli_batch_matches_oracle():
+    """The Toffoli variant of the uneven batch (3-qubit gates in every instance,
+    the 3-bit persistent kernel), at sizes the oracle finishes in seconds."""
+    from vqchip.workloads import config2_toffoli_instances
+    inst = config2_toffoli_instances(count=24, nmin=5, nmax=13, dmax=6)
+    E, grads = Batch(inst).run()
+    worst = 0.0
+    for i, (c, H, n) in enumerate(inst):
+        Er, gr = oracle_eval(c, H, n)
+        worst = max(worst, abs(E[i] - Er) / max(1.0, abs(Er)), float(np.max(np.abs(grads[i] - gr))) / max(1.0, float(np.max(np.abs(gr)))))
+    print(f"toffoli batch: 24 instances, max error vs the oracle {worst:.2e}", flush=True)
+    assert worst < 1e-10

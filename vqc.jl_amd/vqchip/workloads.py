@@ -106,6 +106,34 @@ def config2_instances(seed=DEFAULT_SEED, count=256, nmin=8, nmax=20, dmin=2, dma
     return out
 
 
+def toffoli_matrix():
+    """CCX as an 8 x 8 matrix: the target flips when both controls are 1 (its
+    two highest basis states swap), whatever the order of the three positions."""
+    m = np.eye(8, dtype=complex)
+    m[[6, 7]] = m[[7, 6]]
+    return m
+
+
+def config2_toffoli_instances(seed=DEFAULT_SEED, count=256, nmin=8, nmax=20, dmin=2, dmax=12):
+    """The uneven batch with one Toffoli per layer (on qubits k, k+1, k+2 of
+    layer k, cyclically), so that every instance carries 3-qubit gates."""
+    out = []
+    for i in range(count):
+        rng = np.random.default_rng([seed, i])
+        n = int(rng.integers(nmin, nmax + 1))
+        depth = int(rng.integers(dmin, dmax + 1))
+        thetas = rng.uniform(0, 2 * np.pi, 3 * n * depth)
+        J = rng.standard_normal(n - 1)
+        c = QCircuit()
+        it = iter(range(len(thetas)))
+        for k in range(depth):
+            hea_layer(c, n, thetas, it)
+            q = k % (n - 2)
+            c.append(QuantumGate((q + 1, q + 2, q + 3), toffoli_matrix()))
+        out.append((c, heisenberg(n, J), n))
+    return out
+
+
 def config3(seed=DEFAULT_SEED, n=12, depth=6, p=0.01, gamma=0.005):
     """HEA layers with Depolarizing(p) on each CNOT's target after the CNOT and
     AmplitudeDamping(gamma) on every qubit at the end of each layer; observable
