@@ -48,7 +48,8 @@ struct BInst {
     int32_t rev_begin;
     int32_t row_begin;   // first global partial row
     int32_t nrows;
-    int32_t pad;
+    int32_t pf_n;        // small: pool range [pf_lo, pf_lo + pf_n) of every micro-op's matrices (0: not read ahead)
+    int64_t pf_lo;
 };
 
 struct BatchArgs {
@@ -181,6 +182,7 @@ __device__ __forceinline__ void batch_loop(const BatchArgs& a, int& s_item, int&
             const BInst in = a.inst[it.inst];
             if (it.kind == kItemSmall) {
                 const uint32_t T = 1u << in.nbits;
+                warm_ops(a.mops, in.mop_begin, in.mop_end, a.pool, in.pf_lo, in.pf_n, smem + 2 * T);
                 small_eval_body<MAXKB, RKS>(in.nbits, nullptr, nullptr, a.mops, in.mop_begin, in.ham_begin, in.rev_begin, in.mop_end, a.pool, smem,
                                 smem + T, smem + 2 * T, smem + 2 * T + kRedDoubles2, in.nrows,
                                 a.parts + a.row_off[in.row_begin], 1);
@@ -191,6 +193,7 @@ __device__ __forceinline__ void batch_loop(const BatchArgs& a, int& s_item, int&
                 double2* sb = p.kind == PH_FWD ? smem : smem + T;
                 double2* red = p.kind == PH_FWD ? smem + T : smem + 2 * T;
                 double2* gacc = red + kRedDoubles2;
+                warm_ops(a.mops, p.op_begin, p.op_end, a.pool, p.pf_lo, p.pf_n, red);
                 for (int i = threadIdx.x; i < p.nacc; i += kTileThreads) gacc[i] = make_double2(0.0, 0.0);
                 __syncthreads();
                 AccState st;
@@ -471,6 +474,7 @@ int vqc_batch_build(int ninst, const vqc_instance_t* inst, const vqc_op_t* ops, 
                 bi.rev_begin = bi.mop_begin + rb;
                 pb.mops.insert(pb.mops.end(), sp.mops.begin(), sp.mops.end());
                 bi.mop_end = int(pb.mops.size());
+                vqc_prefetch_range(pb.mops, bi.mop_begin, bi.mop_end, xp.size(), &bi.pf_lo, &bi.pf_n);
                 bi.nrows = sp.nrows;
                 for (int r : sp.row_slot) pb.row_slot.push_back(r);
                 pb.nrows += sp.nrows;

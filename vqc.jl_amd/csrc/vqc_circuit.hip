@@ -819,6 +819,53 @@ static std::vector<MicroOp> fuse_dense(const std::vector<MicroOp>& ph, XPool& xp
     return out;
 }
 
+// The pool range a phase's micro-ops read matrices from (PhaseDesc::pf_lo,
+// pf_n), for the kernels to read ahead once per phase: every micro-op otherwise
+// waits for its descriptor and then for its matrices, two dependent round trips
+// to memory, and on small tiles those waits are most of the micro-op's time.
+// Sizes are upper bounds (4^kb entries per matrix, 16 for register-block steps),
+// clamped to the pool; ranges over kPrefetchMax entries are not read ahead.
+void vqc_prefetch_range(const std::vector<MicroOp>& mops, int op_begin, int op_end, int64_t plen, int64_t* pf_lo, int32_t* pf_n) {
+    int64_t lo = INT64_MAX, hi = 0;
+    auto use = [&](int64_t off, int64_t n) {
+        if (off < 0) return;
+        lo = std::min(lo, off);
+        hi = std::max(hi, off + n);
+    };
+    for (int o = op_begin; o < op_end; ++o) {
+        const MicroOp& m = mops[o];
+        const int64_t D2 = int64_t(1) << (2 * m.kb);
+        switch (m.kind) {
+            case MK_CHAIN:
+            case MK_BLOCK:
+                for (int q = 0; q < 4; ++q) {
+                    use(m.cmat[q], 16);
+                    use(m.gen[q], 16);
+                }
+                break;
+            case MK_GRAD:
+                for (int q = 0; q < m.ng; ++q) use(m.gen[q], D2);
+                break;
+            default:  // MK_APPLY, MK_APPLY2, MK_APPLY2X, MK_HAM, MK_DENSE
+                use(m.mat, D2);
+                use(m.mat2, D2);
+                if (m.kind == MK_APPLY2)
+                    for (int q = 0; q < m.ng; ++q) use(m.gen[q], D2);
+                break;
+        }
+    }
+    hi = std::min(hi, plen);
+    *pf_lo = 0;
+    *pf_n = 0;
+    if (lo < hi && hi - lo <= kPrefetchMax) {
+        *pf_lo = lo;
+        *pf_n = int32_t(hi - lo);
+    }
+}
+static void set_prefetch(PhaseDesc& p, const std::vector<MicroOp>& mops, int64_t plen) {
+    vqc_prefetch_range(mops, p.op_begin, p.op_end, plen, &p.pf_lo, &p.pf_n);
+}
+
 void vqc_fuse_chains(PlanBuilder& pb, size_t phase_begin, size_t phase_end) {
     // Phases own contiguous, ascending micro-op ranges; the range's phases are
     // rewritten in place and later phases are shifted.
@@ -911,6 +958,7 @@ void vqc_fuse_chains(PlanBuilder& pb, size_t phase_begin, size_t phase_end) {
         pb.phases[i].op_end += shift;
     }
     pb.mops.swap(out);
+    for (size_t i = phase_begin; i < phase_end; ++i) set_prefetch(pb.phases[i], pb.mops, xp.size());
 }
 
 int vqc_plan_forward(PlanBuilder& pb, int nbits, int n, int is_dm, int nops, const vqc_op_t* ops, int t, int mlow) {
@@ -1101,6 +1149,7 @@ __global__ void __launch_bounds__(kTileThreads, MAXKB <= 3 ? 4 : 1) k_phase(cons
     double2* sb = p.kind == PH_FWD ? smem : smem + T;
     double2* red = p.kind == PH_FWD ? smem + T : smem + 2 * T;
     double2* gacc = red + kRedDoubles2;
+    warm_ops(b.ops, p.op_begin, p.op_end, b.pool, p.pf_lo, p.pf_n, red);
     for (int i = threadIdx.x; i < p.nacc; i += kTileThreads) gacc[i] = make_double2(0.0, 0.0);
     __syncthreads();
     AccState st;
@@ -1127,12 +1176,15 @@ struct SmallArgs {
     const MicroOp* ops;
     const double2* pool;
     double2* rows;      // [nrows] row sums (one column)
+    int64_t pf_lo;      // pool range every micro-op's matrices lie in (pf_n = 0: not read ahead)
+    int32_t pf_n;
 };
 
 template <int MAXKB, int RKS>
 __global__ void __launch_bounds__(kTileThreads) k_small_eval(SmallArgs a) {
     extern __shared__ double2 smem[];
     const uint32_t T = 1u << a.nbits;
+    warm_ops(a.ops, 0, a.nops, a.pool, a.pf_lo, a.pf_n, smem + 2 * T);
     small_eval_body<MAXKB, RKS>(a.nbits, a.src, a.A, a.ops, 0, a.ham_begin, a.rev_begin, a.nops, a.pool, smem, smem + T, smem + 2 * T,
                     smem + 2 * T + kRedDoubles2, a.nrows, a.rows, 1);
 }
@@ -1802,7 +1854,8 @@ static int sv_adjoint(const vqc_state* psi0, int nops, const vqc_op_t* ops, cons
             const int ham_begin = e->info[1], rev_begin = e->info[2], mk = e->info[3], ks = e->info[4];
             VQC_TRY(ensure(g_wrows, size_t(std::max(pb.nrows, 1)) * 16));
             SmallArgs a{nb, int(pb.mops.size()), ham_begin, rev_begin, pb.nrows, psi0->d, fwd_out, (const MicroOp*)g_wmops.buf,
-                        (const double2*)g_wpool.buf, (double2*)g_wrows.buf};
+                        (const double2*)g_wpool.buf, (double2*)g_wrows.buf, 0, 0};
+            vqc_prefetch_range(pb.mops, 0, int(pb.mops.size()), pb.xp->size(), &a.pf_lo, &a.pf_n);
             VQC_TRY(set_lds_attrs());
             const size_t lds = (2 * L + kRedDoubles2 + size_t(std::max(pb.nrows, 1))) * 16;
 #define VQC_SMALL(M, K) \

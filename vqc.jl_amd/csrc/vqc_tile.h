@@ -102,8 +102,14 @@ struct PhaseDesc {
     int32_t row_begin;   // first partial row of those entries
     int32_t ham_accum;   // PH_HAM: 1 = add onto B already in HBM, 0 = overwrite
     int32_t flags;       // PF_INIT_ZERO: the phase reads |0...0> instead of srcA
+    // pool entries [pf_lo, pf_lo + pf_n) cover every matrix the phase's micro-ops
+    // read (pf_n = 0: too spread out to be worth reading ahead, see warm_phase)
+    int32_t pf_n;
+    int64_t pf_lo;
 };
 enum { PF_INIT_ZERO = 1 };
+// largest pool range (complex entries) a phase reads ahead: 64 KiB
+constexpr int kPrefetchMax = 4096;
 
 template <int K>
 __device__ __forceinline__ uint32_t ins32(uint32_t i, const int32_t* p) {
@@ -171,9 +177,13 @@ __device__ __forceinline__ void wave_partial_re(double2* red, int buf, int q, do
 
 // fold pending partials into gacc (callers order it with barriers). One thread
 // folds them in turn: two partials of one micro-op may belong to the same row
-// (the row and column terms of a density-matrix gradient).
+// (the row and column terms of a density-matrix gradient). The folding thread is
+// lane 0 of the last wave: micro-ops hand out groups from thread 0 up, so on
+// small tiles (a 9-bit tile holds 128 groups of a 2-bit micro-op: waves 0 and 1)
+// the last wave has no group of its own and the fold runs beside the micro-op
+// instead of before it.
 __device__ __forceinline__ void acc_fold(AccState& st, const double2* red, double2* gacc) {
-    if (st.pend_n && threadIdx.x == 0) {
+    if (st.pend_n && threadIdx.x == kTileThreads - 64) {
         for (int q = 0; q < st.pend_n; ++q) {
             const int e = q == 0 ? st.e0 : (q == 1 ? st.e1 : (q == 2 ? st.e2 : st.e3));
             double2 s = gacc[e];
@@ -1008,6 +1018,25 @@ __device__ __forceinline__ void acc_finish(AccState& st, const double2* red, dou
     __syncthreads();
 }
 
+// Read ahead what micro-ops [op_begin, op_end) will read: their descriptors and
+// the pool range [pf_lo, pf_lo + pf_n). Each micro-op reads its descriptor and
+// then its matrices with scalar loads, two dependent trips to memory per
+// micro-op; issued here together by all threads as vector loads, they bring
+// every such line into the L2 in one trip. The values are written to `sink`
+// (64 entries of LDS that the caller overwrites before reading them), so that
+// the loads are kept.
+__device__ __forceinline__ void warm_ops(const MicroOp* __restrict__ ops, int op_begin, int op_end, const double2* __restrict__ pool,
+                                         int64_t pf_lo, int32_t pf_n, double2* sink) {
+    double s = 0.0;
+    const double2* __restrict__ q = pool + pf_lo;
+    for (int i = threadIdx.x; i < pf_n; i += kTileThreads) s += q[i].x;
+    // descriptors as raw 8-byte words (sizeof(MicroOp) is a multiple of 8)
+    const double* __restrict__ od = reinterpret_cast<const double*>(ops + op_begin);
+    const int on = (op_end - op_begin) * int(sizeof(MicroOp) / 8);
+    for (int i = threadIdx.x; i < on; i += kTileThreads) s += od[i];
+    sink[threadIdx.x & 63] = make_double2(s, 0.0);
+}
+
 // Stream one tile of phase p through LDS.
 //  PH_FWD: sa <- srcA tile; ops; dstA tile <- sa
 //  PH_REV: sa <- A, sb <- B; ops; A <- sa, B <- sb
@@ -1189,5 +1218,8 @@ void vqc_fuse_chains(PlanBuilder& pb, size_t phase_begin, size_t phase_end);
 inline int vqc_phase_op_start(const PlanBuilder& pb, size_t i) {
     return i < pb.phases.size() ? pb.phases[i].op_begin : int(pb.mops.size());
 }
+// The pool range [*pf_lo, *pf_lo + *pf_n) holding every matrix that micro-ops
+// [op_begin, op_end) read (*pf_n = 0 when it spans more than kPrefetchMax entries).
+void vqc_prefetch_range(const std::vector<MicroOp>& mops, int op_begin, int op_end, int64_t plen, int64_t* pf_lo, int32_t* pf_n);
 // micro-ops a KS_REV kernel runs
 inline bool vqc_rev_simple(const MicroOp& m) { return m.kind == MK_APPLY2 || m.kind == MK_CHAIN || m.kind == MK_BLOCK; }
