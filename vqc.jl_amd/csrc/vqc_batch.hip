@@ -136,6 +136,15 @@ __device__ __forceinline__ void batch_loop(const BatchArgs& a, int& s_item, int&
     // only lane that would dequeue and signal) leaves it while the other lanes
     // keep re-running the body with a stale item.
     const bool wave0 = (__builtin_amdgcn_readfirstlane(threadIdx.x) >> 6) == 0;
+    // items, instances, phases, micro-ops, the pool and the row offsets are
+    // read-only for the whole launch (the pool's variants and products are made
+    // by the kernels before this one): constant memory, read with scalar loads
+    const MicroOp* const mops = (const MicroOp*)as_const(a.mops);
+    const double2* const pool = (const double2*)as_const(a.pool);
+    const int64_t* const row_off = (const int64_t*)as_const(a.row_off);
+    const BItem* const items = (const BItem*)as_const(a.items);
+    const BInst* const insts = (const BInst*)as_const(a.inst);
+    const PhaseDesc* const phases = (const PhaseDesc*)as_const(a.phases);
     const unsigned one = (threadIdx.x & 63) == 0 ? 1u : 0u;
     for (;;) {
         if (wave0) s_item = int(__builtin_amdgcn_readfirstlane(atomicAdd(&a.counters[0], one)));
@@ -143,13 +152,17 @@ __device__ __forceinline__ void batch_loop(const BatchArgs& a, int& s_item, int&
         const int idx = __builtin_amdgcn_readfirstlane(s_item);
         __syncthreads();
         if (idx >= a.nitems) break;
-        const BItem it = a.items[idx];
+        const BItem it = items[idx];
         if (wave0) {
             int ok = 1;
             const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+            // The polls are relaxed: an acquire load invalidates this XCD's L2 on
+            // every poll, and hundreds of waiting workgroups polling that way kept
+            // the L2 of every XCD cold for the workgroups doing the work. One
+            // acquire fence after the wait orders this item after its producers.
             for (;;) {
                 const unsigned done = __builtin_amdgcn_readfirstlane(
-                    __hip_atomic_load(&a.counters[2 + it.inst], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+                    __hip_atomic_load(&a.counters[2 + it.inst], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
                 if (done >= unsigned(it.need)) break;
                 if (__builtin_amdgcn_s_memrealtime() - t0 > kWaitTicks) {
                     ok = 0;
@@ -179,21 +192,21 @@ __device__ __forceinline__ void batch_loop(const BatchArgs& a, int& s_item, int&
         __syncthreads();
         const int ok = __builtin_amdgcn_readfirstlane(s_ok);
         if (ok) {
-            const BInst in = a.inst[it.inst];
+            const BInst in = insts[it.inst];
             if (it.kind == kItemSmall) {
                 const uint32_t T = 1u << in.nbits;
-                warm_ops(a.mops, in.mop_begin, in.mop_end, a.pool, in.pf_lo, in.pf_n, smem + 2 * T);
-                small_eval_body<MAXKB, RKS>(in.nbits, nullptr, nullptr, a.mops, in.mop_begin, in.ham_begin, in.rev_begin, in.mop_end, a.pool, smem,
+                warm_ops(mops, in.mop_begin, in.mop_end, pool, in.pf_lo, in.pf_n, smem + 2 * T);
+                small_eval_body<MAXKB, RKS>(in.nbits, nullptr, nullptr, mops, in.mop_begin, in.ham_begin, in.rev_begin, in.mop_end, pool, smem,
                                 smem + T, smem + 2 * T, smem + 2 * T + kRedDoubles2, in.nrows,
-                                a.parts + a.row_off[in.row_begin], 1);
+                                a.parts + row_off[in.row_begin], 1);
             } else {
-                const PhaseDesc& p = a.phases[it.phase];
+                const PhaseDesc& p = phases[it.phase];
                 const uint32_t T = 1u << p.t;
                 double2* sa = smem;
                 double2* sb = p.kind == PH_FWD ? smem : smem + T;
                 double2* red = p.kind == PH_FWD ? smem + T : smem + 2 * T;
                 double2* gacc = red + kRedDoubles2;
-                warm_ops(a.mops, p.op_begin, p.op_end, a.pool, p.pf_lo, p.pf_n, red);
+                warm_ops(mops, p.op_begin, p.op_end, pool, p.pf_lo, p.pf_n, red);
                 for (int i = threadIdx.x; i < p.nacc; i += kTileThreads) gacc[i] = make_double2(0.0, 0.0);
                 __syncthreads();
                 AccState st;
@@ -202,25 +215,27 @@ __device__ __forceinline__ void batch_loop(const BatchArgs& a, int& s_item, int&
                 double2* B = a.arena + in.b_off;
                 if (p.kind == PH_FWD) {
                     for (int64_t tile = it.tile_begin; tile < it.tile_end; ++tile)
-                        phase_tile<MAXKB, KS_FWDH>(p, uint64_t(tile), A, A, B, a.mops, a.pool, sa, sb, red, gacc, st);
+                        phase_tile<MAXKB, KS_FWDH>(p, uint64_t(tile), A, A, B, mops, pool, sa, sb, red, gacc, st);
                 } else if (p.kind == PH_HAM) {
                     for (int64_t tile = it.tile_begin; tile < it.tile_end; ++tile)
-                        phase_tile<MAXKB, KS_HAM>(p, uint64_t(tile), A, A, B, a.mops, a.pool, sa, sb, red, gacc, st);
+                        phase_tile<MAXKB, KS_HAM>(p, uint64_t(tile), A, A, B, mops, pool, sa, sb, red, gacc, st);
                 } else {
                     for (int64_t tile = it.tile_begin; tile < it.tile_end; ++tile)
-                        phase_tile<MAXKB, RKS>(p, uint64_t(tile), A, A, B, a.mops, a.pool, sa, sb, red, gacc, st);
+                        phase_tile<MAXKB, RKS>(p, uint64_t(tile), A, A, B, mops, pool, sa, sb, red, gacc, st);
                 }
                 acc_finish(st, red, gacc);
                 for (int e = threadIdx.x; e < p.nacc; e += kTileThreads)
-                    a.parts[a.row_off[p.row_begin + e] + it.chunk] = gacc[e];
+                    a.parts[row_off[p.row_begin + e] + it.chunk] = gacc[e];
             }
         }
         // the barrier makes every wave's stores complete in L2; one agent-scope
-        // release (L2 write-back) by wave 0 then publishes them to other XCDs
+        // release fence (L2 write-back) by wave 0 then publishes them to other
+        // XCDs, and the relaxed increment after it carries that release (a
+        // release increment would write the L2 back a second time)
         __syncthreads();
         if (wave0) {
             __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-            __hip_atomic_fetch_add(&a.counters[2 + it.inst], one, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_fetch_add(&a.counters[2 + it.inst], one, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         __syncthreads();
     }

@@ -1018,6 +1018,18 @@ __device__ __forceinline__ void acc_finish(AccState& st, const double2* red, dou
     __syncthreads();
 }
 
+// Constant-memory view of a buffer that no thread writes while the kernel runs
+// (micro-ops, phases, the matrix pool once the variant kernels are done). A
+// kernel with fences and atomics (the batch work queue) otherwise makes the
+// compiler treat every read as possibly clobbered, so uniform reads become
+// vector loads held in VGPRs; reads through these pointers are scalar loads.
+template <class T>
+using const_ptr = const __attribute__((address_space(4))) T*;
+template <class T>
+__device__ __forceinline__ const_ptr<T> as_const(const T* p) {
+    return (const_ptr<T>)(uintptr_t)p;
+}
+
 // Read ahead what micro-ops [op_begin, op_end) will read: their descriptors and
 // the pool range [pf_lo, pf_lo + pf_n). Each micro-op reads its descriptor and
 // then its matrices with scalar loads, two dependent trips to memory per
