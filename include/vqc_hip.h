@@ -42,6 +42,7 @@ extern "C" {
 #define VQC_ENOMEM       -2   /* device allocation failed or budget exceeded */
 #define VQC_EHIP         -3   /* a HIP runtime call failed */
 #define VQC_EUNSUPPORTED -4   /* valid request this build does not handle */
+#define VQC_EINTERNAL    -5   /* an internal consistency check failed (vqc_plan_check) */
 
 /* ---- named gates (src/applygates/specific/) ------------------------------- */
 #define VQC_GATE_X        1   /* phasegate.jl:27-43   pos[0] */
@@ -328,6 +329,15 @@ typedef struct vqc_plan_info {
     int32_t reused;
 } vqc_plan_info_t;
 int vqc_plan_info(vqc_plan_info_t* out);
+/* Host only, no reference counterpart (the reference has no planner): plans the
+ * fused energy + gradient of a program as vqc_sv_energy_grad (is_dm = 0) or
+ * vqc_dm_energy_grad (is_dm = 1) would for an nqubits-qubit state, checks the
+ * plan's invariants (contiguous phases, tile bits partitioning the index bits,
+ * micro-op bits inside their tile, matrices inside the pool, accumulators inside
+ * their rows) and writes its shape to *out (reused = 0). Makes no device call
+ * and leaves the plan cache alone. VQC_EINTERNAL names a violated invariant. */
+int vqc_plan_check(int is_dm, int nqubits, int nops, const vqc_op_t* ops, const double* pool, int nterms,
+                   const vqc_term_t* terms, const double* hpool, int nparams, int flags, vqc_plan_info_t* out);
 
 /* Energy E = sum_keys <psi|M|psi> with psi = circuit * psi0 and the adjoint
  * gradient dRe(E)/dtheta for every active parameter. Replaces Zygote.gradient(

@@ -16,7 +16,8 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB = os.path.join(HERE, "liboracle.so")
+# VQC_ORACLE_LIB: another build of the same sources (the sanitizer build)
+LIB = os.environ.get("VQC_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
 _lib = None

@@ -67,5 +67,62 @@ def build(verbose=False):
     return LIB
 
 
+# Host-code sanitizer build (CPU tests only: tests/test_sanitizers.py). The
+# translation units holding host logic -- the planner (vqc_circuit), the batch
+# planner and work queue set-up (vqc_batch), the tuning tables (vqc_core) and the
+# serving scheduler (vqc_serve) -- are compiled with AddressSanitizer and
+# UndefinedBehaviorSanitizer on their host side; their device code is built at
+# -O1 (it is never launched from this library: -O0 overflows the kernels' stack
+# frames) and the other units are linked as in the regular build.
+SAN_DIR = os.path.join(OBJ, "san")
+SAN_LIB = os.path.join(SAN_DIR, "libvqc_hip_san.so")
+SAN_UNITS = ("vqc_batch", "vqc_circuit", "vqc_core", "vqc_serve")
+SAN_FLAGS = ["--offload-arch=gfx950", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Xarch_host", "-O1", "-Xarch_host", "-g", "-Xarch_host", "-fno-omit-frame-pointer",
+             "-Xarch_host", "-fsanitize=address,undefined", "-Xarch_host", "-fno-sanitize-recover=undefined",
+             "-Xarch_device", "-O1"]
+
+
+def _compile_san(src):
+    obj = os.path.join(SAN_DIR, os.path.basename(src)[:-4] + ".o")
+    r = subprocess.run([HIPCC] + SAN_FLAGS + ["-c", src, "-o", obj], capture_output=True, text=True)
+    return src, obj, r.returncode, r.stdout + r.stderr
+
+
+def compile_san():
+    """The sanitized objects (independent of build(), so both can run at once)."""
+    os.makedirs(SAN_DIR, exist_ok=True)
+    srcs = [os.path.join(CSRC, u + ".hip") for u in SAN_UNITS]
+    objs = []
+    with concurrent.futures.ThreadPoolExecutor(max_workers=len(srcs)) as ex:
+        for src, obj, rc, out in ex.map(_compile_san, srcs):
+            if rc != 0:
+                raise RuntimeError(f"sanitizer build of {os.path.basename(src)} failed:\n{out}")
+            objs.append(obj)
+    return objs
+
+
+def link_san(san_objs):
+    """libvqc_hip_san.so from the sanitized objects and the regular ones (after build())."""
+    objs = [os.path.join(OBJ, os.path.basename(s)[:-4] + ".o") for s in sources() if os.path.basename(s)[:-4] not in SAN_UNITS]
+    objs += san_objs
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-fsanitize=address,undefined", "-shared-libasan",
+           "-o", SAN_LIB] + objs + ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("sanitizer link failed:\n" + r.stdout + r.stderr)
+    return SAN_LIB
+
+
+def build_all():
+    """build() and the sanitizer build, compiled side by side."""
+    with concurrent.futures.ThreadPoolExecutor(max_workers=1) as ex:
+        san = ex.submit(compile_san)
+        lib = build()
+        link_san(san.result())
+    return lib
+
+
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))

@@ -196,26 +196,27 @@ static int hint_gen(const vqc_op_t& o) {
 // are summed) drops the dependence test.
 constexpr int kLookahead = 128;
 
-// One phase's scan with the high tile bits fixed to inS (bits below mlow are
-// always in the tile): an item is taken when its high bits are in the set, no
-// bit of it belongs to an item skipped before, and the accumulators allow it.
-static size_t scan_fixed(const std::vector<int>& rest, const std::vector<std::vector<int>>& bits, int nbits, int mlow, int a,
-                         const std::vector<char>& inS, std::vector<char>& blocked, std::vector<int>* take,
-                         std::vector<int>* keep, size_t* idx_out, std::vector<int>* cur) {
-    std::fill(blocked.begin(), blocked.end(), 0);
-    int nblocked = 0, acc = 0;
+// One phase's scan with the high tile bits fixed to the set S (bits below mlow
+// are always in the tile): an item is taken when its high bits are in S, no bit
+// of it belongs to an item skipped before, and the accumulators allow it. Items
+// and sets are bit masks (at most 64 index bits), so a scan is a few operations
+// per item: the coverage choice below runs thousands of them per phase.
+static size_t scan_fixed(const std::vector<int>& rest, const std::vector<uint64_t>& mask,
+                         const std::vector<std::vector<int>>& bits, int nbits, int mlow, int a, uint64_t S,
+                         std::vector<int>* take, std::vector<int>* keep, size_t* idx_out, std::vector<int>* cur) {
+    const uint64_t all = nbits >= 64 ? ~uint64_t(0) : (uint64_t(1) << nbits) - 1;
+    const uint64_t hi = all & ~((uint64_t(1) << mlow) - 1);
+    uint64_t blocked = 0;
+    int acc = 0;
     size_t cnt = 0, idx = 0;
-    for (; idx < rest.size() && int(idx) < kLookahead && nblocked < nbits; ++idx) {
+    for (; idx < rest.size() && int(idx) < kLookahead && blocked != all; ++idx) {
         const int j = rest[idx];
-        const std::vector<int>& b = bits[j];
-        bool ok = acc + a <= kMaxAcc;
-        for (int x : b)
-            if (blocked[x] || (x >= mlow && !inS[x])) ok = false;
-        if (ok) {
+        const uint64_t m = mask[j];
+        if (acc + a <= kMaxAcc && !(m & blocked) && !(m & hi & ~S)) {
             ++cnt;
             if (take) take->push_back(j);
             if (cur)
-                for (int x : b)
+                for (int x : bits[j])
                     if (std::find(cur->begin(), cur->end(), x) == cur->end()) cur->push_back(x);
             acc += a;
             if (a > 0 && acc + a > kMaxAcc) {
@@ -224,61 +225,63 @@ static size_t scan_fixed(const std::vector<int>& rest, const std::vector<std::ve
             }
         } else {
             if (keep) keep->push_back(j);
-            for (int x : b)
-                if (!blocked[x]) {
-                    blocked[x] = 1;
-                    ++nblocked;
-                }
+            blocked |= m;
         }
     }
     if (idx_out) *idx_out = idx;
     return cnt;
 }
 
-static size_t cover_scan(const std::vector<int>& rest, const std::vector<std::vector<int>>& bits, int nbits, int t, int mlow,
-                         int a, std::vector<char>& blocked, std::vector<int>& take, std::vector<int>& keep, size_t& idx,
-                         std::vector<int>& cur) {
+// Coverage choice of a phase's high tile bits: starting from none, repeatedly add
+// the move (the missing high bits of one of the next 64 items) that makes the
+// most items of the window executable per added bit, while the tile has room.
+static size_t cover_scan(const std::vector<int>& rest, const std::vector<uint64_t>& mask,
+                         const std::vector<std::vector<int>>& bits, int nbits, int t, int mlow, int a,
+                         std::vector<int>& take, std::vector<int>& keep, size_t& idx, std::vector<int>& cur) {
     const int budget = t - std::min(mlow, t);
-    std::vector<char> inS(size_t(nbits), 0);
+    const uint64_t all = nbits >= 64 ? ~uint64_t(0) : (uint64_t(1) << nbits) - 1;
+    const uint64_t hi = all & ~((uint64_t(1) << mlow) - 1);
+    uint64_t S = 0;
     int used = 0;
-    size_t base = scan_fixed(rest, bits, nbits, mlow, a, inS, blocked, nullptr, nullptr, nullptr, nullptr);
+    size_t base = scan_fixed(rest, mask, bits, nbits, mlow, a, S, nullptr, nullptr, nullptr, nullptr);
     const size_t W = std::min<size_t>(rest.size(), 64);
+    std::vector<uint64_t> seen;
     for (;;) {
-        std::vector<int> best;
+        uint64_t best = 0;
         double best_score = 0;
         size_t best_cnt = base;
-        std::vector<std::vector<int>> seen;
+        seen.clear();
         for (size_t w = 0; w < W; ++w) {
-            std::vector<int> mv;
-            for (int x : bits[rest[w]])
-                if (x >= mlow && !inS[x]) mv.push_back(x);
-            if (mv.empty() || used + int(mv.size()) > budget) continue;
-            std::sort(mv.begin(), mv.end());
+            const uint64_t mv = mask[rest[w]] & hi & ~S;
+            const int nmv = __builtin_popcountll(mv);
+            if (!mv || used + nmv > budget) continue;
             if (std::find(seen.begin(), seen.end(), mv) != seen.end()) continue;
             seen.push_back(mv);
-            for (int x : mv) inS[x] = 1;
-            const size_t c = scan_fixed(rest, bits, nbits, mlow, a, inS, blocked, nullptr, nullptr, nullptr, nullptr);
-            for (int x : mv) inS[x] = 0;
-            const double score = double(c - base) / double(mv.size());
+            const size_t c = scan_fixed(rest, mask, bits, nbits, mlow, a, S | mv, nullptr, nullptr, nullptr, nullptr);
+            const double score = double(c - base) / double(nmv);
             if (c > base && score > best_score) {
                 best = mv;
                 best_score = score;
                 best_cnt = c;
             }
         }
-        if (best.empty()) break;
-        for (int x : best) inS[x] = 1;
-        used += int(best.size());
+        if (!best) break;
+        S |= best;
+        used += __builtin_popcountll(best);
         base = best_cnt;
     }
-    return scan_fixed(rest, bits, nbits, mlow, a, inS, blocked, &take, &keep, &idx, &cur);
+    return scan_fixed(rest, mask, bits, nbits, mlow, a, S, &take, &keep, &idx, &cur);
 }
 
 template <class BitsFn, class EmitFn>
 static void plan_groups(PlanBuilder& pb, int kind, int nbits, int nitems, int t, int mlow, int max_acc_per_item,
                         BitsFn bits_of, EmitFn emit, bool all_commute = false) {
     std::vector<std::vector<int>> bits(nitems);
-    for (int j = 0; j < nitems; ++j) bits[j] = bits_of(j);
+    std::vector<uint64_t> mask(nitems, 0);
+    for (int j = 0; j < nitems; ++j) {
+        bits[j] = bits_of(j);
+        for (int x : bits[j]) mask[j] |= uint64_t(1) << x;
+    }
     const int a = std::max(max_acc_per_item, 0);
     std::vector<int> rest(nitems);
     for (int j = 0; j < nitems; ++j) rest[j] = j;
@@ -322,7 +325,7 @@ static void plan_groups(PlanBuilder& pb, int kind, int nbits, int nitems, int t,
             // It replaces the in-order first fit when it runs more items.
             std::vector<int> take2, keep2, cur2;
             size_t idx2 = 0;
-            if (cover_scan(rest, bits, nbits, t, mlow, a, blocked, take2, keep2, idx2, cur2) > take.size()) {
+            if (cover_scan(rest, mask, bits, nbits, t, mlow, a, take2, keep2, idx2, cur2) > take.size()) {
                 take.swap(take2);
                 keep.swap(keep2);
                 cur.swap(cur2);
@@ -1952,6 +1955,28 @@ static int dm_seed(double2* Dl, int n, const vqc_term_t& t, const double2* M, hi
 
 int vqc_dm_gate_sweep(double2* v, int n, int k, const int32_t* pos, const double* U, hipStream_t st);
 
+// The fused density-matrix adjoint's plan (vec(rho) on 2n index bits): forward
+// phases, then the reverse ones; rows 0..nterms-1 are the energies, which the
+// trace kernels compute.
+static int plan_dm(PlanEntry& e, int n, int nops, const vqc_op_t* ops, int nterms, const CkptCut& cut) {
+    const int nb = 2 * n;
+    const bool cuts = cut.begin.size() > 1 || cut.chan[0];
+    const TileTuning tt = vqc_tile_tuning(nb);
+    PlanBuilder& np = e.pb;
+    np.dense_fwd = true;
+    if (cuts) VQC_TRY(plan_forward_cut(e, cut, nb, n, 1, nops, ops, tt.t_fwd, tt.mlow));
+    else VQC_TRY(vqc_plan_forward(np, nb, n, 1, nops, ops, tt.t_fwd, tt.mlow));
+    vqc_fuse_chains(np, 0, np.phases.size());
+    const size_t nf = np.phases.size();
+    np.nrows = nterms;
+    for (int t = 0; t < nterms; ++t) np.row_slot.push_back(-1 - t);
+    if (cuts) VQC_TRY(plan_reverse_cut(e, cut, nb, n, 1, nops, ops, tt.t_rev, tt.mlow));
+    else VQC_TRY(vqc_plan_reverse(np, nb, n, 1, nops, ops, tt.t_rev, tt.mlow));
+    vqc_fuse_chains(np, nf, np.phases.size());
+    e.info[0] = int(nf);
+    return VQC_OK;
+}
+
 // The density-matrix adjoint sweep; `seed` / `seed_out` as for sv_adjoint
 // (the seed replaces Delta = H^dagger).
 static int dm_adjoint(const vqc_state* rho0, int nops, const vqc_op_t* ops, const double* pool, int nterms,
@@ -2001,24 +2026,11 @@ static int dm_adjoint(const vqc_state* rho0, int nops, const vqc_op_t* ops, cons
                 }
     } else {
         const CkptCut cut = ckpt_cut(nops, ops, pool, 1, flags);
-        const bool cuts = cut.begin.size() > 1 || cut.chan[0];
         std::string key = plan_key('D', nb, n, seed ? 1 : 0, nparams, nops, ops, plen, nterms, terms, hlen) + cut_key(cut);
         e = plan_find(key, pool, plen, hpool, hlen);
         if (!e) {
-            const TileTuning tt = vqc_tile_tuning(nb);
             std::unique_ptr<PlanEntry> ne = plan_make(std::move(key), pool, plen, hpool, hlen);
-            PlanBuilder& np = ne->pb;
-            np.dense_fwd = true;
-            if (cuts) VQC_TRY(plan_forward_cut(*ne, cut, nb, n, 1, nops, ops, tt.t_fwd, tt.mlow));
-            else VQC_TRY(vqc_plan_forward(np, nb, n, 1, nops, ops, tt.t_fwd, tt.mlow));
-            vqc_fuse_chains(np, 0, np.phases.size());
-            const size_t nf = np.phases.size();
-            np.nrows = nterms;  // rows 0..nterms-1: energies (DM energies come from the trace kernels)
-            for (int t = 0; t < nterms; ++t) np.row_slot.push_back(-1 - t);
-            if (cuts) VQC_TRY(plan_reverse_cut(*ne, cut, nb, n, 1, nops, ops, tt.t_rev, tt.mlow));
-            else VQC_TRY(vqc_plan_reverse(np, nb, n, 1, nops, ops, tt.t_rev, tt.mlow));
-            vqc_fuse_chains(np, nf, np.phases.size());
-            ne->info[0] = int(nf);
+            VQC_TRY(plan_dm(*ne, n, nops, ops, nterms, cut));
             e = plan_store(std::move(ne));
         }
     }
@@ -2125,6 +2137,123 @@ extern "C" int vqc_dm_energy_grad(const vqc_state* rho0, int nops, const vqc_op_
                                   vqc_state* rho_out, int flags) {
     VQC_API_LOCK;
     return dm_adjoint(rho0, nops, ops, pool, nterms, terms, hpool, nparams, energy2, grad, rho_out, flags, nullptr, nullptr);
+}
+
+// ------------------------------- plan checks --------------------------------
+// Invariants of a finished plan on nb index bits over a pool of plen complex
+// entries; the first violation is described in *why.
+static bool plan_invariants(const PlanEntry& e, int nb, int nterms, int nparams, std::string* why) {
+    const PlanBuilder& pb = e.pb;
+    const int64_t plen = e.xp.size();
+    char buf[256];
+    auto fail = [&](const char* fmt, size_t i, int o, long long a, long long b) {
+        std::snprintf(buf, sizeof buf, fmt, i, o, a, b);
+        *why = buf;
+        return false;
+    };
+    const bool small = e.info[0] == 1 && pb.phases.size() && pb.phases[0].t == nb && nb <= kMaxTileBits2;
+    if (int(pb.row_slot.size()) != pb.nrows) return fail("phase %zu op %d: %lld row slots for %lld rows", 0, 0, (long long)pb.row_slot.size(), pb.nrows);
+    for (size_t r = 0; r < pb.row_slot.size(); ++r) {
+        const int sl = pb.row_slot[r];
+        if (sl >= nparams || sl < -nterms - 1) return fail("phase %zu op %d: row %lld has slot %lld", 0, 0, (long long)r, sl);
+    }
+    int prev_end = 0;
+    for (size_t i = 0; i < pb.phases.size(); ++i) {
+        const PhaseDesc& p = pb.phases[i];
+        if (p.op_begin != prev_end || p.op_end < p.op_begin || p.op_end > int(pb.mops.size()))
+            return fail("phase %zu op %d: micro-op range [%lld, %lld) is not the next one", i, prev_end, p.op_begin, p.op_end);
+        prev_end = p.op_end;
+        const int tmax = p.kind == PH_FWD ? kMaxTileBits : kMaxTileBits2;
+        if (p.nbits != nb || p.t < 1 || p.t > std::min(nb, tmax)) return fail("phase %zu op %d: %lld tile bits of %lld", i, 0, p.t, p.nbits);
+        if (p.m + p.nhb != p.t || p.nob != nb - p.t || p.m < 0) return fail("phase %zu op %d: tile split %lld + %lld", i, 0, p.m, p.nhb);
+        std::vector<char> seen(size_t(nb), 0);
+        for (int b = 0; b < p.m; ++b) seen[size_t(b)] = 1;
+        for (int k = 0; k < p.nhb; ++k) {
+            const int b = p.hb[k];
+            if (b < p.m || b >= nb || seen[size_t(b)] || (k && b <= p.hb[k - 1])) return fail("phase %zu op %d: high tile bit %lld (%lld)", i, 0, b, k);
+            seen[size_t(b)] = 1;
+        }
+        for (int k = 0; k < p.nob; ++k) {
+            const int b = p.ob[k];
+            if (b < 0 || b >= nb || seen[size_t(b)] || (k && b <= p.ob[k - 1])) return fail("phase %zu op %d: outside bit %lld (%lld)", i, 0, b, k);
+            seen[size_t(b)] = 1;
+        }
+        if (p.nacc < 0 || p.nacc > kMaxAcc || (!small && p.row_begin + p.nacc > pb.nrows))
+            return fail("phase %zu op %d: %lld accumulators from row %lld", i, 0, p.nacc, p.row_begin);
+        if (p.pf_n < 0 || p.pf_lo < 0 || p.pf_lo + p.pf_n > plen) return fail("phase %zu op %d: read-ahead range %lld + %lld", i, 0, p.pf_lo, p.pf_n);
+        for (int o = p.op_begin; o < p.op_end; ++o) {
+            const MicroOp& m = pb.mops[size_t(o)];
+            if (m.kind < MK_APPLY || m.kind > MK_DENSE || m.kb < 1 || m.kb > 5) return fail("phase %zu op %d: kind %lld on %lld bits", i, o, m.kind, m.kb);
+            for (int j = 0; j < m.kb; ++j)
+                if (m.lpos[j] < 0 || m.lpos[j] >= p.t || (j && m.lpos[j] <= m.lpos[j - 1]))
+                    return fail("phase %zu op %d: local bit %lld (%lld)", i, o, m.lpos[j], j);
+            if (m.kind == MK_DENSE && p.t - m.kb < 7) return fail("phase %zu op %d: dense block of %lld bits in a %lld-bit tile", i, o, m.kb, p.t);
+            const int64_t D2 = int64_t(1) << (2 * m.kb);
+            auto in_pool = [&](int64_t off, int64_t n) { return off >= 0 && off + n <= plen; };
+            bool ok = true;
+            switch (m.kind) {
+                case MK_CHAIN:
+                case MK_BLOCK:
+                    for (int q = 0; q < m.nchain && q < 4; ++q) {
+                        const int64_t sz = m.kind == MK_CHAIN ? 4 : 16;
+                        if (m.cmat[q] >= 0 && !in_pool(m.cmat[q], m.kind == MK_BLOCK && ((m.smask >> (2 * q)) & 3) != BS_BOTH ? 4 : sz)) ok = false;
+                        if (m.gen[q] >= 0 && !in_pool(m.gen[q], 4)) ok = false;
+                    }
+                    break;
+                case MK_GRAD:
+                    for (int q = 0; q < m.ng; ++q) ok = ok && in_pool(m.gen[q], D2);
+                    break;
+                default:
+                    ok = in_pool(m.mat, D2) && (m.mat2 < 0 || in_pool(m.mat2, D2));
+                    if (m.kind == MK_APPLY2)
+                        for (int q = 0; q < m.ng; ++q) ok = ok && in_pool(m.gen[q], D2);
+                    break;
+            }
+            if (!ok) return fail("phase %zu op %d: a matrix of kind %lld lies outside the pool of %lld entries", i, o, m.kind, plen);
+            if (m.ng < 0 || m.ng > 4) return fail("phase %zu op %d: %lld partials (%lld)", i, o, m.ng, 0);
+            for (int q = 0; q < m.ng; ++q) {
+                const int a = m.acc[q];
+                if (a < 0 || (small ? a >= pb.nrows : a >= p.nacc)) return fail("phase %zu op %d: accumulator %lld of %lld", i, o, a, small ? pb.nrows : p.nacc);
+            }
+        }
+    }
+    if (prev_end != int(pb.mops.size())) return fail("phase %zu op %d: %lld micro-ops past the last phase", pb.phases.size(), prev_end, (long long)pb.mops.size(), 0);
+    return true;
+}
+
+// Host only: plans the fused energy + gradient of a program as
+// vqc_sv_energy_grad (is_dm = 0) or vqc_dm_energy_grad (is_dm = 1) would for an
+// n-qubit state, checks the plan's invariants (contiguous phases, tile bits that
+// partition the index bits, micro-op bits inside their tile, every matrix inside
+// the pool, accumulators inside their rows) and returns its shape. It makes no
+// device call and leaves the plan cache alone, so the planner can be exercised
+// on a machine without a GPU, and under the sanitizers.
+extern "C" int vqc_plan_check(int is_dm, int nqubits, int nops, const vqc_op_t* ops, const double* pool, int nterms,
+                              const vqc_term_t* terms, const double* hpool, int nparams, int flags, vqc_plan_info_t* out) {
+    VQC_API_LOCK;
+    if (!out) return vqc_fail(VQC_EINVAL, "null plan info");
+    if (nqubits < 1 || nqubits > (is_dm ? 17 : 34)) return vqc_fail(VQC_EINVAL, "unsupported state size: %d qubits", nqubits);
+    if (nparams < 0) return vqc_fail(VQC_EINVAL, "bad parameter count");
+    const int nb = is_dm ? 2 * nqubits : nqubits;
+    int64_t plen = 0, hlen = 0;
+    VQC_TRY(check_program(nqubits, is_dm, nops, ops, &plen, nparams));
+    VQC_TRY(check_terms(nqubits, nterms, terms, &hlen));
+    if ((plen && !pool) || (hlen && !hpool)) return vqc_fail(VQC_EINVAL, "null matrix pool");
+    const CkptCut cut = ckpt_cut(nops, ops, pool, is_dm, flags);
+    std::unique_ptr<PlanEntry> e = plan_make(std::string(), pool, plen, hpool, hlen);
+    if (is_dm) VQC_TRY(plan_dm(*e, nqubits, nops, ops, nterms, cut));
+    else VQC_TRY(plan_sv(*e, nb, nops, ops, nterms, terms, plen, hlen, true, cut));
+    std::string why;
+    if (!plan_invariants(*e, nb, nterms, nparams, &why)) return vqc_fail(VQC_EINTERNAL, "plan check: %s", why.c_str());
+    memset(out, 0, sizeof *out);
+    for (const PhaseDesc& p : e->pb.phases) {
+        if (p.kind == PH_FWD) ++out->phases_fwd;
+        else if (p.kind == PH_HAM) ++out->phases_ham;
+        else ++out->phases_rev;
+    }
+    for (const MicroOp& m : e->pb.mops)
+        if (m.kind >= 0 && m.kind < 8) ++out->mops[m.kind];
+    return VQC_OK;
 }
 
 extern "C" int vqc_circuit_vjp(const vqc_state* x, int nops, const vqc_op_t* ops, const double* pool, int nparams,

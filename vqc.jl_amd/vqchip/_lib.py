@@ -9,7 +9,9 @@ import os
 import numpy as np
 
 _PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(_PKG_DIR, "libvqc_hip.so")
+# VQC_LIB_PATH: another build of the same sources (the host sanitizer build the
+# CPU tests load); the default is the in-tree library
+LIB_PATH = os.environ.get("VQC_LIB_PATH") or os.path.join(_PKG_DIR, "libvqc_hip.so")
 
 VQC_OK = 0
 VQC_FLAG_UNFUSED = 1
@@ -186,6 +188,8 @@ SIGNATURES = {
     "vqc_dm_fidelity": [_P, _P, _D],
     "vqc_apply_circuit": [_P, _I, ctypes.POINTER(Op), _D],
     "vqc_plan_info": [ctypes.POINTER(PlanInfo)],
+    "vqc_plan_check": [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Op), _D, ctypes.c_int, ctypes.POINTER(Term), _D,
+                       ctypes.c_int, ctypes.c_int, ctypes.POINTER(PlanInfo)],
     "vqc_sv_energy_grad": [_P, _I, ctypes.POINTER(Op), _D, _I, ctypes.POINTER(Term), _D, _I, _D, _D, _P, _I],
     "vqc_circuit_vjp": [_P, _I, ctypes.POINTER(Op), _D, _I, _P, _D, _P, _P, _I],
     "vqc_dm_energy_grad": [_P, _I, ctypes.POINTER(Op), _D, _I, ctypes.POINTER(Term), _D, _I, _D, _D, _P, _I],
