@@ -211,17 +211,18 @@ __device__ __forceinline__ void batch_loop(const BatchArgs& a, int& s_item, int&
                 __syncthreads();
                 AccState st;
                 acc_init(st);
+                const uint64_t tdep = thread_deposit(p);
                 double2* A = a.arena + in.a_off;
                 double2* B = a.arena + in.b_off;
                 if (p.kind == PH_FWD) {
                     for (int64_t tile = it.tile_begin; tile < it.tile_end; ++tile)
-                        phase_tile<MAXKB, KS_FWDH>(p, uint64_t(tile), A, A, B, mops, pool, sa, sb, red, gacc, st);
+                        phase_tile<MAXKB, KS_FWDH>(p, uint64_t(tile), tdep, A, A, B, mops, pool, sa, sb, red, gacc, st);
                 } else if (p.kind == PH_HAM) {
                     for (int64_t tile = it.tile_begin; tile < it.tile_end; ++tile)
-                        phase_tile<MAXKB, KS_HAM>(p, uint64_t(tile), A, A, B, mops, pool, sa, sb, red, gacc, st);
+                        phase_tile<MAXKB, KS_HAM>(p, uint64_t(tile), tdep, A, A, B, mops, pool, sa, sb, red, gacc, st);
                 } else {
                     for (int64_t tile = it.tile_begin; tile < it.tile_end; ++tile)
-                        phase_tile<MAXKB, RKS>(p, uint64_t(tile), A, A, B, mops, pool, sa, sb, red, gacc, st);
+                        phase_tile<MAXKB, RKS>(p, uint64_t(tile), tdep, A, A, B, mops, pool, sa, sb, red, gacc, st);
                 }
                 acc_finish(st, red, gacc);
                 for (int e = threadIdx.x; e < p.nacc; e += kTileThreads)

@@ -1158,10 +1158,11 @@ __global__ void __launch_bounds__(kTileThreads, MAXKB <= 3 ? 4 : 1) k_phase(cons
     AccState st;
     acc_init(st);
     const uint64_t ntiles = uint64_t(1) << (p.nbits - p.t);
+    const uint64_t tdep = thread_deposit(p);
     // tiles in XCD-contiguous order: consecutive tiles hold neighbouring runs of
     // amplitudes, so each XCD streams one region
     for (uint64_t tile = xcd_block(); tile < ntiles; tile += gridDim.x)
-        phase_tile<MAXKB, KS>(p, tile, b.srcA, b.A, b.B, b.ops, b.pool, sa, sb, red, gacc, st);
+        phase_tile<MAXKB, KS>(p, tile, tdep, b.srcA, b.A, b.B, b.ops, b.pool, sa, sb, red, gacc, st);
     acc_finish(st, red, gacc);
     for (int e = threadIdx.x; e < p.nacc; e += kTileThreads) b.parts[size_t(p.row_begin + e) * b.stride + blockIdx.x] = gacc[e];
 }

@@ -134,11 +134,21 @@ __device__ __forceinline__ uint64_t tile_base(uint64_t tile, const PhaseDesc& p)
     for (int i = 0; i < p.nob; ++i) b |= ((tile >> i) & 1) << p.ob[i];
     return b;
 }
-__device__ __forceinline__ uint64_t tile_global(uint32_t j, const PhaseDesc& p, uint64_t base) {
-    uint64_t idx = base | (j & ((1u << p.m) - 1));
-    const uint32_t h = j >> p.m;
-    for (int i = 0; i < p.nhb; ++i) idx |= uint64_t((h >> i) & 1) << p.hb[i];
-    return idx;
+// State-index bits of the tile-local bits [b0, b1) of j: tile-local bit b sits at
+// state bit b below p.m and at p.hb[b - p.m] above. The deposit is linear over
+// disjoint bits, so the index j = u * kTileThreads + threadIdx.x of a thread's
+// u-th amplitude is the OR of a per-thread part (the bits below
+// kTileThreadBits, once per tile) and a part uniform over the workgroup (the bits
+// of u, scalar work): a few instructions per amplitude instead of a loop over the
+// tile's high bits.
+constexpr int kTileThreadBits = 9;
+static_assert((1 << kTileThreadBits) == kTileThreads, "kTileThreadBits must match kTileThreads");
+__device__ __forceinline__ uint64_t tile_deposit(uint32_t j, int b0, int b1, const PhaseDesc& p) {
+    const int lo = std::min(b1, p.m);  // bits below p.m stay where they are
+    uint64_t r = b0 < lo ? uint64_t(j & ((1u << lo) - 1) & ~((1u << b0) - 1)) : 0;
+    for (int b = std::max(b0, p.m); b < b1; ++b)
+        if (j >> b & 1) r |= uint64_t(1) << p.hb[b - p.m];
+    return r;
 }
 
 // Reductions of the gradient / energy terms. A micro-op leaves one partial sum
@@ -1049,6 +1059,11 @@ __device__ __forceinline__ void warm_ops(const MicroOp* __restrict__ ops, int op
     sink[threadIdx.x & 63] = make_double2(s, 0.0);
 }
 
+// this thread's part of every tile index of phase p (see tile_deposit)
+__device__ __forceinline__ uint64_t thread_deposit(const PhaseDesc& p) {
+    return tile_deposit(threadIdx.x, 0, std::min(p.t, kTileThreadBits), p);
+}
+
 // Stream one tile of phase p through LDS.
 //  PH_FWD: sa <- srcA tile; ops; dstA tile <- sa
 //  PH_REV: sa <- A, sb <- B; ops; A <- sa, B <- sb
@@ -1056,14 +1071,16 @@ __device__ __forceinline__ void warm_ops(const MicroOp* __restrict__ ops, int op
 // Global loads are issued kLoadBatch at a time per thread before any LDS store
 // so that each wave keeps several 1 KiB requests in flight.
 template <int MAXKB, int KS>
-__device__ __forceinline__ void phase_tile(const PhaseDesc& p, uint64_t tile, const double2* srcA, double2* A, double2* B,
-                                           const MicroOp* __restrict__ ops, const double2* __restrict__ pool, double2* sa,
-                                           double2* sb, double2* red, double2* gacc, AccState& st) {
-    const uint64_t base = tile_base(tile, p);
+__device__ __forceinline__ void phase_tile(const PhaseDesc& p, uint64_t tile, uint64_t tdep, const double2* srcA, double2* A,
+                                           double2* B, const MicroOp* __restrict__ ops, const double2* __restrict__ pool,
+                                           double2* sa, double2* sb, double2* red, double2* gacc, AccState& st) {
     const uint32_t T = 1u << p.t;
     const bool loadB = p.kind == PH_REV || (p.kind == PH_HAM && p.ham_accum);
     const bool zeroB = p.kind == PH_HAM && !p.ham_accum;
     const bool init = p.flags & PF_INIT_ZERO;
+    // this thread's state-index bits: the tile's base and the deposit of threadIdx.x
+    // (tdep = thread_deposit(p), computed once per phase by the caller)
+    const uint64_t tbase = tile_base(tile, p) | tdep;
     // per thread: T / kTileThreads amplitudes (uniform), read kLoadBatch at a time
     const uint32_t per = (T + kTileThreads - 1) / kTileThreads;
     for (uint32_t u0 = 0; u0 < per; u0 += kLoadBatch) {
@@ -1074,7 +1091,7 @@ __device__ __forceinline__ void phase_tile(const PhaseDesc& p, uint64_t tile, co
             vb[u] = make_double2(0.0, 0.0);
             const uint32_t j = (u0 + u) * kTileThreads + threadIdx.x;
             if (u0 + u < per && j < T) {
-                const uint64_t gi = tile_global(j, p, base);
+                const uint64_t gi = tbase | tile_deposit((u0 + u) << kTileThreadBits, kTileThreadBits, p.t, p);
                 if (init) va[u].x = gi == 0 ? 1.0 : 0.0;
                 else va[u] = srcA[gi];
                 if (loadB) vb[u] = B[gi];
@@ -1091,10 +1108,13 @@ __device__ __forceinline__ void phase_tile(const PhaseDesc& p, uint64_t tile, co
     }
     __syncthreads();
     for (int o = p.op_begin; o < p.op_end; ++o) run_mop<MAXKB, KS>(sa, sb, p.t, ops[o], pool, red, gacc, st);
-    for (uint32_t j = threadIdx.x; j < T; j += kTileThreads) {
-        const uint64_t gi = tile_global(j, p, base);
-        if (p.kind != PH_HAM) A[gi] = sa[j];
-        if (p.kind != PH_FWD) B[gi] = sb[j];
+    for (uint32_t u = 0; u < per; ++u) {
+        const uint32_t j = u * kTileThreads + threadIdx.x;
+        if (j < T) {
+            const uint64_t gi = tbase | tile_deposit(u << kTileThreadBits, kTileThreadBits, p.t, p);
+            if (p.kind != PH_HAM) A[gi] = sa[j];
+            if (p.kind != PH_FWD) B[gi] = sb[j];
+        }
     }
     __syncthreads();
 }
