@@ -164,13 +164,28 @@ def bench_config0(vq, _lib):
     info = _lib.PlanInfo()
     _lib.call("vqc_plan_info", ctypes.byref(info))
     Ep, gp = _eval(vq, _lib, psi0, args, False)
+    rate_default = _time_evals(lambda: _eval(vq, _lib, psi0, args, True))
+    # the autotuner picks this size's tile shape (forward / reverse tile bits, low
+    # bits) into the library's per-size tile table; the table is restored after
+    from vqchip import autotune
+    table_before = autotune.get_tile_table()
+    t_tune = time.perf_counter()
+    tuned = autotune.tune_sizes([n], workload=lambda _n: (c, H, psi0), reps=5, lows=(2, 3, 4, 5), spread=(-1, 0, 1))
+    tune_s = time.perf_counter() - t_tune
+    Et, gt = _eval(vq, _lib, psi0, args, True)
+    _lib.call("vqc_plan_info", ctypes.byref(info))
     rate = _time_evals(lambda: _eval(vq, _lib, psi0, args, True))
+    autotune.set_tile_table({n: table_before.get(n, (0, 0, 0))})
     out = {
         "workload": "12 qubits, 4 layers Rx/Ry/Rz + CNOT ladder, 144 angles, TFIM (23 keys)",
         "fused_evals_per_s": round(rate, 1),
+        "fused_evals_per_s_default_tiles": round(rate_default, 1),
+        "autotuned_tiles": {"fwd_bits": tuned[n]["best"][0], "rev_bits": tuned[n]["best"][1], "low_bits": tuned[n]["best"][2],
+                            "tune_s": round(tune_s, 2)},
         "plain_evals_per_s": round(_time_evals(lambda: _eval(vq, _lib, psi0, args, False)), 1),
         "energy": Ef.real,
-        "max_abs_diff_plain_vs_fused": max(abs(Ep - Ef), float(np.max(np.abs(gp - gf)))),
+        "max_abs_diff_plain_vs_fused": max(abs(Ep - Ef), float(np.max(np.abs(gp - gf))), abs(Et - Ef),
+                                           float(np.max(np.abs(gt - gf)))),
         "roofline_fused": roofline_of(1.0 / rate, eval_flops(c, H, n), fused_hbm_bytes(info, n)),
         "tile_passes": {"fwd": info.phases_fwd, "ham": info.phases_ham, "rev": info.phases_rev},
     }

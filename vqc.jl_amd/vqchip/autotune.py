@@ -52,6 +52,35 @@ def set_tuning(**kw):
     _lib.call("vqc_set_tuning", ctypes.byref(t))
 
 
+def _prepared_eval(H, circuit, state0):
+    """A callable running the fused energy + gradient of a packed program and
+    returning the energy: the candidates are timed on the library call alone,
+    without packing the circuit each time (host work of the same order as a small
+    evaluation, which would hide the differences being measured). Operators with
+    keys of more than 5 sites take the general route of energy_and_gradient."""
+    from .api import LARGEST_SUPPORTED_NTERMS, _as_operator, energy_and_gradient
+    from .circuit import Program
+    from .gates import QuantumMap
+    from .operators import pack_terms
+    H = _as_operator(H)
+    if H.largest_nterm() > LARGEST_SUPPORTED_NTERMS or (not state0.is_dm and any(
+            isinstance(g, QuantumMap) for g in circuit)):
+        return lambda: energy_and_gradient(H, circuit, state0)[0]
+    dm = bool(state0.is_dm)
+    prog = Program(circuit, n=state0.nqubits(), dm=dm)
+    ops, pool = prog.op_array(), prog.pool()
+    terms, nt, hpool = pack_terms(H)
+    grad = np.zeros(max(prog.nparams, 1))
+    e2 = np.zeros(2)
+    fn = "vqc_dm_energy_grad" if dm else "vqc_sv_energy_grad"
+
+    def run():
+        _lib.call(fn, state0.handle, len(prog.ops), ops, _lib.dptr(pool), nt, terms, _lib.dptr(hpool), prog.nparams,
+                  _lib.dptr(e2), _lib.dptr(grad), None, 0)
+        return complex(e2[0], e2[1])
+    return run
+
+
 def _time(fn, reps):
     fn()
     ts = []
@@ -66,7 +95,7 @@ def tune_energy_grad(H, circuit, state0, fwd=(10, 11, 12), rev=(10, 11, 12), low
                      cache_key=None):
     """Sweep the tile shape of the fused energy + gradient path on this workload.
     Returns {"best": {...}, "seconds": t, "table": [(params, seconds), ...]}."""
-    from .api import energy_and_gradient
+    evaluate = _prepared_eval(H, circuit, state0)
     before = get_tuning()
     table = []
     ref = None
@@ -79,7 +108,7 @@ def tune_energy_grad(H, circuit, state0, fwd=(10, 11, 12), rev=(10, 11, 12), low
                 out = {}
 
                 def run():
-                    out["E"] = energy_and_gradient(H, circuit, state0)[0]
+                    out["E"] = evaluate()
 
                 t = _time(run, reps)
                 if ref is None:
@@ -190,7 +219,6 @@ def tune_sizes(sizes, workload=None, reps=3, lows=(3, 4, 5), spread=(-1, 0, 1), 
     built-in rule and the contiguous low bits given, and the fastest shape becomes
     the size's entry of the library's tile table. Sizes of at most 9 bits run in
     one workgroup and are left out. Returns {n: {"best": (fwd, rev, low), "table": [...]}}."""
-    from .api import energy_and_gradient
     from .state import StateVector
     from .workloads import hea_circuit, tfim
     if workload is None:
@@ -204,6 +232,7 @@ def tune_sizes(sizes, workload=None, reps=3, lows=(3, 4, 5), spread=(-1, 0, 1), 
         if n <= 9:
             continue
         c, H, s0 = workload(n)
+        evaluate = _prepared_eval(H, c, s0)
         f0, r0 = _rule(n)
         table, ref = [], None
         for df in spread:
@@ -218,7 +247,7 @@ def tune_sizes(sizes, workload=None, reps=3, lows=(3, 4, 5), spread=(-1, 0, 1), 
                     res = {}
 
                     def run():
-                        res["E"] = energy_and_gradient(H, c, s0)[0]
+                        res["E"] = evaluate()
 
                     t = _time(run, reps)
                     if ref is None:
