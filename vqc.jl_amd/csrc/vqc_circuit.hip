@@ -1196,11 +1196,19 @@ __global__ void __launch_bounds__(kTileThreads) k_small_eval(SmallArgs a) {
 // ---------------------------------------------------------------------------
 // host drivers
 // ---------------------------------------------------------------------------
+// Work buffers of the fused paths: one set per host thread (each thread runs its
+// calls on its own stream), freed when the thread ends.
 struct Work {
     void* buf = nullptr;
     size_t cap = 0;
+    Work() = default;
+    Work(const Work&) = delete;
+    Work& operator=(const Work&) = delete;
+    ~Work() {
+        if (buf) (void)hipFree(buf);
+    }
 };
-static Work g_wA, g_wB, g_wparts, g_wpool, g_wmops, g_wrows, g_wphases;
+static thread_local Work g_wA, g_wB, g_wparts, g_wpool, g_wmops, g_wrows, g_wphases;
 
 static int ensure(Work& w, size_t bytes) {
     if (bytes <= w.cap) return VQC_OK;
@@ -1392,7 +1400,7 @@ extern "C" int vqc_plan_info(vqc_plan_info_t* out) {
 }
 
 // cached plan whose micro-ops and phases the device buffers hold (0: unknown)
-static uint64_t g_uploaded_plan = 0;
+static thread_local uint64_t g_uploaded_plan = 0;
 
 // The matrix pool always goes up; the micro-ops and phases only when the device
 // does not hold them already (plan_id: the cached plan they belong to, 0 for an
@@ -1746,7 +1754,7 @@ static int plan_reverse_cut(PlanEntry& e, const CkptCut& cut, int nbits, int n, 
     return VQC_OK;
 }
 // Checkpoint buffers of one call (grown on demand, kept for the next call)
-static Work g_wckpt;
+static thread_local Work g_wckpt;
 static int ckpt_buffers(const PlanEntry& e, uint64_t L, double2** out) {
     const size_t k = e.ck_store.size();
     if (!k) {
@@ -1869,9 +1877,11 @@ static int sv_adjoint(const vqc_state* psi0, int nops, const vqc_op_t* ops, cons
 #undef VQC_SMALL
             VQC_HIP(hipGetLastError());
             std::vector<double> rows(2 * size_t(std::max(pb.nrows, 1)));
-            if (pb.nrows) VQC_TRY(vqc_fetch((double2*)g_wrows.buf, pb.nrows, rows.data()));
+            // the wait releases the API lock: keep what the plan's rows mean first
+            const std::vector<int> row_slot = pb.row_slot;
+            if (pb.nrows) VQC_TRY(vqc_fetch_unlocked((double2*)g_wrows.buf, pb.nrows, rows.data()));
             else VQC_HIP(hipStreamSynchronize(st));
-            finish_rows(rows, pb.row_slot, nparams, energy2, grad);
+            finish_rows(rows, row_slot, nparams, energy2, grad);
             return VQC_OK;
         }
         const size_t nfwd = size_t(e->info[1]), nham_end = size_t(e->info[2]);
@@ -1899,9 +1909,11 @@ static int sv_adjoint(const vqc_state* psi0, int nops, const vqc_op_t* ops, cons
         if (pb.nrows) vqc_reduce_partials(parts, pb.nrows, stride, rowsum, st);
         VQC_HIP(hipGetLastError());
         std::vector<double> rows(2 * size_t(nrows));
-        if (pb.nrows) VQC_TRY(vqc_fetch(rowsum, pb.nrows, rows.data()));
+        // the wait releases the API lock: keep what the plan's rows mean first
+        const std::vector<int> row_slot = pb.row_slot;
+        if (pb.nrows) VQC_TRY(vqc_fetch_unlocked(rowsum, pb.nrows, rows.data()));
         else VQC_HIP(hipStreamSynchronize(st));
-        finish_rows(rows, pb.row_slot, nparams, energy2, grad);
+        finish_rows(rows, row_slot, nparams, energy2, grad);
     }
     return VQC_OK;
 }
@@ -2127,9 +2139,11 @@ static int dm_adjoint(const vqc_state* rho0, int nops, const vqc_op_t* ops, cons
     if (pb.nrows) vqc_reduce_partials(parts, pb.nrows, stride, rowsum, st);
     VQC_HIP(hipGetLastError());
     std::vector<double> rows(2 * size_t(nrows));
-    if (pb.nrows) VQC_TRY(vqc_fetch(rowsum, pb.nrows, rows.data()));
+    // the wait releases the API lock: keep what the plan's rows mean first
+    const std::vector<int> row_slot = pb.row_slot;
+    if (pb.nrows) VQC_TRY(vqc_fetch_unlocked(rowsum, pb.nrows, rows.data()));
     else VQC_HIP(hipStreamSynchronize(st));
-    finish_rows(rows, pb.row_slot, nparams, energy2, grad);
+    finish_rows(rows, row_slot, nparams, energy2, grad);
     return VQC_OK;
 }
 

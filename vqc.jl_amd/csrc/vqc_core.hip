@@ -39,34 +39,63 @@ void vqc_trace(const char* fmt, ...) {
 extern "C" const char* vqc_last_error(void) { return g_err.c_str(); }
 extern "C" int vqc_version(void) { return 10000; }
 
-static VqcCtx g_ctx;
+static VqcCtx g_ctx;  // the device's properties (stream and buffers unused)
 static std::mutex g_ctx_mu;
 
 std::recursive_mutex& vqc_api_mutex() {
     static std::recursive_mutex mu;
     return mu;
 }
+int& vqc_api_depth() {
+    static thread_local int depth = 0;
+    return depth;
+}
+
+// This thread's context: its own stream and staging, released when the thread
+// ends (before any static of the HIP runtime goes away, also for the main thread).
+namespace {
+struct ThreadCtx {
+    VqcCtx c;
+    ~ThreadCtx() {
+        if (c.stream) (void)hipStreamSynchronize(c.stream);
+        if (c.scratch) (void)hipFree(c.scratch);
+        if (c.pinned) (void)hipHostFree(c.pinned);
+        if (c.stream) (void)hipStreamDestroy(c.stream);
+    }
+};
+thread_local ThreadCtx t_ctx;
+}  // namespace
 
 int vqc_ctx(VqcCtx** out) {
-    std::lock_guard<std::mutex> lk(g_ctx_mu);
-    if (g_ctx.device < 0) {
-        int ndev = 0;
-        hipError_t e = hipGetDeviceCount(&ndev);
-        if (e != hipSuccess || ndev < 1)
-            return vqc_fail(VQC_EHIP, "no HIP device available (%s)", hipGetErrorString(e));
-        int dev = 0;
-        VQC_HIP(hipGetDevice(&dev));
-        hipDeviceProp_t prop;
-        VQC_HIP(hipGetDeviceProperties(&prop, dev));
-        if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
-            return vqc_fail(VQC_EUNSUPPORTED, "libvqc_hip is built for gfx950, device is %s", prop.gcnArchName);
-        VQC_HIP(hipStreamCreateWithFlags(&g_ctx.stream, hipStreamNonBlocking));
-        g_ctx.cu_count = prop.multiProcessorCount;
-        g_ctx.lds_per_cu = int(prop.maxSharedMemoryPerMultiProcessor);
-        g_ctx.hbm_bytes = prop.totalGlobalMem;
-        g_ctx.device = dev;
+    VqcCtx& t = t_ctx.c;
+    if (t.device < 0) {
+        {
+            std::lock_guard<std::mutex> lk(g_ctx_mu);
+            if (g_ctx.device < 0) {
+                int ndev = 0;
+                hipError_t e = hipGetDeviceCount(&ndev);
+                if (e != hipSuccess || ndev < 1)
+                    return vqc_fail(VQC_EHIP, "no HIP device available (%s)", hipGetErrorString(e));
+                int dev = 0;
+                VQC_HIP(hipGetDevice(&dev));
+                hipDeviceProp_t prop;
+                VQC_HIP(hipGetDeviceProperties(&prop, dev));
+                if (std::string(prop.gcnArchName).find("gfx950") == std::string::npos)
+                    return vqc_fail(VQC_EUNSUPPORTED, "libvqc_hip is built for gfx950, device is %s", prop.gcnArchName);
+                g_ctx.cu_count = prop.multiProcessorCount;
+                g_ctx.lds_per_cu = int(prop.maxSharedMemoryPerMultiProcessor);
+                g_ctx.hbm_bytes = prop.totalGlobalMem;
+                g_ctx.device = dev;
+            }
+        }
+        VQC_HIP(hipSetDevice(g_ctx.device));
+        VQC_HIP(hipStreamCreateWithFlags(&t.stream, hipStreamNonBlocking));
+        t.cu_count = g_ctx.cu_count;
+        t.lds_per_cu = g_ctx.lds_per_cu;
+        t.hbm_bytes = g_ctx.hbm_bytes;
+        t.device = g_ctx.device;
     }
-    *out = &g_ctx;
+    *out = &t;
     return VQC_OK;
 }
 
@@ -217,6 +246,25 @@ __global__ void __launch_bounds__(256) k_reduce_parts(const double2* __restrict_
 
 void vqc_reduce_partials(const double2* parts, int nterms, int nparts, double2* out, hipStream_t st) {
     hipLaunchKernelGGL(k_reduce_parts, dim3(nterms), dim3(256), 0, st, parts, nparts, out);
+}
+
+int vqc_fetch_unlocked(const double2* dev, int n, double* host) {
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    double* pin;
+    VQC_TRY(vqc_pinned(size_t(n) * 16, &pin));
+    VQC_HIP(hipMemcpyAsync(pin, dev, size_t(n) * 16, hipMemcpyDeviceToHost, c->stream));
+    // other threads' calls may plan and launch on their streams meanwhile: the
+    // stream, the pinned staging and the device buffers are this thread's own
+    const int depth = vqc_api_depth();
+    for (int i = 0; i < depth; ++i) vqc_api_mutex().unlock();
+    vqc_api_depth() = 0;
+    const hipError_t e = hipStreamSynchronize(c->stream);
+    for (int i = 0; i < depth; ++i) vqc_api_mutex().lock();
+    vqc_api_depth() = depth;
+    VQC_HIP(e);
+    for (int i = 0; i < 2 * n; ++i) host[i] = pin[i];
+    return VQC_OK;
 }
 
 int vqc_fetch(const double2* dev, int n, double* host) {

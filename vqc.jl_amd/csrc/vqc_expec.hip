@@ -76,8 +76,7 @@ __global__ void __launch_bounds__(kEB) k_expec(const double2* __restrict__ ket, 
     if (threadIdx.x == 0) parts[blockIdx.x] = make_double2(sr, si);
 }
 
-static void* g_emat = nullptr;
-static size_t g_emat_cap = 0;
+static thread_local DeviceBuffer g_emat;
 
 template <int K>
 static void launch_accum(const double2* in, double2* out, int nbits, const int32_t* pos, const double2* Ud, hipStream_t st) {
@@ -141,8 +140,8 @@ extern "C" int vqc_sv_accum_dense(const vqc_state* in, vqc_state* out, int k, co
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));
     const int D = 1 << k;
-    VQC_TRY(vqc_upload_buffer(&g_emat, &g_emat_cap, M, size_t(D) * D * 16));
-    return vqc_accum_sweep(in->d, out->d, in->nbits, k, pos, (const double2*)g_emat, c->stream);
+    VQC_TRY(vqc_upload_buffer(&g_emat.p, &g_emat.cap, M, size_t(D) * D * 16));
+    return vqc_accum_sweep(in->d, out->d, in->nbits, k, pos, (const double2*)g_emat.p, c->stream);
 }
 
 extern "C" int vqc_sv_expec_dense(const vqc_state* ket, const vqc_state* bra, int k, const int32_t* pos, const double* M,
@@ -155,12 +154,12 @@ extern "C" int vqc_sv_expec_dense(const vqc_state* ket, const vqc_state* bra, in
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));
     const int D = 1 << k;
-    VQC_TRY(vqc_upload_buffer(&g_emat, &g_emat_cap, M, size_t(D) * D * 16));
+    VQC_TRY(vqc_upload_buffer(&g_emat.p, &g_emat.cap, M, size_t(D) * D * 16));
     void* scr;
     VQC_TRY(vqc_scratch(size_t(kMaxParts + 1) * 16, &scr));
     double2* parts = (double2*)scr;
     unsigned grid;
-    VQC_TRY(vqc_expec_sweep(ket->d, bra ? bra->d : nullptr, ket->nbits, k, pos, (const double2*)g_emat, parts, &grid, c->stream));
+    VQC_TRY(vqc_expec_sweep(ket->d, bra ? bra->d : nullptr, ket->nbits, k, pos, (const double2*)g_emat.p, parts, &grid, c->stream));
     vqc_reduce_partials(parts, 1, grid, parts + kMaxParts, c->stream);
     VQC_HIP(hipGetLastError());
     return vqc_fetch(parts + kMaxParts, 1, out2);
@@ -536,8 +535,7 @@ extern "C" int vqc_sv_apply_pauli_sum(const vqc_state* in, vqc_state* out, int n
 
 // out = sum_keys M_key in ; energy = <in|out>  ((m::QubitsOperator)(v) and
 // expectation(m, v), apply_qterms.jl:13-18 and expecs.jl:22-28)
-static void* g_hpool = nullptr;
-static size_t g_hpool_cap = 0;
+static thread_local DeviceBuffer g_hpool;
 
 extern "C" int vqc_sv_apply_terms(const vqc_state* in, vqc_state* out, int nterms, const vqc_term_t* terms, const double* pool,
                                   double* energy2) {
@@ -554,10 +552,10 @@ extern "C" int vqc_sv_apply_terms(const vqc_state* in, vqc_state* out, int nterm
     }
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));
-    VQC_TRY(vqc_upload_buffer(&g_hpool, &g_hpool_cap, pool, size_t(plen) * 16));
+    VQC_TRY(vqc_upload_buffer(&g_hpool.p, &g_hpool.cap, pool, size_t(plen) * 16));
     vqc_zero(out->d, out->len(), c->stream);
     for (int t = 0; t < nterms; ++t)
-        VQC_TRY(vqc_accum_sweep(in->d, out->d, in->nbits, terms[t].nq, terms[t].pos, (const double2*)g_hpool + terms[t].mat, c->stream));
+        VQC_TRY(vqc_accum_sweep(in->d, out->d, in->nbits, terms[t].nq, terms[t].pos, (const double2*)g_hpool.p + terms[t].mat, c->stream));
     if (energy2) return vqc_state_dot(in, out, energy2);
     return VQC_OK;
 }
@@ -622,12 +620,12 @@ extern "C" int vqc_dm_expec_dense(const vqc_state* rho, int k, const int32_t* po
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));
     const int D = 1 << k;
-    VQC_TRY(vqc_upload_buffer(&g_emat, &g_emat_cap, M, size_t(D) * D * 16));
+    VQC_TRY(vqc_upload_buffer(&g_emat.p, &g_emat.cap, M, size_t(D) * D * 16));
     void* scr;
     VQC_TRY(vqc_scratch(size_t(kMaxParts + 1) * 16, &scr));
     double2* parts = (double2*)scr;
     unsigned grid;
-    VQC_TRY(vqc_dm_expec_sweep(rho->d, rho->nqubits, k, pos, (const double2*)g_emat, parts, &grid, c->stream));
+    VQC_TRY(vqc_dm_expec_sweep(rho->d, rho->nqubits, k, pos, (const double2*)g_emat.p, parts, &grid, c->stream));
     vqc_reduce_partials(parts, 1, grid, parts + kMaxParts, c->stream);
     VQC_HIP(hipGetLastError());
     return vqc_fetch(parts + kMaxParts, 1, out2);

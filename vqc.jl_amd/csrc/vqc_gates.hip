@@ -109,8 +109,7 @@ __global__ void __launch_bounds__(kBlock) k_dense_big(double2* __restrict__ v, u
 
 // Device copy of a host matrix for the K >= 4 kernels (stream ordered, so a
 // later call may reuse the buffer).
-static void* g_mat_dev = nullptr;
-static size_t g_mat_cap = 0;
+static thread_local DeviceBuffer g_mat;  // this thread's matrix upload buffer
 
 template <int K>
 static int launch_dense(double2* v, int nbits, const int32_t* pos, const double* Uh, hipStream_t st) {
@@ -145,9 +144,9 @@ static int launch_dense(double2* v, int nbits, const int32_t* pos, const double*
             vqc_trace("k_dense_small<%d,%d,%d> nbits %d lowest target %d grid %u", K, R, int(NT), nbits, gp.p[0],
                       grid_for(ngroups, R * kBlock));
     } else {
-        VQC_TRY(vqc_upload_buffer(&g_mat_dev, &g_mat_cap, Uh, size_t(D) * D * 16));
+        VQC_TRY(vqc_upload_buffer(&g_mat.p, &g_mat.cap, Uh, size_t(D) * D * 16));
         hipLaunchKernelGGL((k_dense_big<K>), dim3(grid_for(ngroups, kBlock)), dim3(kBlock), 0, st, v, ngroups, gp,
-                           (const double2*)g_mat_dev);
+                           (const double2*)g_mat.p);
         if (vqc_trace_on()) vqc_trace("k_dense_big<%d> nbits %d grid %u", K, nbits, grid_for(ngroups, kBlock));
     }
     VQC_HIP(hipGetLastError());

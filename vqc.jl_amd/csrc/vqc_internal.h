@@ -32,16 +32,33 @@ void vqc_trace(const char* fmt, ...) __attribute__((format(printf, 1, 2)));
         if (r_ != VQC_OK) return r_; \
     } while (0)
 
-// One lock for the single-state entry points. They share the context's stream
-// and its scratch, pinned and staging buffers, so calls from several host
-// threads are serialised here (recursive: an entry point may call another).
-// Batches and the server own their buffers and streams and do not take it.
+// One lock for the host work of the single-state entry points: the plan cache,
+// the tuning and tile tables, planning and launching. Each host thread has its
+// own stream and device buffers (VqcCtx, DeviceBuffer below), so the GPU work of
+// calls from different threads overlaps: the fused evaluations release the lock
+// while they wait for their results (vqc_fetch_unlocked). Recursive: an entry
+// point may call another. Batches and the server own their buffers and streams
+// and do not take it.
 std::recursive_mutex& vqc_api_mutex();
-#define VQC_API_LOCK std::lock_guard<std::recursive_mutex> vqc_api_guard_(vqc_api_mutex())
+int& vqc_api_depth();  // times this thread holds the lock
+struct VqcApiGuard {
+    VqcApiGuard() {
+        vqc_api_mutex().lock();
+        ++vqc_api_depth();
+    }
+    ~VqcApiGuard() {
+        --vqc_api_depth();
+        vqc_api_mutex().unlock();
+    }
+    VqcApiGuard(const VqcApiGuard&) = delete;
+    VqcApiGuard& operator=(const VqcApiGuard&) = delete;
+};
+#define VQC_API_LOCK VqcApiGuard vqc_api_guard_
 
 // ---------------------------------------------------------------------------
-// library context: one stream per process on the current device, a growable
-// device scratch area, and pinned host staging for scalar results.
+// library context of the calling host thread: the device's properties, this
+// thread's stream (default priority), a growable device scratch area and pinned
+// host staging for results. Freed when the thread ends.
 // ---------------------------------------------------------------------------
 struct VqcCtx {
     int device = -1;
@@ -55,6 +72,18 @@ struct VqcCtx {
     size_t pinned_bytes = 0;
 };
 int vqc_ctx(VqcCtx** out);
+// A device buffer of one host thread (declared thread_local), grown on demand and
+// freed when the thread ends.
+struct DeviceBuffer {
+    void* p = nullptr;
+    size_t cap = 0;
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer() {
+        if (p) (void)hipFree(p);
+    }
+};
 // the tuning in force (vqc_set_tuning; initial values from the environment)
 vqc_tuning_t vqc_tuning_now();
 // whether tile_low_bits was given (vqc_set_tuning or VQC_TILE_MLOW) rather than defaulted
@@ -197,3 +226,6 @@ int vqc_check_qubits(const vqc_state* s, const char* what);
 void vqc_reduce_partials(const double2* parts, int nterms, int nparts, double2* out, hipStream_t st);
 // Copy n complex device results to the host and wait.
 int vqc_fetch(const double2* dev, int n, double* host);
+// vqc_fetch with the API lock released while it waits for the stream (the
+// caller holds no reference into shared host state across it)
+int vqc_fetch_unlocked(const double2* dev, int n, double* host);
