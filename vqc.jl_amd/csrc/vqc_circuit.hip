@@ -501,6 +501,20 @@ bool XPool::parity_blocks(int64_t off) {
     return p;
 }
 
+// every off-diagonal entry of the D x D matrix is exactly zero
+static bool host_diag(const double* M, int D) {
+    for (int c = 0; c < D; ++c)
+        for (int r = 0; r < D; ++r)
+            if (r != c && (M[2 * (r + c * D)] != 0.0 || M[2 * (r + c * D) + 1] != 0.0)) return false;
+    return true;
+}
+
+bool XPool::diagonal(int64_t off, int D) {
+    const bool d = host_diag(h.data() + 2 * off, D);
+    claims.push_back({off, D, d ? 1 : 0, 2, 0});
+    return d;
+}
+
 bool XPool::reload(const double* pool, int64_t plen, const double* hpool, int64_t hlen) {
     if (plen) std::copy(pool, pool + 2 * plen, h.begin());
     if (hlen) std::copy(hpool, hpool + 2 * hlen, h.begin() + 2 * plen);
@@ -510,7 +524,9 @@ bool XPool::reload(const double* pool, int64_t plen, const double* hpool, int64_
     }
     for (const XPoolBlock& b : blocks) make_block_product(h.data(), b, h.data() + 2 * b.dst);
     for (const XPoolClaim& c : claims) {
-        const int now = c.test == 1 ? int(host_parity4(h.data() + 2 * c.off)) : host_mstruct(h.data() + 2 * c.off, c.D);
+        const int now = c.test == 1   ? int(host_parity4(h.data() + 2 * c.off))
+                        : c.test == 2 ? int(host_diag(h.data() + 2 * c.off, c.D))
+                                      : host_mstruct(h.data() + 2 * c.off, c.D);
         if (now != c.ms) return false;
     }
     return true;
@@ -996,6 +1012,74 @@ int vqc_plan_forward(PlanBuilder& pb, int nbits, int n, int is_dm, int nops, con
     return VQC_OK;
 }
 
+// Rewrite the Hamiltonian phases [first_phase, end) that plan_groups made (one
+// MK_HAM and one energy row per term): in each phase, the terms of at most two
+// bits become one MK_HAMS followed by their MK_HAM records, with one energy row
+// for their sum; wider terms keep their own micro-op and row. Rows and micro-ops
+// of the phases are renumbered in order. A term's matrix is marked diagonal
+// (MS_DIAG) when its values are known and say so.
+static void merge_ham_terms(PlanBuilder& pb, size_t first_phase) {
+    if (first_phase >= pb.phases.size()) return;
+    XPool& xp = *pb.xp;
+    const int op0 = pb.phases[first_phase].op_begin, row0 = pb.phases[first_phase].row_begin;
+    const std::vector<MicroOp> old(pb.mops.begin() + op0, pb.mops.end());
+    const std::vector<int> old_slot(pb.row_slot.begin() + row0, pb.row_slot.end());
+    pb.mops.resize(size_t(op0));
+    pb.row_slot.resize(size_t(row0));
+    pb.nrows = row0;
+    for (size_t i = first_phase; i < pb.phases.size(); ++i) {
+        PhaseDesc& p = pb.phases[i];
+        std::vector<MicroOp> narrow, wide;
+        std::vector<int> narrow_slot, wide_slot;
+        for (int o = p.op_begin; o < p.op_end; ++o) {
+            MicroOp m = old[size_t(o - op0)];
+            const int slot = old_slot[size_t(p.row_begin + m.acc[0] - row0)];
+            if (m.kb <= 2) {
+                narrow.push_back(m);
+                narrow_slot.push_back(slot);
+            } else {
+                wide.push_back(m);
+                wide_slot.push_back(slot);
+            }
+        }
+        if (narrow.size() < 2) {  // nothing to merge: keep the term as it is
+            wide.insert(wide.begin(), narrow.begin(), narrow.end());
+            wide_slot.insert(wide_slot.begin(), narrow_slot.begin(), narrow_slot.end());
+            narrow.clear();
+        }
+        p.op_begin = int(pb.mops.size());
+        p.row_begin = pb.nrows;
+        p.nacc = 0;
+        if (!narrow.empty()) {
+            MicroOp h = mop_init(MK_HAMS);
+            h.nchain = int(narrow.size());
+            h.kb = 1;
+            for (const MicroOp& m : narrow) h.kb = std::max(h.kb, m.kb);
+            for (int j = 0; j < h.kb; ++j) h.lpos[j] = j;
+            h.mat = -1;
+            h.ng = 1;
+            h.acc[0] = p.nacc++;
+            pb.row_slot.push_back(narrow_slot[0]);
+            pb.nrows++;
+            pb.mops.push_back(h);
+            for (MicroOp m : narrow) {
+                m.ng = 0;
+                m.acc[0] = 0;
+                m.mstr = !xp.deferred && xp.diagonal(m.mat, 1 << m.kb) ? MS_DIAG : MS_DENSE;
+                pb.mops.push_back(m);
+            }
+        }
+        for (size_t k = 0; k < wide.size(); ++k) {
+            MicroOp m = wide[k];
+            m.acc[0] = p.nacc++;
+            pb.row_slot.push_back(wide_slot[k]);
+            pb.nrows++;
+            pb.mops.push_back(m);
+        }
+        p.op_end = int(pb.mops.size());
+    }
+}
+
 int vqc_plan_ham(PlanBuilder& pb, int nbits, int nterms, const vqc_term_t* terms, int64_t hoff, int t, int mlow) {
     XPool& xp = *pb.xp;
     const size_t first_phase = pb.phases.size();
@@ -1014,6 +1098,7 @@ int vqc_plan_ham(PlanBuilder& pb, int nbits, int nterms, const vqc_term_t* terms
         },
         true);
     for (size_t i = first_phase; i < pb.phases.size(); ++i) pb.phases[i].ham_accum = i > first_phase;
+    if (!getenv("VQC_NO_HAMS")) merge_ham_terms(pb, first_phase);
     return VQC_OK;
 }
 
@@ -2198,7 +2283,17 @@ static bool plan_invariants(const PlanEntry& e, int nb, int nterms, int nparams,
         if (p.pf_n < 0 || p.pf_lo < 0 || p.pf_lo + p.pf_n > plen) return fail("phase %zu op %d: read-ahead range %lld + %lld", i, 0, p.pf_lo, p.pf_n);
         for (int o = p.op_begin; o < p.op_end; ++o) {
             const MicroOp& m = pb.mops[size_t(o)];
-            if (m.kind < MK_APPLY || m.kind > MK_DENSE || m.kb < 1 || m.kb > 5) return fail("phase %zu op %d: kind %lld on %lld bits", i, o, m.kind, m.kb);
+            if (m.kind < MK_APPLY || m.kind > MK_HAMS || m.kb < 1 || m.kb > 5) return fail("phase %zu op %d: kind %lld on %lld bits", i, o, m.kind, m.kb);
+            if (m.kind == MK_HAMS) {
+                // the header of nchain term records (MK_HAM, 1 or 2 bits, no partials of their own)
+                if (p.kind != PH_HAM || m.nchain < 2 || o + m.nchain >= p.op_end || m.ng != 1)
+                    return fail("phase %zu op %d: merged terms %lld (%lld partials)", i, o, m.nchain, m.ng);
+                for (int q = 1; q <= m.nchain; ++q) {
+                    const MicroOp& r = pb.mops[size_t(o + q)];
+                    if (r.kind != MK_HAM || r.kb < 1 || r.kb > std::min(2, m.kb) || r.ng != 0)
+                        return fail("phase %zu op %d: term record of kind %lld on %lld bits", i, o + q, r.kind, r.kb);
+                }
+            }
             for (int j = 0; j < m.kb; ++j)
                 if (m.lpos[j] < 0 || m.lpos[j] >= p.t || (j && m.lpos[j] <= m.lpos[j - 1]))
                     return fail("phase %zu op %d: local bit %lld (%lld)", i, o, m.lpos[j], j);
@@ -2217,6 +2312,8 @@ static bool plan_invariants(const PlanEntry& e, int nb, int nterms, int nparams,
                     break;
                 case MK_GRAD:
                     for (int q = 0; q < m.ng; ++q) ok = ok && in_pool(m.gen[q], D2);
+                    break;
+                case MK_HAMS:  // no matrix of its own
                     break;
                 default:
                     ok = in_pool(m.mat, D2) && (m.mat2 < 0 || in_pool(m.mat2, D2));

@@ -17,6 +17,9 @@
 //   MK_APPLY2X A <- M A, B <- M2 B                (channel reverse: S^-1, S^dagger)
 //   MK_GRAD    acc += Re sum conj(B) (G A)        (density-matrix gradient terms)
 //   MK_HAM     B += (M + M^dagger) A, acc += sum conj(A) (M A)   (adjoint seed, energy)
+//   MK_HAMS    the MK_HAM of nchain terms of at most 2 bits at once, one amplitude
+//              per thread; the terms are the nchain MK_HAM records that follow it,
+//              and acc receives the sum of their energies
 //   MK_CHAIN   up to 4 one-bit MK_APPLY2 steps on the same bit, done in registers
 //              (one LDS round trip for e.g. the Rz, Ry, Rx of one qubit)
 //   MK_DENSE   A <- M A for a dense 2^K x 2^K matrix on K = 3..5 bits, the product
@@ -36,7 +39,7 @@
 
 #include "vqc_internal.h"
 
-enum { MK_APPLY = 0, MK_APPLY2 = 1, MK_APPLY2X = 2, MK_GRAD = 3, MK_HAM = 4, MK_CHAIN = 5, MK_BLOCK = 6, MK_DENSE = 7 };
+enum { MK_APPLY = 0, MK_APPLY2 = 1, MK_APPLY2X = 2, MK_GRAD = 3, MK_HAM = 4, MK_CHAIN = 5, MK_BLOCK = 6, MK_DENSE = 7, MK_HAMS = 8 };
 enum { BS_BOTH = 0, BS_LO = 1, BS_HI = 2 };  // MK_BLOCK step targets
 // matrix structure; codes 2 and 3 name CNOT patterns for 4x4 matrices and
 // real / "X-like" (real diagonal, imaginary off-diagonal: Rx and its generator)
@@ -47,7 +50,7 @@ enum { PH_FWD = 0, PH_REV = 1, PH_HAM = 2 };
 // kind sets a kernel is built for (bit k = micro-op kind k)
 enum {
     KS_FWD = (1 << MK_APPLY) | (1 << MK_BLOCK),
-    KS_HAM = 1 << MK_HAM,
+    KS_HAM = (1 << MK_HAM) | (1 << MK_HAMS),
     KS_REV = (1 << MK_APPLY2) | (1 << MK_CHAIN) | (1 << MK_BLOCK),  // state-vector reverse
     KS_REVX = (1 << MK_APPLY) | (1 << MK_APPLY2) | (1 << MK_APPLY2X) | (1 << MK_GRAD) | (1 << MK_CHAIN) | (1 << MK_BLOCK),
     // forward, with the straight-line block for the ansatz shape (chosen per
@@ -74,7 +77,7 @@ struct MicroOp {
     int32_t kind;
     int32_t kb;         // local bits touched (1..5)
     int32_t buf;        // MK_APPLY: 0 = A, 1 = B
-    int32_t nchain;     // MK_CHAIN / MK_BLOCK: steps (2..4)
+    int32_t nchain;     // MK_CHAIN / MK_BLOCK: steps (2..4); MK_HAMS: term records that follow
     int32_t lpos[5];    // ascending local bit positions
     int32_t ng;         // partial sums produced (MK_APPLY2: 0..3, MK_GRAD / MK_HAM: 1, MK_CHAIN: 0..4)
     int32_t acc[4];     // accumulator index of each partial
@@ -796,6 +799,65 @@ __device__ __forceinline__ void mop_ham(const double2* __restrict__ sa, double2*
     wave_partial(red, buf, 0, er, ei);
 }
 
+// MK_HAMS: the nrec MK_HAM records at rec (terms of 1 or 2 bits) in one pass.
+// Each thread takes whole amplitudes j of the tile and sums, over the terms, row
+// j of (M + M^dagger) A into B[j] and conj(A[j]) (M A)[j] into one energy
+// partial. A is only read and each B[j] is written by the thread that owns j, so
+// the terms need no barrier between them: one micro-op instead of one per term.
+// A diagonal term (mstr MS_DIAG, read from the values by the planner) reads no
+// other amplitude; a dense one reads the 2 or 4 amplitudes of j's group.
+__device__ __forceinline__ void ham_term1(const double2* __restrict__ sa, uint32_t j, double2 aj, const MicroOp& op,
+                                          const double2* __restrict__ pool, double2& s, double2& h) {
+    const double2* __restrict__ M = pool + op.mat;
+    const double2* __restrict__ Dm = pool + op.mat2;
+    const uint32_t o0 = 1u << op.lpos[0];
+    const bool r = (j & o0) != 0;
+    if (ms_mat(op, 0) == MS_DIAG) {
+        s = cfma(r ? M[3] : M[0], aj, s);
+        h = cfma(r ? Dm[3] : Dm[0], aj, h);
+        return;
+    }
+    // row r of the 2 x 2 products over the pair (a0, a1) holding j
+    const double2 ao = sa[j ^ o0];
+    const double2 a0 = r ? ao : aj, a1 = r ? aj : ao;
+    s = cfma(r ? M[3] : M[2], a1, cfma(r ? M[1] : M[0], a0, s));
+    h = cfma(r ? Dm[3] : Dm[2], a1, cfma(r ? Dm[1] : Dm[0], a0, h));
+}
+__device__ __forceinline__ void ham_term2(const double2* __restrict__ sa, uint32_t j, double2 aj, const MicroOp& op,
+                                          const double2* __restrict__ pool, double2& s, double2& h) {
+    const uint32_t o0 = 1u << op.lpos[0], o1 = 1u << op.lpos[1];
+    const int r = ((j & o0) ? 1 : 0) | ((j & o1) ? 2 : 0);
+    // row r of column-major 4 x 4 matrices: entries r, r + 4, r + 8, r + 12
+    const double2* __restrict__ M = pool + op.mat + r;
+    const double2* __restrict__ Dm = pool + op.mat2 + r;
+    if (ms_mat(op, 0) == MS_DIAG) {
+        s = cfma(M[4 * r], aj, s);  // entry (r, r)
+        h = cfma(Dm[4 * r], aj, h);
+        return;
+    }
+    const uint32_t base = j & ~(o0 | o1);
+    const double2 x0 = sa[base], x1 = sa[base | o0], x2 = sa[base | o1], x3 = sa[base | o0 | o1];
+    s = cfma(M[12], x3, cfma(M[8], x2, cfma(M[4], x1, cfma(M[0], x0, s))));
+    h = cfma(Dm[12], x3, cfma(Dm[8], x2, cfma(Dm[4], x1, cfma(Dm[0], x0, h))));
+}
+__device__ __forceinline__ void mop_hams(const double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp* rec,
+                                         int nrec, const double2* __restrict__ pool, double2* red, int buf) {
+    const uint32_t T = 1u << t;
+    double er = 0.0, ei = 0.0;
+    for (uint32_t j = threadIdx.x; j < T; j += kTileThreads) {
+        const double2 aj = sa[j];
+        double2 s = make_double2(0.0, 0.0), h = make_double2(0.0, 0.0);
+        for (int q = 0; q < nrec; ++q) {
+            if (rec[q].kb == 1) ham_term1(sa, j, aj, rec[q], pool, s, h);
+            else ham_term2(sa, j, aj, rec[q], pool, s, h);
+        }
+        sb[j] = cadd(sb[j], h);
+        er = fma(aj.x, s.x, fma(aj.y, s.y, er));
+        ei = fma(aj.x, s.y, fma(-aj.y, s.x, ei));
+    }
+    wave_partial(red, buf, 0, er, ei);
+}
+
 // MK_DENSE: A <- M A with the 2^K x 2^K complex M (column-major over lpos[0..K),
 // ascending) as a real GEMM on the matrix cores. A group (2^K amplitudes, the
 // other tile bits fixed) is one column of X; in real form M is 2D x 2D with
@@ -917,13 +979,16 @@ __device__ __forceinline__ void mop_dense(double2* __restrict__ s, int t, const 
 // `gacc` the accumulator rows addressed by op.acc[]. MAXKB is the widest
 // micro-op the calling kernel is built for: kernels for phases of at most 3
 // local bits carry no 4/5-bit code, which keeps them free of register spills.
+// `op` is an element of the phase's micro-op array; the return value is the
+// number of records after it that it consumed (the terms of an MK_HAMS), which
+// the caller skips.
 template <int MAXKB, int KS>
-__device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const MicroOp& op, const double2* pool, double2* red,
-                                        double2* gacc, AccState& st) {
+__device__ __forceinline__ int run_mop(double2* sa, double2* sb, int t, const MicroOp& op, const double2* pool, double2* red,
+                                       double2* gacc, AccState& st) {
     acc_fold(st, red, gacc);
     const int kb = op.kb;
     const int buf = st.cur_buf;
-    int nred = 0;
+    int nred = 0, nrec = 0;
     switch (op.kind) {
         case MK_APPLY:
             if constexpr ((KS & (1 << MK_APPLY)) != 0) {
@@ -1017,9 +1082,17 @@ __device__ __forceinline__ void run_mop(double2* sa, double2* sb, int t, const M
             nred = 1;
             }
             break;
+        case MK_HAMS:
+            if constexpr ((KS & (1 << MK_HAMS)) != 0) {
+                nrec = op.nchain;
+                mop_hams(sa, sb, t, &op + 1, nrec, pool, red, buf);
+                nred = 1;
+            }
+            break;
     }
     __syncthreads();
     acc_push(st, nred, op.acc);
+    return nrec;
 }
 
 // Fold the last pending partials; ends with a barrier so gacc is complete.
@@ -1122,7 +1195,7 @@ __device__ __forceinline__ void phase_tile(const PhaseDesc& p, uint64_t tile, ui
         }
     }
     __syncthreads();
-    for (int o = p.op_begin; o < p.op_end; ++o) run_mop<MAXKB, KS>(sa, sb, p.t, ops[o], pool, red, gacc, st);
+    for (int o = p.op_begin; o < p.op_end; ++o) o += run_mop<MAXKB, KS>(sa, sb, p.t, ops[o], pool, red, gacc, st);
     for (uint32_t u = 0; u < per; ++u) {
         const uint32_t j = u * kTileThreads + threadIdx.x;
         if (j < T) {
@@ -1154,12 +1227,12 @@ __device__ __forceinline__ void small_eval_body(int nbits, const double2* src, d
     __syncthreads();
     AccState st;
     acc_init(st);
-    for (int o = mop_begin; o < ham_begin; ++o) run_mop<MAXKB, KS_FWD>(sa, sb, nbits, ops[o], pool, red, gacc, st);
+    for (int o = mop_begin; o < ham_begin; ++o) o += run_mop<MAXKB, KS_FWD>(sa, sb, nbits, ops[o], pool, red, gacc, st);
     if (A) {
         for (uint32_t j = threadIdx.x; j < T; j += kTileThreads) A[j] = sa[j];
     }
-    for (int o = ham_begin; o < rev_begin; ++o) run_mop<MAXKB, KS_HAM>(sa, sb, nbits, ops[o], pool, red, gacc, st);
-    for (int o = rev_begin; o < mop_end; ++o) run_mop<MAXKB, RKS>(sa, sb, nbits, ops[o], pool, red, gacc, st);
+    for (int o = ham_begin; o < rev_begin; ++o) o += run_mop<MAXKB, KS_HAM>(sa, sb, nbits, ops[o], pool, red, gacc, st);
+    for (int o = rev_begin; o < mop_end; ++o) o += run_mop<MAXKB, RKS>(sa, sb, nbits, ops[o], pool, red, gacc, st);
     acc_finish(st, red, gacc);
     for (int e = threadIdx.x; e < nrows; e += kTileThreads) rows[size_t(e) * row_stride] = gacc[e];
     __syncthreads();
@@ -1203,7 +1276,8 @@ struct XPoolBlock {
 struct XPoolClaim {
     int64_t off;
     int32_t D, ms;  // what was found for the D x D matrix at off: its class (MS_*), or for
-    int32_t test;   // test 1, whether it is a parity-block channel (1) or not (0)
+    int32_t test;   // test 1, whether it is a parity-block channel (1) or not (0), for
+                    // test 2, whether it is diagonal (1) or not (0)
     int32_t pad;
 };
 struct XPool {
@@ -1230,6 +1304,8 @@ struct XPool {
     int classify(int64_t off, int D);
     // whether the 4 x 4 matrix at off only couples indices of equal bit parity, recorded
     bool parity_blocks(int64_t off);
+    // whether the D x D matrix at off is diagonal, recorded
+    bool diagonal(int64_t off, int D);
     // New caller matrices (pool, then hpool, the lengths the pool was made with):
     // every derived matrix is recomputed from them in the order it was made.
     // False when a structure class the planner read no longer holds.
