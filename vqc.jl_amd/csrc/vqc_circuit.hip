@@ -1295,6 +1295,50 @@ struct Work {
 };
 static thread_local Work g_wA, g_wB, g_wparts, g_wpool, g_wmops, g_wrows, g_wphases;
 
+// Pinned host staging for the per-call matrix upload: a copy from pageable
+// memory makes the host wait for the transfer before it can launch the first
+// phase; from pinned memory it is queued like a kernel. The event marks the end
+// of the last transfer out of the staging, which the next upload waits for
+// before overwriting it (a call that returns without waiting for its stream may
+// leave it in flight).
+struct UploadStage {
+    double* buf = nullptr;
+    size_t cap = 0;
+    hipEvent_t done = nullptr;
+    bool pending = false;
+    UploadStage() = default;
+    UploadStage(const UploadStage&) = delete;
+    UploadStage& operator=(const UploadStage&) = delete;
+    ~UploadStage() {
+        if (pending) (void)hipEventSynchronize(done);
+        if (done) (void)hipEventDestroy(done);
+        if (buf) (void)hipHostFree(buf);
+    }
+};
+static thread_local UploadStage g_upload;
+
+static int upload_staged(void* dev, const double* host, size_t bytes, hipStream_t st) {
+    UploadStage& u = g_upload;
+    if (u.pending) {
+        VQC_HIP(hipEventSynchronize(u.done));
+        u.pending = false;
+    }
+    if (bytes > u.cap) {
+        if (u.buf) VQC_HIP(hipHostFree(u.buf));
+        u.buf = nullptr;
+        u.cap = 0;
+        const size_t nb = std::max<size_t>(bytes, size_t(1) << 16);
+        VQC_HIP(hipHostMalloc((void**)&u.buf, nb, hipHostMallocDefault));
+        u.cap = nb;
+    }
+    if (!u.done) VQC_HIP(hipEventCreateWithFlags(&u.done, hipEventDisableTiming));
+    memcpy(u.buf, host, bytes);
+    VQC_HIP(hipMemcpyAsync(dev, u.buf, bytes, hipMemcpyHostToDevice, st));
+    VQC_HIP(hipEventRecord(u.done, st));
+    u.pending = true;
+    return VQC_OK;
+}
+
 static int ensure(Work& w, size_t bytes) {
     if (bytes <= w.cap) return VQC_OK;
     VqcCtx* c;
@@ -1442,6 +1486,34 @@ static unsigned phase_grid(const PhaseDesc& p, unsigned max_acc_grid) {
     return unsigned(std::min<uint64_t>(ntiles, max_acc_grid));
 }
 
+// The columns of the partial rows that the phases write: workgroup b of a phase
+// launched on phase_grid(p, stride) workgroups writes column b of each of the
+// phase's rows, so rows come in segments of equal column counts (no zero fill
+// needed). False when there are more segments than a kernel argument holds.
+static bool row_segments(const PlanBuilder& pb, int stride, VqcRowSegs* s) {
+    s->n = 0;
+    int cur = 0;
+    auto push = [&](int end, int cols) {
+        if (end <= cur) return true;
+        if (s->n > 0 && s->cols[s->n - 1] == cols) {
+            s->end[s->n - 1] = end;
+        } else {
+            if (s->n == kMaxRowSegs) return false;
+            s->end[s->n] = end;
+            s->cols[s->n] = cols;
+            ++s->n;
+        }
+        cur = end;
+        return true;
+    };
+    for (const PhaseDesc& p : pb.phases) {
+        if (p.nacc <= 0) continue;
+        if (p.row_begin < cur) return false;  // rows out of order: not the layout assumed here
+        if (!push(p.row_begin, 0) || !push(p.row_begin + p.nacc, int(phase_grid(p, unsigned(stride))))) return false;
+    }
+    return push(pb.nrows, 0);
+}
+
 // Run planned FWD phases from src into A.
 static int run_forward(const PlanBuilder& pb, size_t ph_begin, size_t ph_end, const double2* src, double2* A,
                        const MicroOp* dmops, const double2* dpool, const PhaseDesc* dpd, hipStream_t st) {
@@ -1494,7 +1566,7 @@ static int upload_program(const PlanBuilder& pb, hipStream_t st, uint64_t plan_i
     record_plan(pb);
     const std::vector<double>& h = pb.xp->h;
     VQC_TRY(ensure(g_wpool, h.size() * 8 + 16));
-    if (!h.empty()) VQC_HIP(hipMemcpyAsync(g_wpool.buf, h.data(), h.size() * 8, hipMemcpyHostToDevice, st));
+    if (!h.empty()) VQC_TRY(upload_staged(g_wpool.buf, h.data(), h.size() * 8, st));
     if (plan_id != 0 && plan_id == g_uploaded_plan) return VQC_OK;
     g_uploaded_plan = 0;
     VQC_TRY(ensure(g_wmops, std::max<size_t>(pb.mops.size(), 1) * sizeof(MicroOp)));
@@ -1977,7 +2049,9 @@ static int sv_adjoint(const vqc_state* psi0, int nops, const vqc_op_t* ops, cons
         VQC_TRY(ensure(g_wparts, size_t(nrows) * (stride + 1) * 16));
         double2* parts = (double2*)g_wparts.buf;
         double2* rowsum = parts + size_t(nrows) * stride;
-        vqc_zero(parts, size_t(nrows) * stride, st);
+        VqcRowSegs segs;
+        const bool by_segs = row_segments(pb, stride, &segs);
+        if (!by_segs) vqc_zero(parts, size_t(nrows) * stride, st);
         double2* ck;
         VQC_TRY(ckpt_buffers(*e, L, &ck));
         if (nfwd == 0) VQC_HIP(hipMemcpyAsync(A, psi0->d, L * 16, hipMemcpyDeviceToDevice, st));
@@ -1991,13 +2065,25 @@ static int sv_adjoint(const vqc_state* psi0, int nops, const vqc_op_t* ops, cons
             VQC_TRY(launch_phase(pb, i, (const PhaseDesc*)g_wphases.buf, b, phase_grid(pb.phases[i], stride), st));
         }
         if (seed_out) VQC_HIP(hipMemcpyAsync(seed_out->d, B, L * 16, hipMemcpyDeviceToDevice, st));
-        if (pb.nrows) vqc_reduce_partials(parts, pb.nrows, stride, rowsum, st);
-        VQC_HIP(hipGetLastError());
         std::vector<double> rows(2 * size_t(nrows));
         // the wait releases the API lock: keep what the plan's rows mean first
         const std::vector<int> row_slot = pb.row_slot;
-        if (pb.nrows) VQC_TRY(vqc_fetch_unlocked(rowsum, pb.nrows, rows.data()));
-        else VQC_HIP(hipStreamSynchronize(st));
+        if (pb.nrows && by_segs) {
+            // row sums straight into pinned host memory: no copy after the reduction
+            double* pin;
+            VQC_TRY(vqc_pinned_rows(size_t(pb.nrows) * 16, &pin));
+            vqc_reduce_rows(parts, stride, segs, pb.nrows, (double2*)pin, st);
+            VQC_HIP(hipGetLastError());
+            VQC_TRY(vqc_wait_unlocked());
+            std::copy(pin, pin + 2 * size_t(pb.nrows), rows.begin());
+        } else if (pb.nrows) {
+            vqc_reduce_partials(parts, pb.nrows, stride, rowsum, st);
+            VQC_HIP(hipGetLastError());
+            VQC_TRY(vqc_fetch_unlocked(rowsum, pb.nrows, rows.data()));
+        } else {
+            VQC_HIP(hipGetLastError());
+            VQC_HIP(hipStreamSynchronize(st));
+        }
         finish_rows(rows, row_slot, nparams, energy2, grad);
     }
     return VQC_OK;

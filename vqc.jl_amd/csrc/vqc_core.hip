@@ -60,6 +60,7 @@ struct ThreadCtx {
         if (c.stream) (void)hipStreamSynchronize(c.stream);
         if (c.scratch) (void)hipFree(c.scratch);
         if (c.pinned) (void)hipHostFree(c.pinned);
+        if (c.rows) (void)hipHostFree(c.rows);
         if (c.stream) (void)hipStreamDestroy(c.stream);
     }
 };
@@ -132,6 +133,24 @@ int vqc_pinned(size_t bytes, double** out) {
         c->pinned_bytes = nb;
     }
     *out = c->pinned;
+    return VQC_OK;
+}
+
+int vqc_pinned_rows(size_t bytes, double** out) {
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    if (bytes > c->rows_bytes) {
+        if (c->rows) {
+            VQC_HIP(hipStreamSynchronize(c->stream));
+            VQC_HIP(hipHostFree(c->rows));
+            c->rows = nullptr;
+            c->rows_bytes = 0;
+        }
+        size_t nb = std::max<size_t>(bytes, size_t(1) << 14);
+        VQC_HIP(hipHostMalloc((void**)&c->rows, nb, hipHostMallocCoherent));
+        c->rows_bytes = nb;
+    }
+    *out = c->rows;
     return VQC_OK;
 }
 
@@ -248,12 +267,41 @@ void vqc_reduce_partials(const double2* parts, int nterms, int nparts, double2* 
     hipLaunchKernelGGL(k_reduce_parts, dim3(nterms), dim3(256), 0, st, parts, nparts, out);
 }
 
-int vqc_fetch_unlocked(const double2* dev, int n, double* host) {
+// one wave per row: lane l sums columns l, l + 64, ... of the row's written
+// columns, then the wave's fixed-order sum; lane 0 stores the result with
+// system-scope stores (out may be host memory)
+__global__ void __launch_bounds__(256) k_reduce_rows(const double2* __restrict__ parts, int stride, VqcRowSegs segs, int nrows,
+                                                     double2* out) {
+    const int r = int(blockIdx.x) * 4 + int(threadIdx.x >> 6);
+    if (r >= nrows) return;  // the whole wave
+    const int lane = threadIdx.x & 63;
+    int cols = 0;
+    for (int k = 0; k < segs.n; ++k)
+        if (r < segs.end[k]) {
+            cols = segs.cols[k];
+            break;
+        }
+    const double2* __restrict__ p = parts + size_t(r) * stride;
+    double sr = 0.0, si = 0.0;
+    for (int i = lane; i < cols; i += 64) {
+        sr += p[i].x;
+        si += p[i].y;
+    }
+    sr = wave_sum_full(sr);
+    si = wave_sum_full(si);
+    if (lane == 0) {
+        __hip_atomic_store(&out[r].x, sr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(&out[r].y, si, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+    }
+}
+
+void vqc_reduce_rows(const double2* parts, int stride, const VqcRowSegs& segs, int nrows, double2* out, hipStream_t st) {
+    hipLaunchKernelGGL(k_reduce_rows, dim3((nrows + 3) / 4), dim3(256), 0, st, parts, stride, segs, nrows, out);
+}
+
+int vqc_wait_unlocked() {
     VqcCtx* c;
     VQC_TRY(vqc_ctx(&c));
-    double* pin;
-    VQC_TRY(vqc_pinned(size_t(n) * 16, &pin));
-    VQC_HIP(hipMemcpyAsync(pin, dev, size_t(n) * 16, hipMemcpyDeviceToHost, c->stream));
     // other threads' calls may plan and launch on their streams meanwhile: the
     // stream, the pinned staging and the device buffers are this thread's own
     const int depth = vqc_api_depth();
@@ -263,6 +311,16 @@ int vqc_fetch_unlocked(const double2* dev, int n, double* host) {
     for (int i = 0; i < depth; ++i) vqc_api_mutex().lock();
     vqc_api_depth() = depth;
     VQC_HIP(e);
+    return VQC_OK;
+}
+
+int vqc_fetch_unlocked(const double2* dev, int n, double* host) {
+    VqcCtx* c;
+    VQC_TRY(vqc_ctx(&c));
+    double* pin;
+    VQC_TRY(vqc_pinned(size_t(n) * 16, &pin));
+    VQC_HIP(hipMemcpyAsync(pin, dev, size_t(n) * 16, hipMemcpyDeviceToHost, c->stream));
+    VQC_TRY(vqc_wait_unlocked());
     for (int i = 0; i < 2 * n; ++i) host[i] = pin[i];
     return VQC_OK;
 }

@@ -70,6 +70,8 @@ struct VqcCtx {
     size_t scratch_bytes = 0;
     double* pinned = nullptr;     // host pinned staging
     size_t pinned_bytes = 0;
+    double* rows = nullptr;       // host pinned, device-coherent: results kernels write directly
+    size_t rows_bytes = 0;
 };
 int vqc_ctx(VqcCtx** out);
 // A device buffer of one host thread (declared thread_local), grown on demand and
@@ -92,6 +94,11 @@ bool vqc_tuning_low_bits_given();
 vqc_tile_entry_t vqc_tile_entry(int nbits);
 int vqc_scratch(size_t bytes, void** out);      // device scratch, valid until the next call
 int vqc_pinned(size_t bytes, double** out);     // pinned host staging
+// pinned, device-coherent host memory of this thread that kernels write results
+// into directly (no copy kernel): valid once the stream has been waited for
+int vqc_pinned_rows(size_t bytes, double** out);
+// wait for this thread's stream with the API lock released
+int vqc_wait_unlocked();
 // Upload host data into a dedicated device buffer of at least `bytes` bytes (grown on demand).
 int vqc_upload_buffer(void** dev, size_t* cap, const void* host, size_t bytes);
 
@@ -224,6 +231,17 @@ int vqc_check_qubits(const vqc_state* s, const char* what);
 // Reduce `nparts` per-block complex partials for each of `nterms` terms
 // (layout [term][part]) into `out` (device, nterms complex), in a fixed order.
 void vqc_reduce_partials(const double2* parts, int nterms, int nparts, double2* out, hipStream_t st);
+// Partial rows whose written columns vary by row: segment k holds the rows below
+// end[k] (from end[k-1] on) with cols[k] columns written, the rest never written.
+constexpr int kMaxRowSegs = 64;
+struct VqcRowSegs {
+    int32_t n;
+    int32_t end[kMaxRowSegs];
+    int32_t cols[kMaxRowSegs];
+};
+// out[r] = sum of the written columns of row r (row stride `stride`), in a fixed
+// order; out may be host memory from vqc_pinned_rows
+void vqc_reduce_rows(const double2* parts, int stride, const VqcRowSegs& segs, int nrows, double2* out, hipStream_t st);
 // Copy n complex device results to the host and wait.
 int vqc_fetch(const double2* dev, int n, double* host);
 // vqc_fetch with the API lock released while it waits for the stream (the
