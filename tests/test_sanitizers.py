@@ -1,4 +1,5 @@
-"""Host code under AddressSanitizer and UndefinedBehaviorSanitizer (CPU only).
+"""Host code under AddressSanitizer, UndefinedBehaviorSanitizer and
+ThreadSanitizer (CPU only).
 
 * The oracle: oracle/liboracle_san.so (the same sources, `make liboracle_san.so`)
   runs the oracle's own test modules.
@@ -6,10 +7,14 @@
   the batch planner, the tuning tables and the serving scheduler compiled with the
   sanitizers on their host side, build_lib.build_all) runs the planner tests, which
   make no device call (vqc_plan_check).
+* The same units built with ThreadSanitizer (vqc.jl_amd/_build/tsan/
+  libvqc_hip_tsan.so, gcc's runtime preloaded) run the concurrent-caller tests of
+  tests/test_host_threads.py: planning from several threads while others rewrite
+  the tuning and the tile table.
 
 Each run is a subprocess with the sanitizer runtime preloaded (the Python
 interpreter itself is not instrumented); a report, or any failing test, fails
-the run. Both builds are made by __graft_entry__.build().
+the run. The builds are made by __graft_entry__.build().
 """
 import os
 import subprocess
@@ -20,12 +25,21 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SAN = os.path.join(ROOT, "oracle", "liboracle_san.so")
 LIB_SAN = os.path.join(ROOT, "vqc.jl_amd", "_build", "san", "libvqc_hip_san.so")
+LIB_TSAN = os.path.join(ROOT, "vqc.jl_amd", "_build", "tsan", "libvqc_hip_tsan.so")
 ENV = {"ASAN_OPTIONS": "detect_leaks=0:verify_asan_link_order=0:abort_on_error=0",
-       "UBSAN_OPTIONS": "halt_on_error=1:print_stacktrace=1"}
+       "UBSAN_OPTIONS": "halt_on_error=1:print_stacktrace=1",
+       "TSAN_OPTIONS": "halt_on_error=0:report_signal_unsafe=0"}
+REPORTS = ("AddressSanitizer", "runtime error:", "ThreadSanitizer")
 
 
 def _gcc_asan():
     r = subprocess.run(["gcc", "-print-file-name=libasan.so"], capture_output=True, text=True)
+    p = r.stdout.strip()
+    return p if os.path.isabs(p) and os.path.exists(p) else None
+
+
+def _gcc_tsan():
+    r = subprocess.run(["gcc", "-print-file-name=libtsan.so"], capture_output=True, text=True)
     p = r.stdout.strip()
     return p if os.path.isabs(p) and os.path.exists(p) else None
 
@@ -44,7 +58,7 @@ def _run(preload, extra_env, tests, probe):
     r = subprocess.run([sys.executable, "-m", "pytest", "-q", "-x", "-p", "no:cacheprovider", "-m", "not gpu"] + tests,
                        cwd=ROOT, env=env, capture_output=True, text=True, timeout=900)
     out = r.stdout + r.stderr
-    assert "AddressSanitizer" not in out and "runtime error:" not in out, out[-4000:]
+    assert not any(w in out for w in REPORTS), out[-4000:]
     assert r.returncode == 0, out[-4000:]
     assert " passed" in out, out[-2000:]
 
@@ -66,3 +80,12 @@ def test_planner_under_address_and_undefined_sanitizers():
     probe = ("import sys; sys.path.insert(0, 'vqc.jl_amd'); from vqchip import _lib; _lib.load(); "
              "print(any('libvqc_hip_san.so' in l for l in open('/proc/self/maps')))")
     _run(rt, {"VQC_LIB_PATH": LIB_SAN}, ["tests/test_planner.py"], probe)
+
+
+def test_concurrent_callers_under_thread_sanitizer():
+    rt = _gcc_tsan()
+    if rt is None or not os.path.exists(LIB_TSAN):
+        pytest.skip("no gcc ThreadSanitizer runtime or no libvqc_hip_tsan.so (run __graft_entry__.build())")
+    probe = ("import sys; sys.path.insert(0, 'vqc.jl_amd'); from vqchip import _lib; _lib.load(); "
+             "print(any('libvqc_hip_tsan.so' in l for l in open('/proc/self/maps')))")
+    _run(rt, {"VQC_LIB_PATH": LIB_TSAN}, ["tests/test_host_threads.py", "tests/test_planner.py"], probe)

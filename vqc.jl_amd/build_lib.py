@@ -84,23 +84,44 @@ SAN_FLAGS = ["--offload-arch=gfx950", "-std=c++17", "-fPIC", "-Wall", "-Wno-unus
              "-Xarch_device", "-O1"]
 
 
-def _compile_san(src):
-    obj = os.path.join(SAN_DIR, os.path.basename(src)[:-4] + ".o")
-    r = subprocess.run([HIPCC] + SAN_FLAGS + ["-c", src, "-o", obj], capture_output=True, text=True)
+# ThreadSanitizer build of the same units (host code only). The image has no
+# clang TSan runtime; the process preloads gcc's libtsan, whose interface lacks
+# the __tsan_mem* entry points recent clang emits for memory intrinsics, so those
+# stay plain calls (which the runtime intercepts).
+TSAN_DIR = os.path.join(OBJ, "tsan")
+TSAN_LIB = os.path.join(TSAN_DIR, "libvqc_hip_tsan.so")
+TSAN_FLAGS = ["--offload-arch=gfx950", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+              "-Xarch_host", "-O1", "-Xarch_host", "-g", "-Xarch_host", "-fno-omit-frame-pointer",
+              "-Xarch_host", "-fsanitize=thread", "-mllvm", "-tsan-instrument-memintrinsics=false",
+              "-Xarch_device", "-O1"]
+
+
+def _compile_with(flags, outdir, src):
+    obj = os.path.join(outdir, os.path.basename(src)[:-4] + ".o")
+    r = subprocess.run([HIPCC] + flags + ["-c", src, "-o", obj], capture_output=True, text=True)
     return src, obj, r.returncode, r.stdout + r.stderr
+
+
+def _compile_units(flags, outdir, what):
+    os.makedirs(outdir, exist_ok=True)
+    srcs = [os.path.join(CSRC, u + ".hip") for u in SAN_UNITS]
+    objs = []
+    with concurrent.futures.ThreadPoolExecutor(max_workers=len(srcs)) as ex:
+        for src, obj, rc, out in ex.map(lambda s: _compile_with(flags, outdir, s), srcs):
+            if rc != 0:
+                raise RuntimeError(f"{what} build of {os.path.basename(src)} failed:\n{out}")
+            objs.append(obj)
+    return objs
 
 
 def compile_san():
     """The sanitized objects (independent of build(), so both can run at once)."""
-    os.makedirs(SAN_DIR, exist_ok=True)
-    srcs = [os.path.join(CSRC, u + ".hip") for u in SAN_UNITS]
-    objs = []
-    with concurrent.futures.ThreadPoolExecutor(max_workers=len(srcs)) as ex:
-        for src, obj, rc, out in ex.map(_compile_san, srcs):
-            if rc != 0:
-                raise RuntimeError(f"sanitizer build of {os.path.basename(src)} failed:\n{out}")
-            objs.append(obj)
-    return objs
+    return _compile_units(SAN_FLAGS, SAN_DIR, "sanitizer")
+
+
+def compile_tsan():
+    return _compile_units(TSAN_FLAGS, TSAN_DIR, "thread-sanitizer")
 
 
 def link_san(san_objs):
@@ -115,12 +136,27 @@ def link_san(san_objs):
     return SAN_LIB
 
 
+def link_tsan(tsan_objs):
+    """libvqc_hip_tsan.so: the regular objects and the thread-sanitized ones, linked
+    without a runtime (the process preloads it)."""
+    objs = [os.path.join(OBJ, os.path.basename(s)[:-4] + ".o") for s in sources() if os.path.basename(s)[:-4] not in SAN_UNITS]
+    objs += tsan_objs
+    cmd = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", TSAN_LIB] + objs + \
+          ["-L/opt/rocm/lib", "-Wl,-rpath,/opt/rocm/lib", "-lrocsolver", "-lrocblas"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("thread-sanitizer link failed:\n" + r.stdout + r.stderr)
+    return TSAN_LIB
+
+
 def build_all():
-    """build() and the sanitizer build, compiled side by side."""
-    with concurrent.futures.ThreadPoolExecutor(max_workers=1) as ex:
+    """build() and the two sanitizer builds, compiled side by side."""
+    with concurrent.futures.ThreadPoolExecutor(max_workers=2) as ex:
         san = ex.submit(compile_san)
+        tsan = ex.submit(compile_tsan)
         lib = build()
         link_san(san.result())
+        link_tsan(tsan.result())
     return lib
 
 
