@@ -51,7 +51,9 @@ def _clang_asan():
 
 
 def _run(preload, extra_env, tests, probe):
-    env = dict(os.environ, LD_PRELOAD=preload, **ENV, **extra_env)
+    # the sanitizer runtime first; whatever the environment already preloads stays
+    kept = os.environ.get("LD_PRELOAD", "")
+    env = dict(os.environ, LD_PRELOAD=preload + (":" + kept if kept else ""), **ENV, **extra_env)
     # the sanitized library must be the one the process maps
     r = subprocess.run([sys.executable, "-c", probe], cwd=ROOT, env=env, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0 and r.stdout.strip().endswith("True"), r.stdout + r.stderr
