@@ -250,15 +250,6 @@ __global__ void __launch_bounds__(kTileThreads) k_batch(BatchArgs a) {
     batch_loop<MAXKB, RKS>(a, s_item, s_ok);
 }
 
-// At most 128 VGPRs, so two 512-thread workgroups fit a CU when their LDS does
-// (four waves per SIMD).
-template <int MAXKB, int RKS>
-__global__ void __launch_bounds__(kTileThreads, 4) k_batch2(BatchArgs a) {
-    __shared__ int s_item;
-    __shared__ int s_ok;
-    batch_loop<MAXKB, RKS>(a, s_item, s_ok);
-}
-
 // out[r] = sum of row r's partials, in a fixed order
 __global__ void __launch_bounds__(256) k_reduce_rows(const double2* __restrict__ parts, const int64_t* __restrict__ off,
                                                      const int32_t* __restrict__ cnt, double2* out) {
@@ -333,14 +324,10 @@ struct vqc_batch {
 
 typedef void (*BatchKernel)(BatchArgs);
 static BatchKernel batch_kernel(const vqc_batch* b) {
-    if (b->maxkb <= 2) {
-        // the 128-VGPR variant only when the LDS of two workgroups fits a CU;
-        // otherwise one workgroup per CU, which needs no register cap
-        static const bool occ1 = getenv("VQC_BATCH_OCC1") != nullptr;
-        if (occ1 || 2 * std::max<size_t>(b->lds, 16 * 1024) > 160 * 1024)
-            return b->rev_simple ? k_batch<2, KS_REV> : k_batch<2, KS_REVX>;
-        return b->rev_simple ? k_batch2<2, KS_REV> : k_batch2<2, KS_REVX>;
-    }
+    // One workgroup per CU and no register cap: capped at 128 VGPRs (two workgroups
+    // per CU where their LDS fits) the kernel spills, and it measured no faster on
+    // the serving loop or on 256 instances of 8 to 11 qubits (within 5%).
+    if (b->maxkb <= 2) return b->rev_simple ? k_batch<2, KS_REV> : k_batch<2, KS_REVX>;
     if (b->maxkb <= 3) return b->rev_simple ? k_batch<3, KS_REV> : k_batch<3, KS_REVX>;
     return b->rev_simple ? k_batch<5, KS_REV> : k_batch<5, KS_REVX>;
 }

@@ -334,10 +334,12 @@ __device__ __forceinline__ void mop_parity(double2* __restrict__ sa, double2* __
     }
 }
 
-template <int KB>
+// ROLL keeps the row loop rolled at every width (reverse kernels, where a lone
+// MK_APPLY is rare and unrolled 8 x 8 rows beside the reverse micro-ops spill).
+template <int KB, bool ROLL = false>
 __device__ __forceinline__ void mop_apply(double2* __restrict__ s, int t, const MicroOp& op, const double2* __restrict__ pool) {
     constexpr int D = 1 << KB;
-    constexpr int RU = KB <= 3 ? D : 1;
+    constexpr int RU = KB <= 3 && !ROLL ? D : 1;
     const double2* __restrict__ M = pool + op.mat;
     const uint32_t ng = 1u << (t - KB);
     if constexpr (KB == 1) {
@@ -384,9 +386,77 @@ __device__ __forceinline__ void mop_apply(double2* __restrict__ s, int t, const 
 // matrix (mat2) for B (channels), MK_GRAD applies nothing. Row
 // loops of 2 and more bits stay rolled: the matrix rows are read through the
 // scalar cache one at a time instead of all being hoisted into registers.
+// The same for 4 and 5 bits with one buffer's group in registers at a time: A's
+// D amplitudes, then B's, then A's new ones for the generator terms, each B entry
+// of those read from LDS as its row comes. Holding both groups (2 D complex values,
+// 256 VGPRs at 5 bits) spills.
+template <int KB>
+__device__ __forceinline__ void mop_pair_wide(double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
+                                              const double2* __restrict__ pool, double2* red, int buf) {
+    constexpr int D = 1 << KB;
+    const bool APPLY = op.kind != MK_GRAD;
+    const double2* __restrict__ M = pool + op.mat;
+    const double2* __restrict__ M2 = op.kind == MK_APPLY2X ? pool + op.mat2 : M;
+    const int ngen = op.ng;
+    double gs0 = 0.0, gs1 = 0.0, gs2 = 0.0;
+    const uint32_t ng = 1u << (t - KB);
+    for (uint32_t g = threadIdx.x; g < ng; g += kTileThreads) {
+        const uint32_t base = ins32<KB>(g, op.lpos);
+        double2 x[D];
+        if (APPLY) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) x[j] = sa[base + off32<KB>(j, op.lpos)];
+#pragma unroll 1
+            for (int r = 0; r < D; ++r) {
+                double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+                for (int c = 0; c < D; ++c) s = cfma(M[r + c * D], x[c], s);
+                sa[base + off32<KB>(r, op.lpos)] = s;
+            }
+#pragma unroll
+            for (int j = 0; j < D; ++j) x[j] = sb[base + off32<KB>(j, op.lpos)];
+#pragma unroll 1
+            for (int r = 0; r < D; ++r) {
+                double2 u = make_double2(0.0, 0.0);
+#pragma unroll
+                for (int c = 0; c < D; ++c) u = cfma(M2[r + c * D], x[c], u);
+                sb[base + off32<KB>(r, op.lpos)] = u;
+            }
+        }
+        if (ngen) {
+#pragma unroll
+            for (int j = 0; j < D; ++j) x[j] = sa[base + off32<KB>(j, op.lpos)];
+#pragma unroll
+            for (int q = 0; q < 3; ++q) {
+                if (q >= ngen) break;
+                const double2* __restrict__ G = pool + op.gen[q];
+                double acc = 0.0;
+#pragma unroll 1
+                for (int r = 0; r < D; ++r) {
+                    double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+                    for (int c = 0; c < D; ++c) s = cfma(G[r + c * D], x[c], s);
+                    const double2 br = sb[base + off32<KB>(r, op.lpos)];
+                    acc = fma(br.x, s.x, fma(br.y, s.y, acc));  // Re(conj(b_r) s)
+                }
+                if (q == 0) gs0 += acc;
+                else if (q == 1) gs1 += acc;
+                else gs2 += acc;
+            }
+        }
+    }
+    if (ngen > 0) wave_partial_re(red, buf, 0, gs0);
+    if (ngen > 1) wave_partial_re(red, buf, 1, gs1);
+    if (ngen > 2) wave_partial_re(red, buf, 2, gs2);
+}
+
 template <int KB>
 __device__ __forceinline__ void mop_pair(double2* __restrict__ sa, double2* __restrict__ sb, int t, const MicroOp& op,
                                          const double2* __restrict__ pool, double2* red, int buf) {
+    if constexpr (KB >= 4) {
+        mop_pair_wide<KB>(sa, sb, t, op, pool, red, buf);
+        return;
+    }
     constexpr int D = 1 << KB;
     constexpr int RU = KB == 1 ? D : 1;  // unrolled rows would hoist every matrix out of the group loop
     const bool APPLY = op.kind != MK_GRAD;
@@ -992,15 +1062,16 @@ __device__ __forceinline__ int run_mop(double2* sa, double2* sb, int t, const Mi
     switch (op.kind) {
         case MK_APPLY:
             if constexpr ((KS & (1 << MK_APPLY)) != 0) {
+            constexpr bool roll = (KS & (1 << MK_APPLY2)) != 0;  // a reverse kernel
             double2* s = op.buf ? sb : sa;
-            if (kb == 1) mop_apply<1>(s, t, op, pool);
+            if (kb == 1) mop_apply<1, roll>(s, t, op, pool);
             else if (kb == 2 && (op.mstr & kMsParity)) mop_parity<false>(s, sb, t, op, pool);
-            else if (kb == 2) mop_apply<2>(s, t, op, pool);
+            else if (kb == 2) mop_apply<2, roll>(s, t, op, pool);
             else if constexpr (MAXKB >= 3) {
-                if (kb == 3) mop_apply<3>(s, t, op, pool);
+                if (kb == 3) mop_apply<3, roll>(s, t, op, pool);
                 else if constexpr (MAXKB >= 5) {
-                    if (kb == 4) mop_apply<4>(s, t, op, pool);
-                    else mop_apply<5>(s, t, op, pool);
+                    if (kb == 4) mop_apply<4, roll>(s, t, op, pool);
+                    else mop_apply<5, roll>(s, t, op, pool);
                 }
             }
             }
@@ -1023,7 +1094,8 @@ __device__ __forceinline__ int run_mop(double2* sa, double2* sb, int t, const Mi
                     if (kb == 3) mop_pair<3>(sa, sb, t, op, pool, red, buf);
                     else if constexpr (MAXKB >= 5) {
                         if (kb == 4) mop_pair<4>(sa, sb, t, op, pool, red, buf);
-                        else mop_pair<5>(sa, sb, t, op, pool, red, buf);
+                        // 5 bits: gradient-only terms (MK_GRAD), general reverse kernels only
+                        else if constexpr ((KS & (1 << MK_GRAD)) != 0) mop_pair<5>(sa, sb, t, op, pool, red, buf);
                     }
                 }
                 nred = op.ng;
