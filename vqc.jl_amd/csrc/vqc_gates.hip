@@ -87,6 +87,41 @@ __global__ void __launch_bounds__(kBlock) k_dense_small(double2* __restrict__ v,
     }
 }
 
+// K = 1 with one amplitude per thread: a workgroup holds 128 whole pairs (the 256
+// contiguous amplitudes around a target below bit 8, else 128 consecutive
+// amplitudes from each side of the pair). Each thread loads its own amplitude,
+// takes its partner's from LDS and stores its own row: one load and one store per
+// thread over contiguous addresses, the access shape of a plain in-place stream,
+// whatever the target. Standalone probe on a 28-qubit state: 6.65-6.86 TB/s for
+// targets 0-12 against 6.35-6.72 for the pair-per-thread form (4.2-5.1 for
+// targets 0-2, which had to stay temporal there); mixed above bit 12, where the
+// pair form stays.
+constexpr int kDense1xMaxTarget = 12;
+__global__ void __launch_bounds__(kBlock) k_dense_1x(double2* __restrict__ v, int q, GMat<1> U) {
+    __shared__ double2 xs[kBlock];
+    const uint32_t t = threadIdx.x;
+    const uint64_t b = xcd_block();
+    uint64_t a;
+    uint32_t pt;  // the partner's thread
+    if (q < 8) {
+        a = b * kBlock + t;
+        pt = t ^ (1u << q);
+    } else {
+        const uint64_t g = b * (kBlock / 2) + (t & (kBlock / 2 - 1));
+        const uint64_t lo = g & ((uint64_t(1) << q) - 1);
+        a = (((g - lo) << 1) | lo) + (uint64_t(t / (kBlock / 2)) << q);
+        pt = t ^ (kBlock / 2);
+    }
+    const double2 x = ld<true>(v + a);
+    xs[t] = x;
+    __syncthreads();
+    const double2 y = xs[pt];
+    const bool hi = (a >> q) & 1;  // row of this amplitude: x0 = low, x1 = high of the pair
+    const double2 x0 = hi ? y : x, x1 = hi ? x : y;
+    const double2 r = hi ? cfma(U.m[3], x1, cmul(U.m[1], x0)) : cfma(U.m[2], x1, cmul(U.m[0], x0));
+    st<true>(v + a, r);
+}
+
 // K = 4, 5: the matrix (4 or 16 KiB) is staged once per block in LDS and read
 // as a broadcast; one group per thread.
 template <int K>
@@ -131,6 +166,14 @@ static int launch_dense(double2* v, int nbits, const int32_t* pos, const double*
         // below (bits 0, 13, 27: 5.3 -> 3.5 TB/s), where each lane's accesses
         // fall in a few 128-byte lines; those sweeps stay temporal.
         const vqc_tuning_t tu = vqc_tuning_now();
+        if constexpr (K == 1) {
+            if (nbits >= 8 && gp.p[0] <= kDense1xMaxTarget && tu.dense_groups[0] == 1 && tu.dense_nontemporal[0] != 0) {
+                hipLaunchKernelGGL(k_dense_1x, dim3(unsigned((uint64_t(1) << nbits) / kBlock)), dim3(kBlock), 0, st, v, gp.p[0], U);
+                if (vqc_trace_on()) vqc_trace("k_dense_1x nbits %d target %d", nbits, gp.p[0]);
+                VQC_HIP(hipGetLastError());
+                return VQC_OK;
+            }
+        }
         const int R = tu.dense_groups[K - 1];
         const bool NT = tu.dense_nontemporal[K - 1] != 0 && gp.p[0] >= 3;
         const dim3 blk(kBlock);
