@@ -97,6 +97,7 @@ __global__ void __launch_bounds__(kBlock) k_dense_small(double2* __restrict__ v,
 // targets 0-2, which had to stay temporal there); mixed above bit 12, where the
 // pair form stays.
 constexpr int kDense1xMaxTarget = 12;
+constexpr bool kDense2x = true;
 __global__ void __launch_bounds__(kBlock) k_dense_1x(double2* __restrict__ v, int q, GMat<1> U) {
     __shared__ double2 xs[kBlock];
     const uint32_t t = threadIdx.x;
@@ -120,6 +121,35 @@ __global__ void __launch_bounds__(kBlock) k_dense_1x(double2* __restrict__ v, in
     const double2 x0 = hi ? y : x, x1 = hi ? x : y;
     const double2 r = hi ? cfma(U.m[3], x1, cmul(U.m[1], x0)) : cfma(U.m[2], x1, cmul(U.m[0], x0));
     st<true>(v + a, r);
+}
+
+// K = 2 with one amplitude per thread: an 8-bit tile per workgroup made of the m
+// lowest bits and the targets at or above m (m = 8, 7 or 6 as 0, 1 or 2 targets lie
+// at or above it). Local bits below m are the state's own, so a wave still reads
+// contiguous amplitudes; a target at or above m sits on local bit 6 or 7. Each
+// thread loads its amplitude, reads the group's other three and its matrix row from
+// LDS, and stores its own row (the same sum, in the same order, as the
+// group-per-thread form).
+__global__ void __launch_bounds__(kBlock) k_dense_2x(double2* __restrict__ v, GPos<2> gp, GPos<8> tile, int lp0, int lp1,
+                                                     GMat<2> U) {
+    __shared__ double2 xs[kBlock];
+    __shared__ double2 us[16];
+    const uint32_t t = threadIdx.x;
+    if (t < 16) us[t] = U.m[t];
+    // local bit k of t sits on state bit tile.p[k]
+    uint64_t a = insert_zeros<8>(xcd_block(), tile.p);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a |= uint64_t((t >> k) & 1) << tile.p[k];
+    xs[t] = ld<true>(v + a);
+    __syncthreads();
+    const uint32_t r = ((t >> lp0) & 1) | (((t >> lp1) & 1) << 1);
+    const uint32_t g = t & ~((1u << lp0) | (1u << lp1));
+    const uint32_t o0 = 1u << lp0, o1 = 1u << lp1;
+    double2 s = cfma(us[r], xs[g], make_double2(0.0, 0.0));
+    s = cfma(us[r + 4], xs[g | o0], s);
+    s = cfma(us[r + 8], xs[g | o1], s);
+    s = cfma(us[r + 12], xs[g | o0 | o1], s);
+    st<true>(v + a, s);
 }
 
 // K = 4, 5: the matrix (4 or 16 KiB) is staged once per block in LDS and read
@@ -170,6 +200,28 @@ static int launch_dense(double2* v, int nbits, const int32_t* pos, const double*
             if (nbits >= 8 && gp.p[0] <= kDense1xMaxTarget && tu.dense_groups[0] == 1 && tu.dense_nontemporal[0] != 0) {
                 hipLaunchKernelGGL(k_dense_1x, dim3(unsigned((uint64_t(1) << nbits) / kBlock)), dim3(kBlock), 0, st, v, gp.p[0], U);
                 if (vqc_trace_on()) vqc_trace("k_dense_1x nbits %d target %d", nbits, gp.p[0]);
+                VQC_HIP(hipGetLastError());
+                return VQC_OK;
+            }
+        }
+        if constexpr (K == 2) {
+            if (nbits >= 8 && kDense2x && tu.dense_groups[1] == 1 && tu.dense_nontemporal[1] != 0) {
+                // m low bits plus the targets at or above m: 8 bits in all
+                int m = 8;
+                while ((gp.p[0] >= m) + (gp.p[1] >= m) + m > 8) --m;
+                GPos<8> tile;
+                int lp[2], k = 0;
+                for (int b = 0; b < m; ++b) tile.p[k++] = b;
+                for (int j = 0; j < 2; ++j) {
+                    if (gp.p[j] < m) lp[j] = gp.p[j];
+                    else {
+                        lp[j] = k;
+                        tile.p[k++] = gp.p[j];
+                    }
+                }
+                hipLaunchKernelGGL(k_dense_2x, dim3(unsigned((uint64_t(1) << nbits) / kBlock)), dim3(kBlock), 0, st, v, gp, tile, lp[0],
+                                   lp[1], U);
+                if (vqc_trace_on()) vqc_trace("k_dense_2x nbits %d targets %d %d", nbits, gp.p[0], gp.p[1]);
                 VQC_HIP(hipGetLastError());
                 return VQC_OK;
             }
