@@ -98,6 +98,7 @@ __global__ void __launch_bounds__(kBlock) k_dense_small(double2* __restrict__ v,
 // pair form stays.
 constexpr int kDense1xMaxTarget = 12;
 constexpr bool kDense2x = true;
+constexpr bool kDense3x = true;
 __global__ void __launch_bounds__(kBlock) k_dense_1x(double2* __restrict__ v, int q, GMat<1> U) {
     __shared__ double2 xs[kBlock];
     const uint32_t t = threadIdx.x;
@@ -149,6 +150,28 @@ __global__ void __launch_bounds__(kBlock) k_dense_2x(double2* __restrict__ v, GP
     s = cfma(us[r + 4], xs[g | o0], s);
     s = cfma(us[r + 8], xs[g | o1], s);
     s = cfma(us[r + 12], xs[g | o0 | o1], s);
+    st<true>(v + a, s);
+}
+
+// K = 3 in the same way: the m lowest bits (m = 5..8) and the targets at or above m.
+__global__ void __launch_bounds__(kBlock) k_dense_3x(double2* __restrict__ v, GPos<8> tile, int lp0, int lp1, int lp2,
+                                                     GMat<3> U) {
+    __shared__ double2 xs[kBlock];
+    __shared__ double2 us[64];
+    const uint32_t t = threadIdx.x;
+    if (t < 64) us[t] = U.m[t];
+    uint64_t a = insert_zeros<8>(xcd_block(), tile.p);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) a |= uint64_t((t >> k) & 1) << tile.p[k];
+    xs[t] = ld<true>(v + a);
+    __syncthreads();
+    const uint32_t o0 = 1u << lp0, o1 = 1u << lp1, o2 = 1u << lp2;
+    const uint32_t r = ((t >> lp0) & 1) | (((t >> lp1) & 1) << 1) | (((t >> lp2) & 1) << 2);
+    const uint32_t g = t & ~(o0 | o1 | o2);
+    double2 s = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int c = 0; c < 8; ++c)
+        s = cfma(us[r + 8 * c], xs[g | ((c & 1) ? o0 : 0u) | ((c & 2) ? o1 : 0u) | ((c & 4) ? o2 : 0u)], s);
     st<true>(v + a, s);
 }
 
@@ -222,6 +245,27 @@ static int launch_dense(double2* v, int nbits, const int32_t* pos, const double*
                 hipLaunchKernelGGL(k_dense_2x, dim3(unsigned((uint64_t(1) << nbits) / kBlock)), dim3(kBlock), 0, st, v, gp, tile, lp[0],
                                    lp[1], U);
                 if (vqc_trace_on()) vqc_trace("k_dense_2x nbits %d targets %d %d", nbits, gp.p[0], gp.p[1]);
+                VQC_HIP(hipGetLastError());
+                return VQC_OK;
+            }
+        }
+        if constexpr (K == 3) {
+            if (nbits >= 8 && kDense3x && tu.dense_groups[2] == 1 && tu.dense_nontemporal[2] != 0) {
+                int m = 8;
+                while ((gp.p[0] >= m) + (gp.p[1] >= m) + (gp.p[2] >= m) + m > 8) --m;
+                GPos<8> tile;
+                int lp[3], k = 0;
+                for (int b = 0; b < m; ++b) tile.p[k++] = b;
+                for (int j = 0; j < 3; ++j) {
+                    if (gp.p[j] < m) lp[j] = gp.p[j];
+                    else {
+                        lp[j] = k;
+                        tile.p[k++] = gp.p[j];
+                    }
+                }
+                hipLaunchKernelGGL(k_dense_3x, dim3(unsigned((uint64_t(1) << nbits) / kBlock)), dim3(kBlock), 0, st, v, tile, lp[0], lp[1],
+                                   lp[2], U);
+                if (vqc_trace_on()) vqc_trace("k_dense_3x nbits %d targets %d %d %d", nbits, gp.p[0], gp.p[1], gp.p[2]);
                 VQC_HIP(hipGetLastError());
                 return VQC_OK;
             }
