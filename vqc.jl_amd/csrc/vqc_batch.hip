@@ -196,7 +196,7 @@ __device__ __forceinline__ void batch_loop(const BatchArgs& a, int& s_item, int&
             if (it.kind == kItemSmall) {
                 const uint32_t T = 1u << in.nbits;
                 warm_ops(mops, in.mop_begin, in.mop_end, pool, in.pf_lo, in.pf_n, smem + 2 * T);
-                small_eval_body<MAXKB, RKS>(in.nbits, nullptr, nullptr, mops, in.mop_begin, in.ham_begin, in.rev_begin, in.mop_end, pool, smem,
+                small_eval_body<MAXKB, RKS, KS_HAM1>(in.nbits, nullptr, nullptr, mops, in.mop_begin, in.ham_begin, in.rev_begin, in.mop_end, pool, smem,
                                 smem + T, smem + 2 * T, smem + 2 * T + kRedDoubles2, in.nrows,
                                 a.parts + row_off[in.row_begin], 1);
             } else {
@@ -219,7 +219,7 @@ __device__ __forceinline__ void batch_loop(const BatchArgs& a, int& s_item, int&
                         phase_tile<MAXKB, KS_FWDH>(p, uint64_t(tile), tdep, A, A, B, mops, pool, sa, sb, red, gacc, st);
                 } else if (p.kind == PH_HAM) {
                     for (int64_t tile = it.tile_begin; tile < it.tile_end; ++tile)
-                        phase_tile<MAXKB, KS_HAM>(p, uint64_t(tile), tdep, A, A, B, mops, pool, sa, sb, red, gacc, st);
+                        phase_tile<MAXKB, KS_HAM1>(p, uint64_t(tile), tdep, A, A, B, mops, pool, sa, sb, red, gacc, st);
                 } else {
                     for (int64_t tile = it.tile_begin; tile < it.tile_end; ++tile)
                         phase_tile<MAXKB, RKS>(p, uint64_t(tile), tdep, A, A, B, mops, pool, sa, sb, red, gacc, st);
@@ -737,6 +737,12 @@ int vqc_batch_build(int ninst, const vqc_instance_t* inst, const vqc_op_t* ops, 
         batch_free(b);
         return vqc_fail(VQC_ENOMEM, "vqc_batch_create: device allocation of %zu bytes failed (%s)", off, hipGetErrorString(e));
     }
+    // the batch kernels carry a micro-op per Hamiltonian term only (KS_HAM1)
+    for (const MicroOp& m : pb.mops)
+        if (m.kind == MK_HAMS) {
+            batch_free(b);
+            return vqc_fail(VQC_EINTERNAL, "vqc_batch_create: merged Hamiltonian terms in a batch plan");
+        }
     hipStream_t st = b->stream;
     VQC_HIP(hipMemcpyAsync(b->d_items, items.data(), items.size() * sizeof(BItem), hipMemcpyHostToDevice, st));
     VQC_HIP(hipMemcpyAsync(b->d_inst, binst.data(), binst.size() * sizeof(BInst), hipMemcpyHostToDevice, st));

@@ -1098,7 +1098,9 @@ int vqc_plan_ham(PlanBuilder& pb, int nbits, int nterms, const vqc_term_t* terms
         },
         true);
     for (size_t i = first_phase; i < pb.phases.size(); ++i) pb.phases[i].ham_accum = i > first_phase;
-    if (!getenv("VQC_NO_HAMS")) merge_ham_terms(pb, first_phase);
+    // batches plan before the values exist, so every term would be dense: there the
+    // merged pass measured 1% slower than a micro-op per term (config 2, same box)
+    if (!getenv("VQC_NO_HAMS") && !xp.deferred) merge_ham_terms(pb, first_phase);
     return VQC_OK;
 }
 

@@ -51,6 +51,8 @@ enum { PH_FWD = 0, PH_REV = 1, PH_HAM = 2 };
 enum {
     KS_FWD = (1 << MK_APPLY) | (1 << MK_BLOCK),
     KS_HAM = (1 << MK_HAM) | (1 << MK_HAMS),
+    // batches, which plan without the values and keep a micro-op per term
+    KS_HAM1 = 1 << MK_HAM,
     KS_REV = (1 << MK_APPLY2) | (1 << MK_CHAIN) | (1 << MK_BLOCK),  // state-vector reverse
     KS_REVX = (1 << MK_APPLY) | (1 << MK_APPLY2) | (1 << MK_APPLY2X) | (1 << MK_GRAD) | (1 << MK_CHAIN) | (1 << MK_BLOCK),
     // forward, with the straight-line block for the ansatz shape (chosen per
@@ -1285,7 +1287,7 @@ __device__ __forceinline__ void phase_tile(const PhaseDesc& p, uint64_t tile, ui
 // [rev_begin, mop_end), each stage in its own loop; A receives the forward state. The accumulator index
 // of every micro-op is a local row; row e is written to rows[e * row_stride].
 // LDS: sa, sb (2^nbits each), red (kRedDoubles2), gacc (nrows).
-template <int MAXKB, int RKS>
+template <int MAXKB, int RKS, int HKS = KS_HAM>
 __device__ __forceinline__ void small_eval_body(int nbits, const double2* src, double2* A, const MicroOp* __restrict__ ops,
                                                 int mop_begin, int ham_begin, int rev_begin, int mop_end,
                                                 const double2* __restrict__ pool, double2* sa, double2* sb, double2* red,
@@ -1303,7 +1305,7 @@ __device__ __forceinline__ void small_eval_body(int nbits, const double2* src, d
     if (A) {
         for (uint32_t j = threadIdx.x; j < T; j += kTileThreads) A[j] = sa[j];
     }
-    for (int o = ham_begin; o < rev_begin; ++o) o += run_mop<MAXKB, KS_HAM>(sa, sb, nbits, ops[o], pool, red, gacc, st);
+    for (int o = ham_begin; o < rev_begin; ++o) o += run_mop<MAXKB, HKS>(sa, sb, nbits, ops[o], pool, red, gacc, st);
     for (int o = rev_begin; o < mop_end; ++o) o += run_mop<MAXKB, RKS>(sa, sb, nbits, ops[o], pool, red, gacc, st);
     acc_finish(st, red, gacc);
     for (int e = threadIdx.x; e < nrows; e += kTileThreads) rows[size_t(e) * row_stride] = gacc[e];
