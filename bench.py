@@ -168,6 +168,8 @@ def _threaded_evals(vq, _lib, args, n, nthreads, seconds=1.0):
         while not stop[0]:
             _eval(vq, _lib, s, args, True)
             counts[i] += 1
+        del s
+        _lib.call("vqc_thread_release")  # this thread's stream and buffers, before it ends
 
     ts = [threading.Thread(target=work, args=(i,)) for i in range(nthreads)]
     for t in ts:

@@ -121,6 +121,12 @@ int vqc_sync(void);
 /* The HIP stream every call is ordered on (a hipStream_t), for callers that
  * time or order their own work against the library's (no reference counterpart). */
 int vqc_get_stream(void** stream);
+/* Free the calling thread's stream and work buffers now, after waiting for its
+ * work (no reference counterpart; a thread's next call makes new ones). Threads
+ * of a pool may call it before they end: the same release otherwise runs in the
+ * thread's exit handlers, where a profiler that intercepts HIP calls may already
+ * have torn down its own per-thread state. */
+int vqc_thread_release(void);
 
 /* ---- tuning (no counterpart in the reference) ----------------------------
  * Launch and fusion parameters, process-wide. A field set to -1 keeps its

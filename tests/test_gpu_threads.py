@@ -109,3 +109,44 @@ def test_small_evaluations_overlap_across_threads():
             assert Ea == Eb and np.array_equal(ga, gb)
     print(f"\n12-qubit evaluations: serial {nthreads * reps / ts:.0f}/s, {nthreads} threads {nthreads * reps / tp:.0f}/s",
           flush=True)
+
+
+def test_thread_release_then_next_call():
+    """vqc_thread_release frees the calling thread's stream and buffers; the thread's
+    next call makes new ones and gets the same results. Threads of a pool release
+    before they end."""
+    import threading
+
+    from vqchip import _lib
+
+    n = 12
+    rng = np.random.default_rng(321)
+    c = hea_circuit(n, 3, rng.uniform(0, 6.3, 9 * n))
+    H = tfim(n, 0.8)
+    psi = rand_state(n, rng)
+    E0, g0 = vq.energy_and_gradient(H, c, vq.StateVector(psi))
+    res, errs = [], []
+
+    def work():
+        try:
+            s = vq.StateVector(psi)
+            a = vq.energy_and_gradient(H, c, s)
+            _lib.call("vqc_thread_release")
+            _lib.call("vqc_thread_release")  # a second release is a no-op
+            b = vq.energy_and_gradient(H, c, s)
+            res.append((a, b))
+            del s
+            _lib.call("vqc_thread_release")
+        except Exception as e:  # reported in the main thread
+            errs.append(e)
+
+    ts = [threading.Thread(target=work) for _ in range(3)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errs, errs
+    assert len(res) == 3
+    for a, b in res:
+        for E, g in (a, b):
+            assert E == E0 and np.array_equal(g, g0)

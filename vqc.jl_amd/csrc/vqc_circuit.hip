@@ -1291,9 +1291,12 @@ struct Work {
     Work() = default;
     Work(const Work&) = delete;
     Work& operator=(const Work&) = delete;
-    ~Work() {
+    void release() {
         if (buf) (void)hipFree(buf);
+        buf = nullptr;
+        cap = 0;
     }
+    ~Work() { release(); }
 };
 static thread_local Work g_wA, g_wB, g_wparts, g_wpool, g_wmops, g_wrows, g_wphases;
 
@@ -1311,11 +1314,16 @@ struct UploadStage {
     UploadStage() = default;
     UploadStage(const UploadStage&) = delete;
     UploadStage& operator=(const UploadStage&) = delete;
-    ~UploadStage() {
+    void release() {
         if (pending) (void)hipEventSynchronize(done);
+        pending = false;
         if (done) (void)hipEventDestroy(done);
+        done = nullptr;
         if (buf) (void)hipHostFree(buf);
+        buf = nullptr;
+        cap = 0;
     }
+    ~UploadStage() { release(); }
 };
 static thread_local UploadStage g_upload;
 
@@ -1914,6 +1922,12 @@ static int plan_reverse_cut(PlanEntry& e, const CkptCut& cut, int nbits, int n, 
 }
 // Checkpoint buffers of one call (grown on demand, kept for the next call)
 static thread_local Work g_wckpt;
+
+void vqc_release_circuit_thread() {
+    for (Work* w : {&g_wA, &g_wB, &g_wparts, &g_wpool, &g_wmops, &g_wrows, &g_wphases, &g_wckpt}) w->release();
+    g_upload.release();
+    g_uploaded_plan = 0;  // the micro-op and phase buffers are gone
+}
 static int ckpt_buffers(const PlanEntry& e, uint64_t L, double2** out) {
     const size_t k = e.ck_store.size();
     if (!k) {

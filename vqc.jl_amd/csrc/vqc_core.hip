@@ -56,16 +56,29 @@ int& vqc_api_depth() {
 namespace {
 struct ThreadCtx {
     VqcCtx c;
-    ~ThreadCtx() {
+    void release() {
         if (c.stream) (void)hipStreamSynchronize(c.stream);
         if (c.scratch) (void)hipFree(c.scratch);
         if (c.pinned) (void)hipHostFree(c.pinned);
         if (c.rows) (void)hipHostFree(c.rows);
         if (c.stream) (void)hipStreamDestroy(c.stream);
+        c = VqcCtx{};
     }
+    ~ThreadCtx() { release(); }
 };
 thread_local ThreadCtx t_ctx;
 }  // namespace
+
+extern "C" int vqc_thread_release(void) {
+    // only this thread's own state: no API lock
+    VqcCtx& t = t_ctx.c;
+    if (t.stream) VQC_HIP(hipStreamSynchronize(t.stream));
+    vqc_release_circuit_thread();
+    vqc_release_expec_thread();
+    vqc_release_gates_thread();
+    t_ctx.release();
+    return VQC_OK;
+}
 
 int vqc_ctx(VqcCtx** out) {
     VqcCtx& t = t_ctx.c;

@@ -536,6 +536,10 @@ extern "C" int vqc_sv_apply_pauli_sum(const vqc_state* in, vqc_state* out, int n
 // out = sum_keys M_key in ; energy = <in|out>  ((m::QubitsOperator)(v) and
 // expectation(m, v), apply_qterms.jl:13-18 and expecs.jl:22-28)
 static thread_local DeviceBuffer g_hpool;
+void vqc_release_expec_thread() {
+    g_emat.release();
+    g_hpool.release();
+}
 
 extern "C" int vqc_sv_apply_terms(const vqc_state* in, vqc_state* out, int nterms, const vqc_term_t* terms, const double* pool,
                                   double* energy2) {

@@ -202,6 +202,7 @@ __global__ void __launch_bounds__(kBlock) k_dense_big(double2* __restrict__ v, u
 // Device copy of a host matrix for the K >= 4 kernels (stream ordered, so a
 // later call may reuse the buffer).
 static thread_local DeviceBuffer g_mat;  // this thread's matrix upload buffer
+void vqc_release_gates_thread() { g_mat.release(); }
 
 template <int K>
 static int launch_dense(double2* v, int nbits, const int32_t* pos, const double* Uh, hipStream_t st) {

@@ -82,10 +82,18 @@ struct DeviceBuffer {
     DeviceBuffer() = default;
     DeviceBuffer(const DeviceBuffer&) = delete;
     DeviceBuffer& operator=(const DeviceBuffer&) = delete;
-    ~DeviceBuffer() {
+    void release() {
         if (p) (void)hipFree(p);
+        p = nullptr;
+        cap = 0;
     }
+    ~DeviceBuffer() { release(); }
 };
+// Free the calling thread's buffers of each unit (vqc_thread_release; the
+// thread's stream has been waited for).
+void vqc_release_circuit_thread();
+void vqc_release_expec_thread();
+void vqc_release_gates_thread();
 // the tuning in force (vqc_set_tuning; initial values from the environment)
 vqc_tuning_t vqc_tuning_now();
 // whether tile_low_bits was given (vqc_set_tuning or VQC_TILE_MLOW) rather than defaulted

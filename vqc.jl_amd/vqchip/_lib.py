@@ -146,6 +146,7 @@ SIGNATURES = {
     "vqc_device_info": [ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_uint64)],
     "vqc_sync": [],
     "vqc_get_stream": [ctypes.POINTER(_P)],
+    "vqc_thread_release": [],
     "vqc_set_tuning": [ctypes.POINTER(Tuning)],
     "vqc_get_tuning": [ctypes.POINTER(Tuning)],
     "vqc_set_tile_table": [_I, _I32, ctypes.POINTER(TileEntry)],
