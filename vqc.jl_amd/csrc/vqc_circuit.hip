@@ -25,8 +25,9 @@ void vqc_zero(double2* v, uint64_t len, hipStream_t st);
 
 // Tile shape. Tile-bit fields left at 0 resolve by what is evaluated. One
 // state of nbits index bits (nbits >= 0) runs on one workgroup up to 9 bits,
-// else in tiles of 9 bits up to 14, 10 up to 18, 11 up to 20 and 12 (forward) /
-// 11 (reverse) above: 2^(nbits - t) tiles must keep enough CUs busy, while
+// else in tiles of 9 bits up to 14, 10 up to 18, 11 up to 22 and 12 (forward) /
+// 11 (reverse) above, with 3 contiguous low bits from 21 bits on (22 qubits:
+// 22.2 -> 19.5 ms at 11 / 11 against 12 / 11): 2^(nbits - t) tiles must keep enough CUs busy, while
 // each phase, one launch, covers fewer gates the smaller its tiles (the sizes
 // measured fastest for layered circuits of 8 to 20 qubits on MI355X). A wide
 // batch (nbits < 0), whose instances fill the device together, takes the
@@ -39,7 +40,7 @@ TileTuning vqc_tile_tuning(int nbits) {
     int fwd = kMaxTileBits, rev = kMaxTileBits2, small = 12;
     if (nbits >= 0) {
         small = 9;
-        fwd = nbits <= 14 ? 9 : (nbits <= 18 ? 10 : (nbits <= 20 ? 11 : 12));
+        fwd = nbits <= 14 ? 9 : (nbits <= 18 ? 10 : (nbits <= 22 ? 11 : 12));
         rev = std::min(fwd, 11);
     }
     // the autotuner's entry for this size (vqc_set_tile_table) replaces the rule
@@ -53,6 +54,9 @@ TileTuning vqc_tile_tuning(int nbits) {
     t.t_small = std::min(kMaxTileBits2, std::max(1, g.tile_small_bits ? int(g.tile_small_bits) : small));
     t.mlow = std::min(8, std::max(0, int(g.tile_low_bits)));
     if (nbits < 0 && !vqc_tuning_low_bits_given()) t.mlow = 5;
+    // states of 21 qubits and more: 3 contiguous low bits (HEA energy + gradient, one
+    // box: 24 qubits 75.8 -> 73.4 ms, 26 qubits 375.7 -> 356.9 ms)
+    if (nbits >= 21 && !vqc_tuning_low_bits_given()) t.mlow = 3;
     if (te.low && !vqc_tuning_low_bits_given()) t.mlow = te.low;
     return t;
 }
