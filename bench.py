@@ -364,8 +364,8 @@ def bench_config3(vq, _lib):
     return out
 
 
-# HBM bytes one one-qubit sweep launch moves at 28 qubits (k_dense_1x and
-# k_dense_small<1, 1, true> alike), from two rocprofv3 --pmc passes (FETCH_SIZE x 2.0
+# HBM bytes one one-qubit sweep launch moves at 28 qubits (k_dense_1x; the same
+# for the pair-per-thread form), from two rocprofv3 --pmc passes (FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0, the factors a copy of known size reads;
 # profiles/hbm_traffic_dense_28q.md). Algorithmic: 2 * 16 * 2^28 = 8.590e9.
 TRAFFIC_1Q_BYTES = 8.5900e9
@@ -628,8 +628,8 @@ def main():
     log(f"unfused: {ms / args.steps:.1f} ms/step, {gbs:.0f} GB/s (wall {wall:.2f}s)")
     norm = state.norm()
 
-    # dominant kernel class: the one-qubit dense sweeps, timed alone (k_dense_1x for
-    # targets up to bit 12, k_dense_small<1, 1, true> above)
+    # dominant kernel class: the one-qubit dense sweeps, timed alone (k_dense_1x<256>
+    # for targets up to bit 12, k_dense_1x<512> above)
     one_q = [m for m, g in zip(mats, circuit) if len(g.positions) == 1]
     timer.start()
     for k, p, U, _keep in one_q:
@@ -761,7 +761,7 @@ def main():
                    "qubits": n, "gates": G, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(ach1, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(ach1 / HBM_PEAK_GBS, 4), "traffic": TRAFFIC_1Q_BYTES if n == 28 else None,
-                     "kernel": "1-qubit dense sweep: k_dense_1x (targets 0-12), k_dense_small<1, 1, true> (13-27)",
+                     "kernel": "1-qubit dense sweep: k_dense_1x<256> (targets 0-12), k_dense_1x<512> (13-27)",
                      "avg_launch_ms": round(avg1_ms, 4), "bytes_per_launch": bytes_per_gate},
         "cpu_baseline": cpu,
         "extra": extra,

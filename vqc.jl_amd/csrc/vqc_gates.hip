@@ -87,32 +87,35 @@ __global__ void __launch_bounds__(kBlock) k_dense_small(double2* __restrict__ v,
     }
 }
 
-// K = 1 with one amplitude per thread: a workgroup holds 128 whole pairs (the 256
-// contiguous amplitudes around a target below bit 8, else 128 consecutive
-// amplitudes from each side of the pair). Each thread loads its own amplitude,
-// takes its partner's from LDS and stores its own row: one load and one store per
-// thread over contiguous addresses, the access shape of a plain in-place stream,
-// whatever the target. Standalone probe on a 28-qubit state: 6.65-6.86 TB/s for
-// targets 0-12 against 6.35-6.72 for the pair-per-thread form (4.2-5.1 for
-// targets 0-2, which had to stay temporal there); mixed above bit 12, where the
-// pair form stays.
-constexpr int kDense1xMaxTarget = 12;
+// K = 1 with one amplitude per thread: a workgroup of BS threads holds BS / 2 whole
+// pairs (the BS contiguous amplitudes around a target below log2(BS), else BS / 2
+// consecutive amplitudes from each side of the pair). Each thread loads its own
+// amplitude, takes its partner's from LDS and stores its own row: one load and one
+// store per thread over contiguous addresses, the access shape of a plain in-place
+// stream, whatever the target. Standalone probe on a 28-qubit state: 6.65-6.86
+// TB/s for targets 0-12 with 256 threads, against 6.35-6.72 for the pair-per-thread
+// form (4.2-5.1 for targets 0-2, which had to stay temporal there); for targets
+// 13-27, 512 threads (4 KiB from each side) average 6373 GB/s against 6273 for the
+// pair form and 6289 with 256 threads.
+constexpr int kDense1xWideFrom = 13;  // targets from this bit on take 512 threads
 constexpr bool kDense2x = true;
 constexpr bool kDense3x = true;
-__global__ void __launch_bounds__(kBlock) k_dense_1x(double2* __restrict__ v, int q, GMat<1> U) {
-    __shared__ double2 xs[kBlock];
+template <int BS>
+__global__ void __launch_bounds__(BS) k_dense_1x(double2* __restrict__ v, int q, GMat<1> U) {
+    __shared__ double2 xs[BS];
+    constexpr int LB = BS == 512 ? 9 : 8;  // log2(BS)
     const uint32_t t = threadIdx.x;
     const uint64_t b = xcd_block();
     uint64_t a;
     uint32_t pt;  // the partner's thread
-    if (q < 8) {
-        a = b * kBlock + t;
+    if (q < LB) {
+        a = b * BS + t;
         pt = t ^ (1u << q);
     } else {
-        const uint64_t g = b * (kBlock / 2) + (t & (kBlock / 2 - 1));
+        const uint64_t g = b * (BS / 2) + (t & (BS / 2 - 1));
         const uint64_t lo = g & ((uint64_t(1) << q) - 1);
-        a = (((g - lo) << 1) | lo) + (uint64_t(t / (kBlock / 2)) << q);
-        pt = t ^ (kBlock / 2);
+        a = (((g - lo) << 1) | lo) + (uint64_t(t / (BS / 2)) << q);
+        pt = t ^ (BS / 2);
     }
     const double2 x = ld<true>(v + a);
     xs[t] = x;
@@ -221,8 +224,11 @@ static int launch_dense(double2* v, int nbits, const int32_t* pos, const double*
         // fall in a few 128-byte lines; those sweeps stay temporal.
         const vqc_tuning_t tu = vqc_tuning_now();
         if constexpr (K == 1) {
-            if (nbits >= 8 && gp.p[0] <= kDense1xMaxTarget && tu.dense_groups[0] == 1 && tu.dense_nontemporal[0] != 0) {
-                hipLaunchKernelGGL(k_dense_1x, dim3(unsigned((uint64_t(1) << nbits) / kBlock)), dim3(kBlock), 0, st, v, gp.p[0], U);
+            if (nbits >= 9 && tu.dense_groups[0] == 1 && tu.dense_nontemporal[0] != 0) {
+                if (gp.p[0] < kDense1xWideFrom)
+                    hipLaunchKernelGGL((k_dense_1x<256>), dim3(unsigned((uint64_t(1) << nbits) / 256)), dim3(256), 0, st, v, gp.p[0], U);
+                else
+                    hipLaunchKernelGGL((k_dense_1x<512>), dim3(unsigned((uint64_t(1) << nbits) / 512)), dim3(512), 0, st, v, gp.p[0], U);
                 if (vqc_trace_on()) vqc_trace("k_dense_1x nbits %d target %d", nbits, gp.p[0]);
                 VQC_HIP(hipGetLastError());
                 return VQC_OK;
