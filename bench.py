@@ -368,7 +368,7 @@ def bench_config3(vq, _lib):
 # for the pair-per-thread form), from two rocprofv3 --pmc passes (FETCH_SIZE x 2.0
 # + WRITE_SIZE x 1.0, the factors a copy of known size reads;
 # profiles/hbm_traffic_dense_28q.md). Algorithmic: 2 * 16 * 2^28 = 8.590e9.
-TRAFFIC_1Q_BYTES = 8.5900e9
+TRAFFIC_1Q_BYTES = 8.5901e9
 
 
 def bench_config4(vq, _lib, count, seed=20261203):
