@@ -98,8 +98,7 @@ __global__ void __launch_bounds__(kBlock) k_dense_small(double2* __restrict__ v,
 // 13-27, 512 threads (4 KiB from each side) average 6373 GB/s against 6273 for the
 // pair form and 6289 with 256 threads.
 constexpr int kDense1xWideFrom = 13;  // targets from this bit on take 512 threads
-constexpr bool kDense2x = true;
-constexpr bool kDense3x = true;
+constexpr bool kDenseKx = true;  // K = 2, 3 in the one-amplitude form below
 template <int BS>
 __global__ void __launch_bounds__(BS) k_dense_1x(double2* __restrict__ v, int q, GMat<1> U) {
     __shared__ double2 xs[BS];
@@ -127,55 +126,55 @@ __global__ void __launch_bounds__(BS) k_dense_1x(double2* __restrict__ v, int q,
     st<true>(v + a, r);
 }
 
-// K = 2 with one amplitude per thread: an 8-bit tile per workgroup made of the m
-// lowest bits and the targets at or above m (m = 8, 7 or 6 as 0, 1 or 2 targets lie
-// at or above it). Local bits below m are the state's own, so a wave still reads
-// contiguous amplitudes; a target at or above m sits on local bit 6 or 7. Each
-// thread loads its amplitude, reads the group's other three and its matrix row from
-// LDS, and stores its own row (the same sum, in the same order, as the
-// group-per-thread form).
-__global__ void __launch_bounds__(kBlock) k_dense_2x(double2* __restrict__ v, GPos<2> gp, GPos<8> tile, int lp0, int lp1,
-                                                     GMat<2> U) {
-    __shared__ double2 xs[kBlock];
-    __shared__ double2 us[16];
+// K = 2 and 3 with one amplitude per thread. A workgroup holds a tile of 2^LB
+// amplitudes (LB = 8 for K = 2, 9 for K = 3): the m lowest state bits in place and
+// the targets at or above m on the next local bits (m = LB - that count), so a wave
+// still reads contiguous amplitudes. Each thread loads one amplitude into LDS. Wave w
+// then computes row w of the gate for the 64 groups of the tile, lane l for group l:
+// the row is wave-uniform, so its 2^K entries come from the kernel arguments through
+// scalar loads, and only the group's members are read from LDS. When the targets sit
+// on local bits 6 and up (LP_TOP), the output of (group l, row w) is lane l's own
+// amplitude and is stored at once; otherwise it goes back through LDS to the thread
+// that loaded that position. The sum runs over the same terms in the same order as
+// the group-per-thread form, so the results are bit-identical to it. Standalone probe
+// on a 28-qubit state, 16 random target sets: 6146 -> 6405 GB/s for K = 2 and
+// 5920 -> 6387 for K = 3 against reading the row from LDS as well (the copy-only
+// access pattern of the same tiles: 6398 and 6436).
+template <int K, bool LP_TOP>
+__global__ void __launch_bounds__(64 << K) k_dense_kx(double2* __restrict__ v, GPos<K + 6> tile, GPos<K> lp, GMat<K> U) {
+    constexpr int LB = K + 6;
+    constexpr uint32_t BS = 1u << LB;
+    __shared__ double2 xs[BS];
+    __shared__ double2 ys[LP_TOP ? 1 : BS];
     const uint32_t t = threadIdx.x;
-    if (t < 16) us[t] = U.m[t];
-    // local bit k of t sits on state bit tile.p[k]
-    uint64_t a = insert_zeros<8>(xcd_block(), tile.p);
+    uint64_t a = insert_zeros<LB>(xcd_block(), tile.p);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) a |= uint64_t((t >> k) & 1) << tile.p[k];
+    for (int k = 0; k < LB; ++k) a |= uint64_t((t >> k) & 1) << tile.p[k];
     xs[t] = ld<true>(v + a);
     __syncthreads();
-    const uint32_t r = ((t >> lp0) & 1) | (((t >> lp1) & 1) << 1);
-    const uint32_t g = t & ~((1u << lp0) | (1u << lp1));
-    const uint32_t o0 = 1u << lp0, o1 = 1u << lp1;
-    double2 s = cfma(us[r], xs[g], make_double2(0.0, 0.0));
-    s = cfma(us[r + 4], xs[g | o0], s);
-    s = cfma(us[r + 8], xs[g | o1], s);
-    s = cfma(us[r + 12], xs[g | o0 | o1], s);
-    st<true>(v + a, s);
-}
-
-// K = 3 in the same way: the m lowest bits (m = 5..8) and the targets at or above m.
-__global__ void __launch_bounds__(kBlock) k_dense_3x(double2* __restrict__ v, GPos<8> tile, int lp0, int lp1, int lp2,
-                                                     GMat<3> U) {
-    __shared__ double2 xs[kBlock];
-    __shared__ double2 us[64];
-    const uint32_t t = threadIdx.x;
-    if (t < 64) us[t] = U.m[t];
-    uint64_t a = insert_zeros<8>(xcd_block(), tile.p);
+    const uint32_t w = __builtin_amdgcn_readfirstlane(t >> 6);  // the row: uniform per wave
+    uint32_t o[K];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) a |= uint64_t((t >> k) & 1) << tile.p[k];
-    xs[t] = ld<true>(v + a);
-    __syncthreads();
-    const uint32_t o0 = 1u << lp0, o1 = 1u << lp1, o2 = 1u << lp2;
-    const uint32_t r = ((t >> lp0) & 1) | (((t >> lp1) & 1) << 1) | (((t >> lp2) & 1) << 2);
-    const uint32_t g = t & ~(o0 | o1 | o2);
+    for (int j = 0; j < K; ++j) o[j] = 1u << lp.p[j];
+    const uint32_t g = LP_TOP ? (t & 63u) : uint32_t(insert_zeros<K>(t & 63u, lp.p));
     double2 s = make_double2(0.0, 0.0);
 #pragma unroll
-    for (int c = 0; c < 8; ++c)
-        s = cfma(us[r + 8 * c], xs[g | ((c & 1) ? o0 : 0u) | ((c & 2) ? o1 : 0u) | ((c & 4) ? o2 : 0u)], s);
-    st<true>(v + a, s);
+    for (int c = 0; c < (1 << K); ++c) {
+        uint32_t e = g;
+#pragma unroll
+        for (int j = 0; j < K; ++j) e |= ((c >> j) & 1) ? o[j] : 0u;
+        s = cfma(U.m[w + (1u << K) * c], xs[e], s);
+    }
+    if constexpr (LP_TOP) {
+        st<true>(v + a, s);
+    } else {
+        uint32_t e = g;
+#pragma unroll
+        for (int j = 0; j < K; ++j) e |= ((w >> j) & 1) ? o[j] : 0u;
+        ys[e] = s;
+        __syncthreads();
+        st<true>(v + a, ys[t]);
+    }
 }
 
 // K = 4, 5: the matrix (4 or 16 KiB) is staged once per block in LDS and read
@@ -234,45 +233,37 @@ static int launch_dense(double2* v, int nbits, const int32_t* pos, const double*
                 return VQC_OK;
             }
         }
-        if constexpr (K == 2) {
-            if (nbits >= 8 && kDense2x && tu.dense_groups[1] == 1 && tu.dense_nontemporal[1] != 0) {
-                // m low bits plus the targets at or above m: 8 bits in all
-                int m = 8;
-                while ((gp.p[0] >= m) + (gp.p[1] >= m) + m > 8) --m;
-                GPos<8> tile;
-                int lp[2], k = 0;
+        if constexpr (K == 2 || K == 3) {
+            constexpr int LB = K + 6;
+            if (nbits >= LB && kDenseKx && tu.dense_groups[K - 1] == 1 && tu.dense_nontemporal[K - 1] != 0) {
+                // the m lowest bits plus the targets at or above m: LB bits in all
+                int m = LB;
+                for (;;) {
+                    int above = 0;
+                    for (int j = 0; j < K; ++j) above += gp.p[j] >= m;
+                    if (above + m <= LB) break;
+                    --m;
+                }
+                GPos<LB> tile;
+                GPos<K> lp;
+                int k = 0;
                 for (int b = 0; b < m; ++b) tile.p[k++] = b;
-                for (int j = 0; j < 2; ++j) {
-                    if (gp.p[j] < m) lp[j] = gp.p[j];
+                for (int j = 0; j < K; ++j) {
+                    if (gp.p[j] < m) lp.p[j] = gp.p[j];
                     else {
-                        lp[j] = k;
+                        lp.p[j] = k;
                         tile.p[k++] = gp.p[j];
                     }
                 }
-                hipLaunchKernelGGL(k_dense_2x, dim3(unsigned((uint64_t(1) << nbits) / kBlock)), dim3(kBlock), 0, st, v, gp, tile, lp[0],
-                                   lp[1], U);
-                if (vqc_trace_on()) vqc_trace("k_dense_2x nbits %d targets %d %d", nbits, gp.p[0], gp.p[1]);
-                VQC_HIP(hipGetLastError());
-                return VQC_OK;
-            }
-        }
-        if constexpr (K == 3) {
-            if (nbits >= 8 && kDense3x && tu.dense_groups[2] == 1 && tu.dense_nontemporal[2] != 0) {
-                int m = 8;
-                while ((gp.p[0] >= m) + (gp.p[1] >= m) + (gp.p[2] >= m) + m > 8) --m;
-                GPos<8> tile;
-                int lp[3], k = 0;
-                for (int b = 0; b < m; ++b) tile.p[k++] = b;
-                for (int j = 0; j < 3; ++j) {
-                    if (gp.p[j] < m) lp[j] = gp.p[j];
-                    else {
-                        lp[j] = k;
-                        tile.p[k++] = gp.p[j];
-                    }
-                }
-                hipLaunchKernelGGL(k_dense_3x, dim3(unsigned((uint64_t(1) << nbits) / kBlock)), dim3(kBlock), 0, st, v, tile, lp[0], lp[1],
-                                   lp[2], U);
-                if (vqc_trace_on()) vqc_trace("k_dense_3x nbits %d targets %d %d %d", nbits, gp.p[0], gp.p[1], gp.p[2]);
+                if (k != LB) return vqc_fail(VQC_EINTERNAL, "dense sweep tile of %d bits instead of %d", k, LB);
+                bool top = true;
+                for (int j = 0; j < K; ++j) top = top && lp.p[j] == 6 + j;
+                const dim3 grid(unsigned((uint64_t(1) << nbits) >> LB)), blk(1u << LB);
+                if (top) hipLaunchKernelGGL((k_dense_kx<K, true>), grid, blk, 0, st, v, tile, lp, U);
+                else hipLaunchKernelGGL((k_dense_kx<K, false>), grid, blk, 0, st, v, tile, lp, U);
+                if (vqc_trace_on())
+                    vqc_trace("k_dense_kx<%d,%d> nbits %d targets %d %d %d", K, int(top), nbits, gp.p[0], gp.p[1],
+                              K > 2 ? gp.p[K - 1] : -1);
                 VQC_HIP(hipGetLastError());
                 return VQC_OK;
             }
