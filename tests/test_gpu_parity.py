@@ -27,11 +27,12 @@ def sv_from(v):
     return vq.StateVector(np.ascontiguousarray(v))
 
 
-@pytest.mark.parametrize("k,nb", [(1, 8), (2, 8), (3, 8), (4, 7), (5, 7), (1, 9), (2, 9), (3, 9)])
+@pytest.mark.parametrize("k,nb", [(1, 8), (2, 8), (3, 8), (4, 7), (5, 7), (1, 9), (2, 9), (3, 9), (4, 10)])
 def test_dense_gate_every_target_class(k, nb):
-    """Every target set of a k-qubit gate on nb qubits. At 9 qubits the 3-qubit gate
-    takes the 512-amplitude one-amplitude-per-thread tile (at 8 the group-per-thread
-    form), and the sets with every target at bit 6 or above take its direct-store form."""
+    """Every target set of a k-qubit gate on nb qubits. From k + 6 qubits on, a 2- or
+    3-qubit gate takes the 2^(k+6)-amplitude one-amplitude-per-thread tile (below, the
+    group-per-thread form), and the sets with every target at bit 6 or above take its
+    direct-store form. 4- and 5-qubit gates run group-per-thread at every size."""
     rng = np.random.default_rng(1000 + k + 100 * nb)
     for pos in itertools.combinations(range(nb), k):
         v = rand_state(nb, rng)
@@ -44,6 +45,7 @@ def test_dense_gate_every_target_class(k, nb):
 
 @pytest.mark.parametrize("nb,k,pos", [(18, 1, (0,)), (18, 1, (17,)), (18, 2, (3, 16)), (18, 3, (0, 9, 17)),
                                       (18, 2, (12, 17)), (18, 3, (6, 13, 17)), (18, 3, (2, 4, 5)),
+                                      (18, 4, (6, 9, 12, 17)), (18, 4, (0, 3, 8, 15)),
                                       (17, 4, (1, 5, 9, 16)), (17, 5, (0, 2, 7, 11, 16)), (20, 1, (10,))])
 def test_dense_gate_multiblock(nb, k, pos):
     rng = np.random.default_rng(nb * 7 + k)
